@@ -1,0 +1,123 @@
+/*
+ * libdips_hip.so — C-ABI of the MI355X (gfx950) NBM Poisson training step.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t
+ * (passed as void*), enqueues work on that stream only, never synchronises
+ * unless stated, never allocates device memory, and returns 0 on success or a
+ * nonzero status with a message retrievable through dips_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the JAX-DIPS repository):
+ *   dips_loss_grad   <- Trainer.loss + value_and_grad(loss)
+ *                       (jax_dips/solvers/poisson/trainer.py:786, :892-912;
+ *                        per-point residual discretization.py:299-519)
+ *   dips_opt_step    <- optimizer.update + optax.apply_updates
+ *                       (trainer.py:787-788; solvers/optimizers.py:33-97)
+ *   dips_train_level <- the batch/epoch lax.scan of single_GPU_train for one
+ *                       zoom level (trainer.py:501-591)
+ *   dips_geom_*      <- geometry/geometric_integrations_per_point.py and the
+ *                       regression rows of discretization.py:238-296
+ *                       (see dips_geom.h section below)
+ *   dips_eval        <- Trainer.evaluate_solution_fn / gradients
+ *                       (trainer.py:836-844, :914-977)
+ */
+#ifndef DIPS_HIP_H
+#define DIPS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define DIPS_OK 0
+#define DIPS_EINVAL 1
+#define DIPS_EUNSUPPORTED 2
+#define DIPS_EHIP 3
+
+/* Training grid: float32 axes as built by jnp.linspace (domain/mesh.py:121-153);
+ * point index n = (i*ny + j)*nz + k ('ij' meshgrid, C order). */
+typedef struct {
+  const float* xs;
+  const float* ys;
+  const float* zs;
+  int32_t nx, ny, nz;
+} dips_grid_t;
+
+/* Affine residual records of one zoom level:
+ *   r(row) = row_c[row] + sum_{e in [row_ptr[row], row_ptr[row+1])} ent_w[e] * NN(pos_e)
+ * with pos_e = fl32(p + o_e * h), p = grid point row_point[row], o_e decoded
+ * from ent_code[e] bits 0..6 as 25*(oz+2) + 5*(oy+2) + (ox+2), and NN the
+ * Omega+ network when ent_code[e] bit 7 is set, the Omega- network otherwise.
+ * Rows of one batch are contiguous.  All arrays live in device memory. */
+typedef struct {
+  const int32_t* row_point;
+  const int32_t* row_ptr;
+  const float* row_c;
+  const uint32_t* ent_code;
+  const float* ent_w;
+  int64_t nrows;
+  int64_t nnz;
+  float hx, hy, hz;
+} dips_records_t;
+
+/* Two-phase tanh MLP (jax_dips/nn/mlp/MLP.py:93-139): L hidden layers of width
+ * H per phase.  Flat parameters: the Omega+ net then the Omega- net, each as
+ * W0[3][H], b0[H], (W[H][H], b[H]) * (L-1), Wout[H][1], bout[1]. */
+typedef struct {
+  int32_t Lm, Hm, Lp, Hp;
+} dips_net_t;
+
+/* Optimizer (solvers/optimizers.py:13-97).
+ * kind: 0 = custom (clip_by_global_norm -> adam -> schedule -> scale(-1)),
+ *       1 = adam (constant lr), 2 = rmsprop.
+ * sched: 0 = exponential_decay (smooth), 1 = polynomial (power 1, end 0). */
+typedef struct {
+  int32_t kind;
+  int32_t sched;
+  float lr;
+  float decay_rate;
+  float transition_steps;
+  float max_norm;
+} dips_opt_t;
+
+const char* dips_last_error(void);
+int dips_version(void);
+
+/* Number of float32 parameters of the two-phase network (both phases). */
+int64_t dips_param_count(const dips_net_t* net);
+/* Floats of scratch dips_loss_grad writes for nrows rows: blocks * (P + 1). */
+int64_t dips_partial_size(const dips_net_t* net, int64_t nrows);
+/* 1 if the network shape has a compiled kernel, else 0. */
+int dips_net_supported(const dips_net_t* net);
+
+/* Forward residual + loss (and, when want_grad != 0, the parameter gradient)
+ * of rows [row_begin, row_begin + nrows) of rec.  Writes per-block partials
+ * (gradient sums then 0.5*sum r^2) into partial[0 .. dips_partial_size) and the
+ * number of blocks into *nblk (host pointer). */
+int dips_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
+                   const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
+                   int want_grad, int32_t* nblk, void* stream);
+
+/* Reduce partials in a fixed order (float64 accumulation), scale by
+ * inv_batch (the loss mean and gradient mean), apply the optimizer in place
+ * to params/m/v, write the batch loss to loss_hist[*count] (skipped when
+ * *count is outside [0, loss_cap)) and increment the device counter *count. */
+int dips_opt_step(const dips_net_t* net, const dips_opt_t* opt, const float* partial, int32_t nblk,
+                  float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+                  int32_t loss_cap, void* stream);
+
+/* nepochs epochs of one zoom level: for each epoch, for each batch b, rows
+ * [batch_row_begin[b], batch_row_begin[b] + batch_rows[b]) (host arrays),
+ * one dips_loss_grad + dips_opt_step with inv_batch = 1/batch_size.
+ * loss_hist receives one loss per optimizer step at index *count. */
+int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const int64_t* batch_row_begin,
+                     const int64_t* batch_rows, int32_t nbatches, int64_t batch_size, int32_t nepochs,
+                     const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
+                     int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIPS_HIP_H */

@@ -1,0 +1,123 @@
+// extern "C" entry points of libdips_hip.so (declared in include/dips_hip.h).
+#include <string>
+
+#include "dips_common.h"
+#include "dips_hip.h"
+
+namespace dips {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_status(hipError_t e, const char* what) {
+  if (e == hipSuccess) return DIPS_OK;
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return DIPS_EHIP;
+}
+
+int net_supported(const dips_net_t* n);
+
+static int check_net(const dips_net_t* net) {
+  if (!net || net->Lm < 1 || net->Lp < 1 || net->Hm < 1 || net->Hp < 1) {
+    set_error("invalid network dimensions");
+    return DIPS_EINVAL;
+  }
+  return DIPS_OK;
+}
+
+static int check_records(const dips_grid_t* grid, const dips_records_t* rec) {
+  if (!grid || !rec || !grid->xs || !grid->ys || !grid->zs || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) {
+    set_error("invalid grid");
+    return DIPS_EINVAL;
+  }
+  if (!rec->row_point || !rec->row_ptr || !rec->row_c || rec->nrows < 1 || rec->nnz < 0 ||
+      (rec->nnz > 0 && (!rec->ent_code || !rec->ent_w))) {
+    set_error("invalid records");
+    return DIPS_EINVAL;
+  }
+  return DIPS_OK;
+}
+
+}  // namespace dips
+
+using namespace dips;
+
+extern "C" {
+
+const char* dips_last_error(void) { return g_last_error.c_str(); }
+
+int dips_version(void) { return 1; }
+
+int64_t dips_param_count(const dips_net_t* net) {
+  if (check_net(net)) return -1;
+  return partial_stride(net) - 1;
+}
+
+int64_t dips_partial_size(const dips_net_t* net, int64_t nrows) {
+  if (check_net(net) || nrows < 1) return -1;
+  return partial_blocks(nrows) * (int64_t)partial_stride(net);
+}
+
+int dips_net_supported(const dips_net_t* net) {
+  if (check_net(net)) return 0;
+  return net_supported(net);
+}
+
+int dips_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
+                   const dips_net_t* net, const float* params, float* partial, int64_t partial_cap, int want_grad,
+                   int32_t* nblk, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!params || !partial || !nblk) {
+    set_error("dips_loss_grad: null buffer");
+    return DIPS_EINVAL;
+  }
+  int nb = 0;
+  st = loss_grad_launch(grid, rec, row_begin, nrows, net, params, partial, partial_cap, want_grad, &nb,
+                        (hipStream_t)stream);
+  *nblk = nb;
+  return st;
+}
+
+int dips_opt_step(const dips_net_t* net, const dips_opt_t* opt, const float* partial, int32_t nblk,
+                  float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+                  int32_t loss_cap, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if (!opt || !partial || !params || !m || !v || !count || !loss_hist) {
+    set_error("dips_opt_step: null buffer");
+    return DIPS_EINVAL;
+  }
+  return opt_step_launch(net, opt, partial, nblk, inv_batch, params, m, v, count, loss_hist, loss_cap,
+                         (hipStream_t)stream);
+}
+
+int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const int64_t* batch_row_begin,
+                     const int64_t* batch_rows, int32_t nbatches, int64_t batch_size, int32_t nepochs,
+                     const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
+                     int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
+                     void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!batch_row_begin || !batch_rows || nbatches < 1 || batch_size < 1 || nepochs < 0) {
+    set_error("dips_train_level: invalid batch description");
+    return DIPS_EINVAL;
+  }
+  const float inv_b = 1.0f / (float)batch_size;
+  for (int e = 0; e < nepochs; ++e) {
+    for (int b = 0; b < nbatches; ++b) {
+      int nb = 0;
+      st = loss_grad_launch(grid, rec, batch_row_begin[b], batch_rows[b], net, params, partial, partial_cap, 1, &nb,
+                            (hipStream_t)stream);
+      if (st) return st;
+      st = opt_step_launch(net, opt, partial, nb, inv_b, params, m, v, count, loss_hist, loss_cap, (hipStream_t)stream);
+      if (st) return st;
+    }
+  }
+  return DIPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+"""ctypes binding of libdips_hip.so (the C-ABI declared in include/dips_hip.h).
+
+The library is loaded from the package directory.  There is no fallback: if the
+library is missing or fails to load, every entry point raises.  Device buffers
+are passed as raw pointers (torch tensors are used only to own device memory
+and to name the current HIP stream).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdips_hip.so")
+
+DIPS_OK = 0
+
+
+class DipsError(RuntimeError):
+    pass
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("zs", ctypes.c_void_p),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32)]
+
+
+class Records(ctypes.Structure):
+    _fields_ = [("row_point", ctypes.c_void_p), ("row_ptr", ctypes.c_void_p), ("row_c", ctypes.c_void_p),
+                ("ent_code", ctypes.c_void_p), ("ent_w", ctypes.c_void_p), ("nrows", ctypes.c_int64),
+                ("nnz", ctypes.c_int64), ("hx", ctypes.c_float), ("hy", ctypes.c_float), ("hz", ctypes.c_float)]
+
+
+class Net(ctypes.Structure):
+    _fields_ = [("Lm", ctypes.c_int32), ("Hm", ctypes.c_int32), ("Lp", ctypes.c_int32), ("Hp", ctypes.c_int32)]
+
+
+class Opt(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("sched", ctypes.c_int32), ("lr", ctypes.c_float),
+                ("decay_rate", ctypes.c_float), ("transition_steps", ctypes.c_float), ("max_norm", ctypes.c_float)]
+
+
+# name -> (restype, argtypes); must match include/dips_hip.h
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+SIGNATURES = {
+    "dips_last_error": (ctypes.c_char_p, []),
+    "dips_version": (ctypes.c_int, []),
+    "dips_param_count": (_I64, [ctypes.POINTER(Net)]),
+    "dips_partial_size": (_I64, [ctypes.POINTER(Net), _I64]),
+    "dips_net_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
+    "dips_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _I64, _I64, ctypes.POINTER(Net),
+                                      _P, _P, _I64, ctypes.c_int, ctypes.POINTER(_I32), _P]),
+    "dips_opt_step": (ctypes.c_int, [ctypes.POINTER(Net), ctypes.POINTER(Opt), _P, _I32, ctypes.c_float, _P, _P, _P,
+                                     _P, _P, _I32, _P]),
+    "dips_train_level": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(_I64),
+                                        ctypes.POINTER(_I64), _I32, _I64, _I32, ctypes.POINTER(Net),
+                                        ctypes.POINTER(Opt), _P, _P, _P, _P, _P, _I32, _P, _I64, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load (once) and return the library, raising DipsError if unavailable."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DipsError(f"{LIB_PATH} is not built; run jax-dips_amd/build.py (or __graft_entry__.build())")
+        try:
+            L = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise DipsError(f"cannot load {LIB_PATH}: {e}") from e
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what):
+    if status != DIPS_OK:
+        msg = lib().dips_last_error().decode()
+        raise DipsError(f"{what} failed (status {status}): {msg}")
+
+
+def ptr(t):
+    """Raw device pointer of a torch tensor (or None)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def current_stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

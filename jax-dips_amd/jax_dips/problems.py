@@ -1,0 +1,153 @@
+"""Interface problems of the benchmark workloads, as user callbacks.
+
+Every callback takes coordinate-first positions ``r`` (shape (3, m)) and uses
+the ``xnp`` namespace, so the same function samples on numpy (host) or torch
+(device) arrays.  The manufactured pairs follow the recipe of the reference's
+test configurations (tests/confs/experiment_configs.py:21-412): f = k u - div(mu
+grad u) on each side, alpha = u+ - u-, and beta in the reference's sign
+convention beta = -(mu+ grad u+ - mu- grad u-) . n (experiment_configs.py:85-96,
+the explicit ``* (-1.0)``), with n the normalised level-set gradient and a
+nan_to_num guard where that gradient vanishes (experiment_configs.py:295).
+phi is perturbed by sign+-(phi) * 1e-10 (geometry/level_set.py:34-48).
+"""
+import numpy as np
+
+from . import xnp
+
+
+def perturb(phi):
+    """phi + sign+-(phi) * 1e-10 with sign+-(0) = -1 (level_set.py:34-48)."""
+
+    def f(r):
+        v = phi(r)
+        return v + xnp.where(v > 0, 1.0, -1.0) * 1e-10
+
+    return f
+
+
+def _u_pair():
+    """u- = exp(-|x|^2), u+ = sin x cos y sin z and their gradients/laplacians."""
+
+    def um(r):
+        return xnp.exp(-(r[0] ** 2 + r[1] ** 2 + r[2] ** 2))
+
+    def up(r):
+        return xnp.sin(r[0]) * xnp.cos(r[1]) * xnp.sin(r[2])
+
+    def grad_um(r):
+        e = um(r)
+        return (-2 * r[0] * e, -2 * r[1] * e, -2 * r[2] * e)
+
+    def grad_up(r):
+        return (xnp.cos(r[0]) * xnp.cos(r[1]) * xnp.sin(r[2]),
+                -xnp.sin(r[0]) * xnp.sin(r[1]) * xnp.sin(r[2]),
+                xnp.sin(r[0]) * xnp.cos(r[1]) * xnp.cos(r[2]))
+
+    def lap_um(r):
+        q = r[0] ** 2 + r[1] ** 2 + r[2] ** 2
+        return (4 * q - 6) * um(r)
+
+    def lap_up(r):
+        return -3 * up(r)
+
+    return um, up, grad_um, grad_up, lap_um, lap_up
+
+
+def _const(v):
+    return lambda r: xnp.zeros_like(r[0]) + v
+
+
+def make_problem(phi_raw, grad_phi, mu_m, mu_p, k_m, k_p, name):
+    """Assemble the 11 callbacks for constant mu/k and the standard u pair."""
+    um, up, gum, gup, lum, lup = _u_pair()
+    phi = perturb(phi_raw)
+
+    def f_m(r):
+        return k_m * um(r) - mu_m * lum(r)
+
+    def f_p(r):
+        return k_p * up(r) - mu_p * lup(r)
+
+    def alpha(r):
+        return up(r) - um(r)
+
+    def beta(r):
+        g = grad_phi(r)
+        nn = xnp.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
+        a, b = gup(r), gum(r)
+        v = sum((mu_p * a[i] - mu_m * b[i]) * (g[i] / nn) for i in range(3))
+        return xnp.nan_to_num(-v)
+
+    def exact(r):
+        return xnp.where(phi(r) >= 0, up(r), um(r))
+
+    return dict(
+        name=name,
+        initial_value_fn=_const(0.0),
+        dirichlet_bc_fn=exact,
+        lvl_set_fn=phi,
+        mu_m_fn=_const(mu_m),
+        mu_p_fn=_const(mu_p),
+        k_m_fn=_const(k_m),
+        k_p_fn=_const(k_p),
+        f_m_fn=f_m,
+        f_p_fn=f_p,
+        alpha_fn=alpha,
+        beta_fn=beta,
+        exact_fn=exact,
+        exact_m_fn=um,
+        exact_p_fn=up,
+    )
+
+
+def sphere(radius=0.5, mu_m=1.0, mu_p=10.0, k_m=0.0, k_p=0.0, centre=(0.0, 0.0, 0.0)):
+    """phi = |x - c| - radius (the 16^3 CPU, 64^3 and 128^3 sphere workloads)."""
+    cx, cy, cz = centre
+
+    def phi(r):
+        return xnp.sqrt((r[0] - cx) ** 2 + (r[1] - cy) ** 2 + (r[2] - cz) ** 2) - radius
+
+    def gphi(r):
+        return (r[0] - cx, r[1] - cy, r[2] - cz)
+
+    return make_problem(phi, gphi, mu_m, mu_p, k_m, k_p, "sphere")
+
+
+def star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0):
+    """phi = r - (r0 + amp sin(lobes_t theta) sin(lobes_p varphi)),
+    theta = arccos(z/r), varphi = atan2(y, x); r = 0 guarded (phi = -r0)."""
+
+    def phi(r):
+        rr = xnp.sqrt(r[0] ** 2 + r[1] ** 2 + r[2] ** 2)
+        th = xnp.arccos(xnp.nan_to_num(r[2] / rr))
+        ph = xnp.arctan2(r[1], r[0])
+        return rr - (r0 + amp * xnp.sin(lobes_t * th) * xnp.sin(lobes_p * ph))
+
+    def gphi(r):
+        x, y, z = r[0], r[1], r[2]
+        rr = xnp.sqrt(x ** 2 + y ** 2 + z ** 2)
+        rho = xnp.sqrt(x ** 2 + y ** 2)
+        th = xnp.arccos(xnp.nan_to_num(z / rr))
+        ph = xnp.arctan2(y, x)
+        dth = amp * lobes_t * xnp.cos(lobes_t * th) * xnp.sin(lobes_p * ph)
+        dph = amp * lobes_p * xnp.sin(lobes_t * th) * xnp.cos(lobes_p * ph)
+        # grad theta = (xz, yz, -rho^2) / (r^2 rho); grad varphi = (-y, x, 0) / rho^2
+        gx = x / rr - dth * (x * z) / (rr ** 2 * rho) - dph * (-y) / rho ** 2
+        gy = y / rr - dth * (y * z) / (rr ** 2 * rho) - dph * x / rho ** 2
+        gz = z / rr - dth * (-(rho ** 2)) / (rr ** 2 * rho)
+        return (gx, gy, gz)
+
+    return make_problem(phi, gphi, mu_m, mu_p, k_m, k_p, "star")
+
+
+def grid_axes(n, lo=-1.0, hi=1.0):
+    """float32 linspace as the reference builds grids (README quick example)."""
+    a = np.linspace(lo, hi, n, dtype=np.float32)
+    return a, a.copy(), a.copy()
+
+
+def oracle_callbacks(prob):
+    """The callback dict in the name scheme of the CPU oracle."""
+    return dict(phi=prob["lvl_set_fn"], mu_m=prob["mu_m_fn"], mu_p=prob["mu_p_fn"],
+                k_m=prob["k_m_fn"], k_p=prob["k_p_fn"], f_m=prob["f_m_fn"], f_p=prob["f_p_fn"],
+                alpha=prob["alpha_fn"], beta=prob["beta_fn"], g=prob["dirichlet_bc_fn"])

@@ -1,0 +1,166 @@
+"""Device-resident affine residual records and the training-step engine.
+
+A record row is one training point p of one zoom level (stencil spacing h):
+
+    r(p) = c_p + sum_e w_e * NN_{phase_e}(fl32(p + o_e * h))
+
+with o_e in [-2,2]^3 a lattice offset and phase_e the sign of phi at that
+position (NN+ where phi >= 0, MLP.py:98).  The sum is exactly the reference's
+nan_to_num(lhs/diag) - nan_to_num(rhs/diag) of compute_Ax_and_b_preconditioned_fn
+(discretization.py:299-423), which is affine in the network outputs because
+geometry, regression rows and callbacks are parameter-independent.
+
+Layout in HBM (SoA, one set per zoom level):
+  row_point i32[N]  grid index of the row's point      row_c  f32[N]
+  row_ptr   i32[N+1] CSR offsets into the entry arrays
+  ent_code  u32[nnz] offset code | phase << 7          ent_w  f32[nnz]
+Rows of one batch are contiguous (the reference's contiguous batches,
+data_management.py:79-160) and, inside a batch, ordered by entry count so a
+wavefront sees uniform trip counts (boundary rows: 1 entry, bulk: 7,
+interface: up to 125).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+
+def batch_bounds(n, batch_size):
+    """Contiguous batches (DatasetDict with num_gpus = 1, batch_size clamped to n)."""
+    B = min(int(batch_size), int(n))
+    nb = -(-n // B)
+    begins = np.arange(nb, dtype=np.int64) * B
+    rows = np.minimum(begins + B, n) - begins
+    return B, begins, rows
+
+
+def pack_dense(W125, c, phase125, batch_size):
+    """Pack dense 125-offset records (host arrays) into the CSR device layout.
+
+    W125 (n,125) weights, c (n,) constants, phase125 (n,125) bool.  Returns a
+    dict of host arrays in row order (batches contiguous, rows sorted by entry
+    count inside each batch)."""
+    n = W125.shape[0]
+    B, begins, rows = batch_bounds(n, batch_size)
+    nnz_row = (W125 != 0).sum(axis=1)
+    order = np.empty(n, dtype=np.int64)
+    for b0, nr in zip(begins, rows):
+        seg = np.arange(b0, b0 + nr)
+        order[b0:b0 + nr] = seg[np.argsort(nnz_row[seg], kind="stable")]
+    Wo = W125[order]
+    Po = phase125[order]
+    rr, cc = np.nonzero(Wo)
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(rr, minlength=n), out=row_ptr[1:])
+    if row_ptr[-1] >= 2 ** 31:
+        raise ValueError("too many record entries for int32 offsets")
+    code = cc.astype(np.uint32) | (Po[rr, cc].astype(np.uint32) << np.uint32(7))
+    return dict(row_point=order.astype(np.int32), row_ptr=row_ptr.astype(np.int32),
+                row_c=np.asarray(c, dtype=np.float64)[order].astype(np.float32),
+                ent_code=code, ent_w=Wo[rr, cc].astype(np.float32),
+                batch_size=B, batch_begin=begins, batch_rows=rows)
+
+
+class DeviceGrid:
+    def __init__(self, xs, ys, zs, device="cuda"):
+        import torch
+
+        self.xs = torch.as_tensor(np.asarray(xs, np.float32), device=device)
+        self.ys = torch.as_tensor(np.asarray(ys, np.float32), device=device)
+        self.zs = torch.as_tensor(np.asarray(zs, np.float32), device=device)
+        self.shape = (len(xs), len(ys), len(zs))
+        self.c = N.Grid(self.xs.data_ptr(), self.ys.data_ptr(), self.zs.data_ptr(), *self.shape)
+
+
+class DeviceRecords:
+    """Records of one zoom level in device memory, plus the batch table."""
+
+    def __init__(self, packed, h, device="cuda"):
+        import torch
+
+        def T(a, dt):
+            return torch.as_tensor(np.ascontiguousarray(a), device=device).to(dt)
+
+        self.row_point = T(packed["row_point"], torch.int32)
+        self.row_ptr = T(packed["row_ptr"], torch.int32)
+        self.row_c = T(packed["row_c"], torch.float32)
+        nnz = len(packed["ent_w"])
+        codes = packed["ent_code"].astype(np.int64).astype(np.int32) if nnz else np.zeros(1, np.int32)
+        self.ent_code = T(codes, torch.int32)
+        self.ent_w = T(packed["ent_w"] if nnz else np.zeros(1, np.float32), torch.float32)
+        self.nrows = len(packed["row_c"])
+        self.nnz = nnz
+        self.h = np.asarray(h, np.float32)
+        self.batch_size = int(packed["batch_size"])
+        self.batch_begin = np.asarray(packed["batch_begin"], np.int64)
+        self.batch_rows = np.asarray(packed["batch_rows"], np.int64)
+        self.c = N.Records(self.row_point.data_ptr(), self.row_ptr.data_ptr(), self.row_c.data_ptr(),
+                           self.ent_code.data_ptr(), self.ent_w.data_ptr(), self.nrows, self.nnz,
+                           float(self.h[0]), float(self.h[1]), float(self.h[2]))
+
+
+OPT_KIND = {"custom": 0, "adam": 1, "rmsprop": 2}
+SCHED_KIND = {"exponential": 0, "polynomial": 1}
+
+
+def make_opt(optimizer_dict):
+    name = optimizer_dict.get("optimizer_name", "custom")
+    if name not in OPT_KIND:
+        raise ValueError(f"Unknown optimizer: {name}")
+    sched = optimizer_dict.get("sched", {}) or {}
+    sname = sched.get("scheduler_name", "exponential")
+    if sname not in SCHED_KIND:
+        raise ValueError(f"Unknown scheduler: {sname}")
+    return N.Opt(OPT_KIND[name], SCHED_KIND[sname], float(optimizer_dict.get("learning_rate", 1e-3)),
+                 float(sched.get("decay_rate", 0.9)), 1000.0, 1.0)
+
+
+class Engine:
+    """Parameters, optimizer state and scratch of one two-phase network on the GPU."""
+
+    def __init__(self, dims, params, optimizer_dict, max_rows, max_steps, device="cuda"):
+        import torch
+
+        self.dims = tuple(int(d) for d in dims)
+        self.net = N.Net(*self.dims)
+        L = N.lib()
+        if not L.dips_net_supported(ctypes.byref(self.net)):
+            raise N.DipsError(f"network shape (Lm,Hm,Lp,Hp)={self.dims} has no compiled kernel")
+        self.P = int(L.dips_param_count(ctypes.byref(self.net)))
+        params = np.asarray(params, np.float32)
+        if params.size != self.P:
+            raise ValueError(f"expected {self.P} parameters, got {params.size}")
+        self.device = device
+        self.params = torch.as_tensor(params, device=device).clone()
+        self.m = torch.zeros(self.P, dtype=torch.float32, device=device)
+        self.v = torch.zeros(self.P, dtype=torch.float32, device=device)
+        self.count = torch.zeros(1, dtype=torch.int32, device=device)
+        self.loss_hist = torch.zeros(max(1, int(max_steps)), dtype=torch.float32, device=device)
+        self.partial_cap = int(L.dips_partial_size(ctypes.byref(self.net), int(max_rows)))
+        self.partial = torch.zeros(self.partial_cap, dtype=torch.float32, device=device)
+        self.opt = make_opt(optimizer_dict)
+
+    def loss_grad(self, grid, rec, row_begin, nrows, inv_batch, want_grad=True):
+        """(loss, grad) of rows [row_begin, row_begin+nrows) as float64 numpy."""
+        import torch
+
+        nblk = ctypes.c_int32(0)
+        st = N.lib().dips_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), int(row_begin), int(nrows),
+                                    ctypes.byref(self.net), N.ptr(self.params), N.ptr(self.partial),
+                                    self.partial_cap, int(bool(want_grad)), ctypes.byref(nblk), N.current_stream())
+        N.check(st, "dips_loss_grad")
+        torch.cuda.synchronize()
+        part = self.partial[: nblk.value * (self.P + 1)].view(nblk.value, self.P + 1).double().cpu().numpy()
+        s = part.sum(axis=0) * float(inv_batch)
+        return s[self.P], s[: self.P]
+
+    def train_level(self, grid, rec, nepochs):
+        st = N.lib().dips_train_level(
+            ctypes.byref(grid.c), ctypes.byref(rec.c),
+            (ctypes.c_int64 * len(rec.batch_begin))(*rec.batch_begin.tolist()),
+            (ctypes.c_int64 * len(rec.batch_rows))(*rec.batch_rows.tolist()),
+            len(rec.batch_begin), rec.batch_size, int(nepochs), ctypes.byref(self.net), ctypes.byref(self.opt),
+            N.ptr(self.params), N.ptr(self.m), N.ptr(self.v), N.ptr(self.count), N.ptr(self.loss_hist),
+            int(self.loss_hist.numel()), N.ptr(self.partial), self.partial_cap, N.current_stream())
+        N.check(st, "dips_train_level")

@@ -1,0 +1,124 @@
+"""Loss, gradient, batching, zoom schedule and epoch loop — CPU restatement.
+
+Test infrastructure only (see oracle/__init__.py).
+
+Restates:
+  * Trainer.loss = mean 0.5 (lhs - rhs)^2 ........... trainer.py:892-912
+  * Trainer.update (value_and_grad + optax) ....... trainer.py:783-789
+  * single_GPU_train (scan epochs x batches; loss_epochs[e] = mean of the
+    pre-update batch losses; no shuffling) ........ trainer.py:501-591
+  * alternate_res_sequentially: h = h_tr * 0.5**(epoch // (E // 4))
+                                                  data_management.py:319-326
+  * DatasetDict contiguous batches (+ padding when B does not divide N)
+                                                  data_management.py:79-160
+
+The per-point residual comes from oracle.discretization in affine form; the
+network is evaluated at the canonical float32 lattice positions fl32(p + o h).
+Padding points (B not dividing N) use numpy's generator instead of jax's
+PRNGKey(0) normals (same rescaling rule), so their coordinates differ from the
+reference's; none of the benchmark workloads pads.
+"""
+import numpy as np
+
+from . import discretization as D
+from . import mlp as M
+from . import optim as O
+
+
+def zoom_level(epoch, num_epochs):
+    return epoch // (num_epochs // 4)
+
+
+def level_spacing(h_tr, level):
+    return (np.asarray(h_tr, dtype=np.float32) * np.float32(0.5) ** np.float32(level)).astype(np.float32)
+
+
+def batches(n, batch_size):
+    """Index lists of contiguous batches; last one padded with -1 entries."""
+    B = min(batch_size, n)
+    nb = -(-n // B)
+    out = []
+    for b in range(nb):
+        idx = np.arange(b * B, min((b + 1) * B, n))
+        if idx.size < B:
+            idx = np.concatenate([idx, -np.ones(B - idx.size, dtype=np.int64)])
+        out.append(idx)
+    return out
+
+
+def padding_points(R, count, seed=0):
+    """generate_random_points_like (data_management.py:70-76), numpy generator."""
+    rng = np.random.default_rng(seed)
+    dR = rng.standard_normal((count, 3))
+    lo, hi = R.min(), R.max()
+    return (lo + (hi - lo) * (dR - dR.min()) / (dR.max() - dR.min())).astype(np.float32)
+
+
+class LevelRecords:
+    """Dense 125-offset affine records of all points at one stencil spacing."""
+
+    def __init__(self, P, h, bounds, cb, dt=np.float64):
+        slots = D.point_slots(P, h, bounds, cb, dt)
+        self.W, self.pos = D.merge_slots(slots, P, h)
+        self.c = slots["c"]
+        self.boundary = slots["boundary"]
+        self.dead = slots["dead"]
+        self.slots = slots
+        self.phase = cb("phi", self.pos) >= 0
+        self.h = np.asarray(h, np.float32)
+
+
+def residual(theta, dims, rec, rows, dt=np.float64):
+    """r for the given rows (n,), evaluating the network only where W != 0."""
+    W = rec.W[rows]
+    nz = np.nonzero(W)
+    X = rec.pos[rows][nz]
+    ph = rec.phase[rows][nz]
+    u = M.forward(theta, dims, X, ph, dt)
+    r = rec.c[rows].astype(dt).copy()
+    np.add.at(r, nz[0], W[nz].astype(dt) * u)
+    return r
+
+
+def loss_and_grad(theta, dims, rec, rows, dt=np.float64):
+    """0.5 * mean(r^2) over the batch and its gradient (hand VJP)."""
+    W = rec.W[rows]
+    nz = np.nonzero(W)
+    X = rec.pos[rows][nz]
+    ph = rec.phase[rows][nz]
+    u = M.forward(theta, dims, X, ph, dt)
+    r = rec.c[rows].astype(dt).copy()
+    np.add.at(r, nz[0], W[nz].astype(dt) * u)
+    B = len(rows)
+    loss = np.sum(dt(0.5) * r * r) / dt(B)
+    seed = r[nz[0]] * W[nz].astype(dt) / dt(B)
+    grad = M.vjp(theta, dims, X, ph, seed, dt)
+    return loss, grad, r
+
+
+def train(P, h_tr, bounds, cb, dims, theta0, num_epochs, batch_size, opt_cfg,
+          dt=np.float64, param_dt=np.float32, record_dt=np.float64, records=None):
+    """Epoch loop of single_GPU_train with the zoom schedule.  Returns
+    (theta, loss_epochs, per-step losses)."""
+    n = P.shape[0]
+    bl = batches(n, batch_size)
+    if any((b < 0).any() for b in bl):
+        raise NotImplementedError("padded batches are exercised separately")
+    theta = np.asarray(theta0, dtype=param_dt).copy()
+    state = O.OptState(theta.size, param_dt)
+    recs = {} if records is None else records
+    loss_epochs = np.zeros(num_epochs)
+    steps = []
+    for epoch in range(num_epochs):
+        lvl = zoom_level(epoch, num_epochs)
+        if lvl not in recs:
+            recs[lvl] = LevelRecords(P, level_spacing(h_tr, lvl), bounds, cb, record_dt)
+        rec = recs[lvl]
+        tot = 0.0
+        for rows in bl:
+            loss, grad, _ = loss_and_grad(theta, dims, rec, rows, dt)
+            theta = O.step(theta, grad.astype(param_dt), state, opt_cfg, param_dt)
+            tot += loss
+            steps.append(loss)
+        loss_epochs[epoch] = tot / len(bl)
+    return theta, loss_epochs, np.array(steps), recs
