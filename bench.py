@@ -1,0 +1,198 @@
+"""Benchmark: NBM residual+grad training points/s on one MI355X.
+
+Workload (default): the 128^3 sphere interface problem on [-1,1]^3 (phi = |x| -
+0.5, mu- = 1, mu+ = 10, k = 0, manufactured u- = exp(-|x|^2), u+ = sin x cos y
+sin z), two-phase tanh MLP 1x1 (Omega-) / 2x10 (Omega+), batch 65536, optimizer
+clip -> Adam -> exponential decay (lr 1e-3, decay 0.9), the four zoom levels
+of a 2000-epoch run.  A step is one optimizer step on one batch of 65536
+training points (residual + loss + parameter gradient + Adam), inputs resident
+in HBM.  Timed steps are split evenly over the four zoom levels and walk the
+batches in order, as the training loop does.
+
+Prints one JSON line.  --gpus N > 1 exits: this build targets one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "jax-dips_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def mlp_fwd_flops(L, H):
+    return 2 * (3 * H + (L - 1) * H * H + H)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=128, help="training grid points per axis")
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--roofline-launches", type=int, default=400)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import torch
+
+    from jax_dips import _native as N
+    from jax_dips import problems
+    from jax_dips.level_records import build_level
+    from jax_dips.records import DeviceGrid, Engine
+
+    dev = "cuda"
+    n, B = args.n, args.batch
+    prob = problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0)
+    cbs = problems.oracle_callbacks(prob)
+    x, y, z = problems.grid_axes(n)
+    grid = DeviceGrid(x, y, z)
+    h0 = np.array([x[1] - x[0], y[1] - y[0], z[1] - z[0]], np.float32)
+    bounds = (x.min(), x.max(), y.min(), y.max(), z.min(), z.max())
+    dims = (1, 1, 2, 10)
+    from jax_dips.nn.mlp.MLP import DoubleMLP
+
+    model = DoubleMLP(mlp={"hidden_layers_m": 1, "hidden_dim_m": 1, "activation_m": "jnp.tanh",
+                           "hidden_layers_p": 2, "hidden_dim_p": 10, "activation_p": "jnp.tanh"})
+    theta = model.flatten(model.init(42))
+    opt = {"optimizer_name": "custom", "learning_rate": 1e-3, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
+
+    t0 = time.time()
+    levels = []
+    for lvl in range(4):
+        h = (h0 * np.float32(0.5) ** lvl).astype(np.float32)
+        levels.append(build_level(grid, h, bounds, cbs, B))
+    torch.cuda.synchronize()
+    t_records = time.time() - t0
+    npts = n ** 3
+    nb = levels[0].batch_begin.size
+    print(f"# records: {t_records:.2f} s for 4 levels; {npts} points, {nb} batches; "
+          + "; ".join(f"L{i}: nnz={r.nnz} intf={r.stats['nip']}" for i, r in enumerate(levels)), file=sys.stderr,
+          flush=True)
+
+    K = max(4, args.steps - args.steps % 4)
+    eng = Engine(dims, theta, opt, max_rows=B, max_steps=args.warmup + K + 8)
+    L = N.lib()
+
+    def run_steps(k, stream):
+        # k steps split over the 4 zoom levels, batches in order (cyclic)
+        per = k // 4
+        for li, rec in enumerate(levels):
+            idx = np.arange(per) % nb
+            beg = (ctypes.c_int64 * per)(*rec.batch_begin[idx].tolist())
+            rows = (ctypes.c_int64 * per)(*rec.batch_rows[idx].tolist())
+            st = L.dips_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), beg, rows, per, rec.batch_size, 1,
+                                    ctypes.byref(eng.net), ctypes.byref(eng.opt), N.ptr(eng.params), N.ptr(eng.m),
+                                    N.ptr(eng.v), N.ptr(eng.count), N.ptr(eng.loss_hist), int(eng.loss_hist.numel()),
+                                    N.ptr(eng.partial), eng.partial_cap, stream)
+            N.check(st, "dips_train_level")
+
+    stream = N.current_stream()
+    run_steps(max(4, args.warmup - args.warmup % 4), stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    run_steps(K, stream)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    value = K * B / (ms / 1e3)
+    loss_tail = eng.loss_hist[: int(eng.count.item())][-4:].cpu().numpy().tolist()
+
+    # roofline of the dominant kernel: the fused residual/loss/gradient launch
+    per_launch_flops = []
+    for rec in levels:
+        codes = rec.ent_code[: rec.nnz].cpu().numpy()
+        ptr = rec.row_ptr.cpu().numpy()
+        ph = (codes >> 7) & 1
+        fl = np.where(ph == 1, mlp_fwd_flops(dims[2], dims[3]), mlp_fwd_flops(dims[0], dims[1])) * 3.0
+        cum = np.concatenate([[0.0], np.cumsum(fl)])
+        for b0, nr in zip(rec.batch_begin, rec.batch_rows):
+            per_launch_flops.append(cum[ptr[b0 + nr]] - cum[ptr[b0]])
+    nbl = ctypes.c_int32(0)
+    R = max(4, args.roofline_launches - args.roofline_launches % 4)
+    torch.cuda.synchronize()
+    r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    flops_sum = 0.0
+    r0.record()
+    for i in range(R):
+        li = (4 * i) // R
+        rec = levels[li]
+        b = i % nb
+        N.check(L.dips_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), int(rec.batch_begin[b]),
+                                 int(rec.batch_rows[b]), ctypes.byref(eng.net), N.ptr(eng.params), N.ptr(eng.partial),
+                                 eng.partial_cap, 1, ctypes.byref(nbl), stream), "dips_loss_grad")
+        flops_sum += per_launch_flops[li * nb + b]
+    r1.record()
+    torch.cuda.synchronize()
+    avg_s = r0.elapsed_time(r1) / 1e3 / R
+    achieved = flops_sum / R / avg_s / 1e12
+
+    result = {
+        "metric": "NBM residual+grad points/s vs JAX ref on same GPU at equal Linf error",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"sphere {n}^3, MLP 1x1/2x10 tanh, batch {B}, 4 zoom levels (2000-epoch schedule)",
+                   "global_batch": B, "grid": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "loss_grad_kernel<1,1,2,10,true>", "avg_launch_us": avg_s * 1e6,
+                     "flop_per_launch": flops_sum / R},
+        "records_build_s": t_records,
+        "loss_tail": loss_tail,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, seconds):
+    """The reference's per-step evaluation pattern in numpy on one host core,
+    on a bounded sample (consecutive batches of 2048 points of the same 128^3
+    grid, level-0 spacing) until ~``seconds`` of CPU work."""
+    from jax_dips import problems
+    from oracle import discretization as D
+    from oracle import refpattern
+
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float32)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+    nb = 2048
+    mid = P.shape[0] // 2  # batches through the middle of the box cross the interface
+    done, t0 = 0, time.time()
+    while time.time() - t0 < seconds:
+        s = mid + done
+        refpattern.step(theta, dims, P[s:s + nb], h0, bounds, cb)
+        done += nb
+    dt = time.time() - t0
+    return {"value": done / dt, "unit": "points/s", "cores": 1, "kind": "port",
+            "sample": f"{done} points in batches of {nb} from the middle of the 128^3 grid, level-0 spacing, "
+                      f"{dt:.1f} s, numpy float32 restatement of the reference's per-step pattern"}
+
+
+if __name__ == "__main__":
+    main()

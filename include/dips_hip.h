@@ -14,9 +14,12 @@
  *                       (trainer.py:787-788; solvers/optimizers.py:33-97)
  *   dips_train_level <- the batch/epoch lax.scan of single_GPU_train for one
  *                       zoom level (trainer.py:501-591)
- *   dips_geom_*      <- geometry/geometric_integrations_per_point.py and the
- *                       regression rows of discretization.py:238-296
- *                       (see dips_geom.h section below)
+ *   dips_geom_*      <- compute_face_centroids_values_plus_minus_at_point,
+ *                       integrate_over_interface_at_point and
+ *                       get_regression_coeffs_at_point / u_mp, evaluated once
+ *                       per zoom level instead of every step
+ *                       (geometry/geometric_integrations_per_point.py:201-1009;
+ *                        discretization.py:238-519)
  *   dips_eval        <- Trainer.evaluate_solution_fn / gradients
  *                       (trainer.py:836-844, :914-977)
  */
@@ -116,6 +119,88 @@ int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const i
                      const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
                      int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
                      void* stream);
+
+
+/* Network at m points pts (m,3): u (m), optionally grad_x u (m,3) and, with
+ * normals (m,3), the normal derivative grad u . n (m).  phase (m) uint8:
+ * 1 -> Omega+ network (phi >= 0), 0 -> Omega- network.  Replaces
+ * evaluate_solution_fn and the gradient post-processing of
+ * evaluate_solution_and_gradients (trainer.py:836-844, :914-977). */
+int dips_eval(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase, const float* normals,
+              int64_t m, float* u, float* grad, float* grad_n, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * K1: FP64 cut-cell geometry, regression rows and record assembly.
+ * (geometry/geometric_integrations_per_point.py:32-1009;
+ *  solvers/poisson/discretization.py:164-519)
+ * Callback samples are prepared by the host at the positions the reference
+ * uses (float32 arithmetic: s_e = p + e*h, corners s_e + (+-h/2), cube s_e + X_q).
+ * Layouts: phic (N,7,8) phi at the corners of the 7 stencil cells of every
+ * point (P000,P100,P101,P001,P010,P110,P011,P111); region (N,7) int8.
+ * ------------------------------------------------------------------------- */
+
+/* region[t] = 0 if cell t is crossed (0 < #(phi<0) < 8) else sign(phi(P000))
+ * (per-point file :203-263), t over n7 = N*7 cells. */
+int dips_geom_regions(const double* phic, int64_t n7, int8_t* region, void* stream);
+
+/* Min-Gibou geometry of crossed centre cells (cell of size h around grid point
+ * cell_pt[c]): V- (ncell), A- per face (ncell,6; x-,x+,y-,y+,z-,z+), Gamma
+ * triangles (ncell,10,3,3; slot 2t+s = triangle s of tet t, zero if absent) and
+ * their areas (ncell,10).  phic is the (N,7,8) corner array (cell e = 0). */
+int dips_geom_cells(const dips_grid_t* grid, float hx, float hy, float hz, const int32_t* cell_pt, int64_t ncell,
+                    const double* phic, double* vol_m, double* area_m, double* tri, double* tri_area, void* stream);
+
+/* Regression rows at crossed stencil points (cs_pt[t], stencil e = cs_e[t]):
+ * phi_cube (ncs,27) on s_e + X_q (x fastest), mu_s (ncs,2) = mu-(s), mu+(s).
+ * Outputs the affine forms u-/u+ = a[.,0/1,:] . U + k[.,0/1] * J with
+ * J = alpha(rG) + delta*beta(rG)/mu_sel(rG), mu_sel = mu+ if sel else mu-,
+ * rG = s - delta*n (ncs,3) and delta (ncs).  rcond: pinv cutoff
+ * (10*max(M,N)*eps_f32 = 3.5762786865234375e-06 reproduces jnp.linalg.pinv). */
+int dips_geom_regression(const dips_grid_t* grid, float hx, float hy, float hz, const int32_t* cs_pt,
+                         const int8_t* cs_e, int64_t ncs, const double* phi_cube, const double* mu_s, double rcond,
+                         double* a, double* k, int8_t* sel, double* r_gamma, double* delta, void* stream);
+
+/* Record assembly inputs (all device pointers; N = grid points). */
+typedef struct {
+  const float* xs;
+  const float* ys;
+  const float* zs;
+  int32_t nx, ny, nz;
+  float hx, hy, hz;
+  float bnd[6];             /* level-set grid bounds xmin,xmax,ymin,ymax,zmin,zmax */
+  const double* phis;       /* (N,7) phi at the stencil points */
+  const int8_t* region;     /* (N,7) */
+  const int32_t* cc_slot;   /* (N) crossed-centre slot or -1 */
+  const double* vol_m;      /* (ncc) */
+  const double* area_m;     /* (ncc,6) */
+  const double* tri_a;      /* (ncc,10) */
+  const double* beta_tri;   /* (ncc,10,3) beta at the triangle vertices */
+  int32_t reserved0;
+  const double* mu_face;    /* (N,6,2) mu-, mu+ at the 6 face centres */
+  const double* pt_vals;    /* (N,5) k-, k+, f-, f+, g at p */
+  const int32_t* cs_slot;   /* (N,7) crossed-stencil slot or -1 */
+  const double* cs_a;       /* (ncs,2,27) */
+  const double* cs_k;       /* (ncs,2) */
+  const int8_t* cs_sel;     /* (ncs) */
+  const double* cs_delta;   /* (ncs) */
+  const double* cs_vals;    /* (ncs,4) alpha, beta, mu-, mu+ at r_Gamma */
+  const double* phi_cube;   /* (ncs,27) */
+  double* scratch;          /* (nip,250) for points with a crossed stencil cell */
+  const int32_t* ip_slot;   /* (N) slot into scratch or -1 */
+  int32_t mode;             /* 0: count entries -> nnz; 1: write rows */
+  int32_t* nnz;             /* (N) */
+  const int64_t* ent_off;   /* (N) first entry of the point's row (mode 1) */
+  const int32_t* row_of;    /* (N) row index of the point (mode 1) */
+  uint32_t* ent_code;
+  float* ent_w;
+  float* row_c;
+  int64_t npts;
+} dips_assemble_t;
+
+/* mode 0 counts the entries of every point's row; mode 1 (same inputs, after
+ * mode 0 on the same stream) writes them, in (offset code, phase) order, at
+ * ent_off[pt] and the row constant at row_c[row_of[pt]]. */
+int dips_geom_assemble(const dips_assemble_t* args, void* stream);
 
 #ifdef __cplusplus
 }

@@ -120,4 +120,53 @@ int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const i
   return DIPS_OK;
 }
 
+int dips_geom_regions(const double* phic, int64_t n7, int8_t* region, void* stream) {
+  if (n7 < 0 || (n7 > 0 && (!phic || !region))) {
+    set_error("dips_geom_regions: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return geom_regions(phic, n7, region, (hipStream_t)stream);
+}
+
+int dips_geom_cells(const dips_grid_t* grid, float hx, float hy, float hz, const int32_t* cell_pt, int64_t ncell,
+                    const double* phic, double* vol_m, double* area_m, double* tri, double* tri_area, void* stream) {
+  if (!grid || ncell < 0 || (ncell > 0 && (!cell_pt || !phic || !vol_m || !area_m || !tri || !tri_area))) {
+    set_error("dips_geom_cells: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return geom_cells(grid, hx, hy, hz, cell_pt, ncell, phic, vol_m, area_m, tri, tri_area, (hipStream_t)stream);
+}
+
+int dips_geom_regression(const dips_grid_t* grid, float hx, float hy, float hz, const int32_t* cs_pt,
+                         const int8_t* cs_e, int64_t ncs, const double* phi_cube, const double* mu_s, double rcond,
+                         double* a, double* k, int8_t* sel, double* r_gamma, double* delta, void* stream) {
+  if (!grid || ncs < 0 || (ncs > 0 && (!cs_pt || !cs_e || !phi_cube || !mu_s || !a || !k || !sel || !r_gamma || !delta))) {
+    set_error("dips_geom_regression: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return geom_regression(grid, hx, hy, hz, cs_pt, cs_e, ncs, phi_cube, mu_s, rcond, a, k, sel, r_gamma, delta,
+                         (hipStream_t)stream);
+}
+
+int dips_geom_assemble(const dips_assemble_t* args, void* stream) {
+  if (!args || args->npts < 0 || (args->mode != 0 && args->mode != 1) || !args->xs || !args->region ||
+      !args->phis || !args->mu_face || !args->pt_vals || (args->mode == 0 && !args->nnz) ||
+      (args->mode == 1 && (!args->ent_off || !args->row_of || !args->ent_code || !args->ent_w || !args->row_c))) {
+    set_error("dips_geom_assemble: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return geom_assemble(*args, (hipStream_t)stream);
+}
+
+int dips_eval(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase, const float* normals,
+              int64_t m, float* u, float* grad, float* grad_n, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if (m < 0 || (m > 0 && (!params || !pts || !phase || !u))) {
+    set_error("dips_eval: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return eval_launch(net, params, pts, phase, normals, m, u, grad, grad_n, (hipStream_t)stream);
+}
+
 }  // extern "C"

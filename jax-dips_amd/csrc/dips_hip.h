@@ -22,4 +22,17 @@ int opt_step_launch(const dips_net_t* net, const dips_opt_t* opt, const float* p
                     float* params, float* m, float* v, int32_t* count, float* loss_hist, int loss_cap,
                     hipStream_t stream);
 
+// geom.hip
+int geom_regions(const double* phic, int64_t n7, int8_t* region, hipStream_t st);
+int geom_cells(const dips_grid_t* g, float hx, float hy, float hz, const int32_t* cell_pt, int64_t ncell,
+               const double* phic, double* vol_m, double* area_m, double* tri, double* tri_a, hipStream_t st);
+int geom_regression(const dips_grid_t* g, float hx, float hy, float hz, const int32_t* cs_pt, const int8_t* cs_e,
+                    int64_t ncs, const double* phi_cube, const double* mu_s, double rcond, double* out_a,
+                    double* out_k, int8_t* out_sel, double* out_r, double* out_delta, hipStream_t st);
+int geom_assemble(const dips_assemble_t& a, hipStream_t st);
+
+// eval.hip
+int eval_launch(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase, const float* nrm,
+                int64_t m, float* u, float* grad, float* gradn, hipStream_t st);
+
 }  // namespace dips

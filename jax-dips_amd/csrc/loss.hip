@@ -15,102 +15,11 @@
 // value_and_grad (trainer.py:786); the network is DoubleMLP (nn/mlp/MLP.py:93-139).
 #include "dips_common.h"
 #include "dips_hip.h"
+#include "dips_mlp.h"
 
 namespace dips {
 
 constexpr int kLossThreads = 256;
-
-template <int L, int H>
-__device__ __forceinline__ float mlp_forward(const float* __restrict__ P, float x, float y, float z) {
-  using NL = NetLayout<L, H>;
-  float a[H];
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    float s = P[NL::b(0) + j];
-    s = fmaf(x, P[NL::w(0) + 0 * H + j], s);
-    s = fmaf(y, P[NL::w(0) + 1 * H + j], s);
-    s = fmaf(z, P[NL::w(0) + 2 * H + j], s);
-    a[j] = tanhf(s);
-  }
-#pragma unroll
-  for (int l = 1; l < L; ++l) {
-    float t[H];
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-      float s = P[NL::b(l) + j];
-#pragma unroll
-      for (int i = 0; i < H; ++i) s = fmaf(a[i], P[NL::w(l) + i * H + j], s);
-      t[j] = tanhf(s);
-    }
-#pragma unroll
-    for (int j = 0; j < H; ++j) a[j] = t[j];
-  }
-  float o = P[NL::bout()];
-#pragma unroll
-  for (int i = 0; i < H; ++i) o = fmaf(a[i], P[NL::wout() + i], o);
-  return o;
-}
-
-// Adds seed * d NN(x,y,z) / d theta into G (register array of NL::size()).
-template <int L, int H>
-__device__ __forceinline__ void mlp_backward(const float* __restrict__ P, float x, float y, float z, float seed,
-                                             float* G) {
-  using NL = NetLayout<L, H>;
-  float act[L][H];
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    float s = P[NL::b(0) + j];
-    s = fmaf(x, P[NL::w(0) + 0 * H + j], s);
-    s = fmaf(y, P[NL::w(0) + 1 * H + j], s);
-    s = fmaf(z, P[NL::w(0) + 2 * H + j], s);
-    act[0][j] = tanhf(s);
-  }
-#pragma unroll
-  for (int l = 1; l < L; ++l) {
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-      float s = P[NL::b(l) + j];
-#pragma unroll
-      for (int i = 0; i < H; ++i) s = fmaf(act[l - 1][i], P[NL::w(l) + i * H + j], s);
-      act[l][j] = tanhf(s);
-    }
-  }
-  // output layer
-  float d[H];
-  G[NL::bout()] += seed;
-#pragma unroll
-  for (int i = 0; i < H; ++i) {
-    G[NL::wout() + i] = fmaf(seed, act[L - 1][i], G[NL::wout() + i]);
-    float a = act[L - 1][i];
-    d[i] = seed * P[NL::wout() + i] * (1.0f - a * a);
-  }
-#pragma unroll
-  for (int l = L - 1; l >= 1; --l) {
-    float dn[H];
-#pragma unroll
-    for (int i = 0; i < H; ++i) {
-      float acc = 0.0f;
-#pragma unroll
-      for (int j = 0; j < H; ++j) {
-        G[NL::w(l) + i * H + j] = fmaf(act[l - 1][i], d[j], G[NL::w(l) + i * H + j]);
-        acc = fmaf(P[NL::w(l) + i * H + j], d[j], acc);
-      }
-      float a = act[l - 1][i];
-      dn[i] = acc * (1.0f - a * a);
-    }
-#pragma unroll
-    for (int j = 0; j < H; ++j) G[NL::b(l) + j] += d[j];
-#pragma unroll
-    for (int i = 0; i < H; ++i) d[i] = dn[i];
-  }
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    G[NL::w(0) + 0 * H + j] = fmaf(x, d[j], G[NL::w(0) + 0 * H + j]);
-    G[NL::w(0) + 1 * H + j] = fmaf(y, d[j], G[NL::w(0) + 1 * H + j]);
-    G[NL::w(0) + 2 * H + j] = fmaf(z, d[j], G[NL::w(0) + 2 * H + j]);
-    G[NL::b(0) + j] += d[j];
-  }
-}
 
 template <int Lm, int Hm, int Lp, int Hp, bool kGrad>
 __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
@@ -233,14 +142,6 @@ static void launch_shape(const dips_grid_t* g, const dips_records_t* rec, int64_
                        g->nz, rec->hx, rec->hy, rec->hz, rec->row_point, rec->row_ptr, rec->row_c, row_begin, nrows,
                        rec->ent_code, rec->ent_w, rec->nnz, g->nx * g->ny * g->nz, params, partial);
 }
-
-// Network shapes with a compiled per-lane kernel.  (Lm, Hm, Lp, Hp)
-#define DIPS_SHAPES(X) \
-  X(1, 1, 2, 10)       \
-  X(1, 3, 2, 10)       \
-  X(1, 1, 1, 1)        \
-  X(2, 10, 2, 10)      \
-  X(2, 16, 2, 16)
 
 int net_supported(const dips_net_t* n) {
 #define X(a, b, c, d) \

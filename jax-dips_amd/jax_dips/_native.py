@@ -38,6 +38,23 @@ class Opt(ctypes.Structure):
                 ("decay_rate", ctypes.c_float), ("transition_steps", ctypes.c_float), ("max_norm", ctypes.c_float)]
 
 
+class Assemble(ctypes.Structure):
+    _fields_ = [("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("zs", ctypes.c_void_p),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("hx", ctypes.c_float), ("hy", ctypes.c_float), ("hz", ctypes.c_float),
+                ("bnd", ctypes.c_float * 6),
+                ("phis", ctypes.c_void_p), ("region", ctypes.c_void_p), ("cc_slot", ctypes.c_void_p),
+                ("vol_m", ctypes.c_void_p), ("area_m", ctypes.c_void_p), ("tri_a", ctypes.c_void_p),
+                ("beta_tri", ctypes.c_void_p), ("reserved0", ctypes.c_int32),
+                ("mu_face", ctypes.c_void_p), ("pt_vals", ctypes.c_void_p), ("cs_slot", ctypes.c_void_p),
+                ("cs_a", ctypes.c_void_p), ("cs_k", ctypes.c_void_p), ("cs_sel", ctypes.c_void_p),
+                ("cs_delta", ctypes.c_void_p), ("cs_vals", ctypes.c_void_p), ("phi_cube", ctypes.c_void_p),
+                ("scratch", ctypes.c_void_p), ("ip_slot", ctypes.c_void_p), ("mode", ctypes.c_int32),
+                ("nnz", ctypes.c_void_p), ("ent_off", ctypes.c_void_p), ("row_of", ctypes.c_void_p),
+                ("ent_code", ctypes.c_void_p), ("ent_w", ctypes.c_void_p), ("row_c", ctypes.c_void_p),
+                ("npts", ctypes.c_int64)]
+
+
 # name -> (restype, argtypes); must match include/dips_hip.h
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -55,6 +72,13 @@ SIGNATURES = {
     "dips_train_level": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(_I64),
                                         ctypes.POINTER(_I64), _I32, _I64, _I32, ctypes.POINTER(Net),
                                         ctypes.POINTER(Opt), _P, _P, _P, _P, _P, _I32, _P, _I64, _P]),
+    "dips_eval": (ctypes.c_int, [ctypes.POINTER(Net), _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "dips_geom_regions": (ctypes.c_int, [_P, _I64, _P, _P]),
+    "dips_geom_cells": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _I64,
+                                       _P, _P, _P, _P, _P, _P]),
+    "dips_geom_regression": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_float, ctypes.c_float, _P,
+                                            _P, _I64, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P, _P]),
+    "dips_geom_assemble": (ctypes.c_int, [ctypes.POINTER(Assemble), _P]),
 }
 
 _lib = None

@@ -113,18 +113,20 @@ def sphere(radius=0.5, mu_m=1.0, mu_p=10.0, k_m=0.0, k_p=0.0, centre=(0.0, 0.0, 
     return make_problem(phi, gphi, mu_m, mu_p, k_m, k_p, "sphere")
 
 
-def star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0):
-    """phi = r - (r0 + amp sin(lobes_t theta) sin(lobes_p varphi)),
+def star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0, centre=(0.0, 0.0, 0.0)):
+    """phi = r - (r0 + amp sin(lobes_t theta) sin(lobes_p varphi)) about ``centre``,
     theta = arccos(z/r), varphi = atan2(y, x); r = 0 guarded (phi = -r0)."""
+    cx, cy, cz = centre
 
     def phi(r):
-        rr = xnp.sqrt(r[0] ** 2 + r[1] ** 2 + r[2] ** 2)
-        th = xnp.arccos(xnp.nan_to_num(r[2] / rr))
-        ph = xnp.arctan2(r[1], r[0])
+        x, y, z = r[0] - cx, r[1] - cy, r[2] - cz
+        rr = xnp.sqrt(x ** 2 + y ** 2 + z ** 2)
+        th = xnp.arccos(xnp.nan_to_num(z / rr))
+        ph = xnp.arctan2(y, x)
         return rr - (r0 + amp * xnp.sin(lobes_t * th) * xnp.sin(lobes_p * ph))
 
     def gphi(r):
-        x, y, z = r[0], r[1], r[2]
+        x, y, z = r[0] - cx, r[1] - cy, r[2] - cz
         rr = xnp.sqrt(x ** 2 + y ** 2 + z ** 2)
         rho = xnp.sqrt(x ** 2 + y ** 2)
         th = xnp.arccos(xnp.nan_to_num(z / rr))

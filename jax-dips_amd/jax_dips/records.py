@@ -35,27 +35,27 @@ def batch_bounds(n, batch_size):
     return B, begins, rows
 
 
-def pack_dense(W125, c, phase125, batch_size):
-    """Pack dense 125-offset records (host arrays) into the CSR device layout.
+def pack_dense(W250, c, batch_size):
+    """Pack dense (offset, phase)-keyed records (host arrays) into the CSR
+    device layout.
 
-    W125 (n,125) weights, c (n,) constants, phase125 (n,125) bool.  Returns a
+    W250 (n,250) weights indexed 2*code + phase, c (n,) constants.  Returns a
     dict of host arrays in row order (batches contiguous, rows sorted by entry
     count inside each batch)."""
-    n = W125.shape[0]
+    n = W250.shape[0]
     B, begins, rows = batch_bounds(n, batch_size)
-    nnz_row = (W125 != 0).sum(axis=1)
+    nnz_row = (W250 != 0).sum(axis=1)
     order = np.empty(n, dtype=np.int64)
     for b0, nr in zip(begins, rows):
         seg = np.arange(b0, b0 + nr)
         order[b0:b0 + nr] = seg[np.argsort(nnz_row[seg], kind="stable")]
-    Wo = W125[order]
-    Po = phase125[order]
+    Wo = W250[order]
     rr, cc = np.nonzero(Wo)
     row_ptr = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(np.bincount(rr, minlength=n), out=row_ptr[1:])
     if row_ptr[-1] >= 2 ** 31:
         raise ValueError("too many record entries for int32 offsets")
-    code = cc.astype(np.uint32) | (Po[rr, cc].astype(np.uint32) << np.uint32(7))
+    code = (cc // 2).astype(np.uint32) | ((cc % 2).astype(np.uint32) << np.uint32(7))
     return dict(row_point=order.astype(np.int32), row_ptr=row_ptr.astype(np.int32),
                 row_c=np.asarray(c, dtype=np.float64)[order].astype(np.float32),
                 ent_code=code, ent_w=Wo[rr, cc].astype(np.float32),
@@ -95,9 +95,44 @@ class DeviceRecords:
         self.batch_size = int(packed["batch_size"])
         self.batch_begin = np.asarray(packed["batch_begin"], np.int64)
         self.batch_rows = np.asarray(packed["batch_rows"], np.int64)
+        self._finish()
+
+    @classmethod
+    def from_device(cls, row_point, row_ptr, row_c, ent_code, ent_w, nnz, h, batch_size, batch_begin, batch_rows):
+        """Wrap records already in device memory (built by the geometry kernels)."""
+        self = cls.__new__(cls)
+        self.row_point, self.row_ptr, self.row_c = row_point, row_ptr, row_c
+        self.ent_code, self.ent_w = ent_code, ent_w
+        self.nrows = int(row_c.numel())
+        self.nnz = int(nnz)
+        self.h = np.asarray(h, np.float32)
+        self.batch_size = int(batch_size)
+        self.batch_begin = np.asarray(batch_begin, np.int64)
+        self.batch_rows = np.asarray(batch_rows, np.int64)
+        self._finish()
+        return self
+
+    def _finish(self):
         self.c = N.Records(self.row_point.data_ptr(), self.row_ptr.data_ptr(), self.row_c.data_ptr(),
                            self.ent_code.data_ptr(), self.ent_w.data_ptr(), self.nrows, self.nnz,
                            float(self.h[0]), float(self.h[1]), float(self.h[2]))
+
+    def to_dense(self):
+        """Host copy as dense (N,125) weights, (N,125) phase and (N,) constants in
+        grid-point order (for tests and inspection)."""
+        rp = self.row_point.cpu().numpy().astype(np.int64)
+        ptr = self.row_ptr.cpu().numpy().astype(np.int64)
+        code = self.ent_code.cpu().numpy().astype(np.int64)[: self.nnz]
+        w = self.ent_w.cpu().numpy()[: self.nnz]
+        c = self.row_c.cpu().numpy()
+        n = self.nrows
+        W = np.zeros((n, 250), np.float64)
+        rows = np.repeat(np.arange(n), np.diff(ptr))
+        slot = (code & 127) * 2 + ((code >> 7) & 1)
+        np.add.at(W, (rp[rows], slot), w)
+        C = np.zeros(n, np.float64)
+        C[rp] = c
+        return W, C
 
 
 OPT_KIND = {"custom": 0, "adam": 1, "rmsprop": 2}

@@ -1,0 +1,302 @@
+"""NBM Poisson trainer with the reference's contract, running on the HIP kernels.
+
+``setup(11 callbacks[, nonlinear_op_m, nonlinear_op_p]) -> init_fn``;
+``init_fn(lvl_gstate, tr_gstate, eval_gstate, ...) -> (PoissonSimState, solve_fn)``;
+``solve_fn(sim_state) -> (PoissonSimState, epoch_store, loss_epochs)``
+(jax_dips/solvers/poisson/trainer.py:980-1137).
+
+Callbacks are functions of coordinate-first positions r (shape (3, m)) written
+with the ``jax_dips.xnp`` namespace (or plain numpy), i.e. the reference's
+``r[0], r[1], r[2]`` callbacks with ``jnp`` replaced by ``xnp``.
+
+What runs where (Trainer.solve):
+  * per zoom level (h = h_tr * 0.5**(epoch // (E//4)), data_management.py:319-326)
+    the FP64 geometry kernels build the affine residual records of every
+    training point once (level_records.build_level);
+  * each epoch walks the contiguous batches (no shuffling, trainer.py:539): one
+    fused residual/loss/gradient kernel and one fused optimizer kernel per
+    batch (dips_train_level), losses kept on the device;
+  * loss_epochs[e] = mean of the pre-update batch losses of epoch e
+    (trainer.py:553-554);
+  * the solution, grad u and d_n u on eval_gstate come from dips_eval
+    (trainer.py:960-977).
+Unsupported reference options raise NotImplementedError: algorithm 1, lbfgs,
+the learned preconditioner, nonlinear operators, non-tanh activations and
+padded last batches (batch_size not dividing the number of training points).
+"""
+import logging
+import os
+import pickle
+import signal
+import time
+
+import numpy as np
+
+from ... import _native
+from ...data import data_management
+from ...level_records import build_level, sample
+from ...nn.configure import get_model
+from ...records import DeviceGrid, Engine
+from ..simulation_states import PoissonSimState, PoissonSimStateFn
+from ..._jaxmd_modules import dataclasses
+
+logger = logging.getLogger(__name__)
+
+stop_training = False
+
+
+def signalHandler(signal_num, frame):
+    """SIGINT stops training after the current epoch (trainer.py:68-78)."""
+    global stop_training
+    stop_training = True
+    logger.warning("Signal %s: training will stop after the completion of the current epoch", signal_num)
+
+
+DEFAULT_OPTIMIZER = {"optimizer_name": "custom", "learning_rate": 1e-3,
+                     "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
+DEFAULT_MODEL = {"name": None, "model_type": "mlp",
+                 "mlp": {"hidden_layers_m": 1, "hidden_dim_m": 1, "activation_m": "jnp.tanh",
+                         "hidden_layers_p": 2, "hidden_dim_p": 10, "activation_p": "jnp.tanh"}}
+
+
+def _callbacks(sim_state_fn):
+    return dict(phi=sim_state_fn.phi_fn, mu_m=sim_state_fn.mu_m_fn, mu_p=sim_state_fn.mu_p_fn,
+                k_m=sim_state_fn.k_m_fn, k_p=sim_state_fn.k_p_fn, f_m=sim_state_fn.f_m_fn,
+                f_p=sim_state_fn.f_p_fn, alpha=sim_state_fn.alpha_fn, beta=sim_state_fn.beta_fn,
+                g=sim_state_fn.dir_bc_fn)
+
+
+class Trainer:
+    """Trains the two-phase network of one interface problem on the GPU."""
+
+    def __init__(self, lvl_gstate, tr_gstate, eval_gstate, sim_state, sim_state_fn, algorithm=0,
+                 mgrad_over_pgrad_scalefactor=1, lvl_set_fn=None, num_epochs=1000, multi_gpu=False,
+                 batch_size=131072, checkpoint_dir="./checkpoints", checkpoint_interval=2, results_dir="./",
+                 loss_plot_name="solver_loss", optimizer_dict=DEFAULT_OPTIMIZER, restart=False,
+                 restart_checkpoint_dir="./checkpoints", print_rate=1, model_dict=DEFAULT_MODEL, init_params=None):
+        if algorithm != 0:
+            raise NotImplementedError("algorithm 1 (network-based extrapolation) is not implemented in the reference either")
+        if optimizer_dict.get("optimizer_name") == "lbfgs":
+            raise NotImplementedError("lbfgs (jaxopt) has no device implementation")
+        if (model_dict.get("preconditioner") or {}).get("enable", False):
+            raise NotImplementedError("the learned preconditioner has no device implementation")
+        if sim_state_fn.nonlinear_op_m is not None or sim_state_fn.nonlinear_op_p is not None:
+            raise NotImplementedError("nonlinear operators have no device implementation")
+        if multi_gpu:
+            logger.warning("multi_gpu=True: one GPU is used; the batch walk matches the single-GPU path")
+        self.lvl_gstate, self.tr_gstate, self.eval_gstate = lvl_gstate, tr_gstate, eval_gstate
+        self.sim_state, self.sim_state_fn = sim_state, sim_state_fn
+        self.num_epochs = int(num_epochs)
+        self.batch_size = int(batch_size)
+        self.checkpoint_dir, self.checkpoint_interval = checkpoint_dir, checkpoint_interval
+        self.results_dir, self.loss_plot_name, self.print_rate = results_dir, loss_plot_name, max(1, int(print_rate))
+        self.optimizer_dict = optimizer_dict
+        self.TD = data_management.TrainData(tr_gstate, lvl_set_fn)
+        self.train_points = self.TD.train_points
+        self.train_dx, self.train_dy, self.train_dz = (np.float32(tr_gstate.dx), np.float32(tr_gstate.dy),
+                                                       np.float32(tr_gstate.dz))
+        self.forward, self.framework = get_model(model_dict, model_type=model_dict.get("model_type", "mlp"))
+        self.model = self.forward.model
+        self.cbs = _callbacks(sim_state_fn)
+        self.epoch_start = 0
+        self.opt_restore = None
+        if restart:
+            state = self.fetch_checkpoint(restart_checkpoint_dir)
+            if state is None:
+                raise FileNotFoundError(f"no checkpoint in {restart_checkpoint_dir}")
+            self.params = state["params"]
+            self.opt_restore = state["opt_state"]
+            self.epoch_start = int(state["epoch"])
+            self.batch_size = int(state["batch_size"])
+        elif init_params is not None:
+            self.params = init_params if isinstance(init_params, dict) else self.model.unflatten(init_params)
+        else:
+            self.params = self.forward.init(42)
+        self.params = dict(self.params)
+        self.params.setdefault("preconditioner", {})
+        self.epoch_store = np.arange(self.epoch_start, self.num_epochs)
+        self.loss_epochs = np.zeros(self.num_epochs - self.epoch_start, dtype=np.float32)
+
+    # ------------------------------------------------------------------ checkpoints
+    @staticmethod
+    def fetch_checkpoint(checkpoint_dir):
+        """Latest ``checkpoint_<epoch>`` by epoch number (the reference picks the
+        lexicographic max, trainer.py:334, so checkpoint_100 < checkpoint_20)."""
+        if checkpoint_dir is None or not os.path.exists(checkpoint_dir):
+            return None
+        cks = [p for p in os.listdir(checkpoint_dir) if p.startswith("checkpoint_") and p[11:].isdigit()]
+        if not cks:
+            return None
+        best = max(cks, key=lambda p: int(p[11:]))
+        with open(os.path.join(checkpoint_dir, best), "rb") as f:
+            return pickle.load(f)
+
+    @staticmethod
+    def save_checkpoint(checkpoint_dir, state):
+        """pickle {opt_state, params, epoch, batch_size, resolution} to
+        checkpoint_dir/checkpoint_<epoch> (trainer.py:340-351)."""
+        if checkpoint_dir is None:
+            return None
+        os.makedirs(checkpoint_dir, exist_ok=True)
+        path = os.path.join(checkpoint_dir, "checkpoint_" + str(state["epoch"]))
+        with open(path, "wb") as f:
+            pickle.dump(state, f)
+        return path
+
+    def _state(self, engine, epoch):
+        return {"opt_state": {"mu": self.model.unflatten(engine.m.cpu().numpy()),
+                              "nu": self.model.unflatten(engine.v.cpu().numpy()),
+                              "count": int(engine.count.item())},
+                "params": dict(self.model.unflatten(engine.params.cpu().numpy()), preconditioner={}),
+                "epoch": int(epoch), "batch_size": self.batch_size,
+                "resolution": f"{self.train_dx}, {self.train_dy}, {self.train_dz}"}
+
+    # ------------------------------------------------------------------ training
+    def solve(self):
+        global stop_training
+        import torch
+
+        tr = self.tr_gstate
+        n = len(tr.R)
+        B = min(self.batch_size, n)
+        if n % B:
+            raise NotImplementedError("batch_size must divide the number of training points "
+                                      "(padded last batches are not supported on the device)")
+        nb = n // B
+        grid = DeviceGrid(tr.x, tr.y, tr.z)
+        lv = self.lvl_gstate
+        bounds = (lv.xmin(), lv.xmax(), lv.ymin(), lv.ymax(), lv.zmin(), lv.zmax())
+        E = self.num_epochs
+        steps_total = (E - self.epoch_start) * nb
+        engine = Engine(self.model.dims, self.model.flatten(self.params), self.optimizer_dict, max_rows=B,
+                        max_steps=max(1, steps_total))
+        if self.opt_restore is not None:
+            engine.m.copy_(torch.as_tensor(self.model.flatten(self.opt_restore["mu"]), device="cuda"))
+            engine.v.copy_(torch.as_tensor(self.model.flatten(self.opt_restore["nu"]), device="cuda"))
+            engine.count.fill_(int(self.opt_restore["count"]))
+        count0 = int(engine.count.item())
+        prev = signal.signal(signal.SIGINT, signalHandler)
+        stop_training = False
+        t0 = time.time()
+        done = self.epoch_start
+        self.timing = dict(records_s=0.0, train_s=0.0)
+        try:
+            epoch = self.epoch_start
+            while epoch < E and not stop_training:
+                lvl = epoch // (E // 4)  # ZeroDivisionError for E < 4, as in the reference
+                last = epoch
+                while last + 1 < E and (last + 1) // (E // 4) == lvl:
+                    last += 1
+                h = np.array(self.TD.alternate_res_sequentially(E, epoch, tr.dx, tr.dy, tr.dz), np.float32)
+                tr0 = time.time()
+                rec = build_level(grid, h, bounds, self.cbs, B)
+                torch.cuda.synchronize()
+                self.timing["records_s"] += time.time() - tr0
+                tt0 = time.time()
+                for e in range(epoch, last + 1):
+                    if stop_training:
+                        break
+                    engine.train_level(grid, rec, 1)
+                    done = e + 1
+                torch.cuda.synchronize()
+                self.timing["train_s"] += time.time() - tt0
+                hist = engine.loss_hist.cpu().numpy()
+                for e in range(epoch, done):
+                    k = (e - self.epoch_start) * nb
+                    self.loss_epochs[e - self.epoch_start] = hist[k:k + nb].mean()
+                    if e % self.print_rate == 0:
+                        logger.info(f"epoch # {e} loss is {self.loss_epochs[e - self.epoch_start]:.6e}")
+                del rec
+                epoch = last + 1
+        finally:
+            signal.signal(signal.SIGINT, prev)
+        logger.info(f"solve took {time.time() - t0} (sec)")
+        self.steps = int(engine.count.item()) - count0
+        self.params = dict(self.model.unflatten(engine.params.cpu().numpy()), preconditioner={})
+        self.save_checkpoint(self.checkpoint_dir, self._state(engine, done))
+        self._plot()
+        self.engine = engine
+        u, gu, gnu = self.evaluate_solution_and_gradients(self.params, self.eval_gstate)
+        return u, gu, gnu, self.epoch_store, self.loss_epochs
+
+    def _plot(self):
+        try:
+            import matplotlib
+
+            matplotlib.use("Agg")
+            import matplotlib.pyplot as plt
+        except ImportError:
+            return
+        os.makedirs(self.results_dir, exist_ok=True)
+        fig, ax = plt.subplots(figsize=(6, 4))
+        ax.plot(self.epoch_store, self.loss_epochs)
+        ax.set_yscale("log")
+        ax.set_xlabel("epoch")
+        ax.set_ylabel("loss")
+        fig.savefig(os.path.join(self.results_dir, f"{self.loss_plot_name}.png"), dpi=100)
+        plt.close(fig)
+
+    # ------------------------------------------------------------------ evaluation
+    def evaluate_solution_fn(self, params, R_flat):
+        from ...evaluation import evaluate_points
+        import torch
+
+        R = np.asarray(R_flat, np.float32)
+        phase = sample(self.cbs["phi"], torch.as_tensor(R, device="cuda")).cpu().numpy() >= 0
+        u, _ = evaluate_points(self.model, self.model.flatten(params), R, phase)
+        return u
+
+    def evaluate_solution_and_gradients(self, params, eval_gstate):
+        from ...evaluation import central_normals, evaluate_points
+        import torch
+
+        R = np.asarray(eval_gstate.R, np.float32)
+        phase = sample(self.cbs["phi"], torch.as_tensor(R, device="cuda")).cpu().numpy() >= 0
+        nrm = central_normals(self.cbs["phi"], R, eval_gstate.dx, eval_gstate.dy, eval_gstate.dz)
+        u, g, gn = evaluate_points(self.model, self.model.flatten(params), R, phase, normals=nrm)
+        return u, g, gn
+
+
+def setup(initial_value_fn, dirichlet_bc_fn, lvl_set_fn, mu_m_fn_, mu_p_fn_, k_m_fn_, k_p_fn_, f_m_fn_, f_p_fn_,
+          alpha_fn_, beta_fn_, nonlinear_op_m=None, nonlinear_op_p=None):
+    """Returns init_fn (trainer.py:980-1137).  Missing nonlinear operators are zero."""
+    if nonlinear_op_m is None:
+        logger.warning("nonlinear_op_m(u) is not defined. Setting it to zero.")
+    if nonlinear_op_p is None:
+        logger.warning("nonlinear_op_p(u) is not defined. Setting it to zero.")
+    sim_state_fn = PoissonSimStateFn(initial_value_fn, dirichlet_bc_fn, lvl_set_fn, mu_m_fn_, mu_p_fn_, k_m_fn_,
+                                     k_p_fn_, f_m_fn_, f_p_fn_, alpha_fn_, beta_fn_, nonlinear_op_m, nonlinear_op_p)
+
+    def init_fn(lvl_gstate=None, tr_gstate=None, eval_gstate=None, num_epochs=1000, batch_size=131072, algorithm=0,
+                mgrad_over_pgrad_scalefactor=1, multi_gpu=False, checkpoint_interval=1000,
+                checkpoint_dir="./checkpoints", results_dir="./", loss_plot_name="solver_loss",
+                optimizer_dict=DEFAULT_OPTIMIZER, model_dict=DEFAULT_MODEL, restart=False,
+                restart_checkpoint_dir="./checkpoints", print_rate=1, init_params=None):
+        import torch
+
+        R = torch.as_tensor(np.asarray(eval_gstate.R, np.float32), device="cuda")
+
+        def ev(fn):
+            return sample(fn, R).cpu().numpy().astype(np.float32)
+
+        state = PoissonSimState(ev(lvl_set_fn), ev(initial_value_fn), ev(dirichlet_bc_fn), ev(mu_m_fn_),
+                                ev(mu_p_fn_), ev(k_m_fn_), ev(k_p_fn_), ev(f_m_fn_), ev(f_p_fn_), ev(alpha_fn_),
+                                ev(beta_fn_), None, None)
+
+        def solve_fn(sim_state):
+            trainer = Trainer(lvl_gstate, tr_gstate, eval_gstate, sim_state, sim_state_fn, algorithm,
+                              mgrad_over_pgrad_scalefactor=mgrad_over_pgrad_scalefactor, lvl_set_fn=lvl_set_fn,
+                              num_epochs=num_epochs, multi_gpu=multi_gpu, batch_size=batch_size,
+                              checkpoint_dir=checkpoint_dir, checkpoint_interval=checkpoint_interval,
+                              results_dir=results_dir, loss_plot_name=loss_plot_name, optimizer_dict=optimizer_dict,
+                              restart=restart, restart_checkpoint_dir=restart_checkpoint_dir, print_rate=print_rate,
+                              model_dict=model_dict, init_params=init_params)
+            solve_fn.trainer = trainer
+            final_solution, grad_u, grad_u_n, epoch_store, loss_epochs = trainer.solve()
+            new_state = dataclasses.replace(sim_state, solution=final_solution, grad_solution=grad_u,
+                                            grad_normal_solution=grad_u_n)
+            return new_state, epoch_store, loss_epochs
+
+        return state, solve_fn
+
+    return init_fn

@@ -68,10 +68,12 @@ def where(c, a, b):
     if _first((c, a, b)):
         import torch
 
-        ref = next(x for x in (a, b, c) if _is_torch(x) and x.is_floating_point()) if any(
-            _is_torch(x) and x.is_floating_point() for x in (a, b, c)) else next(x for x in (a, b, c) if _is_torch(x))
-        a = a if _is_torch(a) else torch.as_tensor(a, dtype=ref.dtype, device=ref.device)
-        b = b if _is_torch(b) else torch.as_tensor(b, dtype=ref.dtype, device=ref.device)
+        tens = [x for x in (a, b, c) if _is_torch(x)]
+        flt = [x for x in tens if x.is_floating_point()]
+        dtype = flt[0].dtype if flt else torch.float64
+        dev = tens[0].device
+        a = a if _is_torch(a) else torch.as_tensor(a, dtype=dtype, device=dev)
+        b = b if _is_torch(b) else torch.as_tensor(b, dtype=dtype, device=dev)
         return torch.where(c, a, b)
     return np.where(c, a, b)
 

@@ -250,17 +250,22 @@ def point_slots(P, h, bounds, cb, dt=np.float64):
                 c_m=c_m, c_p=c_p, diag=diag, rhs=rhs)
 
 
-def merge_slots(slots, P, h):
-    """Fold (stencil, cube) slots onto the 125 local lattice offsets.
+def merge_slots(slots, P, h, phi_of):
+    """Fold (stencil, cube) slots onto (lattice offset, phase) keys.
 
-    Returns W125 (n,125) and the canonical float32 positions fl32(p + o*h)
-    (n,125,3)."""
+    A slot's network phase is phi(s_e + X_q) >= 0 at its own float32 position
+    (MLP.py:98); slots landing on the same offset o = e + x_q in [-2,2]^3 with
+    the same phase are summed.  Returns W250 (n,250) indexed 2*code + phase and
+    the canonical float32 positions fl32(p + o*h) (n,125,3) at which the folded
+    entry is evaluated."""
     W = slots["W"]
     n = W.shape[0]
-    W125 = np.zeros((n, 125), dtype=W.dtype)
+    slot_phase = phi_of(slots["pos"]) >= 0  # (n,7,27)
+    W250 = np.zeros((n, 250), dtype=W.dtype)
     for e in range(7):
         for q in range(27):
             code = offset_code(STENCIL[e] + CUBE[q])
-            W125[:, code] += W[:, e, q]
+            key = 2 * code + slot_phase[:, e, q].astype(np.int64)
+            np.add.at(W250, (np.arange(n), key), W[:, e, q])
     pos = f32_shift(np.asarray(P, np.float32)[:, None, :], OFFS125[None], h)
-    return W125, pos
+    return W250, pos

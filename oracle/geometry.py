@@ -37,11 +37,12 @@ FACE_SIGN = np.array([-1, 1, -1, 1, -1, 1])
 
 
 def cell_corners(P, h, dt):
-    """Corner positions p + 0.5*[+-h] in ``dt`` (per-point file :276-290)."""
-    P = np.asarray(P, dtype=dt)
-    half = (np.asarray(h, dtype=dt) * dt(0.5))
-    off = CORNER_SIGNS.astype(dt) * half[None, :]
-    return P[:, None, :] + off[None, :, :]
+    """Corner positions p + 0.5*[+-h], formed in float32 as the reference forms
+    them (per-point file :276-290), returned in ``dt``."""
+    P = np.asarray(P, dtype=np.float32)
+    half = (np.asarray(h, dtype=np.float32) * np.float32(0.5))
+    off = CORNER_SIGNS.astype(np.float32) * half[None, :]
+    return (P[:, None, :] + off[None, :, :]).astype(dt)
 
 
 def region_from_corners(phi_c):
@@ -167,7 +168,7 @@ def cell_geometry(P, h, phi_c, dt=np.float64):
     area_m_cut = np.zeros((n, 6), dtype=dt)
     for f in range(6):
         ax = FACE_AXIS[f]
-        fc = P[:, ax] + FACE_SIGN[f] * (dt(0.5) * h[ax])
+        fc = (np.asarray(P[:, ax], np.float32) + np.float32(FACE_SIGN[f]) * (np.float32(0.5) * np.float32(h[ax]))).astype(dt)
         tot = np.zeros(n, dtype=dt)
         for t in FACE_TETS[f]:
             for k in range(3):

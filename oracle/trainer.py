@@ -55,25 +55,29 @@ def padding_points(R, count, seed=0):
 
 
 class LevelRecords:
-    """Dense 125-offset affine records of all points at one stencil spacing."""
+    """Dense (offset, phase)-keyed affine records of all points at one stencil
+    spacing: r = c + sum_k W[:, k] * NN_{k % 2}(pos[:, k // 2])."""
 
     def __init__(self, P, h, bounds, cb, dt=np.float64):
         slots = D.point_slots(P, h, bounds, cb, dt)
-        self.W, self.pos = D.merge_slots(slots, P, h)
+        self.W, self.pos = D.merge_slots(slots, P, h, lambda R: cb("phi", R))
         self.c = slots["c"]
         self.boundary = slots["boundary"]
         self.dead = slots["dead"]
         self.slots = slots
-        self.phase = cb("phi", self.pos) >= 0
         self.h = np.asarray(h, np.float32)
+
+    def entries(self, rows):
+        W = self.W[rows]
+        nz = np.nonzero(W)
+        X = self.pos[rows][nz[0], nz[1] // 2]
+        ph = (nz[1] % 2) == 1
+        return W, nz, X, ph
 
 
 def residual(theta, dims, rec, rows, dt=np.float64):
     """r for the given rows (n,), evaluating the network only where W != 0."""
-    W = rec.W[rows]
-    nz = np.nonzero(W)
-    X = rec.pos[rows][nz]
-    ph = rec.phase[rows][nz]
+    W, nz, X, ph = rec.entries(rows)
     u = M.forward(theta, dims, X, ph, dt)
     r = rec.c[rows].astype(dt).copy()
     np.add.at(r, nz[0], W[nz].astype(dt) * u)
@@ -82,10 +86,7 @@ def residual(theta, dims, rec, rows, dt=np.float64):
 
 def loss_and_grad(theta, dims, rec, rows, dt=np.float64):
     """0.5 * mean(r^2) over the batch and its gradient (hand VJP)."""
-    W = rec.W[rows]
-    nz = np.nonzero(W)
-    X = rec.pos[rows][nz]
-    ph = rec.phase[rows][nz]
+    W, nz, X, ph = rec.entries(rows)
     u = M.forward(theta, dims, X, ph, dt)
     r = rec.c[rows].astype(dt).copy()
     np.add.at(r, nz[0], W[nz].astype(dt) * u)

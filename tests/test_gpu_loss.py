@@ -27,7 +27,7 @@ DIMS = (1, 1, 2, 10)
 def _device(axes, rec, batch_size):
     from jax_dips.records import DeviceGrid, DeviceRecords, pack_dense
 
-    packed = pack_dense(rec.W, rec.c, rec.phase, batch_size)
+    packed = pack_dense(rec.W, rec.c, batch_size)
     return DeviceGrid(*axes), DeviceRecords(packed, rec.h), packed
 
 
