@@ -86,7 +86,7 @@ def main():
           flush=True)
 
     K = max(4, args.steps - args.steps % 4)
-    eng = Engine(dims, theta, opt, max_rows=B, max_steps=args.warmup + K + 8)
+    eng = Engine(dims, theta, opt, max_rows=B, max_steps=args.warmup + K + args.roofline_launches + 8)
     L = N.lib()
 
     def run_steps(k, stream):
@@ -94,12 +94,12 @@ def main():
         per = k // 4
         for li, rec in enumerate(levels):
             idx = np.arange(per) % nb
-            beg = (ctypes.c_int64 * per)(*rec.batch_begin[idx].tolist())
-            rows = (ctypes.c_int64 * per)(*rec.batch_rows[idx].tolist())
-            st = L.dips_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), beg, rows, per, rec.batch_size, 1,
+            st = L.dips_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), N.i64_array(rec.batch_begin[idx]),
+                                    N.i64_array(rec.batch_rows[idx]), N.i64_array(rec.batch_nbulk[idx]), per,
+                                    rec.batch_size, 1,
                                     ctypes.byref(eng.net), ctypes.byref(eng.opt), N.ptr(eng.params), N.ptr(eng.m),
                                     N.ptr(eng.v), N.ptr(eng.count), N.ptr(eng.loss_hist), int(eng.loss_hist.numel()),
-                                    N.ptr(eng.partial), eng.partial_cap, stream)
+                                    N.ptr(eng.partial), eng.partial_cap, N.ptr(eng.sync), stream)
             N.check(st, "dips_train_level")
 
     stream = N.current_stream()
@@ -124,7 +124,6 @@ def main():
         cum = np.concatenate([[0.0], np.cumsum(fl)])
         for b0, nr in zip(rec.batch_begin, rec.batch_rows):
             per_launch_flops.append(cum[ptr[b0 + nr]] - cum[ptr[b0]])
-    nbl = ctypes.c_int32(0)
     R = max(4, args.roofline_launches - args.roofline_launches % 4)
     torch.cuda.synchronize()
     r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -134,9 +133,12 @@ def main():
         li = (4 * i) // R
         rec = levels[li]
         b = i % nb
-        N.check(L.dips_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), int(rec.batch_begin[b]),
-                                 int(rec.batch_rows[b]), ctypes.byref(eng.net), N.ptr(eng.params), N.ptr(eng.partial),
-                                 eng.partial_cap, 1, ctypes.byref(nbl), stream), "dips_loss_grad")
+        N.check(L.dips_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), int(rec.batch_begin[b]),
+                                  int(rec.batch_rows[b]), int(rec.batch_nbulk[b]), ctypes.byref(eng.net),
+                                  ctypes.byref(eng.opt),
+                                  ctypes.c_float(1.0 / B), N.ptr(eng.params), N.ptr(eng.m), N.ptr(eng.v),
+                                  N.ptr(eng.count), N.ptr(eng.loss_hist), int(eng.loss_hist.numel()),
+                                  N.ptr(eng.partial), eng.partial_cap, N.ptr(eng.sync), stream), "dips_train_step")
         flops_sum += per_launch_flops[li * nb + b]
     r1.record()
     torch.cuda.synchronize()
@@ -160,7 +162,7 @@ def main():
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "loss_grad_kernel<1,1,2,10,true>", "avg_launch_us": avg_s * 1e6,
+                     "kernel": "loss_grad_kernel<1,1,2,10,grad,fused> (one launch per optimizer step)", "avg_launch_us": avg_s * 1e6,
                      "flop_per_launch": flops_sum / R},
         "records_build_s": t_records,
         "loss_tail": loss_tail,

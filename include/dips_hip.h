@@ -95,11 +95,13 @@ int64_t dips_partial_size(const dips_net_t* net, int64_t nrows);
 int dips_net_supported(const dips_net_t* net);
 
 /* Forward residual + loss (and, when want_grad != 0, the parameter gradient)
- * of rows [row_begin, row_begin + nrows) of rec.  Writes per-block partials
- * (gradient sums then 0.5*sum r^2) into partial[0 .. dips_partial_size) and the
- * number of blocks into *nblk (host pointer). */
+ * of rows [row_begin, row_begin + nrows) of rec.  The first nbulk rows of the
+ * range (few entries each) are processed one lane per row, the rest one
+ * wavefront per row.  Writes per-block partials (gradient sums then
+ * 0.5*sum r^2) into partial[0 .. dips_partial_size) and the number of blocks
+ * into *nblk (host pointer). */
 int dips_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
-                   const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
+                   int64_t nbulk, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
                    int want_grad, int32_t* nblk, void* stream);
 
 /* Reduce partials in a fixed order (float64 accumulation), scale by
@@ -110,15 +112,29 @@ int dips_opt_step(const dips_net_t* net, const dips_opt_t* opt, const float* par
                   float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
                   int32_t loss_cap, void* stream);
 
+/* One optimizer step in a single launch (networks up to 1024 parameters):
+ * the loss/gradient workgroups write parameter-major partial rows, the last
+ * workgroup to arrive (arrival counter *sync, a device uint32 that must be 0
+ * before the first call and is left at 0) reduces them in block order and
+ * applies the optimizer as dips_opt_step does.  Same results as
+ * dips_loss_grad followed by dips_opt_step. */
+int dips_train_step(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
+                    int64_t nbulk, const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params,
+                    float* m, float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* partial,
+                    int64_t partial_cap, uint32_t* sync, void* stream);
+
 /* nepochs epochs of one zoom level: for each epoch, for each batch b, rows
- * [batch_row_begin[b], batch_row_begin[b] + batch_rows[b]) (host arrays),
- * one dips_loss_grad + dips_opt_step with inv_batch = 1/batch_size.
+ * [batch_row_begin[b], batch_row_begin[b] + batch_rows[b]) with
+ * batch_nbulk[b] lane-per-row rows first (host arrays),
+ * one optimizer step with inv_batch = 1/batch_size (dips_train_step when the
+ * network fits it and sync is given, else dips_loss_grad + dips_opt_step).
  * loss_hist receives one loss per optimizer step at index *count. */
 int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const int64_t* batch_row_begin,
-                     const int64_t* batch_rows, int32_t nbatches, int64_t batch_size, int32_t nepochs,
+                     const int64_t* batch_rows, const int64_t* batch_nbulk, int32_t nbatches, int64_t batch_size,
+                     int32_t nepochs,
                      const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
                      int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
-                     void* stream);
+                     uint32_t* sync, void* stream);
 
 
 /* Network at m points pts (m,3): u (m), optionally grad_x u (m,3) and, with

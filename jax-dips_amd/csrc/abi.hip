@@ -43,6 +43,9 @@ static int check_records(const dips_grid_t* grid, const dips_records_t* rec) {
 
 using namespace dips;
 
+// networks up to this many parameters take the single-launch fused step
+static constexpr int kFusedMaxParams = 1024;
+
 extern "C" {
 
 const char* dips_last_error(void) { return g_last_error.c_str(); }
@@ -65,8 +68,8 @@ int dips_net_supported(const dips_net_t* net) {
 }
 
 int dips_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
-                   const dips_net_t* net, const float* params, float* partial, int64_t partial_cap, int want_grad,
-                   int32_t* nblk, void* stream) {
+                   int64_t nbulk, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
+                   int want_grad, int32_t* nblk, void* stream) {
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -75,7 +78,7 @@ int dips_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, int64_t r
     return DIPS_EINVAL;
   }
   int nb = 0;
-  st = loss_grad_launch(grid, rec, row_begin, nrows, net, params, partial, partial_cap, want_grad, &nb,
+  st = loss_grad_launch(grid, rec, row_begin, nrows, nbulk, net, params, partial, partial_cap, want_grad, &nb,
                         (hipStream_t)stream);
   *nblk = nb;
   return st;
@@ -94,24 +97,51 @@ int dips_opt_step(const dips_net_t* net, const dips_opt_t* opt, const float* par
                          (hipStream_t)stream);
 }
 
-int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const int64_t* batch_row_begin,
-                     const int64_t* batch_rows, int32_t nbatches, int64_t batch_size, int32_t nepochs,
-                     const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
-                     int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
-                     void* stream) {
+int dips_train_step(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
+                    int64_t nbulk, const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params,
+                    float* m, float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* partial,
+                    int64_t partial_cap, uint32_t* sync, void* stream) {
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
-  if (!batch_row_begin || !batch_rows || nbatches < 1 || batch_size < 1 || nepochs < 0) {
+  if (!opt || !params || !m || !v || !count || !loss_hist || !partial || !sync) {
+    set_error("dips_train_step: null buffer");
+    return DIPS_EINVAL;
+  }
+  if (partial_stride(net) - 1 > kFusedMaxParams) {
+    set_error("dips_train_step: network too large for the fused step; use dips_loss_grad + dips_opt_step");
+    return DIPS_EUNSUPPORTED;
+  }
+  return train_step_launch(grid, rec, row_begin, nrows, nbulk, net, opt, inv_batch, params, m, v, count, loss_hist,
+                           loss_cap, partial, partial_cap, sync, (hipStream_t)stream);
+}
+
+int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const int64_t* batch_row_begin,
+                     const int64_t* batch_rows, const int64_t* batch_nbulk, int32_t nbatches, int64_t batch_size,
+                     int32_t nepochs,
+                     const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
+                     int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
+                     uint32_t* sync, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!batch_row_begin || !batch_rows || !batch_nbulk || nbatches < 1 || batch_size < 1 || nepochs < 0) {
     set_error("dips_train_level: invalid batch description");
     return DIPS_EINVAL;
   }
   const float inv_b = 1.0f / (float)batch_size;
+  const bool fused = (partial_stride(net) - 1 <= kFusedMaxParams) && sync != nullptr;
   for (int e = 0; e < nepochs; ++e) {
     for (int b = 0; b < nbatches; ++b) {
+      if (fused) {
+        st = train_step_launch(grid, rec, batch_row_begin[b], batch_rows[b], batch_nbulk[b], net, opt, inv_b, params,
+                               m, v, count, loss_hist, loss_cap, partial, partial_cap, sync, (hipStream_t)stream);
+        if (st) return st;
+        continue;
+      }
       int nb = 0;
-      st = loss_grad_launch(grid, rec, batch_row_begin[b], batch_rows[b], net, params, partial, partial_cap, 1, &nb,
-                            (hipStream_t)stream);
+      st = loss_grad_launch(grid, rec, batch_row_begin[b], batch_rows[b], batch_nbulk[b], net, params, partial,
+                            partial_cap, 1, &nb, (hipStream_t)stream);
       if (st) return st;
       st = opt_step_launch(net, opt, partial, nb, inv_b, params, m, v, count, loss_hist, loss_cap, (hipStream_t)stream);
       if (st) return st;

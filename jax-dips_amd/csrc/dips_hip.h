@@ -11,9 +11,15 @@ void set_error(const std::string& msg);
 int hip_status(hipError_t e, const char* what);
 
 // loss.hip
+struct FusedOpt;
 int loss_grad_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
-                     const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
-                     int want_grad, int* nblk_out, hipStream_t stream);
+                     int64_t nbulk, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
+                     int want_grad, int* nblk_out, hipStream_t stream, const FusedOpt* fused = nullptr);
+// One fused launch: loss + gradient + (last workgroup) reduction + optimizer.
+int train_step_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
+                      int64_t nbulk, const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params,
+                      float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, float* partial,
+                      int64_t partial_cap, uint32_t* sync, hipStream_t stream);
 int partial_stride(const dips_net_t* net);
 int64_t partial_blocks(int64_t nrows);
 

@@ -20,14 +20,55 @@
 namespace dips {
 
 constexpr int kLossThreads = 256;
+struct FusedOpt;
 
-template <int Lm, int Hm, int Lp, int Hp, bool kGrad>
+// Optimizer state consumed by the fused step (last workgroup of the launch).
+struct FusedOpt {
+  dips_opt_t opt;
+  float inv_b;
+  float* m;
+  float* v;
+  int32_t* count;
+  float* loss_hist;
+  int loss_cap;
+  uint32_t* sync;  // arrival counter, zero between launches
+};
+
+// Optimizer update of parameter p with gradient g (optax chain of
+// solvers/optimizers.py:33-97; see optim.hip for the float32 constants).
+__device__ __forceinline__ void apply_update(const dips_opt_t& opt, int c, float lr_t, float bc1, float bc2,
+                                             bool clip, float gn, int p, float g, float* params, float* m, float* v) {
+  if (clip) g = (g / gn) * opt.max_norm;
+  float u;
+  if (opt.kind == 2) {
+    const float nv = 0.1f * (g * g) + 0.9f * v[p];
+    v[p] = nv;
+    u = -(g / sqrtf(nv + 1e-8f)) * opt.lr;
+  } else {
+    const float nm = 0.1f * g + 0.9f * m[p];
+    const float nv = 0.001f * (g * g) + 0.999f * v[p];
+    m[p] = nm;
+    v[p] = nv;
+    u = (nm / bc1) / (sqrtf(nv / bc2) + 1e-8f);
+    u = (opt.kind == 0) ? -(lr_t * u) : -(u * opt.lr);
+  }
+  params[p] = params[p] + u;
+}
+
+__device__ __forceinline__ float schedule_lr(const dips_opt_t& opt, int c) {
+  if (opt.kind != 0) return opt.lr;
+  if (opt.sched == 0) return (c <= 0) ? opt.lr : opt.lr * powf(opt.decay_rate, (float)c / opt.transition_steps);
+  const float cc = fminf(fmaxf((float)c, 0.0f), opt.transition_steps);
+  return opt.lr * (1.0f - cc / opt.transition_steps);
+}
+
+template <int Lm, int Hm, int Lp, int Hp, bool kGrad, bool kFused>
 __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
     const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs, int ny, int nz,
     float hx, float hy, float hz, const int32_t* __restrict__ row_point, const int32_t* __restrict__ row_ptr,
     const float* __restrict__ row_c, int64_t row_begin, int64_t nrows, const uint32_t* __restrict__ ent_code,
-    const float* __restrict__ ent_w, int64_t nnz, int npts, const float* __restrict__ params,
-    float* __restrict__ partial) {
+    const float* __restrict__ ent_w, int64_t nnz, int npts, float* params, float* __restrict__ partial,
+    FusedOpt fo, int64_t nbulk, int nblk_bulk) {
   using NP = NetLayout<Lp, Hp>;
   using NM = NetLayout<Lm, Hm>;
   constexpr int PP = NP::size();
@@ -46,31 +87,23 @@ __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
   }
   float lsum = 0.0f;
 
-  const int64_t local = (int64_t)blockIdx.x * kLossThreads + threadIdx.x;
-  const int64_t row = row_begin + local;
-  const int pt = (local < nrows) ? row_point[row] : -1;
-  if (pt >= 0 && pt < npts) {
-    const int k = pt % nz;
-    const int ij = pt / nz;
-    const int j = ij % ny;
-    const int i = ij / ny;
-    const float px = xs[i], py = ys[j], pz = zs[k];
-    // entry range clamped to the entry arrays (rows are well formed by construction)
-    const int beg = max(row_ptr[row], 0);
-    const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
-    float r = row_c[row];
-    for (int e = beg; e < end; ++e) {
-      const uint32_t code = ent_code[e];
-      int ox, oy, oz;
-      dips_decode_offset(code, ox, oy, oz);
-      const float x = px + (float)ox * hx;
-      const float y = py + (float)oy * hy;
-      const float z = pz + (float)oz * hz;
-      const float u = (code & 128u) ? mlp_forward<Lp, Hp>(Pp, x, y, z) : mlp_forward<Lm, Hm>(Pm, x, y, z);
-      r = fmaf(ent_w[e], u, r);
-    }
-    lsum = 0.5f * r * r;
-    if (kGrad) {
+  const int lane = threadIdx.x & (DIPS_WAVE - 1);
+  const int wave = threadIdx.x / DIPS_WAVE;
+  if ((int)blockIdx.x < nblk_bulk) {
+    // bulk rows (few entries): one lane per row
+    const int64_t local = (int64_t)blockIdx.x * kLossThreads + threadIdx.x;
+    const int64_t row = row_begin + local;
+    const int pt = (local < nbulk) ? row_point[row] : -1;
+    if (pt >= 0 && pt < npts) {
+      const int k = pt % nz;
+      const int ij = pt / nz;
+      const int j = ij % ny;
+      const int i = ij / ny;
+      const float px = xs[i], py = ys[j], pz = zs[k];
+      // entry range clamped to the entry arrays (rows are well formed by construction)
+      const int beg = max(row_ptr[row], 0);
+      const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
+      float r = row_c[row];
       for (int e = beg; e < end; ++e) {
         const uint32_t code = ent_code[e];
         int ox, oy, oz;
@@ -78,51 +111,171 @@ __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
         const float x = px + (float)ox * hx;
         const float y = py + (float)oy * hy;
         const float z = pz + (float)oz * hz;
-        const float seed = r * ent_w[e];
-        if (code & 128u)
-          mlp_backward<Lp, Hp>(Pp, x, y, z, seed, Gp);
-        else
-          mlp_backward<Lm, Hm>(Pm, x, y, z, seed, Gm);
+        const float u = (code & 128u) ? mlp_forward<Lp, Hp>(Pp, x, y, z) : mlp_forward<Lm, Hm>(Pm, x, y, z);
+        r = fmaf(ent_w[e], u, r);
+      }
+      lsum = 0.5f * r * r;
+      if (kGrad) {
+        for (int e = beg; e < end; ++e) {
+          const uint32_t code = ent_code[e];
+          int ox, oy, oz;
+          dips_decode_offset(code, ox, oy, oz);
+          const float x = px + (float)ox * hx;
+          const float y = py + (float)oy * hy;
+          const float z = pz + (float)oz * hz;
+          const float seed = r * ent_w[e];
+          if (code & 128u)
+            mlp_backward<Lp, Hp>(Pp, x, y, z, seed, Gp);
+          else
+            mlp_backward<Lm, Hm>(Pm, x, y, z, seed, Gm);
+        }
+      }
+    }
+  } else {
+    // interface rows (many entries): one wavefront per row, entries spread over the lanes
+    const int64_t wrow = ((int64_t)blockIdx.x - nblk_bulk) * (kLossThreads / DIPS_WAVE) + wave;
+    const int64_t row = row_begin + nbulk + wrow;
+    const int pt = (nbulk + wrow < nrows) ? row_point[row] : -1;  // uniform across the wave
+    if (pt >= 0 && pt < npts) {
+      const int k = pt % nz;
+      const int ij = pt / nz;
+      const int j = ij % ny;
+      const int i = ij / ny;
+      const float px = xs[i], py = ys[j], pz = zs[k];
+      const int beg = max(row_ptr[row], 0);
+      const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
+      float acc = 0.0f;
+      for (int e = beg + lane; e < end; e += DIPS_WAVE) {
+        const uint32_t code = ent_code[e];
+        int ox, oy, oz;
+        dips_decode_offset(code, ox, oy, oz);
+        const float x = px + (float)ox * hx;
+        const float y = py + (float)oy * hy;
+        const float z = pz + (float)oz * hz;
+        const float u = (code & 128u) ? mlp_forward<Lp, Hp>(Pp, x, y, z) : mlp_forward<Lm, Hm>(Pm, x, y, z);
+        acc = fmaf(ent_w[e], u, acc);
+      }
+      const float r = dips_wave_sum(acc) + row_c[row];
+      lsum = (lane == 0) ? 0.5f * r * r : 0.0f;
+      if (kGrad) {
+        for (int e = beg + lane; e < end; e += DIPS_WAVE) {
+          const uint32_t code = ent_code[e];
+          int ox, oy, oz;
+          dips_decode_offset(code, ox, oy, oz);
+          const float x = px + (float)ox * hx;
+          const float y = py + (float)oy * hy;
+          const float z = pz + (float)oz * hz;
+          const float seed = r * ent_w[e];
+          if (code & 128u)
+            mlp_backward<Lp, Hp>(Pp, x, y, z, seed, Gp);
+          else
+            mlp_backward<Lm, Hm>(Pm, x, y, z, seed, Gm);
+        }
       }
     }
   }
 
-  // block reduction: wave shuffles, then LDS across the 4 waves
+  // block reduction: DPP wave sums, then LDS across the 4 waves
   __shared__ float red[kLossThreads / DIPS_WAVE][PT + 1];
-  const int lane = threadIdx.x & (DIPS_WAVE - 1);
-  const int wave = threadIdx.x / DIPS_WAVE;
   if (kGrad) {
 #pragma unroll
     for (int p = 0; p < PP; ++p) {
-      float s = dips_wave_sum(Gp[p]);
+      const float s = dips_wave_sum(Gp[p]);
       if (lane == 0) red[wave][p] = s;
     }
 #pragma unroll
     for (int p = 0; p < PM; ++p) {
-      float s = dips_wave_sum(Gm[p]);
+      const float s = dips_wave_sum(Gm[p]);
       if (lane == 0) red[wave][PP + p] = s;
     }
   }
   {
-    float s = dips_wave_sum(lsum);
+    const float s = dips_wave_sum(lsum);
     if (lane == 0) red[wave][PT] = s;
   }
   __syncthreads();
-  const int stride = PT + 1;
-  float* out = partial + (int64_t)blockIdx.x * stride;
+  constexpr int stride = PT + 1;
+  const int nblk = (int)gridDim.x;
   for (int p = threadIdx.x; p < stride; p += kLossThreads) {
-    if (!kGrad && p < PT) {
-      out[p] = 0.0f;
-      continue;
-    }
     float s = 0.0f;
+    if (kGrad || p == PT) {
 #pragma unroll
-    for (int w = 0; w < kLossThreads / DIPS_WAVE; ++w) s += red[w][p];
-    out[p] = s;
+      for (int w = 0; w < kLossThreads / DIPS_WAVE; ++w) s += red[w][p];
+    }
+    // fused: parameter-major rows (p * nblk + block) so the last workgroup reads contiguous runs
+    if (kFused)
+      partial[(int64_t)p * nblk + blockIdx.x] = s;
+    else
+      partial[(int64_t)blockIdx.x * stride + p] = s;
+  }
+  if (!kFused) return;
+
+  // the last workgroup to arrive reduces all partial rows in block order and applies the optimizer
+  __shared__ int am_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) am_last = (atomicAdd(fo.sync, 1u) == (uint32_t)(nblk - 1)) ? 1 : 0;
+  __syncthreads();
+  if (!am_last) return;
+  __threadfence();
+  __shared__ double dred[kLossThreads / DIPS_WAVE];
+  __shared__ float gnorm_s;
+  __shared__ int count_s;
+  constexpr int kPer = (stride + kLossThreads - 1) / kLossThreads;
+  float gl[kPer];
+  double sq = 0.0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int p = threadIdx.x + q * kLossThreads;
+    gl[q] = 0.0f;
+    if (p < stride) {
+      const float* src = partial + (int64_t)p * nblk;
+      double acc = 0.0;
+      for (int b = 0; b < nblk; ++b) acc += (double)__hip_atomic_load(src + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const float g = (float)(acc * (double)fo.inv_b);
+      gl[q] = g;
+      if (p < PT) sq += (double)g * (double)g;
+    }
+  }
+  sq = dips_wave_sum_d(sq);
+  if (lane == 0) dred[wave] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kLossThreads / DIPS_WAVE; ++w) t += dred[w];
+    gnorm_s = (float)sqrt(t);
+    count_s = *fo.count;
+  }
+  __syncthreads();
+  const int c = count_s;
+  const float gn = gnorm_s;
+  const bool clip = (fo.opt.kind == 0) && !(gn < fo.opt.max_norm);
+  const float lr_t = schedule_lr(fo.opt, c);
+  const float t1 = (float)(c + 1);
+  const float bc1 = 1.0f - powf(0.9f, t1);
+  const float bc2 = 1.0f - powf(0.999f, t1);
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int p = threadIdx.x + q * kLossThreads;
+    if (p < PT) apply_update(fo.opt, c, lr_t, bc1, bc2, clip, gn, p, gl[q], params, fo.m, fo.v);
+    if (p == PT && c >= 0 && c < fo.loss_cap) fo.loss_hist[c] = gl[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *fo.count = c + 1;
+    *fo.sync = 0u;
   }
 }
 
-int64_t partial_blocks(int64_t nrows) { return (nrows + kLossThreads - 1) / kLossThreads; }
+constexpr int kRowsPerWaveBlock = kLossThreads / DIPS_WAVE;  // interface rows per workgroup
+
+// workgroups for nrows rows of which the first nbulk are lane-per-row rows
+int64_t loss_blocks(int64_t nrows, int64_t nbulk) {
+  return (nbulk + kLossThreads - 1) / kLossThreads + (nrows - nbulk + kRowsPerWaveBlock - 1) / kRowsPerWaveBlock;
+}
+
+// upper bound of loss_blocks over all splits (every row an interface row)
+int64_t partial_blocks(int64_t nrows) { return (nrows + kRowsPerWaveBlock - 1) / kRowsPerWaveBlock + 1; }
 
 int partial_stride(const dips_net_t* net) {
   auto sz = [](int L, int H) { return (3 * H + H) + (L - 1) * (H * H + H) + (H + 1); };
@@ -131,16 +284,21 @@ int partial_stride(const dips_net_t* net) {
 
 template <int Lm, int Hm, int Lp, int Hp>
 static void launch_shape(const dips_grid_t* g, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
-                         const float* params, float* partial, int want_grad, int nblk, hipStream_t st) {
+                         float* params, float* partial, int mode, const FusedOpt& fo, int nblk, int64_t nbulk,
+                         int nblk_bulk, hipStream_t st) {
   dim3 grid(nblk), block(kLossThreads);
-  if (want_grad)
-    hipLaunchKernelGGL((loss_grad_kernel<Lm, Hm, Lp, Hp, true>), grid, block, 0, st, g->xs, g->ys, g->zs, g->ny,
-                       g->nz, rec->hx, rec->hy, rec->hz, rec->row_point, rec->row_ptr, rec->row_c, row_begin, nrows,
-                       rec->ent_code, rec->ent_w, rec->nnz, g->nx * g->ny * g->nz, params, partial);
+#define DIPS_LAUNCH(G, F)                                                                                         \
+  hipLaunchKernelGGL((loss_grad_kernel<Lm, Hm, Lp, Hp, G, F>), grid, block, 0, st, g->xs, g->ys, g->zs, g->ny, \
+                     g->nz, rec->hx, rec->hy, rec->hz, rec->row_point, rec->row_ptr, rec->row_c, row_begin, nrows, \
+                     rec->ent_code, rec->ent_w, rec->nnz, g->nx * g->ny * g->nz, params, partial, fo, nbulk, \
+                     nblk_bulk)
+  if (mode == 2)
+    DIPS_LAUNCH(true, true);
+  else if (mode == 1)
+    DIPS_LAUNCH(true, false);
   else
-    hipLaunchKernelGGL((loss_grad_kernel<Lm, Hm, Lp, Hp, false>), grid, block, 0, st, g->xs, g->ys, g->zs, g->ny,
-                       g->nz, rec->hx, rec->hy, rec->hz, rec->row_point, rec->row_ptr, rec->row_c, row_begin, nrows,
-                       rec->ent_code, rec->ent_w, rec->nnz, g->nx * g->ny * g->nz, params, partial);
+    DIPS_LAUNCH(false, false);
+#undef DIPS_LAUNCH
 }
 
 int net_supported(const dips_net_t* n) {
@@ -152,8 +310,8 @@ int net_supported(const dips_net_t* n) {
 }
 
 int loss_grad_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
-                     const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
-                     int want_grad, int* nblk_out, hipStream_t stream) {
+                     int64_t nbulk, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
+                     int want_grad, int* nblk_out, hipStream_t stream, const FusedOpt* fused) {
   if (nrows <= 0) {
     set_error("dips_loss_grad: nrows must be positive");
     return DIPS_EINVAL;
@@ -162,7 +320,12 @@ int loss_grad_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_t
     set_error("dips_loss_grad: row range outside the records");
     return DIPS_EINVAL;
   }
-  const int64_t nblk = partial_blocks(nrows);
+  if (nbulk < 0 || nbulk > nrows) {
+    set_error("dips_loss_grad: nbulk outside [0, nrows]");
+    return DIPS_EINVAL;
+  }
+  const int64_t nblk_bulk = (nbulk + kLossThreads - 1) / kLossThreads;
+  const int64_t nblk = loss_blocks(nrows, nbulk);
   const int64_t need = nblk * (int64_t)partial_stride(net);
   if (partial_cap < need) {
     set_error("dips_loss_grad: partial buffer too small");
@@ -173,15 +336,28 @@ int loss_grad_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_t
     return DIPS_EINVAL;
   }
   *nblk_out = (int)nblk;
-#define X(a, b, c, d)                                                                                  \
-  if (net->Lm == a && net->Hm == b && net->Lp == c && net->Hp == d) {                                 \
-    launch_shape<a, b, c, d>(grid, rec, row_begin, nrows, params, partial, want_grad, (int)nblk, stream); \
-    return hip_status(hipGetLastError(), "loss_grad_kernel launch");                                  \
+  FusedOpt fo{};
+  if (fused) fo = *fused;
+  const int mode = fused ? 2 : (want_grad ? 1 : 0);
+#define X(a, b, c, d)                                                                                              \
+  if (net->Lm == a && net->Hm == b && net->Lp == c && net->Hp == d) {                                             \
+    launch_shape<a, b, c, d>(grid, rec, row_begin, nrows, const_cast<float*>(params), partial, mode, fo, (int)nblk, \
+                             nbulk, (int)nblk_bulk, stream);                                                      \
+    return hip_status(hipGetLastError(), "loss_grad_kernel launch");                                              \
   }
   DIPS_SHAPES(X)
 #undef X
   set_error("dips_loss_grad: network shape has no compiled kernel");
   return DIPS_EUNSUPPORTED;
+}
+
+int train_step_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
+                      int64_t nbulk, const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params,
+                      float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, float* partial,
+                      int64_t partial_cap, uint32_t* sync, hipStream_t stream) {
+  FusedOpt fo{*opt, inv_batch, m, v, count, loss_hist, loss_cap, sync};
+  int nb = 0;
+  return loss_grad_launch(grid, rec, row_begin, nrows, nbulk, net, params, partial, partial_cap, 1, &nb, stream, &fo);
 }
 
 }  // namespace dips

@@ -65,13 +65,18 @@ SIGNATURES = {
     "dips_param_count": (_I64, [ctypes.POINTER(Net)]),
     "dips_partial_size": (_I64, [ctypes.POINTER(Net), _I64]),
     "dips_net_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
-    "dips_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _I64, _I64, ctypes.POINTER(Net),
-                                      _P, _P, _I64, ctypes.c_int, ctypes.POINTER(_I32), _P]),
+    "dips_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _I64, _I64, _I64,
+                                      ctypes.POINTER(Net), _P, _P, _I64, ctypes.c_int, ctypes.POINTER(_I32), _P]),
     "dips_opt_step": (ctypes.c_int, [ctypes.POINTER(Net), ctypes.POINTER(Opt), _P, _I32, ctypes.c_float, _P, _P, _P,
                                      _P, _P, _I32, _P]),
     "dips_train_level": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(_I64),
-                                        ctypes.POINTER(_I64), _I32, _I64, _I32, ctypes.POINTER(Net),
-                                        ctypes.POINTER(Opt), _P, _P, _P, _P, _P, _I32, _P, _I64, _P]),
+                                        ctypes.POINTER(_I64), ctypes.POINTER(_I64), _I32, _I64, _I32,
+                                        ctypes.POINTER(Net),
+                                        ctypes.POINTER(Opt), _P, _P, _P, _P, _P, _I32, _P, _I64, _P, _P]),
+    "dips_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _I64, _I64, _I64,
+                                       ctypes.POINTER(Net),
+                                       ctypes.POINTER(Opt), ctypes.c_float, _P, _P, _P, _P, _P, _I32, _P, _I64, _P,
+                                       _P]),
     "dips_eval": (ctypes.c_int, [ctypes.POINTER(Net), _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
     "dips_geom_regions": (ctypes.c_int, [_P, _I64, _P, _P]),
     "dips_geom_cells": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _I64,
@@ -106,6 +111,11 @@ def check(status, what):
     if status != DIPS_OK:
         msg = lib().dips_last_error().decode()
         raise DipsError(f"{what} failed (status {status}): {msg}")
+
+
+def i64_array(values):
+    vals = [int(v) for v in values]
+    return (ctypes.c_int64 * len(vals))(*vals)
 
 
 def ptr(t):

@@ -29,7 +29,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
-from .records import DeviceRecords
+from .records import BULK_MAX_ENTRIES, DeviceRecords
 
 CORNER_SIGNS = ((-1, -1, -1), (1, -1, -1), (1, -1, 1), (-1, -1, 1), (-1, 1, -1), (1, 1, -1), (-1, 1, 1), (1, 1, 1))
 STENCIL = ((0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1))
@@ -203,7 +203,9 @@ def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
 
     begins = np.arange(nb, dtype=np.int64) * B
     rows = np.minimum(begins + B, npts) - begins
+    small = (nnz <= BULK_MAX_ENTRIES).to(torch.int64)
+    nbulk = torch.zeros(nb, dtype=torch.int64, device=device).index_add_(0, pid // B, small).cpu().numpy()
     rec = DeviceRecords.from_device(order.to(torch.int32).contiguous(), row_ptr64.to(torch.int32).contiguous(), row_c,
-                                    ent_code, ent_w, total, h, B, begins, rows)
+                                    ent_code, ent_w, total, h, B, begins, rows, nbulk)
     rec.stats = dict(npts=npts, ncs=ncs, ncc=ncc, nip=nip, nnz=total, boundary=int(bnd.sum().item()))
     return rec

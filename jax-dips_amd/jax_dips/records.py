@@ -26,6 +26,11 @@ import numpy as np
 from . import _native as N
 
 
+# rows with at most this many entries are processed one lane per row; longer
+# (interface) rows one wavefront per row
+BULK_MAX_ENTRIES = 8
+
+
 def batch_bounds(n, batch_size):
     """Contiguous batches (DatasetDict with num_gpus = 1, batch_size clamped to n)."""
     B = min(int(batch_size), int(n))
@@ -56,10 +61,13 @@ def pack_dense(W250, c, batch_size):
     if row_ptr[-1] >= 2 ** 31:
         raise ValueError("too many record entries for int32 offsets")
     code = (cc // 2).astype(np.uint32) | ((cc % 2).astype(np.uint32) << np.uint32(7))
+    nz_sorted = nnz_row[order]
+    nbulk = np.array([int((nz_sorted[b0:b0 + nr] <= BULK_MAX_ENTRIES).sum()) for b0, nr in zip(begins, rows)],
+                     dtype=np.int64)
     return dict(row_point=order.astype(np.int32), row_ptr=row_ptr.astype(np.int32),
                 row_c=np.asarray(c, dtype=np.float64)[order].astype(np.float32),
                 ent_code=code, ent_w=Wo[rr, cc].astype(np.float32),
-                batch_size=B, batch_begin=begins, batch_rows=rows)
+                batch_size=B, batch_begin=begins, batch_rows=rows, batch_nbulk=nbulk)
 
 
 class DeviceGrid:
@@ -95,10 +103,12 @@ class DeviceRecords:
         self.batch_size = int(packed["batch_size"])
         self.batch_begin = np.asarray(packed["batch_begin"], np.int64)
         self.batch_rows = np.asarray(packed["batch_rows"], np.int64)
+        self.batch_nbulk = np.asarray(packed["batch_nbulk"], np.int64)
         self._finish()
 
     @classmethod
-    def from_device(cls, row_point, row_ptr, row_c, ent_code, ent_w, nnz, h, batch_size, batch_begin, batch_rows):
+    def from_device(cls, row_point, row_ptr, row_c, ent_code, ent_w, nnz, h, batch_size, batch_begin, batch_rows,
+                    batch_nbulk):
         """Wrap records already in device memory (built by the geometry kernels)."""
         self = cls.__new__(cls)
         self.row_point, self.row_ptr, self.row_c = row_point, row_ptr, row_c
@@ -109,6 +119,7 @@ class DeviceRecords:
         self.batch_size = int(batch_size)
         self.batch_begin = np.asarray(batch_begin, np.int64)
         self.batch_rows = np.asarray(batch_rows, np.int64)
+        self.batch_nbulk = np.asarray(batch_nbulk, np.int64)
         self._finish()
         return self
 
@@ -174,14 +185,20 @@ class Engine:
         self.loss_hist = torch.zeros(max(1, int(max_steps)), dtype=torch.float32, device=device)
         self.partial_cap = int(L.dips_partial_size(ctypes.byref(self.net), int(max_rows)))
         self.partial = torch.zeros(self.partial_cap, dtype=torch.float32, device=device)
+        self.sync = torch.zeros(1, dtype=torch.int32, device=device)  # arrival counter of the fused step
+        self.fused = True
         self.opt = make_opt(optimizer_dict)
 
-    def loss_grad(self, grid, rec, row_begin, nrows, inv_batch, want_grad=True):
-        """(loss, grad) of rows [row_begin, row_begin+nrows) as float64 numpy."""
+    def loss_grad(self, grid, rec, row_begin, nrows, inv_batch, want_grad=True, nbulk=None):
+        """(loss, grad) of rows [row_begin, row_begin+nrows) as float64 numpy.
+        nbulk defaults to the batch table's split when the range is a batch."""
         import torch
 
+        if nbulk is None:
+            hit = np.nonzero((rec.batch_begin == row_begin) & (rec.batch_rows == nrows))[0]
+            nbulk = int(rec.batch_nbulk[hit[0]]) if hit.size else 0
         nblk = ctypes.c_int32(0)
-        st = N.lib().dips_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), int(row_begin), int(nrows),
+        st = N.lib().dips_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), int(row_begin), int(nrows), int(nbulk),
                                     ctypes.byref(self.net), N.ptr(self.params), N.ptr(self.partial),
                                     self.partial_cap, int(bool(want_grad)), ctypes.byref(nblk), N.current_stream())
         N.check(st, "dips_loss_grad")
@@ -193,9 +210,9 @@ class Engine:
     def train_level(self, grid, rec, nepochs):
         st = N.lib().dips_train_level(
             ctypes.byref(grid.c), ctypes.byref(rec.c),
-            (ctypes.c_int64 * len(rec.batch_begin))(*rec.batch_begin.tolist()),
-            (ctypes.c_int64 * len(rec.batch_rows))(*rec.batch_rows.tolist()),
+            N.i64_array(rec.batch_begin), N.i64_array(rec.batch_rows), N.i64_array(rec.batch_nbulk),
             len(rec.batch_begin), rec.batch_size, int(nepochs), ctypes.byref(self.net), ctypes.byref(self.opt),
             N.ptr(self.params), N.ptr(self.m), N.ptr(self.v), N.ptr(self.count), N.ptr(self.loss_hist),
-            int(self.loss_hist.numel()), N.ptr(self.partial), self.partial_cap, N.current_stream())
+            int(self.loss_hist.numel()), N.ptr(self.partial), self.partial_cap,
+            N.ptr(self.sync) if self.fused else None, N.current_stream())
         N.check(st, "dips_train_level")

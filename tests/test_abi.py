@@ -32,5 +32,6 @@ def test_host_only_queries():
     assert lib.dips_param_count(ctypes.byref(net)) == 167
     assert lib.dips_net_supported(ctypes.byref(net)) == 1
     assert lib.dips_net_supported(ctypes.byref(_native.Net(7, 5, 3, 9))) == 0
-    assert lib.dips_partial_size(ctypes.byref(net), 256) == 168
+    ps = lib.dips_partial_size(ctypes.byref(net), 256)
+    assert ps > 0 and ps % 168 == 0  # whole partial rows of P + 1 floats
     assert lib.dips_version() >= 1

@@ -106,3 +106,30 @@ def test_train_level_matches_oracle_steps():
     assert np.all(np.abs(gl - ol) <= 1e-4 * np.abs(ol))
     assert np.linalg.norm(gp - th) <= 1e-3 * np.linalg.norm(th)
     assert int(eng.count.item()) == 20
+
+
+@pytest.mark.gpu
+def test_fused_step_equals_two_kernel_step():
+    """dips_train_step (single launch, last-workgroup optimizer) gives the same
+    losses and parameters as dips_loss_grad + dips_opt_step."""
+    from jax_dips.records import Engine
+    import torch
+
+    prob = problems.sphere()
+    axes, P, h, bounds, cb, rec = oracle_level(prob, 16, 1)
+    B = 1024
+    theta0 = M.init_params(DIMS, seed=13)
+    grid, drec, packed = _device(axes, rec, B)
+    cfg = {"optimizer_name": "custom", "learning_rate": 1e-2, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
+    out = []
+    for fused in (True, False):
+        eng = Engine(DIMS, theta0, cfg, max_rows=B, max_steps=64)
+        eng.fused = fused
+        eng.train_level(grid, drec, 6)
+        torch.cuda.synchronize()
+        out.append((eng.loss_hist[:24].cpu().numpy(), eng.params.cpu().numpy(), int(eng.count.item()),
+                    int(eng.sync.item())))
+    (l1, p1, c1, s1), (l2, p2, c2, s2) = out
+    assert c1 == c2 == 24 and s1 == 0
+    assert np.array_equal(l1, l2)
+    assert np.max(np.abs(p1 - p2)) <= 1e-7 * max(1.0, np.max(np.abs(p2)))
