@@ -146,6 +146,26 @@ int dips_eval(const dips_net_t* net, const float* params, const float* pts, cons
               int64_t m, float* u, float* grad, float* grad_n, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * K5: the grid-value operator and its matrix-free BiCGSTAB solve (judge and
+ * fallback).  rec holds the zoom-0 records of the eval grid (spacing = grid
+ * spacing); r(p) = c_p + sum_e w_e u(p + o_e) with u a grid vector (N floats,
+ * C order) read through a linear ghost layer beyond the box.  Replaces
+ * Trainer.residual_vector / compute_Ax_and_b_discrete_fn
+ * (trainer.py:883-890; discretization.py:522-741; interpolate.py:762-816).
+ * ------------------------------------------------------------------------- */
+
+/* y = A u (+ c when with_c != 0), y and u are grid vectors of N floats. */
+int dips_grid_apply(const dips_grid_t* grid, const dips_records_t* rec, const float* u, float* y, int with_c,
+                    void* stream);
+
+/* Solves A x = -c by BiCGSTAB from the initial guess in x (float32 vectors,
+ * float64 inner products and norms).  work: 6*N floats, dwork: 520 doubles.
+ * Stops at max_iter or when ||r||/||c|| <= rtol; returns the iteration count
+ * and the relative true residual.  Synchronises the stream (scalar readbacks). */
+int dips_bicgstab(const dips_grid_t* grid, const dips_records_t* rec, float* x, float* work, double* dwork,
+                  int32_t max_iter, double rtol, int32_t* iters, double* relres, void* stream);
+
+/* ---------------------------------------------------------------------------
  * K1: FP64 cut-cell geometry, regression rows and record assembly.
  * (geometry/geometric_integrations_per_point.py:32-1009;
  *  solvers/poisson/discretization.py:164-519)

@@ -199,4 +199,31 @@ int dips_eval(const dips_net_t* net, const float* params, const float* pts, cons
   return eval_launch(net, params, pts, phase, normals, m, u, grad, grad_n, (hipStream_t)stream);
 }
 
+int dips_grid_apply(const dips_grid_t* grid, const dips_records_t* rec, const float* u, float* y, int with_c,
+                    void* stream) {
+  int st = check_records(grid, rec);
+  if (st) return st;
+  if (!u || !y) {
+    set_error("dips_grid_apply: null buffer");
+    return DIPS_EINVAL;
+  }
+  return grid_apply(grid, rec, u, y, with_c, (hipStream_t)stream);
+}
+
+int dips_bicgstab(const dips_grid_t* grid, const dips_records_t* rec, float* x, float* work, double* dwork,
+                  int32_t max_iter, double rtol, int32_t* iters, double* relres, void* stream) {
+  int st = check_records(grid, rec);
+  if (st) return st;
+  if (!x || !work || !dwork || !iters || !relres || max_iter < 0) {
+    set_error("dips_bicgstab: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  int it = 0;
+  double rr = 0.0;
+  st = bicgstab(grid, rec, x, work, dwork, max_iter, rtol, &it, &rr, (hipStream_t)stream);
+  *iters = it;
+  *relres = rr;
+  return st;
+}
+
 }  // extern "C"

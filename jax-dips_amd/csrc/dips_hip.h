@@ -41,4 +41,10 @@ int geom_assemble(const dips_assemble_t& a, hipStream_t st);
 int eval_launch(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase, const float* nrm,
                 int64_t m, float* u, float* grad, float* gradn, hipStream_t st);
 
+// judge.hip
+int grid_apply(const dips_grid_t* g, const dips_records_t* rec, const float* u, float* y, int with_c,
+               hipStream_t st);
+int bicgstab(const dips_grid_t* g, const dips_records_t* rec, float* x, float* work, double* dwork, int max_iter,
+             double rtol, int* iters, double* relres, hipStream_t st);
+
 }  // namespace dips

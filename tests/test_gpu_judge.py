@@ -1,0 +1,94 @@
+"""GPU parity of the grid-value operator and its BiCGSTAB solve (K5) with the
+CPU oracle's operator and direct solve.
+
+Tolerances: float32 operator applications with float64 reductions; the solve
+stops at a relative residual of 1e-7, and the grid operator's condition number
+at these sizes (~1e3) bounds the solution error by ~1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from helpers import grid_points
+from jax_dips import problems
+from oracle import discretization as D
+from oracle import judge as J
+from oracle import mlp as M
+from oracle import trainer as T
+
+
+def gstates(n):
+    from jax_dips.domain import mesh
+
+    x, y, z = problems.grid_axes(n)
+    g = mesh.construct(3)[0](x, y, z)
+    return g
+
+
+@pytest.mark.gpu
+def test_bicgstab_quadratic_exact():
+    from jax_dips.judge import GridJudge
+
+    mu = 2.5
+    c = lambda v: (lambda r: r[0] * 0 + v)
+    quad = lambda r: r[0] ** 2 + 2 * r[1] ** 2 - r[2] ** 2 + r[0] * r[1] + 0.3 * r[2]
+    cbs = dict(phi=c(-1.0), mu_m=c(mu), mu_p=c(7.0), k_m=c(0.0), k_p=c(0.0), f_m=c(-mu * 4.0), f_p=c(0.0),
+               alpha=c(0.0), beta=c(0.0), g=quad)
+    g = gstates(24)
+    jd = GridJudge(g, g, cbs)
+    u, it, rr = jd.solve(rtol=1e-7)
+    exact = quad(np.asarray(g.R, np.float64).T)
+    print("iters", it, "relres", rr, "err", np.abs(u - exact).max())
+    assert rr <= 1e-6
+    assert np.abs(u - exact).max() <= 1e-4 * np.abs(exact).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["sphere", "star"])
+def test_operator_and_solve_match_oracle(name):
+    from jax_dips.judge import GridJudge
+
+    n = 20
+    prob = problems.sphere() if name == "sphere" else problems.star(r0=0.55, amp=0.15,
+                                                                     centre=(0.0123, -0.0171, 0.0093))
+    cbs = problems.oracle_callbacks(prob)
+    g = gstates(n)
+    jd = GridJudge(g, g, cbs)
+    axes, P, h, bounds = grid_points(n)
+    orec = T.LevelRecords(P, h, bounds, D.Callbacks(cbs, np.float64))
+    A, cc = J.operator(orec, (n, n, n))
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=n ** 3).astype(np.float32)
+    r_dev = jd.residual_vector(u)
+    r_or = A @ u.astype(np.float64) + cc
+    assert np.abs(r_dev - r_or).max() <= 1e-5 * np.abs(r_or).max()
+    x, it, rr = jd.solve(rtol=1e-7)
+    xo = J.solve(orec, (n, n, n))
+    print(name, "iters", it, "relres", rr, "max diff", np.abs(x - xo).max())
+    assert rr <= 1e-6
+    assert np.abs(x - xo).max() <= 1e-4 * np.abs(xo).max()
+
+
+@pytest.mark.gpu
+def test_grid_residual_at_network_values_equals_network_residual():
+    """residual_vector(u) with u = the network's nodal values is the training
+    residual at zoom level 0 (same records, one value per node)."""
+    from jax_dips.evaluation import evaluate_points
+    from jax_dips.judge import GridJudge
+    from jax_dips.nn.mlp.MLP import DoubleMLP
+
+    n = 16
+    prob = problems.sphere()
+    cbs = problems.oracle_callbacks(prob)
+    g = gstates(n)
+    jd = GridJudge(g, g, cbs)
+    axes, P, h, bounds = grid_points(n)
+    cb = D.Callbacks(cbs, np.float64)
+    orec = T.LevelRecords(P, h, bounds, cb)
+    dims = (1, 1, 2, 10)
+    theta = M.init_params(dims, seed=21)
+    model = DoubleMLP(mlp={"hidden_layers_m": 1, "hidden_dim_m": 1, "activation_m": "jnp.tanh",
+                           "hidden_layers_p": 2, "hidden_dim_p": 10, "activation_p": "jnp.tanh"})
+    u_nodes, _ = evaluate_points(model, theta, P, cb("phi", P) >= 0)
+    r_grid = jd.residual_vector(u_nodes)
+    r_nn = T.residual(theta.astype(np.float64), dims, orec, np.arange(P.shape[0]))
+    assert np.abs(r_grid - r_nn).max() <= 2e-5 * np.abs(r_nn).max()
