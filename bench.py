@@ -42,6 +42,10 @@ def main():
     ap.add_argument("--roofline-launches", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "star"],
+                    help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144")
+    ap.add_argument("--path", default="auto", choices=["auto", "lane", "mfma"],
+                    help="training-step kernels: lane-per-row fused step or MFMA tiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -58,17 +62,25 @@ def main():
 
     dev = "cuda"
     n, B = args.n, args.batch
-    prob = problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0)
+    if args.workload == "star":
+        prob = problems.star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0)
+        if args.batch == 65536:
+            B = 262144
+        mlp = {"hidden_layers_m": 3, "hidden_dim_m": 64, "activation_m": "jnp.tanh",
+               "hidden_layers_p": 3, "hidden_dim_p": 64, "activation_p": "jnp.tanh"}
+    else:
+        prob = problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0)
+        mlp = {"hidden_layers_m": 1, "hidden_dim_m": 1, "activation_m": "jnp.tanh",
+               "hidden_layers_p": 2, "hidden_dim_p": 10, "activation_p": "jnp.tanh"}
     cbs = problems.oracle_callbacks(prob)
     x, y, z = problems.grid_axes(n)
     grid = DeviceGrid(x, y, z)
     h0 = np.array([x[1] - x[0], y[1] - y[0], z[1] - z[0]], np.float32)
     bounds = (x.min(), x.max(), y.min(), y.max(), z.min(), z.max())
-    dims = (1, 1, 2, 10)
     from jax_dips.nn.mlp.MLP import DoubleMLP
 
-    model = DoubleMLP(mlp={"hidden_layers_m": 1, "hidden_dim_m": 1, "activation_m": "jnp.tanh",
-                           "hidden_layers_p": 2, "hidden_dim_p": 10, "activation_p": "jnp.tanh"})
+    model = DoubleMLP(mlp=mlp)
+    dims = model.dims
     theta = model.flatten(model.init(42))
     opt = {"optimizer_name": "custom", "learning_rate": 1e-3, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
 
@@ -88,12 +100,27 @@ def main():
     K = max(4, args.steps - args.steps % 4)
     eng = Engine(dims, theta, opt, max_rows=B, max_steps=args.warmup + K + args.roofline_launches + 8)
     L = N.lib()
+    use_mfma = args.path == "mfma" or (args.path == "auto" and not eng.lane_path)
+    if use_mfma:
+        for rec in levels:
+            eng.mfma_workspace(rec)
 
     def run_steps(k, stream):
         # k steps split over the 4 zoom levels, batches in order (cyclic)
         per = k // 4
         for li, rec in enumerate(levels):
             idx = np.arange(per) % nb
+            if use_mfma:
+                mb = rec.mfma_batches()["batches"]
+                for b in idx:
+                    N.check(L.dips_mfma_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb[b]),
+                                                   ctypes.byref(eng.net), ctypes.byref(eng.opt),
+                                                   ctypes.c_float(1.0 / B), N.ptr(eng.params), N.ptr(eng.m),
+                                                   N.ptr(eng.v), N.ptr(eng.count), N.ptr(eng.loss_hist),
+                                                   int(eng.loss_hist.numel()), N.ptr(eng._mwork),
+                                                   int(eng._mwork.numel()), N.ptr(eng._mdwork),
+                                                   int(eng._mdwork.numel()), stream), "dips_mfma_train_step")
+                continue
             st = L.dips_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), N.i64_array(rec.batch_begin[idx]),
                                     N.i64_array(rec.batch_rows[idx]), N.i64_array(rec.batch_nbulk[idx]), per,
                                     rec.batch_size, 1,
@@ -133,13 +160,22 @@ def main():
         li = (4 * i) // R
         rec = levels[li]
         b = i % nb
+        flops_sum += per_launch_flops[li * nb + b]
+        if use_mfma:
+            mb = rec.mfma_batches()["batches"]
+            N.check(L.dips_mfma_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb[b]),
+                                           ctypes.byref(eng.net), ctypes.byref(eng.opt), ctypes.c_float(1.0 / B),
+                                           N.ptr(eng.params), N.ptr(eng.m), N.ptr(eng.v), N.ptr(eng.count),
+                                           N.ptr(eng.loss_hist), int(eng.loss_hist.numel()), N.ptr(eng._mwork),
+                                           int(eng._mwork.numel()), N.ptr(eng._mdwork), int(eng._mdwork.numel()),
+                                           stream), "dips_mfma_train_step")
+            continue
         N.check(L.dips_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), int(rec.batch_begin[b]),
                                   int(rec.batch_rows[b]), int(rec.batch_nbulk[b]), ctypes.byref(eng.net),
                                   ctypes.byref(eng.opt),
                                   ctypes.c_float(1.0 / B), N.ptr(eng.params), N.ptr(eng.m), N.ptr(eng.v),
                                   N.ptr(eng.count), N.ptr(eng.loss_hist), int(eng.loss_hist.numel()),
                                   N.ptr(eng.partial), eng.partial_cap, N.ptr(eng.sync), stream), "dips_train_step")
-        flops_sum += per_launch_flops[li * nb + b]
     r1.record()
     torch.cuda.synchronize()
     avg_s = r0.elapsed_time(r1) / 1e3 / R
@@ -158,24 +194,27 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"sphere {n}^3, MLP 1x1/2x10 tanh, batch {B}, 4 zoom levels (2000-epoch schedule)",
+        "config": {"workload": f"{args.workload} {n}^3, MLP {dims[0]}x{dims[1]}/{dims[2]}x{dims[3]} tanh, batch {B}, "
+                               f"4 zoom levels (2000-epoch schedule), {'mfma' if use_mfma else 'lane'} kernels",
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "loss_grad_kernel<1,1,2,10,grad,fused> (one launch per optimizer step)", "avg_launch_us": avg_s * 1e6,
+                     "kernel": ("mfma step: fwd + rowsum + bwd + reduce + apply (5 launches)" if use_mfma else
+                                "loss_grad_kernel<grad,fused> (one launch per optimizer step)"),
+                     "avg_launch_us": avg_s * 1e6,
                      "flop_per_launch": flops_sum / R},
         "records_build_s": t_records,
         "loss_tail": loss_tail,
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, args.cpu_seconds, n)
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, seconds):
+def cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, seconds, n=128):
     """The reference's per-step evaluation pattern in numpy on one host core,
-    on a bounded sample (consecutive batches of 2048 points of the same 128^3
-    grid, level-0 spacing) until ~``seconds`` of CPU work."""
+    on a bounded sample (consecutive batches of 2048 points of the same grid,
+    level-0 spacing) until ~``seconds`` of CPU work."""
     from jax_dips import problems
     from oracle import discretization as D
     from oracle import refpattern
@@ -192,7 +231,7 @@ def cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, seconds):
         done += nb
     dt = time.time() - t0
     return {"value": done / dt, "unit": "points/s", "cores": 1, "kind": "port",
-            "sample": f"{done} points in batches of {nb} from the middle of the 128^3 grid, level-0 spacing, "
+            "sample": f"{done} points in batches of {nb} from the middle of the {n}^3 grid, level-0 spacing, "
                       f"{dt:.1f} s, numpy float32 restatement of the reference's per-step pattern"}
 
 

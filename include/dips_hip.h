@@ -146,6 +146,41 @@ int dips_eval(const dips_net_t* net, const float* params, const float* pts, cons
               int64_t m, float* u, float* grad, float* grad_n, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * MFMA path for wide networks (hidden width <= 64; the 3x64 workloads).
+ * The entries of a batch are additionally given in phase-sorted order
+ * (t_* arrays, + entries first) so that 16-entry tiles share one network.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int64_t row_begin, nrows;   /* rows of the batch */
+  int64_t ent_begin, ent_end; /* CSR entry range of those rows */
+  const int32_t* t_row;       /* phase-sorted entries of the level: global row */
+  const uint32_t* t_code;     /* ... entry code (as ent_code) */
+  const float* t_w;           /* ... weight */
+  const int32_t* t_csr;       /* ... index of the entry in the CSR arrays */
+  int64_t t_begin;            /* first phase-sorted entry of this batch */
+  int64_t n_plus, n_minus;    /* + entries [t_begin, t_begin+n_plus), then - entries */
+  int64_t t_count;            /* n_plus + n_minus */
+} dips_mfma_batch_t;
+
+/* 1 if (Lm, ceil(Hm/16), Lp, ceil(Hp/16)) has an MFMA kernel. */
+int dips_mfma_supported(const dips_net_t* net);
+/* float workspace of dips_mfma_train_step for batches of up to max_rows rows
+ * and max_nnz entries (the float64 workspace needs rows/256 + P/256 + 2 doubles). */
+int64_t dips_mfma_workspace(const dips_net_t* net, int64_t max_rows, int64_t max_nnz);
+/* One optimizer step of the batch through the MFMA kernels (forward tiles,
+ * row sums, backward tiles with MFMA weight gradients, fixed-order reduction,
+ * optimizer).  Same loss, gradient and update as dips_loss_grad + dips_opt_step. */
+int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                         const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
+                         float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* work,
+                         int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream);
+/* Loss (device double *loss) and mean gradient (device grad[P]) of the batch
+ * through the MFMA kernels, without an optimizer update. */
+int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                        const dips_net_t* net, float inv_batch, const float* params, float* grad, double* loss,
+                        float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream);
+
+/* ---------------------------------------------------------------------------
  * K5: the grid-value operator and its matrix-free BiCGSTAB solve (judge and
  * fallback).  rec holds the zoom-0 records of the eval grid (spacing = grid
  * spacing); r(p) = c_p + sum_e w_e u(p + o_e) with u a grid vector (N floats,

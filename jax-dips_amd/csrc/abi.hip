@@ -226,4 +226,57 @@ int dips_bicgstab(const dips_grid_t* grid, const dips_records_t* rec, float* x, 
   return st;
 }
 
+int dips_mfma_supported(const dips_net_t* net) {
+  if (check_net(net)) return 0;
+  return mfma_supported(net);
+}
+
+int64_t dips_mfma_workspace(const dips_net_t* net, int64_t max_rows, int64_t max_nnz) {
+  if (check_net(net) || max_rows < 1 || max_nnz < 0) return -1;
+  return mfma_workspace_floats(net, max_rows, max_nnz);
+}
+
+int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                         const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
+                         float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* work,
+                         int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!batch || !opt || !params || !m || !v || !count || !loss_hist || !work || !dwork || batch->nrows < 1 ||
+      batch->row_begin < 0 || batch->row_begin + batch->nrows > rec->nrows || batch->ent_begin < 0 ||
+      batch->ent_end > rec->nnz || batch->ent_begin > batch->ent_end || batch->t_begin < 0 ||
+      batch->t_begin + batch->t_count > rec->nnz || (batch->t_count > 0 && (!batch->t_row || !batch->t_code ||
+      !batch->t_w || !batch->t_csr))) {
+    set_error("dips_mfma_train_step: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  if (!mfma_supported(net)) {
+    set_error("dips_mfma_train_step: network shape has no MFMA kernel");
+    return DIPS_EUNSUPPORTED;
+  }
+  return mfma_step(grid, rec, batch, net, opt, inv_batch, params, m, v, count, loss_hist, loss_cap, work, work_floats,
+                   dwork, dwork_doubles, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                        const dips_net_t* net, float inv_batch, const float* params, float* grad, double* loss,
+                        float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!batch || !params || !grad || !loss || !work || !dwork || batch->nrows < 1 || batch->row_begin < 0 ||
+      batch->row_begin + batch->nrows > rec->nrows || batch->ent_begin < 0 || batch->ent_end > rec->nnz ||
+      batch->ent_begin > batch->ent_end || batch->t_begin < 0 || batch->t_begin + batch->t_count > rec->nnz) {
+    set_error("dips_mfma_loss_grad: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  if (!mfma_supported(net)) {
+    set_error("dips_mfma_loss_grad: network shape has no MFMA kernel");
+    return DIPS_EUNSUPPORTED;
+  }
+  return mfma_step(grid, rec, batch, net, nullptr, inv_batch, const_cast<float*>(params), nullptr, nullptr, nullptr,
+                   nullptr, 0, work, work_floats, dwork, dwork_doubles, grad, loss, (hipStream_t)stream);
+}
+
 }  // extern "C"

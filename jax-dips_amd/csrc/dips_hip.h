@@ -47,4 +47,12 @@ int grid_apply(const dips_grid_t* g, const dips_records_t* rec, const float* u, 
 int bicgstab(const dips_grid_t* g, const dips_records_t* rec, float* x, float* work, double* dwork, int max_iter,
              double rtol, int* iters, double* relres, hipStream_t st);
 
+// mlp_mfma.hip
+int mfma_supported(const dips_net_t* n);
+int64_t mfma_workspace_floats(const dips_net_t* n, int64_t max_rows, int64_t max_nnz);
+int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, const dips_net_t* net,
+              const dips_opt_t* opt, float inv_b, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+              int loss_cap, float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, float* grad_out,
+              double* loss_out, hipStream_t st);
+
 }  // namespace dips

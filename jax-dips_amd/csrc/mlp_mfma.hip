@@ -1,0 +1,694 @@
+// MFMA path of the training step for wide networks (hidden width up to 64).
+//
+// Work unit: a tile of 16 record entries of one phase (entries are sorted by
+// phase inside each batch), evaluated by one wavefront with
+// v_mfma_f32_16x16x4f32.  Activations are kept feature-major: for a layer of
+// width H = 16*NB, lane l holds feature 16*fb + 4*(l/16) + r of tile position
+// l%16 in register (fb, r) -- exactly the accumulator layout of the MFMA.  With
+// the contraction index of the next layer ordered as k = 16*fb + 4*(l/16) + r
+// for step (fb, r), that accumulator is directly the B operand of the next
+// layer, so layers chain without data movement.  Weight operands are packed
+// once per workgroup into LDS in that order.  Weight gradients contract over
+// positions, so the two operands (activation and delta of a tile) pass through
+// a padded LDS transpose and the 16x16 dW blocks accumulate in AGPRs across
+// all tiles of the wavefront.
+//
+// Per optimizer step: fwd (u per entry) -> rowsum (r per row, loss partials)
+// -> bwd (parameter-gradient partials per workgroup) -> reduce -> apply.
+//
+// Reference: the same loss and gradient as loss.hip (trainer.py:783-789,
+// :892-912; DoubleMLP, nn/mlp/MLP.py:93-139).
+#include <math.h>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "dips_common.h"
+#include "dips_hip.h"
+#include "dips_mlp.h"
+
+namespace dips {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMT = 256;  // threads per workgroup (4 wavefronts)
+constexpr int kTP = 17;   // padded row length of a transposed 16-position tile
+constexpr int kMfmaBwdBlocks = 512;  // workgroups of the backward kernel (both phases)
+
+__device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// ---- packed weights of one phase net in LDS ---------------------------------
+// [L0: NB*64] [for hidden layer 1..L-1: fwd NB*NB*64*4, bwd NB*NB*64*4, bias NB*16] [wout NB*16] [bout 1]
+template <int L, int NB>
+struct Packed {
+  static constexpr int H = 16 * NB;
+  static constexpr int l0() { return 0; }
+  static constexpr int fwd(int l) { return NB * 64 + (l - 1) * (2 * NB * NB * 256 + H); }
+  static constexpr int bwd(int l) { return fwd(l) + NB * NB * 256; }
+  static constexpr int bias(int l) { return bwd(l) + NB * NB * 256; }
+  static constexpr int wout() { return NB * 64 + (L - 1) * (2 * NB * NB * 256 + H); }
+  static constexpr int bout() { return wout() + H; }
+  static constexpr int size() { return bout() + 1; }
+};
+
+// flat-parameter offsets of one phase net of true width Hh
+struct FlatNet {
+  int Hh, L, base;
+  __device__ int w(int l) const { return base + (l == 0 ? 0 : 4 * Hh + (l - 1) * (Hh * Hh + Hh)); }
+  __device__ int b(int l) const { return w(l) + (l == 0 ? 3 * Hh : Hh * Hh); }
+  __device__ int wout() const { return base + 4 * Hh + (L - 1) * (Hh * Hh + Hh); }
+  __device__ int bout() const { return wout() + Hh; }
+};
+
+template <int L, int NB>
+__device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
+  using PK = Packed<L, NB>;
+  const int Hh = fn.Hh;
+  for (int idx = threadIdx.x; idx < PK::size(); idx += kMT) {
+    float v = 0.0f;
+    if (idx < NB * 64) {
+      const int ob = idx / 64, lane = idx % 64;
+      const int out = 16 * ob + lane % 16, k = lane / 16;
+      if (out < Hh) v = (k < 3) ? params[fn.w(0) + k * Hh + out] : params[fn.b(0) + out];
+    } else if (idx >= PK::wout()) {
+      const int h = idx - PK::wout();
+      if (h < PK::H) {
+        if (h < Hh) v = params[fn.wout() + h];
+      } else {
+        v = params[fn.bout()];
+      }
+    } else {
+      const int rel = idx - NB * 64;
+      const int per = 2 * NB * NB * 256 + PK::H;
+      const int l = rel / per + 1;
+      const int q = rel % per;
+      if (q < 2 * NB * NB * 256) {
+        const bool bw = q >= NB * NB * 256;
+        const int t = bw ? q - NB * NB * 256 : q;
+        const int r = t % 4, lane = (t / 4) % 64, fb = (t / 256) % NB, xb = t / (256 * NB);
+        // fwd: W[16fb + 4(lane/16) + r][16xb + lane%16] ; bwd: W[16xb + lane%16][16fb + 4(lane/16) + r]
+        const int a = 16 * fb + 4 * (lane / 16) + r, b = 16 * xb + lane % 16;
+        const int in = bw ? b : a, out = bw ? a : b;
+        if (in < Hh && out < Hh) v = params[fn.w(l) + in * Hh + out];
+      } else {
+        const int h = q - 2 * NB * NB * 256;
+        if (h < Hh) v = params[fn.b(l) + h];
+      }
+    }
+    lds[idx] = v;
+  }
+}
+
+// forward of one tile: act[l][fb*4+r] feature-major; returns nothing (u by caller)
+template <int L, int NB>
+__device__ __forceinline__ void tile_forward(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
+  using PK = Packed<L, NB>;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mfma4(W[PK::l0() + ob * 64 + lane], xin, c);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh(c[r]);
+  }
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    f4 c[NB];
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) c[ob][r] = W[PK::bias(l) + 16 * ob + 4 * (lane / 16) + r];
+    }
+#pragma unroll
+    for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) {
+        const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::fwd(l) + ((ob * NB + fb) * 64 + lane) * 4]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) c[ob] = mfma4(w4[r], act[l - 1][fb * 4 + r], c[ob]);
+      }
+    }
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) act[l][ob * 4 + r] = dips_tanh(c[ob][r]);
+    }
+  }
+}
+
+// output u of the tile position l%16 (identical in the 4 lanes of that position)
+template <int L, int NB>
+__device__ __forceinline__ float tile_output(const float* __restrict__ W, const float (&act)[L][4 * NB]) {
+  using PK = Packed<L, NB>;
+  const int lane = threadIdx.x & 63;
+  float s = 0.0f;
+#pragma unroll
+  for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s = fmaf(act[L - 1][fb * 4 + r], W[PK::wout() + 16 * fb + 4 * (lane / 16) + r], s);
+  }
+  s += __shfl_xor(s, 16, 64);
+  s += __shfl_xor(s, 32, 64);
+  return s + W[PK::bout()];
+}
+
+// ---- tile entry data ---------------------------------------------------------
+struct TileArgs {
+  const float* xs;
+  const float* ys;
+  const float* zs;
+  int ny, nz, npts;
+  float hx, hy, hz;
+  const int32_t* row_point;  // global row -> grid point
+  const int32_t* t_row;      // phase-sorted entries: global row
+  const uint32_t* t_code;    // phase-sorted entries: code (bit 7 = phase)
+  const float* t_w;          // phase-sorted entries: weight
+  const int32_t* t_csr;      // phase-sorted entries: CSR entry index
+};
+
+// position of phase-sorted entry e; valid = false for padding lanes
+__device__ __forceinline__ bool entry_pos(const TileArgs& T, int64_t e, int64_t e_end, float& x, float& y, float& z,
+                                          int& row) {
+  row = -1;
+  x = y = z = 0.0f;
+  if (e >= e_end) return false;
+  row = T.t_row[e];
+  const int pt = T.row_point[row];
+  if (pt < 0 || pt >= T.npts) return false;
+  int ox, oy, oz;
+  dips_decode_offset(T.t_code[e], ox, oy, oz);
+  const int k = pt % T.nz, ij = pt / T.nz, j = ij % T.ny, i = ij / T.ny;
+  x = T.xs[i] + (float)ox * T.hx;
+  y = T.ys[j] + (float)oy * T.hy;
+  z = T.zs[k] + (float)oz * T.hz;
+  return true;
+}
+
+// ---- kernel 1: u for every entry of the batch ----------------------------------
+// tiles [0, ntp) hold + entries [e0, e0+np), tiles [ntp, ntp+ntm) hold - entries
+template <int Lm, int NBm, int Lp, int NBp>
+__global__ __launch_bounds__(kMT) void mfma_fwd_kernel(TileArgs T, const float* __restrict__ params, int Hm, int Hp,
+                                                       int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_out) {
+  extern __shared__ float lds[];
+  const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  // a workgroup works on one phase: the first ceil(ntp/4) workgroups take + tiles
+  const int64_t gp = (ntp + 3) / 4;
+  const bool plus = (int64_t)blockIdx.x < gp;
+  const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
+  if (plus)
+    pack_weights<Lp, NBp>(params, FlatNet{Hp, Lp, 0}, lds);
+  else
+    pack_weights<Lm, NBm>(params, FlatNet{Hm, Lm, PPp}, lds);
+  __syncthreads();
+  const int64_t tile = plus ? (int64_t)blockIdx.x * 4 + wave : ((int64_t)blockIdx.x - gp) * 4 + wave;
+  if (plus ? tile >= ntp : tile >= ntm) return;
+  const int64_t ebeg = plus ? e0 + tile * 16 : e0 + np_ + tile * 16;
+  const int64_t eend = plus ? e0 + np_ : e0 + np_ + nm_;
+  float x, y, z;
+  int row;
+  const bool valid = entry_pos(T, ebeg + (lane & 15), eend, x, y, z, row);
+  const int k = lane / 16;
+  const float xin = (k == 0) ? x : (k == 1) ? y : (k == 2) ? z : 1.0f;
+  float u;
+  if (plus) {
+    float act[Lp][4 * NBp];
+    tile_forward<Lp, NBp>(lds, xin, act);
+    u = tile_output<Lp, NBp>(lds, act);
+  } else {
+    float act[Lm][4 * NBm];
+    tile_forward<Lm, NBm>(lds, xin, act);
+    u = tile_output<Lm, NBm>(lds, act);
+  }
+  if (valid && lane < 16) u_out[T.t_csr[ebeg + lane]] = u;
+}
+
+// ---- kernel 2: r per row, loss partials ---------------------------------------
+__global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restrict__ row_ptr,
+                                                          const float* __restrict__ row_c, int64_t row_begin,
+                                                          int64_t nrows, const float* __restrict__ ent_w, int64_t nnz,
+                                                          const float* __restrict__ u, float* __restrict__ r_out,
+                                                          double* __restrict__ loss_part) {
+  const int64_t i = (int64_t)blockIdx.x * kMT + threadIdx.x;
+  double l = 0.0;
+  if (i < nrows) {
+    const int64_t row = row_begin + i;
+    const int beg = max(row_ptr[row], 0);
+    const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
+    float r = row_c[row];
+    for (int e = beg; e < end; ++e) r = fmaf(ent_w[e], u[e], r);
+    r_out[i] = r;
+    l = 0.5 * (double)r * (double)r;
+  }
+  l = dips_wave_sum_d(l);
+  __shared__ double red[kMT / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ---- kernel 3: parameter-gradient partials ---------------------------------------
+template <int L, int NB>
+struct GradAcc {
+  f4 dW[L > 1 ? L - 1 : 1][NB * NB];  // hidden layers 1..L-1, block (ib, ob)
+  f4 dW0[NB];                          // rows x,y,z,1 (padded to 16) x out block
+  float db[L][4 * NB];                 // per-lane partial bias sums (feature-major)
+  float dwo[4 * NB];                   // per-lane partial output-weight sums
+  float dbo;
+};
+
+// transposes a feature-major tile matrix (NB*16 features x 16 positions) through LDS
+template <int NB>
+__device__ __forceinline__ void to_lds(float* __restrict__ buf, const float (&m)[4 * NB]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[(16 * fb + 4 * (lane / 16) + r) * kTP + (lane & 15)] = m[fb * 4 + r];
+  }
+}
+
+template <int L, int NB>
+__device__ __forceinline__ void tile_backward(const float* __restrict__ W, float seed, float xin_t[4],
+                                              const float (&act)[L][4 * NB], float* __restrict__ tbuf,
+                                              GradAcc<L, NB>& G) {
+  using PK = Packed<L, NB>;
+  const int lane = threadIdx.x & 63;
+  float* bufA = tbuf;                 // activation operand (features x 16 positions)
+  float* bufD = tbuf + 16 * NB * kTP;  // delta operand
+  // output layer
+  float d[4 * NB];
+#pragma unroll
+  for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float a = act[L - 1][fb * 4 + r];
+      const float wo = W[PK::wout() + 16 * fb + 4 * (lane / 16) + r];
+      G.dwo[fb * 4 + r] = fmaf(a, seed, G.dwo[fb * 4 + r]);
+      d[fb * 4 + r] = seed * wo * (1.0f - a * a);
+    }
+  }
+  if (lane < 16) G.dbo += seed;
+#pragma unroll
+  for (int l = L - 1; l >= 0; --l) {
+    if (l > 0) {  // b_0 is row 3 of the layer-0 weight gradient
+#pragma unroll
+      for (int q = 0; q < 4 * NB; ++q) G.db[l][q] += d[q];
+    }
+    // dW_l += A_{l-1} D_l^T over the 16 positions (K = positions, 4 steps)
+    to_lds<NB>(bufD, d);
+    if (l > 0) to_lds<NB>(bufA, act[l - 1]);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int pos = 4 * s + lane / 16;
+      float dB[NB];
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) dB[ob] = bufD[(16 * ob + (lane & 15)) * kTP + pos];
+      if (l > 0) {
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+          const float aA = bufA[(16 * ib + (lane & 15)) * kTP + pos];
+#pragma unroll
+          for (int ob = 0; ob < NB; ++ob) G.dW[l - 1][ib * NB + ob] = mfma4(aA, dB[ob], G.dW[l - 1][ib * NB + ob]);
+        }
+      } else {
+        const float aA = ((lane & 15) < 4) ? xin_t[s] : 0.0f;
+#pragma unroll
+        for (int ob = 0; ob < NB; ++ob) G.dW0[ob] = mfma4(aA, dB[ob], G.dW0[ob]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (l > 0) {
+      // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2)
+      f4 c[NB];
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) c[ib] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+          const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::bwd(l) + ((ib * NB + fb) * 64 + lane) * 4]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) c[ib] = mfma4(w4[r], d[fb * 4 + r], c[ib]);
+        }
+      }
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float a = act[l - 1][ib * 4 + r];
+          d[ib * 4 + r] = c[ib][r] * (1.0f - a * a);
+        }
+      }
+    }
+  }
+}
+
+// writes the workgroup's gradient of one phase net into out[0 .. flat size)
+template <int L, int NB>
+__device__ void flush_grad(GradAcc<L, NB>& G, FlatNet fn, float* __restrict__ red, float* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+  const int Hh = fn.Hh;
+  const int P = 4 * Hh + (L - 1) * (Hh * Hh + Hh) + Hh + 1;  // parameters of this net
+  const int base = fn.base;
+  // zero the LDS accumulator
+  for (int i = threadIdx.x; i < P; i += kMT) red[i] = 0.0f;
+  __syncthreads();
+  // each wave adds its part in wave order (fixed order -> deterministic)
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+      // hidden dW blocks: lane holds dW[16ib + 4(lane/16) + r][16ob + lane%16]
+#pragma unroll
+      for (int l = 1; l < L; ++l) {
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+          for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int in = 16 * ib + 4 * (lane / 16) + r, o = 16 * ob + (lane & 15);
+              if (in < Hh && o < Hh) red[fn.w(l) - base + in * Hh + o] += G.dW[l - 1][ib * NB + ob][r];
+            }
+          }
+        }
+      }
+      // layer 0: rows 0..2 -> W0, row 3 -> b0
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 4 * (lane / 16) + r, o = 16 * ob + (lane & 15);
+          if (row < 3 && o < Hh) red[fn.w(0) - base + row * Hh + o] += G.dW0[ob][r];
+          if (row == 3 && o < Hh) red[fn.b(0) - base + o] += G.dW0[ob][r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // per-lane partial sums over positions: reduce the 16 lanes of each row group with DPP-free LDS adds
+  for (int w = 0; w < 4; ++w) {
+    for (int g = 0; g < 16; ++g) {
+      if (wave == w && (lane & 15) == g) {
+#pragma unroll
+        for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int h = 16 * fb + 4 * (lane / 16) + r;
+            if (h < Hh) {
+              red[fn.wout() - base + h] += G.dwo[fb * 4 + r];
+#pragma unroll
+              for (int l = 1; l < L; ++l) red[fn.b(l) - base + h] += G.db[l][fb * 4 + r];
+            }
+          }
+        }
+        if (lane < 16) red[fn.bout() - base] += G.dbo;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < P; i += kMT) out[i] = red[i];
+}
+
+template <int Lm, int NBm, int Lp, int NBp>
+__global__ __launch_bounds__(kMT) void mfma_bwd_kernel(TileArgs T, const float* __restrict__ params, int Hm, int Hp,
+                                                       int64_t e0, int64_t np_, int64_t nm_, int64_t row_begin,
+                                                       const float* __restrict__ r_row, float* __restrict__ gpart,
+                                                       int nblk_p, int stride) {
+  extern __shared__ float lds[];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const bool plus = (int)blockIdx.x < nblk_p;
+  const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
+  const int PPm = 4 * Hm + (Lm - 1) * (Hm * Hm + Hm) + Hm + 1;
+  constexpr int PKsz = (Packed<Lp, NBp>::size() > Packed<Lm, NBm>::size()) ? Packed<Lp, NBp>::size()
+                                                                            : Packed<Lm, NBm>::size();
+  constexpr int NBx = NBp > NBm ? NBp : NBm;
+  float* tbuf = lds + PKsz + wave * (2 * 16 * NBx * kTP);
+  const int64_t ntiles = plus ? (np_ + 15) / 16 : (nm_ + 15) / 16;
+  const int nblk = plus ? nblk_p : (int)gridDim.x - nblk_p;
+  const int bid = plus ? (int)blockIdx.x : (int)blockIdx.x - nblk_p;
+  const int64_t eb = plus ? e0 : e0 + np_;
+  const int64_t ee = plus ? e0 + np_ : e0 + np_ + nm_;
+  float* out = gpart + (int64_t)blockIdx.x * stride;
+  auto run = [&](auto tagL, auto tagNB, FlatNet fn) {
+    constexpr int L = decltype(tagL)::value;
+    constexpr int NB = decltype(tagNB)::value;
+    pack_weights<L, NB>(params, fn, lds);
+    __syncthreads();
+    GradAcc<L, NB> G;
+#pragma unroll
+    for (int l = 0; l < (L > 1 ? L - 1 : 1); ++l)
+#pragma unroll
+      for (int q = 0; q < NB * NB; ++q) G.dW[l][q] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < NB; ++q) G.dW0[q] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+      for (int q = 0; q < 4 * NB; ++q) G.db[l][q] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4 * NB; ++q) G.dwo[q] = 0.0f;
+    G.dbo = 0.0f;
+    for (int64_t tile = (int64_t)bid * 4 + wave; tile < ntiles; tile += (int64_t)nblk * 4) {
+      const int64_t ebeg = eb + tile * 16;
+      float x, y, z;
+      int row;
+      const bool valid = entry_pos(T, ebeg + (lane & 15), ee, x, y, z, row);
+      const float seed = valid ? r_row[row - row_begin] * T.t_w[ebeg + (lane & 15)] : 0.0f;
+      const int kq = lane / 16;
+      const float xin = (kq == 0) ? x : (kq == 1) ? y : (kq == 2) ? z : 1.0f;
+      // position-major copies of the tile inputs for the layer-0 weight gradient
+      float xin_t[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int src = 4 * s + lane / 16;  // tile position needed in step s
+        const float sx = __shfl(x, src, 64), sy = __shfl(y, src, 64), sz = __shfl(z, src, 64);
+        const int c = lane & 15;
+        xin_t[s] = (c == 0) ? sx : (c == 1) ? sy : (c == 2) ? sz : 1.0f;
+      }
+      float act[L][4 * NB];
+      tile_forward<L, NB>(lds, xin, act);
+      tile_backward<L, NB>(lds, seed, xin_t, act, tbuf, G);
+    }
+    __syncthreads();  // all waves done with the packed weights: reuse LDS for the reduction
+    flush_grad<L, NB>(G, fn, lds, out);
+  };
+  if (plus)
+    run(std::integral_constant<int, Lp>{}, std::integral_constant<int, NBp>{}, FlatNet{Hp, Lp, 0});
+  else
+    run(std::integral_constant<int, Lm>{}, std::integral_constant<int, NBm>{}, FlatNet{Hm, Lm, PPp});
+  (void)PPm;
+}
+
+// ---- kernel 4/5: reduction of gradient partials and optimizer -------------------
+// g[p] = inv_b * sum over the workgroups of its phase (fixed order); sumsq partial per workgroup
+__global__ __launch_bounds__(256) void mfma_reduce_kernel(const float* __restrict__ gpart, int stride, int nblk_p,
+                                                          int nblk_m, int PPp, int PPm, float inv_b,
+                                                          float* __restrict__ g, double* __restrict__ sq_part) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (p < PPp + PPm) {
+    const bool plus = p < PPp;
+    const int b0 = plus ? 0 : nblk_p, b1 = plus ? nblk_p : nblk_p + nblk_m;
+    const int q = plus ? p : p - PPp;
+    double acc = 0.0;
+    for (int b = b0; b < b1; ++b) acc += (double)gpart[(int64_t)b * stride + q];
+    const float gg = (float)(acc * (double)inv_b);
+    g[p] = gg;
+    v = (double)gg * (double)gg;
+  }
+  v = dips_wave_sum_d(v);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) sq_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void mfma_apply_kernel(const float* __restrict__ g, int P,
+                                                         const double* __restrict__ sq_part, int nsq,
+                                                         const double* __restrict__ loss_part, int nloss, float inv_b,
+                                                         dips_opt_t opt, float* __restrict__ params,
+                                                         float* __restrict__ m, float* __restrict__ v,
+                                                         const int32_t* __restrict__ count_in,
+                                                         float* __restrict__ loss_hist, int loss_cap) {
+  __shared__ float gn_s;
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int b = 0; b < nsq; ++b) t += sq_part[b];
+    gn_s = (float)sqrt(t);
+  }
+  __syncthreads();
+  const int c = *count_in;
+  const float gn = gn_s;
+  const bool clip = (opt.kind == 0) && !(gn < opt.max_norm);
+  float lr_t = opt.lr;
+  if (opt.kind == 0) {
+    if (opt.sched == 0)
+      lr_t = (c <= 0) ? opt.lr : opt.lr * powf(opt.decay_rate, (float)c / opt.transition_steps);
+    else
+      lr_t = opt.lr * (1.0f - fminf(fmaxf((float)c, 0.0f), opt.transition_steps) / opt.transition_steps);
+  }
+  const float t1 = (float)(c + 1);
+  const float bc1 = 1.0f - powf(0.9f, t1), bc2 = 1.0f - powf(0.999f, t1);
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p < P) {
+    float gg = g[p];
+    if (clip) gg = (gg / gn) * opt.max_norm;
+    float u;
+    if (opt.kind == 2) {
+      const float nv = 0.1f * (gg * gg) + 0.9f * v[p];
+      v[p] = nv;
+      u = -(gg / sqrtf(nv + 1e-8f)) * opt.lr;
+    } else {
+      const float nm = 0.1f * gg + 0.9f * m[p];
+      const float nv = 0.001f * (gg * gg) + 0.999f * v[p];
+      m[p] = nm;
+      v[p] = nv;
+      u = (nm / bc1) / (sqrtf(nv / bc2) + 1e-8f);
+      u = (opt.kind == 0) ? -(lr_t * u) : -(u * opt.lr);
+    }
+    params[p] = params[p] + u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && c >= 0 && c < loss_cap) {
+    double l = 0.0;
+    for (int b = 0; b < nloss; ++b) l += loss_part[b];
+    loss_hist[c] = (float)(l * (double)inv_b);
+  }
+}
+
+__global__ void count_inc_kernel(int32_t* count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = *count + 1;
+}
+
+// loss = inv_b * sum of the row-sum partials (fixed order), gradient copy-out
+__global__ void mfma_loss_out_kernel(const double* __restrict__ loss_part, int nloss, float inv_b,
+                                     double* __restrict__ loss_out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double l = 0.0;
+    for (int b = 0; b < nloss; ++b) l += loss_part[b];
+    *loss_out = l * (double)inv_b;
+  }
+}
+
+// ---- host side ----------------------------------------------------------------
+static int nb_of(int H) { return (H + 15) / 16; }
+
+#define DIPS_MFMA_SHAPES(X) \
+  X(1, 1, 2, 1)             \
+  X(2, 1, 2, 1)             \
+  X(3, 4, 3, 4)             \
+  X(1, 1, 3, 4)             \
+  X(2, 2, 2, 2)             \
+  X(3, 2, 3, 2)
+
+int mfma_supported(const dips_net_t* n) {
+  if (n->Hm > 64 || n->Hp > 64) return 0;
+  const int a = n->Lm, b = nb_of(n->Hm), c = n->Lp, d = nb_of(n->Hp);
+#define X(A, B, C, D) \
+  if (a == A && b == B && c == C && d == D) return 1;
+  DIPS_MFMA_SHAPES(X)
+#undef X
+  return 0;
+}
+
+static int net_params(int L, int H) { return 4 * H + (L - 1) * (H * H + H) + H + 1; }
+
+int64_t mfma_workspace_floats(const dips_net_t* n, int64_t max_rows, int64_t max_nnz) {
+  // u per entry, r per row, gradient partials (bwd workgroups x max net size), g
+  const int P = net_params(n->Lm, n->Hm) + net_params(n->Lp, n->Hp);
+  const int stride = std::max(net_params(n->Lm, n->Hm), net_params(n->Lp, n->Hp));
+  return max_nnz + max_rows + (int64_t)kMfmaBwdBlocks * stride + P + 64;
+}
+
+// opt == nullptr: stop after the reduction (gradient in grad_out, loss in loss_out)
+int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, const dips_net_t* net,
+              const dips_opt_t* opt, float inv_b, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+              int loss_cap, float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, float* grad_out,
+              double* loss_out, hipStream_t st) {
+  const int Lm = net->Lm, Hm = net->Hm, Lp = net->Lp, Hp = net->Hp;
+  const int PPp = net_params(Lp, Hp), PPm = net_params(Lm, Hm);
+  const int P = PPp + PPm;
+  const int stride = std::max(PPp, PPm);
+  const int64_t nrows = bt->nrows, np_ = bt->n_plus, nm_ = bt->n_minus;
+  const int64_t nent = bt->ent_end - bt->ent_begin;
+  if (np_ + nm_ != bt->t_count || bt->t_count < 0) {
+    set_error("dips_mfma_train_step: phase counts do not add up");
+    return DIPS_EINVAL;
+  }
+  // workspace layout (floats): u over the batch's CSR entries | r rows | gradient partials | g
+  float* u = work;  // indexed by CSR entry - ent_begin (offset by the caller through t_csr)
+  float* r = u + nent;
+  float* gpart = r + nrows;
+  float* gvec = gpart + (int64_t)kMfmaBwdBlocks * stride;
+  if ((int64_t)(gvec + P - work) > work_floats) {
+    set_error("dips_mfma_train_step: workspace too small");
+    return DIPS_EINVAL;
+  }
+  const int nrb = (int)((nrows + kMT - 1) / kMT);
+  const int nred = (P + 255) / 256;
+  double* loss_part = dwork;
+  double* sq_part = dwork + nrb;
+  if (nrb + nred + 1 > dwork_doubles) {
+    set_error("dips_mfma_train_step: float64 workspace too small");
+    return DIPS_EINVAL;
+  }
+  TileArgs T{g->xs, g->ys, g->zs, g->ny, g->nz, g->nx * g->ny * g->nz, rec->hx, rec->hy, rec->hz,
+             rec->row_point, bt->t_row, bt->t_code, bt->t_w, bt->t_csr};
+  const int a = Lm, b = nb_of(Hm), c = Lp, d = nb_of(Hp);
+  const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
+  const int fwd_blocks = (int)((ntp + 3) / 4 + (ntm + 3) / 4);
+  // bwd workgroups split by phase in proportion to the tiles (each at least 1 if it has tiles)
+  int nbp = (ntp > 0) ? std::max<int64_t>(1, (int64_t)((double)kMfmaBwdBlocks * ntp / std::max<int64_t>(1, ntp + ntm)))
+                      : 0;
+  int nbm = (ntm > 0) ? std::max(1, kMfmaBwdBlocks - nbp) : 0;
+  if (nbp + nbm > kMfmaBwdBlocks) nbp = kMfmaBwdBlocks - nbm;
+  bool launched = false;
+#define X(A, B, C, D)                                                                                              \
+  if (!launched && a == A && b == B && c == C && d == D) {                                                      \
+    launched = true;                                                                                               \
+    constexpr int PK = Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size() : Packed<A, B>::size(); \
+    constexpr int NBX = D > B ? D : B;                                                                             \
+    const size_t lds_f = sizeof(float) * PK;                                                                       \
+    const size_t lds_b = sizeof(float) * (PK + 4 * 2 * 16 * NBX * kTP);                                           \
+    const size_t red_b = sizeof(float) * std::max(PPp, PPm);                                                       \
+    const size_t bwd_lds = std::max(lds_b, red_b);                                                                 \
+    if (bwd_lds > 160 * 1024 || lds_f > 160 * 1024) {                                                              \
+      set_error("dips_mfma_train_step: network too large for the LDS of one workgroup");                            \
+      return DIPS_EUNSUPPORTED;                                                                                    \
+    }                                                                                                              \
+    hipFuncSetAttribute((const void*)mfma_fwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                        (int)lds_f);                                                                               \
+    hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                        (int)bwd_lds);                                                                             \
+    if (fwd_blocks > 0)                                                                                            \
+      hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(fwd_blocks), dim3(kMT), lds_f, st, T, params, Hm, Hp,  \
+                         bt->t_begin, np_, nm_, u - bt->ent_begin);                                                \
+    hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, rec->row_ptr, rec->row_c, bt->row_begin,   \
+                       nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part);                              \
+    if (nbp + nbm > 0)                                                                                             \
+      hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kMT), bwd_lds, st, T,                 \
+                         params, Hm, Hp, bt->t_begin, np_, nm_, bt->row_begin, r, gpart, nbp, stride);             \
+  }
+  DIPS_MFMA_SHAPES(X)
+#undef X
+  if (!launched) {
+    set_error("dips_mfma_train_step: network shape has no MFMA kernel");
+    return DIPS_EUNSUPPORTED;
+  }
+  int status = hip_status(hipGetLastError(), "mfma kernels launch");
+  if (status) return status;
+  // the reduction reads the + net partials from workgroups [0, nbp) and the - net from [nbp, nbp+nbm);
+  // parameters of the + net come first in the flat vector
+  hipLaunchKernelGGL(mfma_reduce_kernel, dim3(nred), dim3(256), 0, st, gpart, stride, nbp, nbm, PPp, PPm, inv_b,
+                     grad_out ? grad_out : gvec, sq_part);
+  if (!opt) {
+    hipLaunchKernelGGL(mfma_loss_out_kernel, dim3(1), dim3(64), 0, st, loss_part, nrb, inv_b, loss_out);
+    return hip_status(hipGetLastError(), "mfma reduce launch");
+  }
+  hipLaunchKernelGGL(mfma_apply_kernel, dim3(nred), dim3(256), 0, st, gvec, P, sq_part, nred, loss_part, nrb, inv_b,
+                     *opt, params, m, v, count, loss_hist, loss_cap);
+  hipLaunchKernelGGL(count_inc_kernel, dim3(1), dim3(64), 0, st, count);
+  return hip_status(hipGetLastError(), "mfma reduce/apply launch");
+}
+
+}  // namespace dips

@@ -38,6 +38,13 @@ class Opt(ctypes.Structure):
                 ("decay_rate", ctypes.c_float), ("transition_steps", ctypes.c_float), ("max_norm", ctypes.c_float)]
 
 
+class MfmaBatch(ctypes.Structure):
+    _fields_ = [("row_begin", ctypes.c_int64), ("nrows", ctypes.c_int64), ("ent_begin", ctypes.c_int64),
+                ("ent_end", ctypes.c_int64), ("t_row", ctypes.c_void_p), ("t_code", ctypes.c_void_p),
+                ("t_w", ctypes.c_void_p), ("t_csr", ctypes.c_void_p), ("t_begin", ctypes.c_int64),
+                ("n_plus", ctypes.c_int64), ("n_minus", ctypes.c_int64), ("t_count", ctypes.c_int64)]
+
+
 class Assemble(ctypes.Structure):
     _fields_ = [("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("zs", ctypes.c_void_p),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
@@ -78,6 +85,13 @@ SIGNATURES = {
                                        ctypes.POINTER(Opt), ctypes.c_float, _P, _P, _P, _P, _P, _I32, _P, _I64, _P,
                                        _P]),
     "dips_eval": (ctypes.c_int, [ctypes.POINTER(Net), _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "dips_mfma_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
+    "dips_mfma_workspace": (_I64, [ctypes.POINTER(Net), _I64, _I64]),
+    "dips_mfma_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
+                                            ctypes.POINTER(Net), ctypes.POINTER(Opt), ctypes.c_float, _P, _P, _P, _P,
+                                            _P, _I32, _P, _I64, _P, _I64, _P]),
+    "dips_mfma_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
+                                           ctypes.POINTER(Net), ctypes.c_float, _P, _P, _P, _P, _I64, _P, _I64, _P]),
     "dips_grid_apply": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _P, ctypes.c_int, _P]),
     "dips_bicgstab": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _P, _P, _I32, ctypes.c_double,
                                      ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double), _P]),

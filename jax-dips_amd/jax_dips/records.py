@@ -128,6 +128,39 @@ class DeviceRecords:
                            self.ent_code.data_ptr(), self.ent_w.data_ptr(), self.nrows, self.nnz,
                            float(self.h[0]), float(self.h[1]), float(self.h[2]))
 
+    def mfma_batches(self):
+        """Phase-sorted entry arrays (+ entries first inside each batch) and one
+        dips_mfma_batch_t per batch, for the MFMA kernels (built once, cached)."""
+        import torch
+
+        if getattr(self, "_mfma", None) is not None:
+            return self._mfma
+        dev = self.row_ptr.device
+        ptr = self.row_ptr.to(torch.int64)
+        counts = ptr[1:] - ptr[:-1]
+        n = self.nnz
+        ent_row = torch.repeat_interleave(torch.arange(self.nrows, device=dev, dtype=torch.int64), counts)
+        code = self.ent_code[:n].to(torch.int64)
+        phase = (code >> 7) & 1
+        batch = ent_row // self.batch_size
+        key = batch * 2 + (1 - phase)
+        _, perm = torch.sort(key, stable=True)
+        t_row = ent_row[perm].to(torch.int32).contiguous()
+        t_code = self.ent_code[:n][perm].contiguous()
+        t_w = self.ent_w[:n][perm].contiguous()
+        t_csr = perm.to(torch.int32).contiguous()
+        nplus = torch.zeros(len(self.batch_begin), dtype=torch.int64, device=dev).index_add_(0, batch, phase).cpu().numpy()
+        ptr_h = ptr.cpu().numpy()
+        batches = []
+        for b, (b0, nr) in enumerate(zip(self.batch_begin, self.batch_rows)):
+            e0, e1 = int(ptr_h[b0]), int(ptr_h[b0 + nr])
+            batches.append(N.MfmaBatch(int(b0), int(nr), e0, e1, t_row.data_ptr(), t_code.data_ptr(), t_w.data_ptr(),
+                                       t_csr.data_ptr(), e0, int(nplus[b]), int(e1 - e0 - nplus[b]), int(e1 - e0)))
+        self._mfma = dict(t_row=t_row, t_code=t_code, t_w=t_w, t_csr=t_csr, batches=batches,
+                          max_nnz=int(max(int(ptr_h[b0 + nr] - ptr_h[b0]) for b0, nr in
+                                          zip(self.batch_begin, self.batch_rows))))
+        return self._mfma
+
     def to_dense(self):
         """Host copy as dense (N,125) weights, (N,125) phase and (N,) constants in
         grid-point order (for tests and inspection)."""
@@ -171,7 +204,9 @@ class Engine:
         self.dims = tuple(int(d) for d in dims)
         self.net = N.Net(*self.dims)
         L = N.lib()
-        if not L.dips_net_supported(ctypes.byref(self.net)):
+        self.lane_path = bool(L.dips_net_supported(ctypes.byref(self.net)))
+        self.mfma_path = bool(L.dips_mfma_supported(ctypes.byref(self.net)))
+        if not (self.lane_path or self.mfma_path):
             raise N.DipsError(f"network shape (Lm,Hm,Lp,Hp)={self.dims} has no compiled kernel")
         self.P = int(L.dips_param_count(ctypes.byref(self.net)))
         params = np.asarray(params, np.float32)
@@ -183,7 +218,8 @@ class Engine:
         self.v = torch.zeros(self.P, dtype=torch.float32, device=device)
         self.count = torch.zeros(1, dtype=torch.int32, device=device)
         self.loss_hist = torch.zeros(max(1, int(max_steps)), dtype=torch.float32, device=device)
-        self.partial_cap = int(L.dips_partial_size(ctypes.byref(self.net), int(max_rows)))
+        # per-workgroup partial rows of the lane-per-row kernel (the MFMA path keeps its own workspace)
+        self.partial_cap = int(L.dips_partial_size(ctypes.byref(self.net), int(max_rows))) if self.lane_path else 1
         self.partial = torch.zeros(self.partial_cap, dtype=torch.float32, device=device)
         self.sync = torch.zeros(1, dtype=torch.int32, device=device)  # arrival counter of the fused step
         self.fused = True
@@ -207,7 +243,48 @@ class Engine:
         s = part.sum(axis=0) * float(inv_batch)
         return s[self.P], s[: self.P]
 
+    def mfma_workspace(self, rec):
+        import torch
+
+        mb = rec.mfma_batches()
+        need = int(N.lib().dips_mfma_workspace(ctypes.byref(self.net), int(rec.batch_size), mb["max_nnz"]))
+        if getattr(self, "_mwork", None) is None or self._mwork.numel() < need:
+            self._mwork = torch.empty(need, dtype=torch.float32, device=self.device)
+            self._mdwork = torch.empty(int(rec.batch_size) // 256 + self.P // 256 + 64, dtype=torch.float64,
+                                       device=self.device)
+        return mb
+
+    def mfma_loss_grad(self, grid, rec, b, inv_batch):
+        """(loss, grad) of batch b through the MFMA kernels, float64 numpy."""
+        import torch
+
+        mb = self.mfma_workspace(rec)
+        g = torch.empty(self.P, dtype=torch.float32, device=self.device)
+        loss = torch.empty(1, dtype=torch.float64, device=self.device)
+        N.check(N.lib().dips_mfma_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb["batches"][b]),
+                                            ctypes.byref(self.net), ctypes.c_float(inv_batch), N.ptr(self.params),
+                                            N.ptr(g), N.ptr(loss), N.ptr(self._mwork), int(self._mwork.numel()),
+                                            N.ptr(self._mdwork), int(self._mdwork.numel()), N.current_stream()),
+                "dips_mfma_loss_grad")
+        torch.cuda.synchronize()
+        return float(loss.item()), g.double().cpu().numpy()
+
+    def train_level_mfma(self, grid, rec, nepochs):
+        mb = self.mfma_workspace(rec)
+        L = N.lib()
+        inv_b = ctypes.c_float(1.0 / rec.batch_size)
+        for _ in range(int(nepochs)):
+            for bt in mb["batches"]:
+                N.check(L.dips_mfma_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(bt),
+                                               ctypes.byref(self.net), ctypes.byref(self.opt), inv_b,
+                                               N.ptr(self.params), N.ptr(self.m), N.ptr(self.v), N.ptr(self.count),
+                                               N.ptr(self.loss_hist), int(self.loss_hist.numel()), N.ptr(self._mwork),
+                                               int(self._mwork.numel()), N.ptr(self._mdwork),
+                                               int(self._mdwork.numel()), N.current_stream()), "dips_mfma_train_step")
+
     def train_level(self, grid, rec, nepochs):
+        if not self.lane_path:
+            return self.train_level_mfma(grid, rec, nepochs)
         st = N.lib().dips_train_level(
             ctypes.byref(grid.c), ctypes.byref(rec.c),
             N.i64_array(rec.batch_begin), N.i64_array(rec.batch_rows), N.i64_array(rec.batch_nbulk),
