@@ -44,8 +44,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="sphere", choices=["sphere", "star"],
                     help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144")
-    ap.add_argument("--path", default="auto", choices=["auto", "lane", "mfma"],
-                    help="training-step kernels: lane-per-row fused step or MFMA tiles")
+    ap.add_argument("--path", default="auto", choices=["auto", "tile", "lane", "mfma"],
+                    help="training-step kernels: packed-tile fused step (auto for the small networks), "
+                         "lane-per-row fused step, or MFMA layer tiles (auto for the wide networks)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -100,10 +101,17 @@ def main():
     K = max(4, args.steps - args.steps % 4)
     eng = Engine(dims, theta, opt, max_rows=B, max_steps=args.warmup + K + args.roofline_launches + 8)
     L = N.lib()
-    use_mfma = args.path == "mfma" or (args.path == "auto" and not eng.lane_path)
+    if args.path == "auto":
+        path = "tile" if eng.tile_path else ("lane" if eng.lane_path else "mfma")
+    else:
+        path = args.path
+    use_mfma = path == "mfma"
     if use_mfma:
         for rec in levels:
             eng.mfma_workspace(rec)
+    if path == "tile":
+        for rec in levels:
+            rec.tiles()
 
     def run_steps(k, stream):
         # k steps split over the 4 zoom levels, batches in order (cyclic)
@@ -120,6 +128,21 @@ def main():
                                                    int(eng.loss_hist.numel()), N.ptr(eng._mwork),
                                                    int(eng._mwork.numel()), N.ptr(eng._mdwork),
                                                    int(eng._mdwork.numel()), stream), "dips_mfma_train_step")
+                continue
+            if path == "tile":
+                tl = rec.tiles()
+                btb = tl["batch_tile_begin"]
+                full, part = divmod(per, nb)
+                for ne, nbat in ((full, nb), (1 if part else 0, part)):
+                    if ne == 0:
+                        continue
+                    N.check(L.dips_tile_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(tl["c"]),
+                                                    N.i64_array(btb[: nbat + 1]), nbat, rec.batch_size, ne,
+                                                    ctypes.byref(eng.net), ctypes.byref(eng.opt), N.ptr(eng.params),
+                                                    N.ptr(eng.m), N.ptr(eng.v), N.ptr(eng.count),
+                                                    N.ptr(eng.loss_hist), int(eng.loss_hist.numel()),
+                                                    N.ptr(eng.partial), eng.partial_cap, N.ptr(eng.sync), stream),
+                            "dips_tile_train_level")
                 continue
             st = L.dips_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), N.i64_array(rec.batch_begin[idx]),
                                     N.i64_array(rec.batch_rows[idx]), N.i64_array(rec.batch_nbulk[idx]), per,
@@ -170,6 +193,16 @@ def main():
                                            int(eng._mwork.numel()), N.ptr(eng._mdwork), int(eng._mdwork.numel()),
                                            stream), "dips_mfma_train_step")
             continue
+        if path == "tile":
+            tl = rec.tiles()
+            tb = int(tl["batch_tile_begin"][b])
+            nt = int(tl["batch_tile_begin"][b + 1]) - tb - 1
+            N.check(L.dips_tile_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(tl["c"]), tb, nt,
+                                           ctypes.byref(eng.net), ctypes.byref(eng.opt), ctypes.c_float(1.0 / B),
+                                           N.ptr(eng.params), N.ptr(eng.m), N.ptr(eng.v), N.ptr(eng.count),
+                                           N.ptr(eng.loss_hist), int(eng.loss_hist.numel()), N.ptr(eng.partial),
+                                           eng.partial_cap, N.ptr(eng.sync), stream), "dips_tile_train_step")
+            continue
         N.check(L.dips_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), int(rec.batch_begin[b]),
                                   int(rec.batch_rows[b]), int(rec.batch_nbulk[b]), ctypes.byref(eng.net),
                                   ctypes.byref(eng.opt),
@@ -195,12 +228,13 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} {n}^3, MLP {dims[0]}x{dims[1]}/{dims[2]}x{dims[3]} tanh, batch {B}, "
-                               f"4 zoom levels (2000-epoch schedule), {'mfma' if use_mfma else 'lane'} kernels",
+                               f"4 zoom levels (2000-epoch schedule), {path} kernels",
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": ("mfma step: fwd + rowsum + bwd + reduce + apply (5 launches)" if use_mfma else
-                                "loss_grad_kernel<grad,fused> (one launch per optimizer step)"),
+                     "kernel": {"mfma": "mfma step: fwd + rowsum + bwd + reduce + apply (5 launches)",
+                                "lane": "loss_grad_kernel<grad,fused> (one launch per optimizer step)",
+                                "tile": "tile_step_kernel<grad,fused> (one launch per optimizer step)"}[path],
                      "avg_launch_us": avg_s * 1e6,
                      "flop_per_launch": flops_sum / R},
         "records_build_s": t_records,

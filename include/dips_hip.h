@@ -137,6 +137,56 @@ int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const i
                      uint32_t* sync, void* stream);
 
 
+/* ---------------------------------------------------------------------------
+ * Entry-parallel step over packed tiles (the default path for the networks
+ * of dips_net_supported).  The rows of every batch are packed into tiles of
+ * whole rows with at most 64 entries and 64 rows (a row with more entries is
+ * a tile of its own); one lane evaluates one entry, row sums are segmented
+ * scans over the lanes, and activations are kept for the backward pass.
+ * Replaces the same reference interfaces as dips_loss_grad / dips_train_step
+ * / dips_train_level (trainer.py:501-591, :783-789, :892-912).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  const int32_t* tile_row; /* device: per batch, ntiles_b + 1 row boundaries; tile t = rows [tile_row[t], tile_row[t+1]) */
+  const int32_t* ent_row;  /* device (nnz): row index of every entry */
+  int64_t nbounds;         /* length of tile_row */
+} dips_tiles_t;
+
+/* Host arrays: row_ptr (nrows+1) of the records, batches [batch_row_begin[b],
+ * + batch_rows[b]).  Writes the boundaries of every batch's tiles into
+ * tile_row (capacity tile_cap; pass NULL to count only) and the index of each
+ * batch's first boundary into batch_tile_begin[0..nbatches] (the last element
+ * is the total).  Returns the number of boundaries, or -1 on error. */
+int64_t dips_pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_row_begin,
+                        const int64_t* batch_rows, int32_t nbatches, int32_t* tile_row, int64_t tile_cap,
+                        int64_t* batch_tile_begin);
+/* ent_row[e] = row of entry e for every entry of rec (device, nnz int32). */
+int dips_entry_rows(const dips_records_t* rec, int32_t* ent_row, void* stream);
+/* 1 if the network shape has a compiled tile kernel, else 0. */
+int dips_tile_supported(const dips_net_t* net);
+/* Floats of partial scratch the tile entry points need (any batch size). */
+int64_t dips_tile_partial_size(const dips_net_t* net);
+/* Loss (and gradient when want_grad != 0) of the ntiles tiles whose first
+ * boundary is tiles->tile_row[tile_begin]: per-workgroup partial rows
+ * (gradient sums then 0.5*sum r^2, block-major, *nblk rows) for dips_opt_step. */
+int dips_tile_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
+                        int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const float* params,
+                        float* partial, int64_t partial_cap, int want_grad, int32_t* nblk, void* stream);
+/* One optimizer step in one launch (as dips_train_step; sync a zeroed device
+ * uint32 arrival counter, left at zero). */
+int dips_tile_train_step(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
+                         int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const dips_opt_t* opt,
+                         float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+                         int32_t loss_cap, float* partial, int64_t partial_cap, uint32_t* sync, void* stream);
+/* nepochs epochs over the nbatches batches of dips_pack_tiles' batch_tile_begin
+ * (host, nbatches + 1), inv_batch = 1/batch_size; fused steps when sync is
+ * given, else dips_tile_loss_grad + dips_opt_step. */
+int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
+                          const int64_t* batch_tile_begin, int32_t nbatches, int64_t batch_size, int32_t nepochs,
+                          const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
+                          int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
+                          uint32_t* sync, void* stream);
+
 /* Network at m points pts (m,3): u (m), optionally grad_x u (m,3) and, with
  * normals (m,3), the normal derivative grad u . n (m).  phase (m) uint8:
  * 1 -> Omega+ network (phi >= 0), 0 -> Omega- network.  Replaces

@@ -150,6 +150,106 @@ int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const i
   return DIPS_OK;
 }
 
+int64_t dips_pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_row_begin,
+                        const int64_t* batch_rows, int32_t nbatches, int32_t* tile_row, int64_t tile_cap,
+                        int64_t* batch_tile_begin) {
+  return pack_tiles(row_ptr, nrows, batch_row_begin, batch_rows, nbatches, tile_row, tile_cap, batch_tile_begin);
+}
+
+int dips_entry_rows(const dips_records_t* rec, int32_t* ent_row, void* stream) {
+  if (!rec || !rec->row_ptr || rec->nrows < 1 || rec->nnz < 0 || (rec->nnz > 0 && !ent_row)) {
+    set_error("dips_entry_rows: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return entry_rows_launch(rec, ent_row, (hipStream_t)stream);
+}
+
+int dips_tile_supported(const dips_net_t* net) {
+  if (check_net(net)) return 0;
+  return tile_supported(net);
+}
+
+int64_t dips_tile_partial_size(const dips_net_t* net) {
+  if (check_net(net)) return -1;
+  return tile_partial_size(net);
+}
+
+int dips_tile_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
+                        int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const float* params,
+                        float* partial, int64_t partial_cap, int want_grad, int32_t* nblk, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!params || !partial || !nblk) {
+    set_error("dips_tile_loss_grad: null buffer");
+    return DIPS_EINVAL;
+  }
+  int nb = 0;
+  st = tile_step_launch(grid, rec, tiles, tile_begin, ntiles, net, params, partial, partial_cap, want_grad, &nb,
+                        (hipStream_t)stream);
+  *nblk = nb;
+  return st;
+}
+
+int dips_tile_train_step(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
+                         int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const dips_opt_t* opt,
+                         float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+                         int32_t loss_cap, float* partial, int64_t partial_cap, uint32_t* sync, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!opt || !params || !m || !v || !count || !loss_hist || !partial || !sync) {
+    set_error("dips_tile_train_step: null buffer");
+    return DIPS_EINVAL;
+  }
+  if (partial_stride(net) - 1 > kFusedMaxParams) {
+    set_error("dips_tile_train_step: network too large for the fused step; use dips_tile_loss_grad + dips_opt_step");
+    return DIPS_EUNSUPPORTED;
+  }
+  return tile_train_step(grid, rec, tiles, tile_begin, ntiles, net, opt, inv_batch, params, m, v, count, loss_hist,
+                         loss_cap, partial, partial_cap, sync, (hipStream_t)stream);
+}
+
+int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
+                          const int64_t* batch_tile_begin, int32_t nbatches, int64_t batch_size, int32_t nepochs,
+                          const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
+                          int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
+                          uint32_t* sync, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!tiles || !batch_tile_begin || nbatches < 1 || batch_size < 1 || nepochs < 0 || !opt || !params || !m || !v ||
+      !count || !loss_hist || !partial) {
+    set_error("dips_tile_train_level: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  for (int b = 0; b < nbatches; ++b) {
+    if (batch_tile_begin[b + 1] - batch_tile_begin[b] < 2) {
+      set_error("dips_tile_train_level: batch without tiles");
+      return DIPS_EINVAL;
+    }
+  }
+  const float inv_b = 1.0f / (float)batch_size;
+  const bool fused = (partial_stride(net) - 1 <= kFusedMaxParams) && sync != nullptr;
+  for (int e = 0; e < nepochs; ++e) {
+    for (int b = 0; b < nbatches; ++b) {
+      const int64_t tb = batch_tile_begin[b], nt = batch_tile_begin[b + 1] - tb - 1;
+      if (fused) {
+        st = tile_train_step(grid, rec, tiles, tb, nt, net, opt, inv_b, params, m, v, count, loss_hist, loss_cap,
+                             partial, partial_cap, sync, (hipStream_t)stream);
+        if (st) return st;
+        continue;
+      }
+      int nb = 0;
+      st = tile_step_launch(grid, rec, tiles, tb, nt, net, params, partial, partial_cap, 1, &nb, (hipStream_t)stream);
+      if (st) return st;
+      st = opt_step_launch(net, opt, partial, nb, inv_b, params, m, v, count, loss_hist, loss_cap, (hipStream_t)stream);
+      if (st) return st;
+    }
+  }
+  return DIPS_OK;
+}
+
 int dips_geom_regions(const double* phic, int64_t n7, int8_t* region, void* stream) {
   if (n7 < 0 || (n7 > 0 && (!phic || !region))) {
     set_error("dips_geom_regions: invalid arguments");

@@ -23,6 +23,20 @@ int train_step_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_
 int partial_stride(const dips_net_t* net);
 int64_t partial_blocks(int64_t nrows);
 
+// tiles.hip
+int tile_step_launch(const dips_grid_t* g, const dips_records_t* rec, const dips_tiles_t* tiles, int64_t tile_begin,
+                     int64_t ntiles, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
+                     int want_grad, int* nblk_out, hipStream_t stream, const FusedOpt* fused = nullptr);
+int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_tiles_t* tiles, int64_t tile_begin,
+                    int64_t ntiles, const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params,
+                    float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, float* partial,
+                    int64_t partial_cap, uint32_t* sync, hipStream_t stream);
+int64_t tile_partial_size(const dips_net_t* net);
+int tile_supported(const dips_net_t* net);
+int entry_rows_launch(const dips_records_t* rec, int32_t* ent_row, hipStream_t st);
+int64_t pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_row_begin, const int64_t* batch_rows,
+                   int32_t nbatches, int32_t* tile_row, int64_t tile_cap, int64_t* batch_tile_begin);
+
 // optim.hip
 int opt_step_launch(const dips_net_t* net, const dips_opt_t* opt, const float* partial, int nblk, float inv_batch,
                     float* params, float* m, float* v, int32_t* count, float* loss_hist, int loss_cap,

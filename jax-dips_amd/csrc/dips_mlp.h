@@ -63,12 +63,12 @@ __device__ __forceinline__ float mlp_forward(const float* __restrict__ P, float 
   return o;
 }
 
-// Adds seed * d NN(x,y,z) / d theta into G (register array of NL::size()).
+// Forward pass that keeps the hidden activations (for a backward pass without
+// recomputation); returns the network output.
 template <int L, int H>
-__device__ __forceinline__ void mlp_backward(const float* __restrict__ P, float x, float y, float z, float seed,
-                                             float* G) {
+__device__ __forceinline__ float mlp_forward_keep(const float* __restrict__ P, float x, float y, float z,
+                                                  float (&act)[L][H]) {
   using NL = NetLayout<L, H>;
-  float act[L][H];
 #pragma unroll
   for (int j = 0; j < H; ++j) {
     float s = P[NL::b(0) + j];
@@ -87,7 +87,18 @@ __device__ __forceinline__ void mlp_backward(const float* __restrict__ P, float 
       act[l][j] = dips_tanh(s);
     }
   }
-  // output layer
+  float o = P[NL::bout()];
+#pragma unroll
+  for (int i = 0; i < H; ++i) o = fmaf(act[L - 1][i], P[NL::wout() + i], o);
+  return o;
+}
+
+// Adds seed * d NN(x,y,z) / d theta into G (register array of NL::size()),
+// given the activations of mlp_forward_keep at the same input.
+template <int L, int H>
+__device__ __forceinline__ void mlp_backward_kept(const float* __restrict__ P, float x, float y, float z, float seed,
+                                                  const float (&act)[L][H], float* G) {
+  using NL = NetLayout<L, H>;
   float d[H];
   G[NL::bout()] += seed;
 #pragma unroll
@@ -122,6 +133,39 @@ __device__ __forceinline__ void mlp_backward(const float* __restrict__ P, float 
     G[NL::w(0) + 2 * H + j] = fmaf(z, d[j], G[NL::w(0) + 2 * H + j]);
     G[NL::b(0) + j] += d[j];
   }
+}
+
+// Pre-activation gradients of every hidden layer for the output seed:
+// dl[L-1] = seed * Wout * (1 - a^2), dl[l-1] = (W_l dl[l]) * (1 - a_{l-1}^2).
+template <int L, int H>
+__device__ __forceinline__ void mlp_deltas(const float* __restrict__ P, float seed, const float (&act)[L][H],
+                                           float (&dl)[L][H]) {
+  using NL = NetLayout<L, H>;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const float a = act[L - 1][i];
+    dl[L - 1][i] = seed * P[NL::wout() + i] * (1.0f - a * a);
+  }
+#pragma unroll
+  for (int l = L - 1; l >= 1; --l) {
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int j = 0; j < H; ++j) acc = fmaf(P[NL::w(l) + i * H + j], dl[l][j], acc);
+      const float a = act[l - 1][i];
+      dl[l - 1][i] = acc * (1.0f - a * a);
+    }
+  }
+}
+
+// Adds seed * d NN(x,y,z) / d theta into G, recomputing the forward activations.
+template <int L, int H>
+__device__ __forceinline__ void mlp_backward(const float* __restrict__ P, float x, float y, float z, float seed,
+                                             float* G) {
+  float act[L][H];
+  (void)mlp_forward_keep<L, H>(P, x, y, z, act);
+  mlp_backward_kept<L, H>(P, x, y, z, seed, act, G);
 }
 
 

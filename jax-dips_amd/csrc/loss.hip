@@ -16,52 +16,11 @@
 #include "dips_common.h"
 #include "dips_hip.h"
 #include "dips_mlp.h"
+#include "dips_step.h"
 
 namespace dips {
 
 constexpr int kLossThreads = 256;
-struct FusedOpt;
-
-// Optimizer state consumed by the fused step (last workgroup of the launch).
-struct FusedOpt {
-  dips_opt_t opt;
-  float inv_b;
-  float* m;
-  float* v;
-  int32_t* count;
-  float* loss_hist;
-  int loss_cap;
-  uint32_t* sync;  // arrival counter, zero between launches
-};
-
-// Optimizer update of parameter p with gradient g (optax chain of
-// solvers/optimizers.py:33-97; see optim.hip for the float32 constants).
-__device__ __forceinline__ void apply_update(const dips_opt_t& opt, int c, float lr_t, float bc1, float bc2,
-                                             bool clip, float gn, int p, float g, float* params, float* m, float* v) {
-  if (clip) g = (g / gn) * opt.max_norm;
-  float u;
-  if (opt.kind == 2) {
-    const float nv = 0.1f * (g * g) + 0.9f * v[p];
-    v[p] = nv;
-    u = -(g / sqrtf(nv + 1e-8f)) * opt.lr;
-  } else {
-    const float nm = 0.1f * g + 0.9f * m[p];
-    const float nv = 0.001f * (g * g) + 0.999f * v[p];
-    m[p] = nm;
-    v[p] = nv;
-    u = (nm / bc1) / (sqrtf(nv / bc2) + 1e-8f);
-    u = (opt.kind == 0) ? -(lr_t * u) : -(u * opt.lr);
-  }
-  params[p] = params[p] + u;
-}
-
-__device__ __forceinline__ float schedule_lr(const dips_opt_t& opt, int c) {
-  if (opt.kind != 0) return opt.lr;
-  if (opt.sched == 0) return (c <= 0) ? opt.lr : opt.lr * powf(opt.decay_rate, (float)c / opt.transition_steps);
-  const float cc = fminf(fmaxf((float)c, 0.0f), opt.transition_steps);
-  return opt.lr * (1.0f - cc / opt.transition_steps);
-}
-
 template <int Lm, int Hm, int Lp, int Hp, bool kGrad, bool kFused>
 __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
     const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs, int ny, int nz,
@@ -210,14 +169,19 @@ __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
   }
   if (!kFused) return;
 
-  // the last workgroup to arrive reduces all partial rows in block order and applies the optimizer
+  // the last workgroup to arrive reduces all partial rows in block order and
+  // applies the optimizer; one thread per workgroup fences (release after the
+  // barrier, acquire in the last workgroup) and counts
   __shared__ int am_last;
-  __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) am_last = (atomicAdd(fo.sync, 1u) == (uint32_t)(nblk - 1)) ? 1 : 0;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int last = (atomicAdd(fo.sync, 1u) == (uint32_t)(nblk - 1)) ? 1 : 0;
+    if (last) __threadfence();
+    am_last = last;
+  }
   __syncthreads();
   if (!am_last) return;
-  __threadfence();
   __shared__ double dred[kLossThreads / DIPS_WAVE];
   __shared__ float gnorm_s;
   __shared__ int count_s;
@@ -231,7 +195,9 @@ __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
     if (p < stride) {
       const float* src = partial + (int64_t)p * nblk;
       double acc = 0.0;
-      for (int b = 0; b < nblk; ++b) acc += (double)__hip_atomic_load(src + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // plain loads after the acquire fence above, unrolled so several are in flight
+#pragma unroll 8
+      for (int b = 0; b < nblk; ++b) acc += (double)src[b];
       const float g = (float)(acc * (double)fo.inv_b);
       gl[q] = g;
       if (p < PT) sq += (double)g * (double)g;

@@ -41,6 +41,7 @@ __global__ __launch_bounds__(kOptThreads) void opt_step_kernel(const float* __re
     g_r[q] = 0.0f;
     if (p < P) {
       double acc = 0.0;
+#pragma unroll 8
       for (int b = 0; b < nblk; ++b) acc += (double)partial[(int64_t)b * stride + p];
       const float g = (float)(acc * (double)inv_b);
       g_r[q] = g;
@@ -55,6 +56,7 @@ __global__ __launch_bounds__(kOptThreads) void opt_step_kernel(const float* __re
     for (int w = 0; w < kOptThreads / DIPS_WAVE; ++w) t += red[w];
     gnorm_s = (float)sqrt(t);
     double l = 0.0;
+#pragma unroll 8
     for (int b = 0; b < nblk; ++b) l += (double)partial[(int64_t)b * stride + P];
     const int c = *count;
     if (c >= 0 && c < loss_cap) loss_hist[c] = (float)(l * (double)inv_b);

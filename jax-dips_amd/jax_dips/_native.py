@@ -45,6 +45,10 @@ class MfmaBatch(ctypes.Structure):
                 ("n_plus", ctypes.c_int64), ("n_minus", ctypes.c_int64), ("t_count", ctypes.c_int64)]
 
 
+class Tiles(ctypes.Structure):
+    _fields_ = [("tile_row", ctypes.c_void_p), ("ent_row", ctypes.c_void_p), ("nbounds", ctypes.c_int64)]
+
+
 class Assemble(ctypes.Structure):
     _fields_ = [("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("zs", ctypes.c_void_p),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
@@ -85,6 +89,19 @@ SIGNATURES = {
                                        ctypes.POINTER(Opt), ctypes.c_float, _P, _P, _P, _P, _P, _I32, _P, _I64, _P,
                                        _P]),
     "dips_eval": (ctypes.c_int, [ctypes.POINTER(Net), _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "dips_pack_tiles": (_I64, [_P, _I64, _P, _P, _I32, _P, _I64, _P]),
+    "dips_entry_rows": (ctypes.c_int, [ctypes.POINTER(Records), _P, _P]),
+    "dips_tile_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
+    "dips_tile_partial_size": (_I64, [ctypes.POINTER(Net)]),
+    "dips_tile_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(Tiles), _I64,
+                                           _I64, ctypes.POINTER(Net), _P, _P, _I64, ctypes.c_int, ctypes.POINTER(_I32),
+                                           _P]),
+    "dips_tile_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(Tiles),
+                                            _I64, _I64, ctypes.POINTER(Net), ctypes.POINTER(Opt), ctypes.c_float, _P,
+                                            _P, _P, _P, _P, _I32, _P, _I64, _P, _P]),
+    "dips_tile_train_level": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(Tiles),
+                                             ctypes.POINTER(_I64), _I32, _I64, _I32, ctypes.POINTER(Net),
+                                             ctypes.POINTER(Opt), _P, _P, _P, _P, _P, _I32, _P, _I64, _P, _P]),
     "dips_mfma_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
     "dips_mfma_workspace": (_I64, [ctypes.POINTER(Net), _I64, _I64]),
     "dips_mfma_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),

@@ -70,6 +70,27 @@ def pack_dense(W250, c, batch_size):
                 batch_size=B, batch_begin=begins, batch_rows=rows, batch_nbulk=nbulk)
 
 
+def pack_tiles(row_ptr, nrows, batch_begin, batch_rows):
+    """Host packing of the rows of every batch into wavefront tiles
+    (dips_pack_tiles).  Returns (tile_row int32, batch_tile_begin int64 of
+    length nbatches + 1)."""
+    L = N.lib()
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    bb = np.ascontiguousarray(batch_begin, dtype=np.int64)
+    br = np.ascontiguousarray(batch_rows, dtype=np.int64)
+    btb = np.zeros(len(bb) + 1, np.int64)
+    total = L.dips_pack_tiles(row_ptr.ctypes.data, int(nrows), bb.ctypes.data, br.ctypes.data, len(bb), None, 0,
+                              btb.ctypes.data)
+    if total < 0:
+        raise N.DipsError(L.dips_last_error().decode())
+    tile_row = np.zeros(int(total), np.int32)
+    got = L.dips_pack_tiles(row_ptr.ctypes.data, int(nrows), bb.ctypes.data, br.ctypes.data, len(bb),
+                            tile_row.ctypes.data, int(total), btb.ctypes.data)
+    if got != total:
+        raise N.DipsError(L.dips_last_error().decode())
+    return tile_row, btb
+
+
 class DeviceGrid:
     def __init__(self, xs, ys, zs, device="cuda"):
         import torch
@@ -161,6 +182,24 @@ class DeviceRecords:
                                           zip(self.batch_begin, self.batch_rows))))
         return self._mfma
 
+    def tiles(self):
+        """Wavefront tiles of every batch (rows packed into tiles of at most 64
+        entries, dips_pack_tiles) and the entry -> row map (dips_entry_rows),
+        built once per level and cached."""
+        import torch
+
+        if getattr(self, "_tiles", None) is not None:
+            return self._tiles
+        L = N.lib()
+        ent_row = torch.empty(max(1, self.nnz), dtype=torch.int32, device=self.row_ptr.device)
+        N.check(L.dips_entry_rows(ctypes.byref(self.c), N.ptr(ent_row), N.current_stream()), "dips_entry_rows")
+        ptr_h = np.ascontiguousarray(self.row_ptr.cpu().numpy(), dtype=np.int32)
+        tile_row, btb = pack_tiles(ptr_h, self.nrows, self.batch_begin, self.batch_rows)
+        tr = torch.as_tensor(tile_row, device=self.row_ptr.device)
+        self._tiles = dict(ent_row=ent_row, tile_row=tr, batch_tile_begin=btb,
+                           c=N.Tiles(tr.data_ptr(), ent_row.data_ptr(), len(tile_row)))
+        return self._tiles
+
     def to_dense(self):
         """Host copy as dense (N,125) weights, (N,125) phase and (N,) constants in
         grid-point order (for tests and inspection)."""
@@ -205,6 +244,7 @@ class Engine:
         self.net = N.Net(*self.dims)
         L = N.lib()
         self.lane_path = bool(L.dips_net_supported(ctypes.byref(self.net)))
+        self.tile_path = bool(L.dips_tile_supported(ctypes.byref(self.net)))
         self.mfma_path = bool(L.dips_mfma_supported(ctypes.byref(self.net)))
         if not (self.lane_path or self.mfma_path):
             raise N.DipsError(f"network shape (Lm,Hm,Lp,Hp)={self.dims} has no compiled kernel")
@@ -220,9 +260,12 @@ class Engine:
         self.loss_hist = torch.zeros(max(1, int(max_steps)), dtype=torch.float32, device=device)
         # per-workgroup partial rows of the lane-per-row kernel (the MFMA path keeps its own workspace)
         self.partial_cap = int(L.dips_partial_size(ctypes.byref(self.net), int(max_rows))) if self.lane_path else 1
+        if self.tile_path:
+            self.partial_cap = max(self.partial_cap, int(L.dips_tile_partial_size(ctypes.byref(self.net))))
         self.partial = torch.zeros(self.partial_cap, dtype=torch.float32, device=device)
         self.sync = torch.zeros(1, dtype=torch.int32, device=device)  # arrival counter of the fused step
         self.fused = True
+        self.use_tiles = True  # packed-tile kernel when the shape has one, else the per-row kernel
         self.opt = make_opt(optimizer_dict)
 
     def loss_grad(self, grid, rec, row_begin, nrows, inv_batch, want_grad=True, nbulk=None):
@@ -242,6 +285,33 @@ class Engine:
         part = self.partial[: nblk.value * (self.P + 1)].view(nblk.value, self.P + 1).double().cpu().numpy()
         s = part.sum(axis=0) * float(inv_batch)
         return s[self.P], s[: self.P]
+
+    def tile_loss_grad(self, grid, rec, b, inv_batch, want_grad=True):
+        """(loss, grad) of batch b through the tile kernel, float64 numpy."""
+        import torch
+
+        tl = rec.tiles()
+        tb = int(tl["batch_tile_begin"][b])
+        nt = int(tl["batch_tile_begin"][b + 1]) - tb - 1
+        nblk = ctypes.c_int32(0)
+        N.check(N.lib().dips_tile_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(tl["c"]), tb, nt,
+                                            ctypes.byref(self.net), N.ptr(self.params), N.ptr(self.partial),
+                                            self.partial_cap, int(bool(want_grad)), ctypes.byref(nblk),
+                                            N.current_stream()), "dips_tile_loss_grad")
+        torch.cuda.synchronize()
+        part = self.partial[: nblk.value * (self.P + 1)].view(nblk.value, self.P + 1).double().cpu().numpy()
+        s = part.sum(axis=0) * float(inv_batch)
+        return s[self.P], s[: self.P]
+
+    def train_level_tiles(self, grid, rec, nepochs):
+        tl = rec.tiles()
+        nb = len(rec.batch_begin)
+        st = N.lib().dips_tile_train_level(
+            ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(tl["c"]), N.i64_array(tl["batch_tile_begin"]), nb,
+            rec.batch_size, int(nepochs), ctypes.byref(self.net), ctypes.byref(self.opt), N.ptr(self.params),
+            N.ptr(self.m), N.ptr(self.v), N.ptr(self.count), N.ptr(self.loss_hist), int(self.loss_hist.numel()),
+            N.ptr(self.partial), self.partial_cap, N.ptr(self.sync) if self.fused else None, N.current_stream())
+        N.check(st, "dips_tile_train_level")
 
     def mfma_workspace(self, rec):
         import torch
@@ -283,6 +353,8 @@ class Engine:
                                                int(self._mdwork.numel()), N.current_stream()), "dips_mfma_train_step")
 
     def train_level(self, grid, rec, nepochs):
+        if self.tile_path and self.use_tiles:
+            return self.train_level_tiles(grid, rec, nepochs)
         if not self.lane_path:
             return self.train_level_mfma(grid, rec, nepochs)
         st = N.lib().dips_train_level(
