@@ -111,7 +111,7 @@ def main():
             eng.mfma_workspace(rec)
     if path == "tile":
         for rec in levels:
-            rec.tiles()
+            rec.tiles(grid)
 
     def run_steps(k, stream):
         # k steps split over the 4 zoom levels, batches in order (cyclic)
@@ -130,7 +130,7 @@ def main():
                                                    int(eng._mdwork.numel()), stream), "dips_mfma_train_step")
                 continue
             if path == "tile":
-                tl = rec.tiles()
+                tl = rec.tiles(grid)
                 btb = tl["batch_tile_begin"]
                 full, part = divmod(per, nb)
                 for ne, nbat in ((full, nb), (1 if part else 0, part)):
@@ -141,7 +141,7 @@ def main():
                                                     ctypes.byref(eng.net), ctypes.byref(eng.opt), N.ptr(eng.params),
                                                     N.ptr(eng.m), N.ptr(eng.v), N.ptr(eng.count),
                                                     N.ptr(eng.loss_hist), int(eng.loss_hist.numel()),
-                                                    N.ptr(eng.partial), eng.partial_cap, N.ptr(eng.sync), stream),
+                                                    N.ptr(eng.partial), eng.partial_cap, None, stream),
                             "dips_tile_train_level")
                 continue
             st = L.dips_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), N.i64_array(rec.batch_begin[idx]),
@@ -175,6 +175,7 @@ def main():
         for b0, nr in zip(rec.batch_begin, rec.batch_rows):
             per_launch_flops.append(cum[ptr[b0 + nr]] - cum[ptr[b0]])
     R = max(4, args.roofline_launches - args.roofline_launches % 4)
+    nblk_rl = ctypes.c_int32(0)
     torch.cuda.synchronize()
     r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     flops_sum = 0.0
@@ -194,14 +195,14 @@ def main():
                                            stream), "dips_mfma_train_step")
             continue
         if path == "tile":
-            tl = rec.tiles()
+            # the dominant kernel alone: the tile step kernel with gradient (its
+            # optimizer kernel, one workgroup, is not part of the roofline)
+            tl = rec.tiles(grid)
             tb = int(tl["batch_tile_begin"][b])
             nt = int(tl["batch_tile_begin"][b + 1]) - tb - 1
-            N.check(L.dips_tile_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(tl["c"]), tb, nt,
-                                           ctypes.byref(eng.net), ctypes.byref(eng.opt), ctypes.c_float(1.0 / B),
-                                           N.ptr(eng.params), N.ptr(eng.m), N.ptr(eng.v), N.ptr(eng.count),
-                                           N.ptr(eng.loss_hist), int(eng.loss_hist.numel()), N.ptr(eng.partial),
-                                           eng.partial_cap, N.ptr(eng.sync), stream), "dips_tile_train_step")
+            N.check(L.dips_tile_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(tl["c"]), tb, nt,
+                                          ctypes.byref(eng.net), N.ptr(eng.params), N.ptr(eng.partial),
+                                          eng.partial_cap, 1, ctypes.byref(nblk_rl), stream), "dips_tile_loss_grad")
             continue
         N.check(L.dips_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), int(rec.batch_begin[b]),
                                   int(rec.batch_rows[b]), int(rec.batch_nbulk[b]), ctypes.byref(eng.net),
@@ -234,7 +235,8 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": {"mfma": "mfma step: fwd + rowsum + bwd + reduce + apply (5 launches)",
                                 "lane": "loss_grad_kernel<grad,fused> (one launch per optimizer step)",
-                                "tile": "tile_step_kernel<grad,fused> (one launch per optimizer step)"}[path],
+                                "tile": "tile_step_kernel<grad> (gradient launch of a step; the optimizer "
+                                        "runs after it in a one-workgroup tile_apply_kernel)"}[path],
                      "avg_launch_us": avg_s * 1e6,
                      "flop_per_launch": flops_sum / R},
         "records_build_s": t_records,

@@ -38,13 +38,19 @@ extern "C" {
 #define DIPS_EUNSUPPORTED 2
 #define DIPS_EHIP 3
 
-/* Training grid: float32 axes as built by jnp.linspace (domain/mesh.py:121-153);
- * point index n = (i*ny + j)*nz + k ('ij' meshgrid, C order). */
+/* Training points: the grid nodes of float32 axes as built by jnp.linspace
+ * (domain/mesh.py:121-153), point index n = (i*ny + j)*nz + k ('ij' meshgrid,
+ * C order) for n < nx*ny*nz, followed by nextra extra points (device float32
+ * (nextra,3), or NULL with nextra = 0): the random points that pad a short
+ * last batch (data_management.py:70-76, :103-173); extra point q has index
+ * nx*ny*nz + q. */
 typedef struct {
   const float* xs;
   const float* ys;
   const float* zs;
   int32_t nx, ny, nz;
+  const float* extra;
+  int64_t nextra;
 } dips_grid_t;
 
 /* Affine residual records of one zoom level:
@@ -139,17 +145,27 @@ int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const i
 
 /* ---------------------------------------------------------------------------
  * Entry-parallel step over packed tiles (the default path for the networks
- * of dips_net_supported).  The rows of every batch are packed into tiles of
- * whole rows with at most 64 entries and 64 rows (a row with more entries is
- * a tile of its own); one lane evaluates one entry, row sums are segmented
- * scans over the lanes, and activations are kept for the backward pass.
- * Replaces the same reference interfaces as dips_loss_grad / dips_train_step
- * / dips_train_level (trainer.py:501-591, :783-789, :892-912).
+ * of dips_tile_supported).  The rows of every batch are packed into tiles of
+ * whole rows with at most 64 slots (one per entry, one for a row without
+ * entries) and 64 rows; a row with more than 64 entries is a tile of its own.
+ * Each tile is laid out once per zoom level as 64 slots (dips_tile_slots);
+ * one lane evaluates one slot, row sums are segmented scans over the lanes,
+ * and activations are kept for the backward pass.  Replaces the same
+ * reference interfaces as dips_loss_grad / dips_train_step / dips_train_level
+ * (trainer.py:501-591, :783-789, :892-912).
  * ------------------------------------------------------------------------- */
 typedef struct {
-  const int32_t* tile_row; /* device: per batch, ntiles_b + 1 row boundaries; tile t = rows [tile_row[t], tile_row[t+1]) */
-  const int32_t* ent_row;  /* device (nnz): row index of every entry */
-  int64_t nbounds;         /* length of tile_row */
+  const int32_t* tile_row;   /* device: per batch, ntiles_b + 1 row boundaries; tile at boundary index k =
+                                rows [tile_row[k], tile_row[k+1]) */
+  int64_t nbounds;           /* length of tile_row */
+  const float* slot_x;       /* device (nbounds * 64): slots of the tile at boundary index k are
+                                k*64 .. k*64+63: entry position fl32(p + o*h) */
+  const float* slot_y;
+  const float* slot_z;
+  const float* slot_w;       /* entry weight (0 for the slot of a row without entries) */
+  const float* slot_c;       /* row constant in the row's last slot, else 0 */
+  const uint32_t* slot_info; /* bits 0-5 first slot of the row, 6-11 last slot, 12 phase (1: Omega+ net),
+                                13 slot in use */
 } dips_tiles_t;
 
 /* Host arrays: row_ptr (nrows+1) of the records, batches [batch_row_begin[b],
@@ -160,8 +176,11 @@ typedef struct {
 int64_t dips_pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_row_begin,
                         const int64_t* batch_rows, int32_t nbatches, int32_t* tile_row, int64_t tile_cap,
                         int64_t* batch_tile_begin);
-/* ent_row[e] = row of entry e for every entry of rec (device, nnz int32). */
-int dips_entry_rows(const dips_records_t* rec, int32_t* ent_row, void* stream);
+/* Lays out the slots of every tile of tile_row (device, nbounds boundaries)
+ * into the six slot arrays (device, nbounds * 64 each). */
+int dips_tile_slots(const dips_grid_t* grid, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
+                    float* slot_x, float* slot_y, float* slot_z, float* slot_w, float* slot_c, uint32_t* slot_info,
+                    void* stream);
 /* 1 if the network shape has a compiled tile kernel, else 0. */
 int dips_tile_supported(const dips_net_t* net);
 /* Floats of partial scratch the tile entry points need (any batch size). */
@@ -172,15 +191,18 @@ int64_t dips_tile_partial_size(const dips_net_t* net);
 int dips_tile_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
                         int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const float* params,
                         float* partial, int64_t partial_cap, int want_grad, int32_t* nblk, void* stream);
-/* One optimizer step in one launch (as dips_train_step; sync a zeroed device
- * uint32 arrival counter, left at zero). */
+/* One optimizer step: with sync (a zeroed device uint32 arrival counter, left
+ * at zero) in one launch whose last workgroup reduces and applies the
+ * optimizer (as dips_train_step); with sync == NULL as two launches, the
+ * step kernel and a one-workgroup reduction + optimizer kernel.  Same results
+ * either way. */
 int dips_tile_train_step(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
                          int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const dips_opt_t* opt,
                          float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
                          int32_t loss_cap, float* partial, int64_t partial_cap, uint32_t* sync, void* stream);
 /* nepochs epochs over the nbatches batches of dips_pack_tiles' batch_tile_begin
- * (host, nbatches + 1), inv_batch = 1/batch_size; fused steps when sync is
- * given, else dips_tile_loss_grad + dips_opt_step. */
+ * (host, nbatches + 1), inv_batch = 1/batch_size: one dips_tile_train_step
+ * per batch (single-launch steps when sync is given). */
 int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
                           const int64_t* batch_tile_begin, int32_t nbatches, int64_t batch_size, int32_t nepochs,
                           const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
@@ -315,7 +337,9 @@ typedef struct {
   uint32_t* ent_code;
   float* ent_w;
   float* row_c;
-  int64_t npts;
+  int64_t npts;             /* N = nx*ny*nz + nextra */
+  const float* extra;       /* (nextra,3) extra points after the grid nodes (as dips_grid_t) */
+  int64_t nextra;
 } dips_assemble_t;
 
 /* mode 0 counts the entries of every point's row; mode 1 (same inputs, after

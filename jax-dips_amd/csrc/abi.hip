@@ -27,7 +27,8 @@ static int check_net(const dips_net_t* net) {
 }
 
 static int check_records(const dips_grid_t* grid, const dips_records_t* rec) {
-  if (!grid || !rec || !grid->xs || !grid->ys || !grid->zs || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) {
+  if (!grid || !rec || !grid->xs || !grid->ys || !grid->zs || grid->nx < 1 || grid->ny < 1 || grid->nz < 1 ||
+      grid->nextra < 0 || (grid->nextra > 0 && !grid->extra)) {
     set_error("invalid grid");
     return DIPS_EINVAL;
   }
@@ -156,12 +157,17 @@ int64_t dips_pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* ba
   return pack_tiles(row_ptr, nrows, batch_row_begin, batch_rows, nbatches, tile_row, tile_cap, batch_tile_begin);
 }
 
-int dips_entry_rows(const dips_records_t* rec, int32_t* ent_row, void* stream) {
-  if (!rec || !rec->row_ptr || rec->nrows < 1 || rec->nnz < 0 || (rec->nnz > 0 && !ent_row)) {
-    set_error("dips_entry_rows: invalid arguments");
+int dips_tile_slots(const dips_grid_t* grid, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
+                    float* slot_x, float* slot_y, float* slot_z, float* slot_w, float* slot_c, uint32_t* slot_info,
+                    void* stream) {
+  int st = check_records(grid, rec);
+  if (st) return st;
+  if (!tile_row || nbounds < 1 || !slot_x || !slot_y || !slot_z || !slot_w || !slot_c || !slot_info) {
+    set_error("dips_tile_slots: invalid arguments");
     return DIPS_EINVAL;
   }
-  return entry_rows_launch(rec, ent_row, (hipStream_t)stream);
+  return tile_slots_launch(grid, rec, tile_row, nbounds, slot_x, slot_y, slot_z, slot_w, slot_c, slot_info,
+                           (hipStream_t)stream);
 }
 
 int dips_tile_supported(const dips_net_t* net) {
@@ -198,13 +204,9 @@ int dips_tile_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
-  if (!opt || !params || !m || !v || !count || !loss_hist || !partial || !sync) {
+  if (!opt || !params || !m || !v || !count || !loss_hist || !partial) {
     set_error("dips_tile_train_step: null buffer");
     return DIPS_EINVAL;
-  }
-  if (partial_stride(net) - 1 > kFusedMaxParams) {
-    set_error("dips_tile_train_step: network too large for the fused step; use dips_tile_loss_grad + dips_opt_step");
-    return DIPS_EUNSUPPORTED;
   }
   return tile_train_step(grid, rec, tiles, tile_begin, ntiles, net, opt, inv_batch, params, m, v, count, loss_hist,
                          loss_cap, partial, partial_cap, sync, (hipStream_t)stream);
@@ -230,20 +232,11 @@ int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, co
     }
   }
   const float inv_b = 1.0f / (float)batch_size;
-  const bool fused = (partial_stride(net) - 1 <= kFusedMaxParams) && sync != nullptr;
   for (int e = 0; e < nepochs; ++e) {
     for (int b = 0; b < nbatches; ++b) {
       const int64_t tb = batch_tile_begin[b], nt = batch_tile_begin[b + 1] - tb - 1;
-      if (fused) {
-        st = tile_train_step(grid, rec, tiles, tb, nt, net, opt, inv_b, params, m, v, count, loss_hist, loss_cap,
-                             partial, partial_cap, sync, (hipStream_t)stream);
-        if (st) return st;
-        continue;
-      }
-      int nb = 0;
-      st = tile_step_launch(grid, rec, tiles, tb, nt, net, params, partial, partial_cap, 1, &nb, (hipStream_t)stream);
-      if (st) return st;
-      st = opt_step_launch(net, opt, partial, nb, inv_b, params, m, v, count, loss_hist, loss_cap, (hipStream_t)stream);
+      st = tile_train_step(grid, rec, tiles, tb, nt, net, opt, inv_b, params, m, v, count, loss_hist, loss_cap,
+                           partial, partial_cap, sync, (hipStream_t)stream);
       if (st) return st;
     }
   }
@@ -303,6 +296,10 @@ int dips_grid_apply(const dips_grid_t* grid, const dips_records_t* rec, const fl
                     void* stream) {
   int st = check_records(grid, rec);
   if (st) return st;
+  if (grid->extra && grid->nextra > 0) {
+    set_error("dips_grid_apply: grid functions live on grid nodes only (no extra points)");
+    return DIPS_EINVAL;
+  }
   if (!u || !y) {
     set_error("dips_grid_apply: null buffer");
     return DIPS_EINVAL;
@@ -314,6 +311,10 @@ int dips_bicgstab(const dips_grid_t* grid, const dips_records_t* rec, float* x, 
                   int32_t max_iter, double rtol, int32_t* iters, double* relres, void* stream) {
   int st = check_records(grid, rec);
   if (st) return st;
+  if (grid->extra && grid->nextra > 0) {
+    set_error("dips_bicgstab: grid functions live on grid nodes only (no extra points)");
+    return DIPS_EINVAL;
+  }
   if (!x || !work || !dwork || !iters || !relres || max_iter < 0) {
     set_error("dips_bicgstab: invalid arguments");
     return DIPS_EINVAL;

@@ -5,6 +5,39 @@
 
 #define DIPS_WAVE 64
 
+// Training points: grid nodes (index (i*ny + j)*nz + k) followed by extra
+// points stored as float32 triples (index ngrid + q).
+struct PointSet {
+  const float* xs;
+  const float* ys;
+  const float* zs;
+  int ny, nz;
+  int64_t ngrid;
+  const float* extra;
+  int64_t nextra;
+};
+
+// Coordinates of point pt; false if pt is not a point of the set.
+__device__ __forceinline__ bool point_xyz(const PointSet& s, int64_t pt, float& x, float& y, float& z) {
+  if (pt < 0) return false;
+  if (pt < s.ngrid) {
+    const int k = (int)(pt % s.nz);
+    const int64_t ij = pt / s.nz;
+    const int j = (int)(ij % s.ny);
+    const int i = (int)(ij / s.ny);
+    x = s.xs[i];
+    y = s.ys[j];
+    z = s.zs[k];
+    return true;
+  }
+  const int64_t q = pt - s.ngrid;
+  if (q >= s.nextra) return false;
+  x = s.extra[3 * q];
+  y = s.extra[3 * q + 1];
+  z = s.extra[3 * q + 2];
+  return true;
+}
+
 // Per-entry code of a record: bits 0..6 = lattice offset o in [-2,2]^3 encoded
 // as 25*(oz+2) + 5*(oy+2) + (ox+2); bit 7 = network phase (1 -> Omega+ net).
 __device__ __forceinline__ void dips_decode_offset(uint32_t code, int& ox, int& oy, int& oz) {
@@ -28,6 +61,12 @@ __device__ __forceinline__ float dips_wave_sum(float v) {
 }
 
 __device__ __forceinline__ double dips_wave_sum_d(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, DIPS_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ int dips_wave_sum_i(int v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, DIPS_WAVE);
   return v;

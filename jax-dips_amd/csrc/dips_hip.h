@@ -4,10 +4,16 @@
 #include <string>
 
 #include "../../include/dips_hip.h"
+#include "dips_common.h"
 
 namespace dips {
 
 void set_error(const std::string& msg);
+
+inline PointSet point_set(const dips_grid_t* g) {
+  return PointSet{g->xs, g->ys, g->zs, g->ny, g->nz, (int64_t)g->nx * g->ny * g->nz, g->extra,
+                  g->extra ? g->nextra : 0};
+}
 int hip_status(hipError_t e, const char* what);
 
 // loss.hip
@@ -33,7 +39,8 @@ int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_
                     int64_t partial_cap, uint32_t* sync, hipStream_t stream);
 int64_t tile_partial_size(const dips_net_t* net);
 int tile_supported(const dips_net_t* net);
-int entry_rows_launch(const dips_records_t* rec, int32_t* ent_row, hipStream_t st);
+int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
+                      float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, hipStream_t st);
 int64_t pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_row_begin, const int64_t* batch_rows,
                    int32_t nbatches, int32_t* tile_row, int64_t tile_cap, int64_t* batch_tile_begin);
 

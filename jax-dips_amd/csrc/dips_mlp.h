@@ -1,10 +1,19 @@
 // Per-lane two-phase tanh MLP forward / backward used by the loss and eval kernels.
-// Weights are read through uniform (scalar-cache) loads from the flat parameter
-// vector; activations and gradient accumulators live in registers.
+// The weight pointer type is a template parameter: kernels pass a pointer in
+// the constant address space (dips_weights) so that the uniform weight reads
+// become scalar loads through the scalar cache, with the weights used as
+// scalar operands; activations and gradient accumulators live in registers.
 #pragma once
 #include "dips_common.h"
 
 namespace dips {
+
+// Weights seen through the constant address space: uniform reads are scalar
+// loads.  Valid while the kernel does not write the parameters before reading
+// them (the optimizer writes them only after the last read of the step).
+typedef const __attribute__((address_space(4))) float dips_weight_t;
+
+__device__ __forceinline__ dips_weight_t* dips_weights(const float* p) { return (dips_weight_t*)p; }
 
 // tanh in float32 without branches: x + x^3 g(x^2) for |x| < 0.55 (degree-4
 // minimax polynomial g fitted for this library, <= 0.9 ulp) and
@@ -32,8 +41,8 @@ __device__ __forceinline__ float dips_tanh(float x) {
   X(2, 16, 2, 16)
 
 
-template <int L, int H>
-__device__ __forceinline__ float mlp_forward(const float* __restrict__ P, float x, float y, float z) {
+template <int L, int H, typename WP>
+__device__ __forceinline__ float mlp_forward(WP P, float x, float y, float z) {
   using NL = NetLayout<L, H>;
   float a[H];
 #pragma unroll
@@ -65,9 +74,8 @@ __device__ __forceinline__ float mlp_forward(const float* __restrict__ P, float 
 
 // Forward pass that keeps the hidden activations (for a backward pass without
 // recomputation); returns the network output.
-template <int L, int H>
-__device__ __forceinline__ float mlp_forward_keep(const float* __restrict__ P, float x, float y, float z,
-                                                  float (&act)[L][H]) {
+template <int L, int H, typename WP>
+__device__ __forceinline__ float mlp_forward_keep(WP P, float x, float y, float z, float (&act)[L][H]) {
   using NL = NetLayout<L, H>;
 #pragma unroll
   for (int j = 0; j < H; ++j) {
@@ -95,8 +103,8 @@ __device__ __forceinline__ float mlp_forward_keep(const float* __restrict__ P, f
 
 // Adds seed * d NN(x,y,z) / d theta into G (register array of NL::size()),
 // given the activations of mlp_forward_keep at the same input.
-template <int L, int H>
-__device__ __forceinline__ void mlp_backward_kept(const float* __restrict__ P, float x, float y, float z, float seed,
+template <int L, int H, typename WP>
+__device__ __forceinline__ void mlp_backward_kept(WP P, float x, float y, float z, float seed,
                                                   const float (&act)[L][H], float* G) {
   using NL = NetLayout<L, H>;
   float d[H];
@@ -137,9 +145,8 @@ __device__ __forceinline__ void mlp_backward_kept(const float* __restrict__ P, f
 
 // Pre-activation gradients of every hidden layer for the output seed:
 // dl[L-1] = seed * Wout * (1 - a^2), dl[l-1] = (W_l dl[l]) * (1 - a_{l-1}^2).
-template <int L, int H>
-__device__ __forceinline__ void mlp_deltas(const float* __restrict__ P, float seed, const float (&act)[L][H],
-                                           float (&dl)[L][H]) {
+template <int L, int H, typename WP>
+__device__ __forceinline__ void mlp_deltas(WP P, float seed, const float (&act)[L][H], float (&dl)[L][H]) {
   using NL = NetLayout<L, H>;
 #pragma unroll
   for (int i = 0; i < H; ++i) {
@@ -160,9 +167,8 @@ __device__ __forceinline__ void mlp_deltas(const float* __restrict__ P, float se
 }
 
 // Adds seed * d NN(x,y,z) / d theta into G, recomputing the forward activations.
-template <int L, int H>
-__device__ __forceinline__ void mlp_backward(const float* __restrict__ P, float x, float y, float z, float seed,
-                                             float* G) {
+template <int L, int H, typename WP>
+__device__ __forceinline__ void mlp_backward(WP P, float x, float y, float z, float seed, float* G) {
   float act[L][H];
   (void)mlp_forward_keep<L, H>(P, x, y, z, act);
   mlp_backward_kept<L, H>(P, x, y, z, seed, act, G);
@@ -170,8 +176,8 @@ __device__ __forceinline__ void mlp_backward(const float* __restrict__ P, float 
 
 
 // Forward value and input gradient d NN / d (x, y, z).
-template <int L, int H>
-__device__ __forceinline__ float mlp_value_grad(const float* __restrict__ P, float x, float y, float z, float g[3]) {
+template <int L, int H, typename WP>
+__device__ __forceinline__ float mlp_value_grad(WP P, float x, float y, float z, float g[3]) {
   using NL = NetLayout<L, H>;
   float act[L][H];
 #pragma unroll

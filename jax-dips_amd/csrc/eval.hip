@@ -21,8 +21,8 @@ __global__ __launch_bounds__(256) void eval_kernel(const float* __restrict__ par
   if (t >= m) return;
   const float x = pts[t * 3 + 0], y = pts[t * 3 + 1], z = pts[t * 3 + 2];
   float g[3];
-  const float v = phase[t] ? mlp_value_grad<Lp, Hp>(params, x, y, z, g)
-                           : mlp_value_grad<Lm, Hm>(params + NP::size(), x, y, z, g);
+  dips_weight_t* W = dips_weights(params);
+  const float v = phase[t] ? mlp_value_grad<Lp, Hp>(W, x, y, z, g) : mlp_value_grad<Lm, Hm>(W + NP::size(), x, y, z, g);
   u[t] = v;
   if (grad) {
     grad[t * 3 + 0] = g[0];

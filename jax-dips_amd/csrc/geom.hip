@@ -160,17 +160,15 @@ __global__ void regions_kernel(const double* __restrict__ phic, int64_t n7, int8
 
 // ----------------------------------------------------------------------------
 // Full geometry of crossed centre cells.
-__global__ void cells_kernel(const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
-                             int ny, int nz, float hx, float hy, float hz, const int32_t* __restrict__ cell_pt,
+__global__ void cells_kernel(PointSet ps, float hx, float hy, float hz, const int32_t* __restrict__ cell_pt,
                              int64_t ncell, const double* __restrict__ phic_pt /* (npts,7,8), centre = e 0 */,
-                             int npts, double* __restrict__ vol_m, double* __restrict__ area_m /* (ncell,6) */,
+                             double* __restrict__ vol_m, double* __restrict__ area_m /* (ncell,6) */,
                              double* __restrict__ tri /* (ncell,10,3,3) */, double* __restrict__ tri_a /* (ncell,10) */) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int pt = cell_pt[c];
-  if (pt < 0 || pt >= npts) return;
-  const int k = pt % nz, ij = pt / nz, j = ij % ny, i = ij / ny;
-  const float ctr[3] = {xs[i], ys[j], zs[k]};
+  float ctr[3];
+  if (!point_xyz(ps, pt, ctr[0], ctr[1], ctr[2])) return;
   const float half[3] = {hx * 0.5f, hy * 0.5f, hz * 0.5f};
   const double* ph = phic_pt + (int64_t)pt * 56;
   V3 C[8];
@@ -284,10 +282,9 @@ __device__ void pinv_sym3(const double Min[3][3], double rcond, double Pi[3][3])
 // out_a: (n,2,27) coefficients of U_q for u- and u+; out_k: (n,2) multipliers of
 // J = alpha(rG) + delta*beta(rG)/mu_sel(rG) in u- and u+; out_sel: 1 -> mu_sel = mu+
 // (mu- > mu+ branch), 0 -> mu_sel = mu-; out_r: (n,3) r_Gamma; out_delta: delta.
-__global__ void regression_kernel(const float* __restrict__ xs, const float* __restrict__ ys,
-                                  const float* __restrict__ zs, int ny, int nz, float hx, float hy, float hz,
-                                  const int32_t* __restrict__ cs_pt, const int8_t* __restrict__ cs_e, int64_t ncs,
-                                  int npts, const double* __restrict__ phi_cube /* (ncs,27) */,
+__global__ void regression_kernel(PointSet ps, float hx, float hy, float hz, const int32_t* __restrict__ cs_pt,
+                                  const int8_t* __restrict__ cs_e, int64_t ncs,
+                                  const double* __restrict__ phi_cube /* (ncs,27) */,
                                   const double* __restrict__ mu_s /* (ncs,2): mu-, mu+ at s */, double rcond,
                                   double* __restrict__ out_a, double* __restrict__ out_k, int8_t* __restrict__ out_sel,
                                   double* __restrict__ out_r, double* __restrict__ out_delta) {
@@ -295,10 +292,10 @@ __global__ void regression_kernel(const float* __restrict__ xs, const float* __r
   if (t >= ncs) return;
   const int pt = cs_pt[t];
   const int e = cs_e[t];
-  if (pt < 0 || pt >= npts || e < 0 || e > 6) return;
-  const int k = pt % nz, ij = pt / nz, j = ij % ny, i = ij / ny;
-  const float s32[3] = {xs[i] + (float)kStencil[e][0] * hx, ys[j] + (float)kStencil[e][1] * hy,
-                        zs[k] + (float)kStencil[e][2] * hz};
+  float px, py, pz;
+  if (e < 0 || e > 6 || !point_xyz(ps, pt, px, py, pz)) return;
+  const float s32[3] = {px + (float)kStencil[e][0] * hx, py + (float)kStencil[e][1] * hy,
+                        pz + (float)kStencil[e][2] * hz};
   const double h[3] = {hx, hy, hz};
   const double* ph = phi_cube + t * 27;
   // cube offsets X_q (x fastest) in float32 then widened, as get_Xijk builds them
@@ -416,8 +413,9 @@ __device__ __forceinline__ int offset_code(int ox, int oy, int oz) { return 25 *
 __global__ void assemble_kernel(AssembleArgs A) {
   const int64_t pt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (pt >= A.npts) return;
-  const int k = (int)(pt % A.nz), ij = (int)(pt / A.nz), j = ij % A.ny, i = ij / A.ny;
-  const float p[3] = {A.xs[i], A.ys[j], A.zs[k]};
+  const PointSet ps{A.xs, A.ys, A.zs, A.ny, A.nz, (int64_t)A.nx * A.ny * A.nz, A.extra, A.extra ? A.nextra : 0};
+  float p[3];
+  if (!point_xyz(ps, pt, p[0], p[1], p[2])) return;
   const float h[3] = {A.hx, A.hy, A.hz};
   const double vol = (double)A.hx * (double)A.hy * (double)A.hz;
   bool boundary = false;
@@ -624,8 +622,8 @@ int geom_cells(const dips_grid_t* g, float hx, float hy, float hz, const int32_t
                const double* phic, double* vol_m, double* area_m, double* tri, double* tri_a, hipStream_t st) {
   if (ncell <= 0) return DIPS_OK;
   const int64_t nb = (ncell + kGeomThreads - 1) / kGeomThreads;
-  hipLaunchKernelGGL(cells_kernel, dim3((unsigned)nb), dim3(kGeomThreads), 0, st, g->xs, g->ys, g->zs, g->ny, g->nz,
-                     hx, hy, hz, cell_pt, ncell, phic, g->nx * g->ny * g->nz, vol_m, area_m, tri, tri_a);
+  hipLaunchKernelGGL(cells_kernel, dim3((unsigned)nb), dim3(kGeomThreads), 0, st, point_set(g), hx, hy, hz, cell_pt,
+                     ncell, phic, vol_m, area_m, tri, tri_a);
   return hip_status(hipGetLastError(), "cells_kernel launch");
 }
 
@@ -634,9 +632,8 @@ int geom_regression(const dips_grid_t* g, float hx, float hy, float hz, const in
                     double* out_k, int8_t* out_sel, double* out_r, double* out_delta, hipStream_t st) {
   if (ncs <= 0) return DIPS_OK;
   const int64_t nb = (ncs + 63) / 64;
-  hipLaunchKernelGGL(regression_kernel, dim3((unsigned)nb), dim3(64), 0, st, g->xs, g->ys, g->zs, g->ny, g->nz, hx,
-                     hy, hz, cs_pt, cs_e, ncs, g->nx * g->ny * g->nz, phi_cube, mu_s, rcond, out_a, out_k, out_sel,
-                     out_r, out_delta);
+  hipLaunchKernelGGL(regression_kernel, dim3((unsigned)nb), dim3(64), 0, st, point_set(g), hx, hy, hz, cs_pt, cs_e,
+                     ncs, phi_cube, mu_s, rcond, out_a, out_k, out_sel, out_r, out_delta);
   return hip_status(hipGetLastError(), "regression_kernel launch");
 }
 

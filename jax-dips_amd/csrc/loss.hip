@@ -23,18 +23,18 @@ namespace dips {
 constexpr int kLossThreads = 256;
 template <int Lm, int Hm, int Lp, int Hp, bool kGrad, bool kFused>
 __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
-    const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs, int ny, int nz,
+    PointSet ps,
     float hx, float hy, float hz, const int32_t* __restrict__ row_point, const int32_t* __restrict__ row_ptr,
     const float* __restrict__ row_c, int64_t row_begin, int64_t nrows, const uint32_t* __restrict__ ent_code,
-    const float* __restrict__ ent_w, int64_t nnz, int npts, float* params, float* __restrict__ partial,
+    const float* __restrict__ ent_w, int64_t nnz, float* params, float* __restrict__ partial,
     FusedOpt fo, int64_t nbulk, int nblk_bulk) {
   using NP = NetLayout<Lp, Hp>;
   using NM = NetLayout<Lm, Hm>;
   constexpr int PP = NP::size();
   constexpr int PM = NM::size();
   constexpr int PT = PP + PM;
-  const float* __restrict__ Pp = params;
-  const float* __restrict__ Pm = params + PP;
+  dips_weight_t* Pp = dips_weights(params);
+  dips_weight_t* Pm = Pp + PP;
 
   float Gp[kGrad ? PP : 1];
   float Gm[kGrad ? PM : 1];
@@ -53,12 +53,8 @@ __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
     const int64_t local = (int64_t)blockIdx.x * kLossThreads + threadIdx.x;
     const int64_t row = row_begin + local;
     const int pt = (local < nbulk) ? row_point[row] : -1;
-    if (pt >= 0 && pt < npts) {
-      const int k = pt % nz;
-      const int ij = pt / nz;
-      const int j = ij % ny;
-      const int i = ij / ny;
-      const float px = xs[i], py = ys[j], pz = zs[k];
+    float px, py, pz;
+    if (point_xyz(ps, pt, px, py, pz)) {
       // entry range clamped to the entry arrays (rows are well formed by construction)
       const int beg = max(row_ptr[row], 0);
       const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
@@ -95,12 +91,8 @@ __global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(
     const int64_t wrow = ((int64_t)blockIdx.x - nblk_bulk) * (kLossThreads / DIPS_WAVE) + wave;
     const int64_t row = row_begin + nbulk + wrow;
     const int pt = (nbulk + wrow < nrows) ? row_point[row] : -1;  // uniform across the wave
-    if (pt >= 0 && pt < npts) {
-      const int k = pt % nz;
-      const int ij = pt / nz;
-      const int j = ij % ny;
-      const int i = ij / ny;
-      const float px = xs[i], py = ys[j], pz = zs[k];
+    float px, py, pz;
+    if (point_xyz(ps, pt, px, py, pz)) {
       const int beg = max(row_ptr[row], 0);
       const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
       float acc = 0.0f;
@@ -254,9 +246,9 @@ static void launch_shape(const dips_grid_t* g, const dips_records_t* rec, int64_
                          int nblk_bulk, hipStream_t st) {
   dim3 grid(nblk), block(kLossThreads);
 #define DIPS_LAUNCH(G, F)                                                                                         \
-  hipLaunchKernelGGL((loss_grad_kernel<Lm, Hm, Lp, Hp, G, F>), grid, block, 0, st, g->xs, g->ys, g->zs, g->ny, \
-                     g->nz, rec->hx, rec->hy, rec->hz, rec->row_point, rec->row_ptr, rec->row_c, row_begin, nrows, \
-                     rec->ent_code, rec->ent_w, rec->nnz, g->nx * g->ny * g->nz, params, partial, fo, nbulk, \
+  hipLaunchKernelGGL((loss_grad_kernel<Lm, Hm, Lp, Hp, G, F>), grid, block, 0, st, point_set(g), rec->hx, rec->hy, \
+                     rec->hz, rec->row_point, rec->row_ptr, rec->row_c, row_begin, nrows, \
+                     rec->ent_code, rec->ent_w, rec->nnz, params, partial, fo, nbulk, \
                      nblk_bulk)
   if (mode == 2)
     DIPS_LAUNCH(true, true);

@@ -153,11 +153,8 @@ __device__ __forceinline__ float tile_output(const float* __restrict__ W, const 
 }
 
 // ---- tile entry data ---------------------------------------------------------
-struct TileArgs {
-  const float* xs;
-  const float* ys;
-  const float* zs;
-  int ny, nz, npts;
+struct MfmaTileArgs {
+  PointSet ps;
   float hx, hy, hz;
   const int32_t* row_point;  // global row -> grid point
   const int32_t* t_row;      // phase-sorted entries: global row
@@ -167,27 +164,27 @@ struct TileArgs {
 };
 
 // position of phase-sorted entry e; valid = false for padding lanes
-__device__ __forceinline__ bool entry_pos(const TileArgs& T, int64_t e, int64_t e_end, float& x, float& y, float& z,
+__device__ __forceinline__ bool entry_pos(const MfmaTileArgs& T, int64_t e, int64_t e_end, float& x, float& y, float& z,
                                           int& row) {
   row = -1;
   x = y = z = 0.0f;
   if (e >= e_end) return false;
   row = T.t_row[e];
   const int pt = T.row_point[row];
-  if (pt < 0 || pt >= T.npts) return false;
+  float px, py, pz;
+  if (!point_xyz(T.ps, pt, px, py, pz)) return false;
   int ox, oy, oz;
   dips_decode_offset(T.t_code[e], ox, oy, oz);
-  const int k = pt % T.nz, ij = pt / T.nz, j = ij % T.ny, i = ij / T.ny;
-  x = T.xs[i] + (float)ox * T.hx;
-  y = T.ys[j] + (float)oy * T.hy;
-  z = T.zs[k] + (float)oz * T.hz;
+  x = px + (float)ox * T.hx;
+  y = py + (float)oy * T.hy;
+  z = pz + (float)oz * T.hz;
   return true;
 }
 
 // ---- kernel 1: u for every entry of the batch ----------------------------------
 // tiles [0, ntp) hold + entries [e0, e0+np), tiles [ntp, ntp+ntm) hold - entries
 template <int Lm, int NBm, int Lp, int NBp>
-__global__ __launch_bounds__(kMT) void mfma_fwd_kernel(TileArgs T, const float* __restrict__ params, int Hm, int Hp,
+__global__ __launch_bounds__(kMT) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_out) {
   extern __shared__ float lds[];
   const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
@@ -411,7 +408,7 @@ __device__ void flush_grad(GradAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
 }
 
 template <int Lm, int NBm, int Lp, int NBp>
-__global__ __launch_bounds__(kMT) void mfma_bwd_kernel(TileArgs T, const float* __restrict__ params, int Hm, int Hp,
+__global__ __launch_bounds__(kMT) void mfma_bwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_, int64_t row_begin,
                                                        const float* __restrict__ r_row, float* __restrict__ gpart,
                                                        int nblk_p, int stride) {
@@ -632,8 +629,8 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("dips_mfma_train_step: float64 workspace too small");
     return DIPS_EINVAL;
   }
-  TileArgs T{g->xs, g->ys, g->zs, g->ny, g->nz, g->nx * g->ny * g->nz, rec->hx, rec->hy, rec->hz,
-             rec->row_point, bt->t_row, bt->t_code, bt->t_w, bt->t_csr};
+  MfmaTileArgs T{point_set(g), rec->hx, rec->hy, rec->hz, rec->row_point, bt->t_row, bt->t_code, bt->t_w,
+                 bt->t_csr};
   const int a = Lm, b = nb_of(Hm), c = Lp, d = nb_of(Hp);
   const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
   const int fwd_blocks = (int)((ntp + 3) / 4 + (ntm + 3) / 4);

@@ -1,25 +1,26 @@
 // Entry-parallel fused training step over packed wavefront tiles (K2 + K3 + K4).
 //
 // The rows of a batch are packed (on the host, once per zoom level) into tiles
-// of whole rows holding at most 64 entries and 64 rows; a row with more than 64
-// entries forms a tile of its own.  A wavefront takes one tile at a time and
-// gives each lane one entry:
-//   1. the lane evaluates the phase network of its entry at fl32(p + o*h) and
-//      keeps the hidden activations in registers;
-//   2. a segmented inclusive scan over the lanes (bounded by the first lane of
-//      each row, fixed summation tree) forms the row sums, so the last lane of
-//      a row holds r = c + sum_e w_e NN(pos_e) and adds 0.5 r^2 to the loss;
-//   3. every lane back-propagates r * w_e through its kept activations into
-//      per-lane gradient registers of both phase networks.
-// Tiles of one long row are processed one chunk of 64 entries at a time (the
-// forward pass is recomputed for the gradient).  The grid is persistent (at
-// most one workgroup of 8 wavefronts per CU, wavefronts stride over the tiles),
-// so the gradient registers are reduced once per wavefront: DPP wave sums, LDS
-// across the 8 wavefronts, one partial row per workgroup.  In the fused mode
-// the last workgroup to arrive reduces the partial rows (parameter-major, each
-// wavefront a set of parameters, lanes over workgroups with coalesced loads,
-// float64 in a fixed order) and applies the optimizer, so one launch is one
-// optimizer step and results are bitwise reproducible.
+// of whole rows holding at most 64 slots, one slot per entry and one for a row
+// without entries; a row with more than 64 entries forms a tile of its own.
+// Each tile is laid out once per level as 64 slots (SoA: position, weight, row
+// constant at the row's last slot, head/last lane and phase), so a wavefront
+// reads its tile with six coalesced loads and no dependent lookups:
+//   1. every lane evaluates the phase network of its slot at the stored
+//      position fl32(p + o*h) and keeps the hidden activations in registers;
+//   2. a segmented inclusive scan over the lanes (bounded by the row's first
+//      lane, fixed summation tree) forms r = c + sum_e w_e NN(pos_e) at the
+//      row's last lane, which adds 0.5 r^2 to the loss and broadcasts r;
+//   3. every lane back-propagates r * w_e through its kept activations.
+// Weight gradients of networks of width 8..12 accumulate on the matrix cores
+// (K = 64 contraction over the tile's slots), smaller networks per lane.
+// Tiles of one long row are processed one chunk of 64 entries at a time from
+// the CSR records (the forward pass is recomputed for the gradient).  The grid
+// is persistent (one workgroup per CU, wavefronts stride over the tiles), so
+// the gradients are reduced once per wavefront, then across the workgroup in
+// LDS into one partial row.  In the fused mode the last workgroup to arrive
+// reduces the partial rows in a fixed order (float64) and applies the
+// optimizer: one launch is one optimizer step, bitwise reproducible.
 //
 // Reference: loss = mean(optax.l2_loss(lhs, rhs)) over the batch
 // (jax_dips/solvers/poisson/trainer.py:892-912), value_and_grad and the optax
@@ -47,11 +48,8 @@ constexpr int kStageFloats = 2 * 16 * kLdsStride;     // A and B operand stacks 
   X(1, 1, 1, 1)             \
   X(2, 10, 2, 10)
 
-struct TileArgs {
-  const float* xs;
-  const float* ys;
-  const float* zs;
-  int ny, nz, npts;
+struct TileStepArgs {
+  PointSet ps;
   float hx, hy, hz;
   const int32_t* row_point;
   const int32_t* row_ptr;
@@ -61,10 +59,20 @@ struct TileArgs {
   const float* ent_w;
   int64_t nnz;
   const int32_t* tile_row;
-  const int32_t* ent_row;
+  const float* sx;  // slot arrays, 64 per tile boundary index
+  const float* sy;
+  const float* sz;
+  const float* sw;
+  const float* sc;
+  const uint32_t* sinfo;
   int64_t tile_begin;  // first boundary of the batch's tiles in tile_row
   int ntiles;
 };
+
+// slot_info bits: 0..5 first lane of the row, 6..11 last lane, 12 phase (+),
+// 13 slot in use
+constexpr uint32_t kSlotPlus = 1u << 12;
+constexpr uint32_t kSlotUsed = 1u << 13;
 
 __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -134,8 +142,17 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
   if (threadIdx.x == 0) {
     if (c >= 0 && c < fo.loss_cap) fo.loss_hist[c] = gsh[PT];
     *fo.count = c + 1;
-    *fo.sync = 0u;
+    if (fo.sync) *fo.sync = 0u;
   }
+}
+
+// Reduction of the parameter-major partial rows of a tile_step_kernel launch
+// (kOut = 1) and the optimizer, as a kernel of its own: the kernel boundary
+// orders the partial rows before their reads.
+template <int kStride>
+__global__ __launch_bounds__(1024) void tile_apply_kernel(const float* partial, int nblk, FusedOpt fo,
+                                                         float* params) {
+  tile_reduce_apply<kStride, 16>(partial, nblk, fo, params);
 }
 
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -235,11 +252,11 @@ struct NetGrad {
 
   // Gradient contribution of one chunk of entries; mine = this lane's entry
   // belongs to this network (act holds its activations).  Wave-uniform call.
-  __device__ __forceinline__ void add(const float* P_, float* AT, float* BT, int lane, bool mine, float x, float y,
+  __device__ __forceinline__ void add(dips_weight_t* P_, float* AT, float* BT, int lane, bool mine, float x, float y,
                                       float z, float seed, const float (&act)[L][H]) {
     // a fresh (opaque) weight pointer: the backward pass reloads the weights
-    // instead of keeping the forward pass's loads live in vector registers
-    const float* Q = P_;
+    // instead of keeping the forward pass's loads live in registers
+    dips_weight_t* Q = P_;
     asm volatile("" : "+s"(Q));
     if constexpr (kGrad && kMfma) {
       float dl[L][H] = {};
@@ -281,9 +298,12 @@ struct TileCfg {
   static constexpr size_t kLdsBytes = sizeof(float) * kStageFloats * kWaves;
 };
 
-template <int Lm, int Hm, int Lp, int Hp, bool kGrad, bool kFused>
+// kOut: 0 = block-major partial rows (for dips_opt_step), 1 = parameter-major
+// rows (for tile_apply_kernel), 2 = parameter-major rows and the fused
+// last-workgroup reduction + optimizer.
+template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut>
 __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step_kernel(
-    TileArgs a, float* params, float* __restrict__ partial, FusedOpt fo) {
+    TileStepArgs a, float* params, float* __restrict__ partial, FusedOpt fo) {
   constexpr int kTileThreads = TileCfg<Lm, Hm, Lp, Hp>::kThreads;
   constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp>::kWaves;
   using NP = NetLayout<Lp, Hp>;
@@ -310,72 +330,36 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
   for (int t = (int)blockIdx.x * kTileWaves + wave; t < a.ntiles; t += nw) {
     // weights are re-read per tile (scalar loads that hit the scalar cache)
     // rather than held in vector registers across the loop
-    const float* Pp = params;
-    asm volatile("" : "+s"(Pp));
-    const float* Pm = Pp + PP;
-    const int64_t tb = a.tile_begin + t;
-    const int64_t r0 = clamp64(a.tile_row[tb], 0, a.nrows);
-    const int64_t r1 = clamp64(a.tile_row[tb + 1], r0, a.nrows);
-    const int64_t e0 = clamp64(a.row_ptr[r0], 0, a.nnz);
-    const int64_t e1 = clamp64(a.row_ptr[r1], e0, a.nnz);
-    if (e1 - e0 <= kTileCap && r1 - r0 <= kTileCap) {
-      const int n = (int)(e1 - e0);
-      const int nr = (int)(r1 - r0);
-      // rows without entries contribute their constant only
-      if (lane < nr) {
-        const int64_t row = r0 + lane;
-        const int pt = a.row_point[row];
-        if (a.row_ptr[row + 1] <= a.row_ptr[row] && pt >= 0 && pt < a.npts) {
-          const float c = a.row_c[row];
-          lsum = fmaf(0.5f * c, c, lsum);
-        }
-      }
-      const bool active = lane < n;
-      float v = 0.0f, w = 0.0f, c = 0.0f, x = 0.0f, y = 0.0f, z = 0.0f;
-      int head = lane, last = lane;
-      bool valid = false, plus = false;
-      float actp[Lp][Hp] = {};
-      float actm[Lm][Hm] = {};
-      if (active) {
-        const int64_t e = e0 + lane;
-        const int64_t row = clamp64(a.ent_row[e], r0, r1 - 1);
-        head = (int)clamp64((int64_t)a.row_ptr[row] - e0, 0, lane);
-        last = (int)clamp64((int64_t)a.row_ptr[row + 1] - 1 - e0, lane, n - 1);
-        const int pt = a.row_point[row];
-        valid = pt >= 0 && pt < a.npts;
-        if (valid) {
-          const int k = pt % a.nz;
-          const int ij = pt / a.nz;
-          const int j = ij % a.ny;
-          const int i = ij / a.ny;
-          const uint32_t code = a.ent_code[e];
-          w = a.ent_w[e];
-          c = a.row_c[row];
-          int ox, oy, oz;
-          dips_decode_offset(code, ox, oy, oz);
-          x = a.xs[i] + (float)ox * a.hx;
-          y = a.ys[j] + (float)oy * a.hy;
-          z = a.zs[k] + (float)oz * a.hz;
-          plus = (code & 128u) != 0;
-          const float u = plus ? mlp_forward_keep<Lp, Hp>(Pp, x, y, z, actp)
-                               : mlp_forward_keep<Lm, Hm>(Pm, x, y, z, actm);
-          v = w * u;
-        }
-      }
-      // segmented inclusive scan: lane l ends with the sum of lanes head(l)..l
-#pragma unroll
-      for (int d = 1; d < DIPS_WAVE; d <<= 1) {
-        const float o = __shfl_up(v, d, DIPS_WAVE);
-        if (lane - d >= head) v += o;
-      }
-      const float tot = __shfl(v, last, DIPS_WAVE);
-      const bool on = active && valid;
-      const float r = tot + c;
-      if (on && lane == last) lsum = fmaf(0.5f * r, r, lsum);
-      const float seed = r * w;
-      gp.add(Pp, AT, BT, lane, on && plus, x, y, z, seed, actp);
-      gm.add(Pm, AT, BT, lane, on && !plus, x, y, z, seed, actm);
+    const float* Praw = params;
+    asm volatile("" : "+s"(Praw));
+    dips_weight_t* Pp = dips_weights(Praw);
+    dips_weight_t* Pm = Pp + PP;
+    const int64_t slot = (a.tile_begin + t) * kTileCap + lane;
+    const uint32_t info = a.sinfo[slot];
+    const float x = a.sx[slot], y = a.sy[slot], z = a.sz[slot];
+    const float w = a.sw[slot], c = a.sc[slot];
+    const bool on = (info & kSlotUsed) != 0;
+    const bool plus = (info & kSlotPlus) != 0;
+    const int head = on ? (int)(info & 63u) : lane;
+    const int last = on ? (int)((info >> 6) & 63u) : lane;
+    float actp[Lp][Hp] = {};
+    float actm[Lm][Hm] = {};
+    float v = 0.0f;
+    if (on) {
+      const float u = plus ? mlp_forward_keep<Lp, Hp>(Pp, x, y, z, actp) : mlp_forward_keep<Lm, Hm>(Pm, x, y, z, actm);
+      v = fmaf(w, u, c);  // c is nonzero only in the row's last slot
     }
+    // segmented inclusive scan: lane l ends with the sum of lanes head(l)..l
+#pragma unroll
+    for (int d = 1; d < DIPS_WAVE; d <<= 1) {
+      const float o = __shfl_up(v, d, DIPS_WAVE);
+      if (lane - d >= head) v += o;
+    }
+    const float r = __shfl(v, last, DIPS_WAVE);
+    if (on && lane == last) lsum = fmaf(0.5f * r, r, lsum);
+    const float seed = r * w;
+    gp.add(Pp, AT, BT, lane, on && plus, x, y, z, seed, actp);
+    gm.add(Pm, AT, BT, lane, on && !plus, x, y, z, seed, actm);
   }
   // long rows (tiles skipped above): one wavefront per row, 64 entries per
   // chunk; the gradient pass recomputes the forward activations.  A loop of
@@ -385,26 +369,26 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
     const int64_t tb = a.tile_begin + t;
     const int64_t r0 = clamp64(a.tile_row[tb], 0, a.nrows);
     const int64_t r1 = clamp64(a.tile_row[tb + 1], r0, a.nrows);
-    const int64_t e0 = clamp64(a.row_ptr[r0], 0, a.nnz);
-    const int64_t e1 = clamp64(a.row_ptr[r1], e0, a.nnz);
-    if (e1 - e0 <= kTileCap && r1 - r0 <= kTileCap) continue;
+    // the tiles without slots: more than 64 rows or slots (tile_slots_kernel)
+    if (r1 - r0 <= kTileCap) {
+      int len = 0;
+      if (lane < r1 - r0) len = max(a.row_ptr[r0 + lane + 1] - a.row_ptr[r0 + lane], 1);
+      if (dips_wave_sum_i(len) <= kTileCap) continue;
+    }
     {
 #pragma unroll 1
       for (int64_t row = r0; row < r1; ++row) {
         const int pt = a.row_point[row];
-        if (pt < 0 || pt >= a.npts) continue;
-        const int k = pt % a.nz;
-        const int ij = pt / a.nz;
-        const int j = ij % a.ny;
-        const int i = ij / a.ny;
-        const float px = a.xs[i], py = a.ys[j], pz = a.zs[k];
+        float px, py, pz;
+        if (!point_xyz(a.ps, pt, px, py, pz)) continue;
         const int64_t beg = clamp64(a.row_ptr[row], 0, a.nnz);
         const int64_t end = clamp64(a.row_ptr[row + 1], beg, a.nnz);
         float acc = 0.0f;
 #pragma unroll 1
         for (int64_t e = beg + lane; e < end; e += DIPS_WAVE) {
-          const float* Wp = params;  // opaque per chunk: no weights hoisted into registers
-          asm volatile("" : "+s"(Wp));
+          const float* Wraw = params;  // opaque per chunk: no weights hoisted into registers
+          asm volatile("" : "+s"(Wraw));
+          dips_weight_t* Wp = dips_weights(Wraw);
           const uint32_t code = a.ent_code[e];
           int ox, oy, oz;
           dips_decode_offset(code, ox, oy, oz);
@@ -419,8 +403,9 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
         if (!kGrad) continue;
 #pragma unroll 1
         for (int64_t base = beg; base < end; base += DIPS_WAVE) {
-          const float* Wp = params;
-          asm volatile("" : "+s"(Wp));
+          const float* Wraw = params;
+          asm volatile("" : "+s"(Wraw));
+          dips_weight_t* Wp = dips_weights(Wraw);
           const int64_t e = base + lane;
           const bool on = e < end;
           bool plus = false;
@@ -467,12 +452,12 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
 #pragma unroll
       for (int w = 0; w < kTileWaves; ++w) s += red[w * stride + p];
     }
-    if (kFused)
-      partial[(int64_t)p * nblk + blockIdx.x] = s;
-    else
+    if (kOut == 0)
       partial[(int64_t)blockIdx.x * stride + p] = s;
+    else
+      partial[(int64_t)p * nblk + blockIdx.x] = s;
   }
-  if (!kFused) return;
+  if (kOut != 2) return;
 
   // One thread per workgroup fences and counts: the barrier completes the
   // workgroup's partial stores, the agent-scope release makes them visible to
@@ -508,7 +493,7 @@ int tile_supported(const dips_net_t* n) {
 int64_t tile_partial_size(const dips_net_t* net) { return (int64_t)kMaxTileBlocks * partial_stride(net); }
 
 template <int Lm, int Hm, int Lp, int Hp>
-static void tile_launch_shape(const TileArgs& a, float* params, float* partial, int mode, const FusedOpt& fo,
+static void tile_launch_shape(const TileStepArgs& a, float* params, float* partial, int mode, const FusedOpt& fo,
                               int nblk, hipStream_t st) {
   dim3 grid(nblk), block(TileCfg<Lm, Hm, Lp, Hp>::kThreads);
   const size_t lds = TileCfg<Lm, Hm, Lp, Hp>::kLdsBytes;
@@ -520,11 +505,15 @@ static void tile_launch_shape(const TileArgs& a, float* params, float* partial, 
   (void)lds_set_##G##F;                                                                                          \
   hipLaunchKernelGGL((tile_step_kernel<Lm, Hm, Lp, Hp, G, F>), grid, block, lds, st, a, params, partial, fo)
   if (mode == 2) {
-    DIPS_TILE_LAUNCH(true, true);
+    DIPS_TILE_LAUNCH(true, 2);
+  } else if (mode == 3) {
+    DIPS_TILE_LAUNCH(true, 1);
+    constexpr int kStride = NetLayout<Lp, Hp>::size() + NetLayout<Lm, Hm>::size() + 1;
+    hipLaunchKernelGGL((tile_apply_kernel<kStride>), dim3(1), dim3(1024), 0, st, partial, nblk, fo, params);
   } else if (mode == 1) {
-    DIPS_TILE_LAUNCH(true, false);
+    DIPS_TILE_LAUNCH(true, 0);
   } else {
-    DIPS_TILE_LAUNCH(false, false);
+    DIPS_TILE_LAUNCH(false, 0);
   }
 #undef DIPS_TILE_LAUNCH
 }
@@ -532,7 +521,8 @@ static void tile_launch_shape(const TileArgs& a, float* params, float* partial, 
 int tile_step_launch(const dips_grid_t* g, const dips_records_t* rec, const dips_tiles_t* tiles, int64_t tile_begin,
                      int64_t ntiles, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
                      int want_grad, int* nblk_out, hipStream_t stream, const FusedOpt* fused) {
-  if (!tiles || !tiles->tile_row || (rec->nnz > 0 && !tiles->ent_row)) {
+  if (!tiles || !tiles->tile_row || !tiles->slot_x || !tiles->slot_y || !tiles->slot_z || !tiles->slot_w ||
+      !tiles->slot_c || !tiles->slot_info) {
     set_error("tile step: invalid tiles");
     return DIPS_EINVAL;
   }
@@ -540,17 +530,20 @@ int tile_step_launch(const dips_grid_t* g, const dips_records_t* rec, const dips
     set_error("tile step: tile range outside the tile table");
     return DIPS_EINVAL;
   }
-  if (rec->nrows >= INT_MAX || (int64_t)g->nx * g->ny * g->nz >= INT_MAX) {
+  if (rec->nrows >= INT_MAX || point_set(g).ngrid + point_set(g).nextra >= INT_MAX) {
     set_error("tile step: grid or records too large for 32-bit point indices");
     return DIPS_EINVAL;
   }
-  TileArgs a{g->xs,         g->ys,          g->zs,       g->ny,          g->nz,       g->nx * g->ny * g->nz,
-             rec->hx,       rec->hy,        rec->hz,     rec->row_point, rec->row_ptr, rec->row_c,
-             rec->nrows,    rec->ent_code,  rec->ent_w,  rec->nnz,       tiles->tile_row, tiles->ent_row,
-             tile_begin,    (int)ntiles};
+  TileStepArgs a{point_set(g),
+             rec->hx,        rec->hy,        rec->hz,        rec->row_point, rec->row_ptr,  rec->row_c,
+             rec->nrows,     rec->ent_code,  rec->ent_w,     rec->nnz,       tiles->tile_row, tiles->slot_x,
+             tiles->slot_y,  tiles->slot_z,  tiles->slot_w,  tiles->slot_c,  tiles->slot_info, tile_begin,
+             (int)ntiles};
   FusedOpt fo{};
   if (fused) fo = *fused;
-  const int mode = fused ? 2 : (want_grad ? 1 : 0);
+  // with an optimizer: one fused launch when an arrival counter is given,
+  // else the step kernel followed by tile_apply_kernel
+  const int mode = fused ? (fused->sync ? 2 : 3) : (want_grad ? 1 : 0);
 #define X(A, B, C, D)                                                                                   \
   if (net->Lm == A && net->Hm == B && net->Lp == C && net->Hp == D) {                                  \
     const int nblk = tile_blocks(ntiles, TileCfg<A, B, C, D>::kWaves);                                 \
@@ -577,25 +570,113 @@ int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_
   return tile_step_launch(g, rec, tiles, tile_begin, ntiles, net, params, partial, partial_cap, 1, &nb, stream, &fo);
 }
 
-// ent_row[e] = row of entry e (one thread per row)
-__global__ void entry_rows_kernel(const int32_t* __restrict__ row_ptr, int64_t nrows, int64_t nnz,
-                                  int32_t* __restrict__ ent_row) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= nrows) return;
-  const int64_t beg = clamp64(row_ptr[row], 0, nnz);
-  const int64_t end = clamp64(row_ptr[row + 1], beg, nnz);
-  for (int64_t e = beg; e < end; ++e) ent_row[e] = (int32_t)row;
+// Slot layout of every tile (once per zoom level): one wavefront per tile
+// boundary index k, tile k = rows [tile_row[k], tile_row[k+1]).  Lane s of
+// the tile fills slot s: its row j (rows take max(entries, 1) consecutive
+// slots), the entry's position fl32(p + o*h), weight, phase, the row's first
+// and last lane, and the row constant in the row's last slot.  Tiles with more
+// than 64 rows or slots (single long rows) and the empty groups between
+// batches get unused slots.
+constexpr int kSlotWaves = 4;
+
+struct SlotArgs {
+  PointSet ps;
+  float hx, hy, hz;
+  const int32_t* row_point;
+  const int32_t* row_ptr;
+  const float* row_c;
+  int64_t nrows;
+  const uint32_t* ent_code;
+  const float* ent_w;
+  int64_t nnz;
+  const int32_t* tile_row;
+  int64_t ngroups;  // nbounds (the last boundary starts no tile)
+  float* sx;
+  float* sy;
+  float* sz;
+  float* sw;
+  float* sc;
+  uint32_t* sinfo;
+};
+
+__global__ __launch_bounds__(kSlotWaves * DIPS_WAVE) void tile_slots_kernel(SlotArgs a) {
+  __shared__ int rowof[kSlotWaves][kTileCap];
+  const int lane = threadIdx.x & (DIPS_WAVE - 1);
+  const int wave = threadIdx.x / DIPS_WAVE;
+  const int64_t k = (int64_t)blockIdx.x * kSlotWaves + wave;
+  if (k >= a.ngroups) return;  // whole wavefront
+  const int64_t r0 = clamp64(a.tile_row[k], 0, a.nrows);
+  const int64_t r1 = (k + 1 < a.ngroups) ? clamp64(a.tile_row[k + 1], r0, a.nrows) : r0;
+  const int64_t nr = r1 - r0;
+  int len = 0, tl = 0;
+  if (nr <= kTileCap && lane < nr) {
+    tl = max(a.row_ptr[r0 + lane + 1] - a.row_ptr[r0 + lane], 0);
+    len = max(tl, 1);
+  }
+  // inclusive scan of the slot counts over the lanes
+  int incl = len;
+#pragma unroll
+  for (int d = 1; d < DIPS_WAVE; d <<= 1) {
+    const int o = __shfl_up(incl, d, DIPS_WAVE);
+    if (lane >= d) incl += o;
+  }
+  const int start = incl - len;
+  const int total = __shfl(incl, DIPS_WAVE - 1, DIPS_WAVE);
+  const bool ok = nr <= kTileCap && total <= kTileCap;
+  rowof[wave][lane] = -1;
+  __builtin_amdgcn_wave_barrier();
+  if (ok && lane < nr)
+    for (int q = 0; q < len; ++q) rowof[wave][start + q] = lane;
+  __builtin_amdgcn_wave_barrier();
+  const int j = ok ? rowof[wave][lane] : -1;
+  const int jj = j < 0 ? 0 : j;
+  const int sj = __shfl(start, jj, DIPS_WAVE);
+  const int lj = __shfl(len, jj, DIPS_WAVE);
+  const int tj = __shfl(tl, jj, DIPS_WAVE);
+  uint32_t info = 0u;
+  float x = 0.0f, y = 0.0f, z = 0.0f, w = 0.0f, c = 0.0f;
+  if (j >= 0) {
+    const int64_t row = r0 + j;
+    const int pt = a.row_point[row];
+    if (point_xyz(a.ps, pt, x, y, z)) {
+      const int head = sj, last = sj + lj - 1;
+      info = kSlotUsed | (uint32_t)head | ((uint32_t)last << 6);
+      if (tj > 0) {
+        const int64_t e = clamp64((int64_t)a.row_ptr[row] + (lane - sj), 0, a.nnz - 1);
+        const uint32_t code = a.ent_code[e];
+        int ox, oy, oz;
+        dips_decode_offset(code, ox, oy, oz);
+        x = x + (float)ox * a.hx;
+        y = y + (float)oy * a.hy;
+        z = z + (float)oz * a.hz;
+        w = a.ent_w[e];
+        if (code & 128u) info |= kSlotPlus;
+      }
+      if (lane == last) c = a.row_c[row];
+    }
+  }
+  const int64_t s = k * kTileCap + lane;
+  a.sx[s] = x;
+  a.sy[s] = y;
+  a.sz[s] = z;
+  a.sw[s] = w;
+  a.sc[s] = c;
+  a.sinfo[s] = info;
 }
 
-int entry_rows_launch(const dips_records_t* rec, int32_t* ent_row, hipStream_t st) {
-  if (rec->nrows >= INT_MAX) {
-    set_error("dips_entry_rows: too many rows for 32-bit row indices");
+int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
+                      float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, hipStream_t st) {
+  if (rec->nrows >= INT_MAX || point_set(g).ngrid + point_set(g).nextra >= INT_MAX) {
+    set_error("dips_tile_slots: grid or records too large for 32-bit indices");
     return DIPS_EINVAL;
   }
-  const int64_t nb = (rec->nrows + 255) / 256;
-  hipLaunchKernelGGL(entry_rows_kernel, dim3((unsigned)nb), dim3(256), 0, st, rec->row_ptr, rec->nrows, rec->nnz,
-                     ent_row);
-  return hip_status(hipGetLastError(), "entry_rows_kernel launch");
+  SlotArgs a{point_set(g),
+             rec->hx,    rec->hy,       rec->hz,    rec->row_point, rec->row_ptr, rec->row_c,
+             rec->nrows, rec->ent_code, rec->ent_w, rec->nnz, tile_row, nbounds,
+             sx,         sy,            sz,         sw,       sc,       sinfo};
+  const int64_t nb = (nbounds + kSlotWaves - 1) / kSlotWaves;
+  hipLaunchKernelGGL(tile_slots_kernel, dim3((unsigned)nb), dim3(kSlotWaves * DIPS_WAVE), 0, st, a);
+  return hip_status(hipGetLastError(), "tile_slots_kernel launch");
 }
 
 // Host: pack the rows of each batch into tiles (see the file comment).
@@ -644,8 +725,9 @@ int64_t pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_r
           set_error("dips_pack_tiles: row offsets decrease");
           return -1;
         }
-        if (len > kTileCap || n + len > kTileCap) break;
-        n += len;
+        const int64_t slots = len > 0 ? len : 1;  // a row without entries takes one slot
+        if (len > kTileCap || n + slots > kTileCap) break;
+        n += slots;
         ++r;
         ++rows;
       }

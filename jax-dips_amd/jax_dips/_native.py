@@ -20,7 +20,8 @@ class DipsError(RuntimeError):
 
 class Grid(ctypes.Structure):
     _fields_ = [("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("zs", ctypes.c_void_p),
-                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32)]
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("extra", ctypes.c_void_p), ("nextra", ctypes.c_int64)]
 
 
 class Records(ctypes.Structure):
@@ -46,7 +47,9 @@ class MfmaBatch(ctypes.Structure):
 
 
 class Tiles(ctypes.Structure):
-    _fields_ = [("tile_row", ctypes.c_void_p), ("ent_row", ctypes.c_void_p), ("nbounds", ctypes.c_int64)]
+    _fields_ = [("tile_row", ctypes.c_void_p), ("nbounds", ctypes.c_int64), ("slot_x", ctypes.c_void_p),
+                ("slot_y", ctypes.c_void_p), ("slot_z", ctypes.c_void_p), ("slot_w", ctypes.c_void_p),
+                ("slot_c", ctypes.c_void_p), ("slot_info", ctypes.c_void_p)]
 
 
 class Assemble(ctypes.Structure):
@@ -63,7 +66,7 @@ class Assemble(ctypes.Structure):
                 ("scratch", ctypes.c_void_p), ("ip_slot", ctypes.c_void_p), ("mode", ctypes.c_int32),
                 ("nnz", ctypes.c_void_p), ("ent_off", ctypes.c_void_p), ("row_of", ctypes.c_void_p),
                 ("ent_code", ctypes.c_void_p), ("ent_w", ctypes.c_void_p), ("row_c", ctypes.c_void_p),
-                ("npts", ctypes.c_int64)]
+                ("npts", ctypes.c_int64), ("extra", ctypes.c_void_p), ("nextra", ctypes.c_int64)]
 
 
 # name -> (restype, argtypes); must match include/dips_hip.h
@@ -90,7 +93,8 @@ SIGNATURES = {
                                        _P]),
     "dips_eval": (ctypes.c_int, [ctypes.POINTER(Net), _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
     "dips_pack_tiles": (_I64, [_P, _I64, _P, _P, _I32, _P, _I64, _P]),
-    "dips_entry_rows": (ctypes.c_int, [ctypes.POINTER(Records), _P, _P]),
+    "dips_tile_slots": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _I64, _P, _P, _P, _P, _P,
+                                       _P, _P]),
     "dips_tile_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
     "dips_tile_partial_size": (_I64, [ctypes.POINTER(Net)]),
     "dips_tile_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(Tiles), _I64,

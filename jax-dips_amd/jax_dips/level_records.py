@@ -69,7 +69,7 @@ def _offsets(vals, device):
 def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
     """DeviceRecords of one zoom level.
 
-    grid: records.DeviceGrid; h: (3,) float32 stencil spacing; bounds: level-set
+    grid: records.DeviceGrid (grid nodes and extra points, one row each); h: (3,) float32 stencil spacing; bounds: level-set
     grid bounds; cbs: dict of callbacks (phi, mu_m, mu_p, k_m, k_p, f_m, f_p,
     alpha, beta, g), coordinate-first."""
     import torch
@@ -79,10 +79,8 @@ def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
     h = np.asarray(h, np.float32)
     ht = torch.tensor(h, dtype=torch.float32, device=device)
     nx, ny, nz = grid.shape
-    npts = nx * ny * nz
-    X, Y, Z = torch.meshgrid(grid.xs, grid.ys, grid.zs, indexing="ij")
-    P = torch.stack([X.reshape(-1), Y.reshape(-1), Z.reshape(-1)], dim=1)  # (N,3) f32
-    del X, Y, Z
+    npts = grid.npts  # grid nodes, then the extra (batch padding) points
+    P = grid.points()  # (N,3) f32
 
     # stencil points and the corners of their cells (float32, reference order of operations)
     S = P[:, None, :] + _offsets(STENCIL, device)[None] * ht  # (N,7,3)
@@ -173,6 +171,7 @@ def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
     A.cs_a, A.cs_k, A.cs_sel, A.cs_delta = cs_a.data_ptr(), cs_k.data_ptr(), cs_sel.data_ptr(), cs_delta.data_ptr()
     A.cs_vals, A.phi_cube, A.scratch, A.ip_slot = cs_vals.data_ptr(), phi_cube.data_ptr(), scratch.data_ptr(), ip_slot.data_ptr()
     A.npts = npts
+    A.extra, A.nextra = (grid.extra.data_ptr() if grid.extra is not None else None), grid.nextra
     A.mode = 0
     A.nnz = nnz.data_ptr()
     N.check(L.dips_geom_assemble(ctypes.byref(A), st), "dips_geom_assemble(count)")

@@ -92,14 +92,32 @@ def pack_tiles(row_ptr, nrows, batch_begin, batch_rows):
 
 
 class DeviceGrid:
-    def __init__(self, xs, ys, zs, device="cuda"):
+    """Training points on the device: the nodes of the float32 axes ('ij' C
+    order) followed by optional extra points (nextra, 3), the random points that
+    pad a short last batch."""
+
+    def __init__(self, xs, ys, zs, extra=None, device="cuda"):
         import torch
 
         self.xs = torch.as_tensor(np.asarray(xs, np.float32), device=device)
         self.ys = torch.as_tensor(np.asarray(ys, np.float32), device=device)
         self.zs = torch.as_tensor(np.asarray(zs, np.float32), device=device)
         self.shape = (len(xs), len(ys), len(zs))
-        self.c = N.Grid(self.xs.data_ptr(), self.ys.data_ptr(), self.zs.data_ptr(), *self.shape)
+        self.ngrid = len(xs) * len(ys) * len(zs)
+        ex = np.zeros((0, 3), np.float32) if extra is None else np.asarray(extra, np.float32).reshape(-1, 3)
+        self.extra = torch.as_tensor(np.ascontiguousarray(ex), device=device) if len(ex) else None
+        self.nextra = int(len(ex))
+        self.npts = self.ngrid + self.nextra
+        self.c = N.Grid(self.xs.data_ptr(), self.ys.data_ptr(), self.zs.data_ptr(), *self.shape,
+                        self.extra.data_ptr() if self.extra is not None else None, self.nextra)
+
+    def points(self):
+        """(npts, 3) float32 device tensor of every training point."""
+        import torch
+
+        X, Y, Z = torch.meshgrid(self.xs, self.ys, self.zs, indexing="ij")
+        P = torch.stack([X.reshape(-1), Y.reshape(-1), Z.reshape(-1)], dim=1)
+        return P if self.extra is None else torch.cat([P, self.extra], dim=0)
 
 
 class DeviceRecords:
@@ -182,22 +200,27 @@ class DeviceRecords:
                                           zip(self.batch_begin, self.batch_rows))))
         return self._mfma
 
-    def tiles(self):
+    def tiles(self, grid):
         """Wavefront tiles of every batch (rows packed into tiles of at most 64
-        entries, dips_pack_tiles) and the entry -> row map (dips_entry_rows),
-        built once per level and cached."""
+        slots, dips_pack_tiles) and their slot layout (dips_tile_slots), built
+        once per level and cached."""
         import torch
 
         if getattr(self, "_tiles", None) is not None:
             return self._tiles
         L = N.lib()
-        ent_row = torch.empty(max(1, self.nnz), dtype=torch.int32, device=self.row_ptr.device)
-        N.check(L.dips_entry_rows(ctypes.byref(self.c), N.ptr(ent_row), N.current_stream()), "dips_entry_rows")
         ptr_h = np.ascontiguousarray(self.row_ptr.cpu().numpy(), dtype=np.int32)
         tile_row, btb = pack_tiles(ptr_h, self.nrows, self.batch_begin, self.batch_rows)
-        tr = torch.as_tensor(tile_row, device=self.row_ptr.device)
-        self._tiles = dict(ent_row=ent_row, tile_row=tr, batch_tile_begin=btb,
-                           c=N.Tiles(tr.data_ptr(), ent_row.data_ptr(), len(tile_row)))
+        dev = self.row_ptr.device
+        tr = torch.as_tensor(tile_row, device=dev)
+        nb = len(tile_row)
+        slots = {k: torch.empty(nb * 64, dtype=torch.float32, device=dev) for k in ("x", "y", "z", "w", "c")}
+        slots["info"] = torch.empty(nb * 64, dtype=torch.int32, device=dev)
+        N.check(L.dips_tile_slots(ctypes.byref(grid.c), ctypes.byref(self.c), N.ptr(tr), nb,
+                                  *(N.ptr(slots[k]) for k in ("x", "y", "z", "w", "c", "info")), N.current_stream()),
+                "dips_tile_slots")
+        c = N.Tiles(tr.data_ptr(), nb, *(slots[k].data_ptr() for k in ("x", "y", "z", "w", "c", "info")))
+        self._tiles = dict(tile_row=tr, batch_tile_begin=btb, slots=slots, c=c)
         return self._tiles
 
     def to_dense(self):
@@ -266,6 +289,9 @@ class Engine:
         self.sync = torch.zeros(1, dtype=torch.int32, device=device)  # arrival counter of the fused step
         self.fused = True
         self.use_tiles = True  # packed-tile kernel when the shape has one, else the per-row kernel
+        # tile steps as two launches (step kernel, then a one-workgroup reduction + optimizer kernel),
+        # which is faster than the single launch whose last workgroup applies the optimizer
+        self.tile_single_launch = False
         self.opt = make_opt(optimizer_dict)
 
     def loss_grad(self, grid, rec, row_begin, nrows, inv_batch, want_grad=True, nbulk=None):
@@ -290,7 +316,7 @@ class Engine:
         """(loss, grad) of batch b through the tile kernel, float64 numpy."""
         import torch
 
-        tl = rec.tiles()
+        tl = rec.tiles(grid)
         tb = int(tl["batch_tile_begin"][b])
         nt = int(tl["batch_tile_begin"][b + 1]) - tb - 1
         nblk = ctypes.c_int32(0)
@@ -304,13 +330,14 @@ class Engine:
         return s[self.P], s[: self.P]
 
     def train_level_tiles(self, grid, rec, nepochs):
-        tl = rec.tiles()
+        tl = rec.tiles(grid)
         nb = len(rec.batch_begin)
         st = N.lib().dips_tile_train_level(
             ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(tl["c"]), N.i64_array(tl["batch_tile_begin"]), nb,
             rec.batch_size, int(nepochs), ctypes.byref(self.net), ctypes.byref(self.opt), N.ptr(self.params),
             N.ptr(self.m), N.ptr(self.v), N.ptr(self.count), N.ptr(self.loss_hist), int(self.loss_hist.numel()),
-            N.ptr(self.partial), self.partial_cap, N.ptr(self.sync) if self.fused else None, N.current_stream())
+            N.ptr(self.partial), self.partial_cap, N.ptr(self.sync) if self.tile_single_launch else None,
+            N.current_stream())
         N.check(st, "dips_tile_train_level")
 
     def mfma_workspace(self, rec):

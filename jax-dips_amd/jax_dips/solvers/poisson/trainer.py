@@ -14,15 +14,17 @@ What runs where (Trainer.solve):
     the FP64 geometry kernels build the affine residual records of every
     training point once (level_records.build_level);
   * each epoch walks the contiguous batches (no shuffling, trainer.py:539): one
-    fused residual/loss/gradient kernel and one fused optimizer kernel per
-    batch (dips_train_level), losses kept on the device;
+    packed-tile residual/loss/gradient kernel and one reduction + optimizer
+    kernel per batch (dips_tile_train_level; dips_train_level or the MFMA
+    kernels for the other network shapes), losses kept on the device;
   * loss_epochs[e] = mean of the pre-update batch losses of epoch e
     (trainer.py:553-554);
   * the solution, grad u and d_n u on eval_gstate come from dips_eval
     (trainer.py:960-977).
+A short last batch is padded with random points in the data's range, as the
+reference pads it (its draws come from jax's PRNGKey(0), these from numpy).
 Unsupported reference options raise NotImplementedError: algorithm 1, lbfgs,
-the learned preconditioner, nonlinear operators, non-tanh activations and
-padded last batches (batch_size not dividing the number of training points).
+the learned preconditioner, nonlinear operators and non-tanh activations.
 """
 import logging
 import os
@@ -34,6 +36,7 @@ import numpy as np
 
 from ... import _native
 from ...data import data_management
+from ...data.data_management import generate_random_points_like
 from ...level_records import build_level, sample
 from ...nn.configure import get_model
 from ...records import DeviceGrid, Engine
@@ -159,11 +162,12 @@ class Trainer:
         tr = self.tr_gstate
         n = len(tr.R)
         B = min(self.batch_size, n)
-        if n % B:
-            raise NotImplementedError("batch_size must divide the number of training points "
-                                      "(padded last batches are not supported on the device)")
-        nb = n // B
-        grid = DeviceGrid(tr.x, tr.y, tr.z)
+        nb = -(-n // B)
+        # a short last batch is padded with random points in the data's range
+        # (DatasetDict.gpu_padded_batches, data_management.py:103-173); they are
+        # training points like the others and sit after the grid nodes
+        extra = generate_random_points_like(np.asarray(tr.R), nb * B - n) if nb * B > n else None
+        grid = DeviceGrid(tr.x, tr.y, tr.z, extra=extra)
         lv = self.lvl_gstate
         bounds = (lv.xmin(), lv.xmax(), lv.ymin(), lv.ymax(), lv.zmin(), lv.zmax())
         E = self.num_epochs

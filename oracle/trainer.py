@@ -100,11 +100,16 @@ def loss_and_grad(theta, dims, rec, rows, dt=np.float64):
 def train(P, h_tr, bounds, cb, dims, theta0, num_epochs, batch_size, opt_cfg,
           dt=np.float64, param_dt=np.float32, record_dt=np.float64, records=None):
     """Epoch loop of single_GPU_train with the zoom schedule.  Returns
-    (theta, loss_epochs, per-step losses)."""
+    (theta, loss_epochs, per-step losses).  A short last batch is padded with
+    padding_points (DatasetDict.gpu_padded_batches, data_management.py:103-173),
+    which become training points after the grid points."""
     n = P.shape[0]
     bl = batches(n, batch_size)
-    if any((b < 0).any() for b in bl):
-        raise NotImplementedError("padded batches are exercised separately")
+    npad = int(sum((b < 0).sum() for b in bl))
+    if npad:
+        base = P
+        P = np.concatenate([P, padding_points(base, npad)]).astype(np.float32)
+        bl = [np.arange(k * len(bl[0]), (k + 1) * len(bl[0])) for k in range(len(bl))]
     theta = np.asarray(theta0, dtype=param_dt).copy()
     state = O.OptState(theta.size, param_dt)
     recs = {} if records is None else records

@@ -81,3 +81,49 @@ def test_loss_on_device_records_matches_oracle():
         ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, orec, np.arange(b0, b0 + nr))
         assert abs(loss - ol) <= 1e-5 * abs(ol)
         assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,level", [("sphere", 0), ("star", 2)])
+def test_padded_batch_points(name, level):
+    """B not dividing N: the short last batch is padded with random points in
+    the data's range (data_management.py:70-76, :103-173).  Their records come
+    from the same kernels at off-grid positions and equal the oracle's records
+    of those positions; the padded batch's loss and gradient through the tile
+    kernel equal the oracle's."""
+    from jax_dips.data.data_management import generate_random_points_like
+    from jax_dips.level_records import build_level
+    from jax_dips.records import DeviceGrid, Engine
+    from oracle import discretization as D
+    from oracle import mlp as M
+    from oracle import trainer as T
+
+    prob = make(name)
+    n = 16 if name == "sphere" else 18
+    axes, P, h, bounds = grid_points(n)
+    B = 1000
+    nb = -(-len(P) // B)
+    extra = generate_random_points_like(P, nb * B - len(P))
+    assert len(extra) > 0 and extra.min() >= P.min() and extra.max() <= P.max()
+    grid = DeviceGrid(*axes, extra=extra)
+    hl = T.level_spacing(h, level)
+    drec = build_level(grid, hl, bounds, problems.oracle_callbacks(prob), B)
+    assert drec.nrows == nb * B and list(drec.batch_rows) == [B] * nb
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    P_all = np.concatenate([P, extra]).astype(np.float32)
+    orec = T.LevelRecords(P_all, hl, bounds, cb, np.float64)
+    W, C = drec.to_dense()
+    scale = np.maximum(np.abs(orec.W).max(axis=1), 1.0)
+    err = np.abs(W - orec.W).max(axis=1) / scale
+    cerr = np.abs(C - orec.c) / np.maximum(np.abs(orec.c), scale)
+    print(name, level, "padding", len(extra), "max w err", err.max(), "(padding rows", err[len(P):].max(), ")",
+          "max c err", cerr.max())
+    assert err.max() <= 1e-6 and cerr.max() <= 1e-6
+    dims = (1, 1, 2, 10)
+    theta = M.init_params(dims, seed=23)
+    eng = Engine(dims, theta, {}, max_rows=B, max_steps=2)
+    b = nb - 1
+    loss, grad = eng.tile_loss_grad(grid, drec, b, 1.0 / B)
+    ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, orec, np.arange(b * B, (b + 1) * B))
+    assert abs(loss - ol) <= 1e-5 * abs(ol)
+    assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)

@@ -88,6 +88,7 @@ def test_train_level_matches_oracle_steps():
     grid, drec, packed = _device(axes, rec, B)
     cfg = {"optimizer_name": "custom", "learning_rate": 1e-2, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
     eng = Engine(DIMS, theta0, cfg, max_rows=B, max_steps=64)
+    eng.use_tiles = False  # the per-row kernel (the tile kernel has its own tests)
     eng.train_level(grid, drec, 5)
     torch.cuda.synchronize()
     gl = eng.loss_hist[:20].cpu().numpy().astype(np.float64)
@@ -125,6 +126,7 @@ def test_fused_step_equals_two_kernel_step():
     for fused in (True, False):
         eng = Engine(DIMS, theta0, cfg, max_rows=B, max_steps=64)
         eng.fused = fused
+        eng.use_tiles = False
         eng.train_level(grid, drec, 6)
         torch.cuda.synchronize()
         out.append((eng.loss_hist[:24].cpu().numpy(), eng.params.cpu().numpy(), int(eng.count.item()),

@@ -96,7 +96,7 @@ def test_tile_long_and_empty_rows():
     B = 1024
     theta = M.init_params(DIMS, seed=17)
     grid, drec, _ = _device(axes, rec, B)
-    tl = drec.tiles()
+    tl = drec.tiles(grid)
     ptr = drec.row_ptr.cpu().numpy().astype(np.int64)
     trow = tl["tile_row"].cpu().numpy().astype(np.int64)
     long_tiles = sum(1 for b in range(len(drec.batch_begin))
@@ -165,8 +165,9 @@ def test_tile_train_level_matches_oracle_steps():
 
 @pytest.mark.gpu
 def test_tile_fused_equals_two_kernel_and_row_kernel():
-    """The fused tile step equals dips_tile_loss_grad + dips_opt_step bit for bit
-    in the losses, and tracks the per-row kernel's trajectory."""
+    """The single-launch tile step (last workgroup applies the optimizer) equals
+    the two-launch step bit for bit in the losses, and tracks the per-row
+    kernel's trajectory."""
     from jax_dips.records import Engine
     import torch
 
@@ -178,9 +179,9 @@ def test_tile_fused_equals_two_kernel_and_row_kernel():
     cfg = {"optimizer_name": "custom", "learning_rate": 1e-2,
            "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
     out = []
-    for fused, tiles in ((True, True), (False, True), (True, False)):
+    for single, tiles in ((True, True), (False, True), (True, False)):
         eng = Engine(DIMS, theta0, cfg, max_rows=B, max_steps=64)
-        eng.fused, eng.use_tiles = fused, tiles
+        eng.tile_single_launch, eng.use_tiles = single, tiles
         eng.train_level(grid, drec, 6)
         torch.cuda.synchronize()
         out.append((eng.loss_hist[:24].cpu().numpy(), eng.params.cpu().numpy(), int(eng.count.item())))

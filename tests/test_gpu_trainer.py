@@ -77,3 +77,22 @@ def test_eval_gradients_match_finite_differences():
         e[a] = eps
         fd = (M.forward(theta, model.dims, X + e, phase) - M.forward(theta, model.dims, X - e, phase)) / (2 * eps)
         assert np.max(np.abs(gr[:, a] - fd)) <= 1e-4
+
+
+@pytest.mark.gpu
+def test_trainer_padded_batches(tmp_path):
+    """batch_size not dividing N (16^3 = 4096 points, B = 1500): the last batch
+    is padded with random points in the data's range on both sides; per-epoch
+    losses and parameters follow the oracle loop."""
+    prob = problems.sphere()
+    dims = (1, 1, 2, 10)
+    theta0 = M.init_params(dims, seed=4)
+    g, st, es, le, tr = run_trainer(prob, 16, 8, 1500, theta0, tmp_path)
+    axes, P, h, bounds = grid_points(16)
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    th, ole, steps, _ = T.train(P, h, bounds, cb, dims, theta0, 8, 1500, OPT)
+    print("device", le, "oracle", ole)
+    assert len(steps) == 8 * 3
+    assert np.all(np.abs(le - ole) <= 1e-4 * np.abs(ole))
+    dev_theta = tr.model.flatten(tr.params)
+    assert np.linalg.norm(dev_theta - th) <= 1e-4 * np.linalg.norm(th)

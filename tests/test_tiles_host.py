@@ -2,9 +2,10 @@
 
 Properties checked on random record layouts (rows of 0..120 entries, batches
 of uneven size): every batch's rows are covered exactly once by consecutive
-tiles; a tile holds at most 64 entries and 64 rows unless it is a single row
-with more than 64 entries; packing is greedy (the next row would not have
-fitted); and the batch table indexes the boundary runs.  A brute-force
+tiles; a tile holds at most 64 slots (one per entry, one for a row without
+entries) and 64 rows unless it is a single row with more than 64 entries;
+packing is greedy (the next row would not have fitted); and the batch table
+indexes the boundary runs.  A brute-force
 re-packing in Python gives the same boundaries.
 """
 import numpy as np
@@ -25,9 +26,9 @@ def _greedy(ptr, begin, rows):
         n = k = 0
         while r < end and k < CAP:
             ln = ptr[r + 1] - ptr[r]
-            if ln > CAP or n + ln > CAP:
+            if ln > CAP or n + max(ln, 1) > CAP:
                 break
-            n += ln
+            n += max(ln, 1)
             r += 1
             k += 1
         out.append(r)
@@ -53,7 +54,8 @@ def test_pack_tiles_properties(seed):
             ne = ptr[r1] - ptr[r0]
             if r1 - r0 == 1 and ne > CAP:
                 continue
-            assert ne <= CAP and r1 - r0 <= CAP
+            slots = np.maximum(np.diff(ptr[r0:r1 + 1]), 1).sum()
+            assert slots <= CAP and r1 - r0 <= CAP
 
 
 def test_pack_tiles_all_empty_rows_and_errors():
