@@ -32,6 +32,22 @@ def mlp_fwd_flops(L, H):
     return 2 * (3 * H + (L - 1) * H * H + H)
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3
+    counter summaries (FETCH_SIZE and WRITE_SIZE passes calibrated on a plain
+    copy, profiles/*.md lines 'traffic_bytes_per_launch <key>: <bytes>'), or None."""
+    import glob
+    import re
+
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(f) as fh:
+            for line in fh:
+                m = re.match(r"\s*traffic_bytes_per_launch (.+): (\d+)\s*$", line)
+                if m and m.group(1) == key:
+                    return int(m.group(2))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +60,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="sphere", choices=["sphere", "star"],
                     help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device-to-device copy, 5 times (for scaling HBM counters)")
     ap.add_argument("--path", default="auto", choices=["auto", "tile", "lane", "mfma"],
                     help="training-step kernels: packed-tile fused step (auto for the small networks), "
                          "lane-per-row fused step, or MFMA layer tiles (auto for the wide networks)")
@@ -55,6 +73,17 @@ def main():
     import ctypes
 
     import torch
+
+    if args.copy_calibration:
+        # a copy kernel of known size: 256 MiB read and 256 MiB written per launch
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}), flush=True)
+        return
 
     from jax_dips import _native as N
     from jax_dips import problems
@@ -215,6 +244,7 @@ def main():
     avg_s = r0.elapsed_time(r1) / 1e3 / R
     achieved = flops_sum / R / avg_s / 1e12
 
+    traffic = measured_traffic(f"{path} {args.workload}{n}") if args.batch == 65536 else None
     result = {
         "metric": "NBM residual+grad points/s vs JAX ref on same GPU at equal Linf error",
         "value": value,
@@ -232,7 +262,7 @@ def main():
                                f"4 zoom levels (2000-epoch schedule), {path} kernels",
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": {"mfma": "mfma step: fwd + rowsum + bwd + reduce + apply (5 launches)",
                                 "lane": "loss_grad_kernel<grad,fused> (one launch per optimizer step)",
                                 "tile": "tile_step_kernel<grad> (gradient launch of a step; the optimizer "

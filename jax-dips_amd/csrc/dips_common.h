@@ -60,10 +60,27 @@ __device__ __forceinline__ float dips_wave_sum(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
+template <int Ctrl, int RowMask, bool BoundCtrl>
+__device__ __forceinline__ double dips_dpp_d(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, Ctrl, RowMask, 0xF, BoundCtrl);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), Ctrl, RowMask, 0xF, BoundCtrl);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// float64 wavefront sum with the same DPP tree as dips_wave_sum (fixed order,
+// no LDS traffic); every lane receives the total.
 __device__ __forceinline__ double dips_wave_sum_d(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, DIPS_WAVE);
-  return v;
+  v += dips_dpp_d<0x111, 0xF, true>(v);
+  v += dips_dpp_d<0x112, 0xF, true>(v);
+  v += dips_dpp_d<0x114, 0xF, true>(v);
+  v += dips_dpp_d<0x118, 0xF, true>(v);
+  v += dips_dpp_d<0x142, 0xA, false>(v);
+  v += dips_dpp_d<0x143, 0xC, false>(v);
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
 __device__ __forceinline__ int dips_wave_sum_i(int v) {

@@ -489,7 +489,8 @@ __global__ __launch_bounds__(256) void mfma_reduce_kernel(const float* __restric
     const int b0 = plus ? 0 : nblk_p, b1 = plus ? nblk_p : nblk_p + nblk_m;
     const int q = plus ? p : p - PPp;
     double acc = 0.0;
-    for (int b = b0; b < b1; ++b) acc += (double)gpart[(int64_t)b * stride + q];
+#pragma unroll 16
+    for (int b = b0; b < b1; ++b) acc += (double)gpart[(int64_t)b * stride + q];  // 16 loads in flight
     const float gg = (float)(acc * (double)inv_b);
     g[p] = gg;
     v = (double)gg * (double)gg;

@@ -83,23 +83,48 @@ template <int kStride, int kTileWaves>
 __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt& fo, float* params) {
   constexpr int kTileThreads = kTileWaves * DIPS_WAVE;
   constexpr int PT = kStride - 1;
-  constexpr int kPer = (kStride + kTileWaves - 1) / kTileWaves;
+  constexpr int kPer = (kStride + kTileWaves - 1) / kTileWaves;  // parameters per wavefront
+  constexpr int kOwn = (PT + kTileThreads - 1) / kTileThreads;   // parameters updated per thread
   __shared__ float gsh[kStride];
   __shared__ double dred[kTileWaves];
-  __shared__ float gnorm_s;
-  __shared__ int count_s;
+  __shared__ float sched[3];
   const int lane = threadIdx.x & (DIPS_WAVE - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
-  for (int q0 = 0; q0 < kPer; q0 += kReduceChunk) {
-    float vals[kReduceChunk][kBlocksPerLane];
+  // every global read is issued up front so that their latencies overlap: the
+  // optimizer state of the parameters this thread updates, the step counter,
+  // and the partial rows (plain, unconditional index-clamped loads after the
+  // caller's acquire; each wavefront reduces a set of parameters, its lanes
+  // stride over the workgroups)
+  float pv[kOwn], mv[kOwn], vv[kOwn];
 #pragma unroll
-    for (int qq = 0; qq < kReduceChunk; ++qq) {
+  for (int q = 0; q < kOwn; ++q) {
+    const int p = threadIdx.x + q * kTileThreads;
+    const int pc = p < PT ? p : PT - 1;
+    pv[q] = params[pc];
+    mv[q] = fo.m[pc];
+    vv[q] = fo.v[pc];
+  }
+  const int c = *fo.count;
+  if (wave == 0) {
+    // step-dependent scalars, computed once (powf is long) while the partial
+    // rows are loaded by the other wavefronts
+    const float t1 = (float)(c + 1);
+    if (lane == 0) {
+      sched[0] = schedule_lr(fo.opt, c);
+      sched[1] = 1.0f - powf(0.9f, t1);
+      sched[2] = 1.0f - powf(0.999f, t1);
+    }
+  }
+  // one round of loads when the wavefront's parameters fit (<= 16 per wavefront)
+  constexpr int kChunk = kPer <= 2 * kReduceChunk ? kPer : kReduceChunk;
+  for (int q0 = 0; q0 < kPer; q0 += kChunk) {
+    float vals[kChunk][kBlocksPerLane];
+#pragma unroll
+    for (int qq = 0; qq < kChunk; ++qq) {
       const int p = wave + (q0 + qq) * kTileWaves;
 #pragma unroll
       for (int i = 0; i < kBlocksPerLane; ++i) {
         const int b = lane + i * DIPS_WAVE;
-        // plain, unconditional (index-clamped) loads after the acquire fence of
-        // the caller, so that all loads of a round are in flight together
         const int pc = p < kStride ? p : kStride - 1;
         const int bc = b < nblk ? b : nblk - 1;
         const float val = partial[(int64_t)pc * nblk + bc];
@@ -107,7 +132,7 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
       }
     }
 #pragma unroll
-    for (int qq = 0; qq < kReduceChunk; ++qq) {
+    for (int qq = 0; qq < kChunk; ++qq) {
       const int p = wave + (q0 + qq) * kTileWaves;
       double acc = 0.0;
 #pragma unroll
@@ -122,23 +147,22 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
   sq = dips_wave_sum_d(sq);
   if (lane == 0) dred[wave] = sq;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < kTileWaves; ++w) t += dred[w];
-    gnorm_s = (float)sqrt(t);
-    count_s = *fo.count;
-  }
-  __syncthreads();
-  const int c = count_s;
-  const float gn = gnorm_s;
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kTileWaves; ++w) t += dred[w];
+  const float gn = (float)sqrt(t);
   const bool clip = (fo.opt.kind == 0) && !(gn < fo.opt.max_norm);
-  const float lr_t = schedule_lr(fo.opt, c);
-  const float t1 = (float)(c + 1);
-  const float bc1 = 1.0f - powf(0.9f, t1);
-  const float bc2 = 1.0f - powf(0.999f, t1);
-  for (int p = threadIdx.x; p < PT; p += kTileThreads)
-    apply_update(fo.opt, c, lr_t, bc1, bc2, clip, gn, p, gsh[p], params, fo.m, fo.v);
-  __syncthreads();
+  const float lr_t = sched[0], bc1 = sched[1], bc2 = sched[2];
+#pragma unroll
+  for (int q = 0; q < kOwn; ++q) {
+    const int p = threadIdx.x + q * kTileThreads;
+    if (p < PT) {
+      apply_update_value(fo.opt, lr_t, bc1, bc2, clip, gn, gsh[p], pv[q], mv[q], vv[q]);
+      params[p] = pv[q];
+      fo.m[p] = mv[q];
+      fo.v[p] = vv[q];
+    }
+  }
   if (threadIdx.x == 0) {
     if (c >= 0 && c < fo.loss_cap) fo.loss_hist[c] = gsh[PT];
     *fo.count = c + 1;
@@ -187,32 +211,33 @@ __device__ __forceinline__ void stage_mfma(const float* AT, const float* BT, int
   }
 }
 
-// Accumulate one chunk of 64 entries (lanes with on == false contribute zero)
-// into the L accumulator blocks of a network.  Called in wave-uniform control flow.
+// Accumulate one chunk of 64 entries into the L accumulator blocks of a
+// network.  Lanes that do not contribute carry seed 0 (hence zero deltas) and
+// finite activations and positions, so their products vanish without masking.
+// Called in wave-uniform control flow.
 template <int L, int H>
-__device__ __forceinline__ void mfma_grad_chunk(float* AT, float* BT, int lane, bool on, float x, float y, float z,
+__device__ __forceinline__ void mfma_grad_chunk(float* AT, float* BT, int lane, float x, float y, float z,
                                                 const float (&act)[L][H], const float (&dl)[L][H], float seed,
                                                 f4 (&acc)[L]) {
-  const float one = on ? 1.0f : 0.0f;
-  AT[0 * kLdsStride + lane] = on ? x : 0.0f;
-  AT[1 * kLdsStride + lane] = on ? y : 0.0f;
-  AT[2 * kLdsStride + lane] = on ? z : 0.0f;
-  AT[3 * kLdsStride + lane] = one;
+  AT[0 * kLdsStride + lane] = x;
+  AT[1 * kLdsStride + lane] = y;
+  AT[2 * kLdsStride + lane] = z;
+  AT[3 * kLdsStride + lane] = 1.0f;
 #pragma unroll
-  for (int i = 0; i < H; ++i) AT[(4 + i) * kLdsStride + lane] = on ? act[L - 1][i] : 0.0f;
+  for (int i = 0; i < H; ++i) AT[(4 + i) * kLdsStride + lane] = act[L - 1][i];
 #pragma unroll
-  for (int j = 0; j < H; ++j) BT[j * kLdsStride + lane] = on ? dl[0][j] : 0.0f;
-  BT[H * kLdsStride + lane] = on ? seed : 0.0f;
+  for (int j = 0; j < H; ++j) BT[j * kLdsStride + lane] = dl[0][j];
+  BT[H * kLdsStride + lane] = seed;
   __builtin_amdgcn_wave_barrier();
   stage_mfma(AT, BT, lane, acc[0]);
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int l = 1; l < L; ++l) {
 #pragma unroll
-    for (int i = 0; i < H; ++i) AT[i * kLdsStride + lane] = on ? act[l - 1][i] : 0.0f;
-    AT[H * kLdsStride + lane] = one;
+    for (int i = 0; i < H; ++i) AT[i * kLdsStride + lane] = act[l - 1][i];
+    AT[H * kLdsStride + lane] = 1.0f;
 #pragma unroll
-    for (int j = 0; j < H; ++j) BT[j * kLdsStride + lane] = on ? dl[l][j] : 0.0f;
+    for (int j = 0; j < H; ++j) BT[j * kLdsStride + lane] = dl[l][j];
     __builtin_amdgcn_wave_barrier();
     stage_mfma(AT, BT, lane, acc[l]);
     __builtin_amdgcn_wave_barrier();
@@ -251,7 +276,8 @@ struct NetGrad {
   }
 
   // Gradient contribution of one chunk of entries; mine = this lane's entry
-  // belongs to this network (act holds its activations).  Wave-uniform call.
+  // belongs to this network (act holds its activations; the activations of
+  // other lanes must be finite, e.g. zero).  Wave-uniform call.
   __device__ __forceinline__ void add(dips_weight_t* P_, float* AT, float* BT, int lane, bool mine, float x, float y,
                                       float z, float seed, const float (&act)[L][H]) {
     // a fresh (opaque) weight pointer: the backward pass reloads the weights
@@ -259,9 +285,10 @@ struct NetGrad {
     dips_weight_t* Q = P_;
     asm volatile("" : "+s"(Q));
     if constexpr (kGrad && kMfma) {
-      float dl[L][H] = {};
-      if (mine) mlp_deltas<L, H>(Q, seed, act, dl);
-      mfma_grad_chunk<L, H>(AT, BT, lane, mine, x, y, z, act, dl, seed, acc);
+      const float s = mine ? seed : 0.0f;
+      float dl[L][H];
+      mlp_deltas<L, H>(Q, s, act, dl);
+      mfma_grad_chunk<L, H>(AT, BT, lane, x, y, z, act, dl, s, acc);
     } else if constexpr (kGrad) {
       if (mine) mlp_backward_kept<L, H>(Q, x, y, z, seed, act, g);
     }
@@ -327,17 +354,31 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
   float lsum = 0.0f;
 
   const int nw = (int)gridDim.x * kTileWaves;
-  for (int t = (int)blockIdx.x * kTileWaves + wave; t < a.ntiles; t += nw) {
+  const int t0 = (int)blockIdx.x * kTileWaves + wave;
+  // slot data of the wavefront's next tile is loaded one tile ahead (the last
+  // prefetch re-reads the final tile, unused)
+  auto slot_of = [&](int t) { return (a.tile_begin + min(t, a.ntiles - 1)) * kTileCap + lane; };
+  int64_t s0 = slot_of(t0);
+  uint32_t n_info = a.sinfo[s0];
+  float n_x = a.sx[s0], n_y = a.sy[s0], n_z = a.sz[s0], n_w = a.sw[s0], n_c = a.sc[s0];
+  for (int t = t0; t < a.ntiles; t += nw) {
     // weights are re-read per tile (scalar loads that hit the scalar cache)
     // rather than held in vector registers across the loop
     const float* Praw = params;
     asm volatile("" : "+s"(Praw));
     dips_weight_t* Pp = dips_weights(Praw);
     dips_weight_t* Pm = Pp + PP;
-    const int64_t slot = (a.tile_begin + t) * kTileCap + lane;
-    const uint32_t info = a.sinfo[slot];
-    const float x = a.sx[slot], y = a.sy[slot], z = a.sz[slot];
-    const float w = a.sw[slot], c = a.sc[slot];
+    const uint32_t info = n_info;
+    const float x = n_x, y = n_y, z = n_z, w = n_w, c = n_c;
+    {
+      const int64_t sn = slot_of(t + nw);
+      n_info = a.sinfo[sn];
+      n_x = a.sx[sn];
+      n_y = a.sy[sn];
+      n_z = a.sz[sn];
+      n_w = a.sw[sn];
+      n_c = a.sc[sn];
+    }
     const bool on = (info & kSlotUsed) != 0;
     const bool plus = (info & kSlotPlus) != 0;
     const int head = on ? (int)(info & 63u) : lane;

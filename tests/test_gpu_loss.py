@@ -112,7 +112,9 @@ def test_train_level_matches_oracle_steps():
 @pytest.mark.gpu
 def test_fused_step_equals_two_kernel_step():
     """dips_train_step (single launch, last-workgroup optimizer) gives the same
-    losses and parameters as dips_loss_grad + dips_opt_step."""
+    losses and parameters as dips_loss_grad + dips_opt_step.  The two reduce the
+    gradient norm over different thread trees (float64), so agreement is to
+    float32 rounding rather than bitwise."""
     from jax_dips.records import Engine
     import torch
 
@@ -133,5 +135,6 @@ def test_fused_step_equals_two_kernel_step():
                     int(eng.sync.item())))
     (l1, p1, c1, s1), (l2, p2, c2, s2) = out
     assert c1 == c2 == 24 and s1 == 0
-    assert np.array_equal(l1, l2)
-    assert np.max(np.abs(p1 - p2)) <= 1e-7 * max(1.0, np.max(np.abs(p2)))
+    assert l1[0] == l2[0]
+    assert np.all(np.abs(l1 - l2) <= 1e-6 * np.abs(l2))
+    assert np.max(np.abs(p1 - p2)) <= 1e-6 * max(1.0, np.max(np.abs(p2)))
