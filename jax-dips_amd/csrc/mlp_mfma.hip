@@ -182,42 +182,46 @@ __device__ __forceinline__ bool entry_pos(const MfmaTileArgs& T, int64_t e, int6
 }
 
 // ---- kernel 1: u for every entry of the batch ----------------------------------
-// tiles [0, ntp) hold + entries [e0, e0+np), tiles [ntp, ntp+ntm) hold - entries
+// Persistent: workgroups [0, nblk_p) take the + tiles (+ entries [e0, e0+np)),
+// the others the - tiles; each packs its network's weights into LDS once and
+// its wavefronts stride over the tiles.
 template <int Lm, int NBm, int Lp, int NBp>
 __global__ __launch_bounds__(kMT) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
-                                                       int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_out) {
+                                                       int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_out,
+                                                       int nblk_p) {
   extern __shared__ float lds[];
-  const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-  // a workgroup works on one phase: the first ceil(ntp/4) workgroups take + tiles
-  const int64_t gp = (ntp + 3) / 4;
-  const bool plus = (int64_t)blockIdx.x < gp;
+  const bool plus = (int)blockIdx.x < nblk_p;
+  const int64_t ntiles = plus ? (np_ + 15) / 16 : (nm_ + 15) / 16;
+  const int nblk = plus ? nblk_p : (int)gridDim.x - nblk_p;
+  const int bid = plus ? (int)blockIdx.x : (int)blockIdx.x - nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
   if (plus)
     pack_weights<Lp, NBp>(params, FlatNet{Hp, Lp, 0}, lds);
   else
     pack_weights<Lm, NBm>(params, FlatNet{Hm, Lm, PPp}, lds);
   __syncthreads();
-  const int64_t tile = plus ? (int64_t)blockIdx.x * 4 + wave : ((int64_t)blockIdx.x - gp) * 4 + wave;
-  if (plus ? tile >= ntp : tile >= ntm) return;
-  const int64_t ebeg = plus ? e0 + tile * 16 : e0 + np_ + tile * 16;
-  const int64_t eend = plus ? e0 + np_ : e0 + np_ + nm_;
-  float x, y, z;
-  int row;
-  const bool valid = entry_pos(T, ebeg + (lane & 15), eend, x, y, z, row);
-  const int k = lane / 16;
-  const float xin = (k == 0) ? x : (k == 1) ? y : (k == 2) ? z : 1.0f;
-  float u;
-  if (plus) {
-    float act[Lp][4 * NBp];
-    tile_forward<Lp, NBp>(lds, xin, act);
-    u = tile_output<Lp, NBp>(lds, act);
-  } else {
-    float act[Lm][4 * NBm];
-    tile_forward<Lm, NBm>(lds, xin, act);
-    u = tile_output<Lm, NBm>(lds, act);
+  const int64_t eb = plus ? e0 : e0 + np_;
+  const int64_t ee = plus ? e0 + np_ : e0 + np_ + nm_;
+  for (int64_t tile = (int64_t)bid * 4 + wave; tile < ntiles; tile += (int64_t)nblk * 4) {
+    const int64_t ebeg = eb + tile * 16;
+    float x, y, z;
+    int row;
+    const bool valid = entry_pos(T, ebeg + (lane & 15), ee, x, y, z, row);
+    const int k = lane / 16;
+    const float xin = (k == 0) ? x : (k == 1) ? y : (k == 2) ? z : 1.0f;
+    float u;
+    if (plus) {
+      float act[Lp][4 * NBp];
+      tile_forward<Lp, NBp>(lds, xin, act);
+      u = tile_output<Lp, NBp>(lds, act);
+    } else {
+      float act[Lm][4 * NBm];
+      tile_forward<Lm, NBm>(lds, xin, act);
+      u = tile_output<Lm, NBm>(lds, act);
+    }
+    if (valid && lane < 16) u_out[T.t_csr[ebeg + lane]] = u;
   }
-  if (valid && lane < 16) u_out[T.t_csr[ebeg + lane]] = u;
 }
 
 // ---- kernel 2: r per row, loss partials ---------------------------------------
@@ -509,12 +513,17 @@ __global__ __launch_bounds__(256) void mfma_apply_kernel(const float* __restrict
                                                          float* __restrict__ m, float* __restrict__ v,
                                                          const int32_t* __restrict__ count_in,
                                                          float* __restrict__ loss_hist, int loss_cap) {
+  // fixed-order float64 sums spread over the workgroup: per thread a strided
+  // subset in order, then the DPP wave tree, then the 4 waves in order
+  __shared__ double red[4];
   __shared__ float gn_s;
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int b = 0; b < nsq; ++b) t += sq_part[b];
-    gn_s = (float)sqrt(t);
-  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nsq; b += 256) t += sq_part[b];
+  t = dips_wave_sum_d(t);
+  if (lane == 0) red[wave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) gn_s = (float)sqrt(((red[0] + red[1]) + red[2]) + red[3]);
   __syncthreads();
   const int c = *count_in;
   const float gn = gn_s;
@@ -547,10 +556,15 @@ __global__ __launch_bounds__(256) void mfma_apply_kernel(const float* __restrict
     }
     params[p] = params[p] + u;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && c >= 0 && c < loss_cap) {
+  if (blockIdx.x == 0) {
     double l = 0.0;
-    for (int b = 0; b < nloss; ++b) l += loss_part[b];
-    loss_hist[c] = (float)(l * (double)inv_b);
+    for (int b = threadIdx.x; b < nloss; b += 256) l += loss_part[b];
+    l = dips_wave_sum_d(l);
+    __syncthreads();
+    if (lane == 0) red[wave] = l;
+    __syncthreads();
+    if (threadIdx.x == 0 && c >= 0 && c < loss_cap)
+      loss_hist[c] = (float)((((red[0] + red[1]) + red[2]) + red[3]) * (double)inv_b);
   }
 }
 
@@ -559,13 +573,16 @@ __global__ void count_inc_kernel(int32_t* count) {
 }
 
 // loss = inv_b * sum of the row-sum partials (fixed order), gradient copy-out
-__global__ void mfma_loss_out_kernel(const double* __restrict__ loss_part, int nloss, float inv_b,
-                                     double* __restrict__ loss_out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double l = 0.0;
-    for (int b = 0; b < nloss; ++b) l += loss_part[b];
-    *loss_out = l * (double)inv_b;
-  }
+__global__ __launch_bounds__(256) void mfma_loss_out_kernel(const double* __restrict__ loss_part, int nloss,
+                                                            float inv_b, double* __restrict__ loss_out) {
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+  double l = 0.0;
+  for (int b = threadIdx.x; b < nloss; b += 256) l += loss_part[b];
+  l = dips_wave_sum_d(l);
+  if (lane == 0) red[wave] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) *loss_out = (((red[0] + red[1]) + red[2]) + red[3]) * (double)inv_b;
 }
 
 // ---- host side ----------------------------------------------------------------
@@ -634,7 +651,6 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                  bt->t_csr};
   const int a = Lm, b = nb_of(Hm), c = Lp, d = nb_of(Hp);
   const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
-  const int fwd_blocks = (int)((ntp + 3) / 4 + (ntm + 3) / 4);
   // bwd workgroups split by phase in proportion to the tiles (each at least 1 if it has tiles)
   int nbp = (ntp > 0) ? std::max<int64_t>(1, (int64_t)((double)kMfmaBwdBlocks * ntp / std::max<int64_t>(1, ntp + ntm)))
                       : 0;
@@ -658,9 +674,9 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                         (int)lds_f);                                                                               \
     hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                         (int)bwd_lds);                                                                             \
-    if (fwd_blocks > 0)                                                                                            \
-      hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(fwd_blocks), dim3(kMT), lds_f, st, T, params, Hm, Hp,  \
-                         bt->t_begin, np_, nm_, u - bt->ent_begin);                                                \
+    if (nbp + nbm > 0)                                                                                             \
+      hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kMT), lds_f, st, T, params, Hm, Hp,   \
+                         bt->t_begin, np_, nm_, u - bt->ent_begin, nbp);                                           \
     hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, rec->row_ptr, rec->row_c, bt->row_begin,   \
                        nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part);                              \
     if (nbp + nbm > 0)                                                                                             \
@@ -680,7 +696,7 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   hipLaunchKernelGGL(mfma_reduce_kernel, dim3(nred), dim3(256), 0, st, gpart, stride, nbp, nbm, PPp, PPm, inv_b,
                      grad_out ? grad_out : gvec, sq_part);
   if (!opt) {
-    hipLaunchKernelGGL(mfma_loss_out_kernel, dim3(1), dim3(64), 0, st, loss_part, nrb, inv_b, loss_out);
+    hipLaunchKernelGGL(mfma_loss_out_kernel, dim3(1), dim3(256), 0, st, loss_part, nrb, inv_b, loss_out);
     return hip_status(hipGetLastError(), "mfma reduce launch");
   }
   hipLaunchKernelGGL(mfma_apply_kernel, dim3(nred), dim3(256), 0, st, gvec, P, sq_part, nred, loss_part, nrb, inv_b,
