@@ -68,13 +68,23 @@ typedef struct {
   int64_t nrows;
   int64_t nnz;
   float hx, hy, hz;
+  const float* row_coeffs;  /* (nrows,26) cell coefficients of each row's point, or NULL (preconditioner input:
+                               mu*A/h of the 12 (face, phase) pairs, V-, V+, the 12 face areas) */
 } dips_records_t;
 
 /* Two-phase tanh MLP (jax_dips/nn/mlp/MLP.py:93-139): L hidden layers of width
  * H per phase.  Flat parameters: the Omega+ net then the Omega- net, each as
- * W0[3][H], b0[H], (W[H][H], b[H]) * (L-1), Wout[H][1], bout[1]. */
+ * W0[3][H], b0[H], (W[H][H], b[H]) * (L-1), Wout[H][1], bout[1].
+ * Optional learned preconditioner (nn/preconditioner.py:10-35; used at
+ * discretization.py:339, :418-419): pc_w1 > 0 enables
+ * P = 0.5 + pc_scale * sigmoid(MLP(c)) on the 26 cell coefficients c of each
+ * row (hidden widths pc_w1 and, if > 0, pc_w2, tanh), which multiplies the
+ * row's residual; its parameters follow the two networks in the flat vector as
+ * W1[26][pc_w1], b1, (W2[pc_w1][pc_w2], b2), Wout[.][1], bout. */
 typedef struct {
   int32_t Lm, Hm, Lp, Hp;
+  int32_t pc_w1, pc_w2;
+  float pc_scale;
 } dips_net_t;
 
 /* Optimizer (solvers/optimizers.py:13-97).
@@ -166,6 +176,7 @@ typedef struct {
   const float* slot_c;       /* row constant in the row's last slot, else 0 */
   const uint32_t* slot_info; /* bits 0-5 first slot of the row, 6-11 last slot, 12 phase (1: Omega+ net),
                                 13 slot in use */
+  const int32_t* slot_row;   /* row of each used slot (needed by the preconditioner), or NULL */
 } dips_tiles_t;
 
 /* Host arrays: row_ptr (nrows+1) of the records, batches [batch_row_begin[b],
@@ -177,10 +188,10 @@ int64_t dips_pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* ba
                         const int64_t* batch_rows, int32_t nbatches, int32_t* tile_row, int64_t tile_cap,
                         int64_t* batch_tile_begin);
 /* Lays out the slots of every tile of tile_row (device, nbounds boundaries)
- * into the six slot arrays (device, nbounds * 64 each). */
+ * into the slot arrays (device, nbounds * 64 each; slot_row may be NULL). */
 int dips_tile_slots(const dips_grid_t* grid, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
                     float* slot_x, float* slot_y, float* slot_z, float* slot_w, float* slot_c, uint32_t* slot_info,
-                    void* stream);
+                    int32_t* slot_row, void* stream);
 /* 1 if the network shape has a compiled tile kernel, else 0. */
 int dips_tile_supported(const dips_net_t* net);
 /* Floats of partial scratch the tile entry points need (any batch size). */
@@ -340,6 +351,8 @@ typedef struct {
   int64_t npts;             /* N = nx*ny*nz + nextra */
   const float* extra;       /* (nextra,3) extra points after the grid nodes (as dips_grid_t) */
   int64_t nextra;
+  float* row_coeffs;        /* (N,26) per row (mode 1, at row_of[pt]): cell coefficients, or NULL; crossed
+                               centre cells of box points then need cc_slot entries too */
 } dips_assemble_t;
 
 /* mode 0 counts the entries of every point's row; mode 1 (same inputs, after

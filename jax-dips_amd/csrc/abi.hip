@@ -1,4 +1,5 @@
 // extern "C" entry points of libdips_hip.so (declared in include/dips_hip.h).
+#include <cmath>
 #include <string>
 
 #include "dips_common.h"
@@ -21,6 +22,11 @@ int net_supported(const dips_net_t* n);
 static int check_net(const dips_net_t* net) {
   if (!net || net->Lm < 1 || net->Lp < 1 || net->Hm < 1 || net->Hp < 1) {
     set_error("invalid network dimensions");
+    return DIPS_EINVAL;
+  }
+  if (net->pc_w1 < 0 || net->pc_w2 < 0 || (net->pc_w2 > 0 && net->pc_w1 == 0) ||
+      (net->pc_w1 > 0 && !std::isfinite(net->pc_scale))) {
+    set_error("invalid preconditioner dimensions");
     return DIPS_EINVAL;
   }
   return DIPS_OK;
@@ -159,7 +165,7 @@ int64_t dips_pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* ba
 
 int dips_tile_slots(const dips_grid_t* grid, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
                     float* slot_x, float* slot_y, float* slot_z, float* slot_w, float* slot_c, uint32_t* slot_info,
-                    void* stream) {
+                    int32_t* slot_row, void* stream) {
   int st = check_records(grid, rec);
   if (st) return st;
   if (!tile_row || nbounds < 1 || !slot_x || !slot_y || !slot_z || !slot_w || !slot_c || !slot_info) {
@@ -167,7 +173,7 @@ int dips_tile_slots(const dips_grid_t* grid, const dips_records_t* rec, const in
     return DIPS_EINVAL;
   }
   return tile_slots_launch(grid, rec, tile_row, nbounds, slot_x, slot_y, slot_z, slot_w, slot_c, slot_info,
-                           (hipStream_t)stream);
+                           slot_row, (hipStream_t)stream);
 }
 
 int dips_tile_supported(const dips_net_t* net) {

@@ -27,6 +27,7 @@ int train_step_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_
                       float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, float* partial,
                       int64_t partial_cap, uint32_t* sync, hipStream_t stream);
 int partial_stride(const dips_net_t* net);
+int precond_params(const dips_net_t* net);
 int64_t partial_blocks(int64_t nrows);
 
 // tiles.hip
@@ -40,7 +41,8 @@ int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_
 int64_t tile_partial_size(const dips_net_t* net);
 int tile_supported(const dips_net_t* net);
 int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
-                      float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, hipStream_t st);
+                      float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, int32_t* srow,
+                      hipStream_t st);
 int64_t pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_row_begin, const int64_t* batch_rows,
                    int32_t nbatches, int32_t* tile_row, int64_t tile_cap, int64_t* batch_tile_begin);
 

@@ -428,6 +428,53 @@ __global__ void assemble_kernel(AssembleArgs A) {
   if (A.mode == 1) wpos = A.ent_off[pt];
   const int row = (A.mode == 1) ? A.row_of[pt] : 0;
 
+  // the 26 cell coefficients of the point (preconditioner input), for box
+  // points too: mu*A/h of the 12 (face, phase) pairs, V-, V+, the 12 face
+  // areas (compute_interface_faces / compute_domain_faces,
+  // geometric_integrations_per_point.py:873-900, :966-996), stored float32
+  if (A.mode == 1 && A.row_coeffs) {
+    const int8_t rc0 = A.region[pt * 7];
+    const double vol0 = vol;
+    const double ff[6] = {(double)A.hy * A.hz, (double)A.hy * A.hz, (double)A.hx * A.hz,
+                          (double)A.hx * A.hz, (double)A.hx * A.hy, (double)A.hx * A.hy};
+    const double* mf = A.mu_face + pt * 12;
+    float* out = A.row_coeffs + (int64_t)row * 26;
+    const int s0 = (rc0 == 0) ? A.cc_slot[pt] : -1;
+    if (rc0 == 0 && s0 < 0) {
+      for (int q = 0; q < 26; ++q) out[q] = __builtin_nanf("");  // crossed cell without geometry
+    } else {
+      double vm, vp;
+      double am[6], ap[6];
+      if (rc0 == 0) {
+        vm = A.vol_m[s0];
+        vp = vol0 - vm;
+        if (vp < 0) vp = 0;
+        for (int f = 0; f < 6; ++f) {
+          am[f] = A.area_m[s0 * 6 + f];
+          ap[f] = ff[f] - am[f];
+          if (ap[f] < 0) ap[f] = 0;
+        }
+      } else {
+        const double mm = (rc0 == -1) ? 1.0 : 0.0, pm = (rc0 == 1) ? 1.0 : 0.0;
+        vm = vol0 * mm;
+        vp = vol0 * pm;
+        for (int f = 0; f < 6; ++f) {
+          am[f] = ff[f] * mm;
+          ap[f] = ff[f] * pm;
+        }
+      }
+      for (int f = 0; f < 6; ++f) {
+        const double hf = (double)h[f >> 1];
+        out[2 * f] = (float)(am[f] * mf[f * 2 + 0] / hf);
+        out[2 * f + 1] = (float)(ap[f] * mf[f * 2 + 1] / hf);
+        out[14 + 2 * f] = (float)am[f];
+        out[15 + 2 * f] = (float)ap[f];
+      }
+      out[12] = (float)vm;
+      out[13] = (float)vp;
+    }
+  }
+
   if (boundary) {
     // (u h^3)/h^3 - (g h^3)/h^3 : weight 1 on NN(p), phase from phi(p)
     if (A.mode == 0) {

@@ -235,9 +235,15 @@ int64_t loss_blocks(int64_t nrows, int64_t nbulk) {
 // upper bound of loss_blocks over all splits (every row an interface row)
 int64_t partial_blocks(int64_t nrows) { return (nrows + kRowsPerWaveBlock - 1) / kRowsPerWaveBlock + 1; }
 
+int precond_params(const dips_net_t* net) {
+  const int w1 = net->pc_w1, w2 = net->pc_w2;
+  if (w1 <= 0) return 0;
+  return 26 * w1 + w1 + (w2 > 0 ? w1 * w2 + w2 + w2 + 1 : w1 + 1);
+}
+
 int partial_stride(const dips_net_t* net) {
   auto sz = [](int L, int H) { return (3 * H + H) + (L - 1) * (H * H + H) + (H + 1); };
-  return sz(net->Lp, net->Hp) + sz(net->Lm, net->Hm) + 1;
+  return sz(net->Lp, net->Hp) + sz(net->Lm, net->Hm) + precond_params(net) + 1;
 }
 
 template <int Lm, int Hm, int Lp, int Hp>
@@ -260,6 +266,7 @@ static void launch_shape(const dips_grid_t* g, const dips_records_t* rec, int64_
 }
 
 int net_supported(const dips_net_t* n) {
+  if (n->pc_w1 > 0) return 0;  // the preconditioner runs in the tile kernel only
 #define X(a, b, c, d) \
   if (n->Lm == a && n->Hm == b && n->Lp == c && n->Hp == d) return 1;
   DIPS_SHAPES(X)
@@ -270,6 +277,10 @@ int net_supported(const dips_net_t* n) {
 int loss_grad_launch(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
                      int64_t nbulk, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
                      int want_grad, int* nblk_out, hipStream_t stream, const FusedOpt* fused) {
+  if (net->pc_w1 > 0) {
+    set_error("dips_loss_grad: the preconditioner runs in the tile kernel only (dips_tile_*)");
+    return DIPS_EUNSUPPORTED;
+  }
   if (nrows <= 0) {
     set_error("dips_loss_grad: nrows must be positive");
     return DIPS_EINVAL;

@@ -41,12 +41,18 @@ constexpr int kReduceChunk = 8;                       // parameters per wavefron
 constexpr int kLdsStride = 68;                        // floats per staged operand row: 64 entries + 4
 constexpr int kStageFloats = 2 * 16 * kLdsStride;     // A and B operand stacks of one wavefront
 
-// Network shapes with a compiled tile kernel (Lm, Hm, Lp, Hp).
-#define DIPS_TILE_SHAPES(X) \
-  X(1, 1, 2, 10)            \
-  X(1, 3, 2, 10)            \
-  X(1, 1, 1, 1)             \
-  X(2, 10, 2, 10)
+// Network shapes with a compiled tile kernel (Lm, Hm, Lp, Hp) and
+// preconditioner hidden widths (PC1, PC2; 0 = no preconditioner, PC2 = 0 = one
+// hidden layer).  [8, 4] is the preconditioner of the reference's LPBE
+// configuration (examples/benchmark_LPBE/conf/lpbe.yaml:63-67).
+#define DIPS_TILE_SHAPES(X)  \
+  X(1, 1, 2, 10, 0, 0)       \
+  X(1, 3, 2, 10, 0, 0)       \
+  X(1, 1, 1, 1, 0, 0)        \
+  X(2, 10, 2, 10, 0, 0)      \
+  X(1, 1, 2, 10, 8, 4)       \
+  X(1, 1, 2, 10, 8, 0)       \
+  X(2, 10, 2, 10, 8, 4)
 
 struct TileStepArgs {
   PointSet ps;
@@ -65,6 +71,9 @@ struct TileStepArgs {
   const float* sw;
   const float* sc;
   const uint32_t* sinfo;
+  const int32_t* srow;       // row of each used slot (preconditioner only)
+  const float* row_coeffs;   // (nrows,26) cell coefficients (preconditioner only)
+  float pc_scale;
   int64_t tile_begin;  // first boundary of the batch's tiles in tile_row
   int ntiles;
 };
@@ -314,12 +323,173 @@ struct NetGrad {
   }
 };
 
+// Learned preconditioner (nn/preconditioner.py:10-35): P = 0.5 + s*sigmoid(z),
+// z = MLP(c) on the 26 cell coefficients of the row, hidden widths PC1 and
+// (PC2 > 0) PC2, tanh.  Evaluated at the row's last lane; its weight gradients
+// accumulate on the matrix cores in three 16x16 blocks (K = the tile's slots):
+//   0: rows c[0..15]            cols d1          -> W1 rows 0..15
+//   1: rows [c[16..25], 1]      cols d1          -> W1 rows 16..25, b1
+//   2: rows [a1, 1, a2, 1]      cols [d2, dz]    -> W2, b2, Wout, bout  (PC2 > 0)
+//      rows [a1, 1]             cols [dz]        -> Wout, bout          (PC2 = 0)
+template <int PC1, int PC2>
+struct Precond {
+  static constexpr int kIn = 26;
+  static constexpr int kLast = PC2 > 0 ? PC2 : PC1;
+  static constexpr int w1() { return 0; }
+  static constexpr int b1() { return kIn * PC1; }
+  static constexpr int w2() { return b1() + PC1; }
+  static constexpr int b2() { return w2() + PC1 * PC2; }
+  static constexpr int wout() { return PC2 > 0 ? b2() + PC2 : b1() + PC1; }
+  static constexpr int bout() { return wout() + kLast; }
+  static constexpr int size() { return bout() + 1; }
+  static_assert(PC1 == 0 || (PC1 <= 15 && PC2 <= 14 && PC1 + PC2 + 2 <= 16), "preconditioner too wide for the staging");
+
+  template <typename WP>
+  __device__ __forceinline__ static float forward(WP W, const float (&c)[kIn], float (&a1)[PC1 > 0 ? PC1 : 1],
+                                                  float (&a2)[PC2 > 0 ? PC2 : 1], float& sig) {
+#pragma unroll
+    for (int j = 0; j < PC1; ++j) {
+      float s = W[b1() + j];
+#pragma unroll
+      for (int i = 0; i < kIn; ++i) s = fmaf(c[i], W[w1() + i * PC1 + j], s);
+      a1[j] = dips_tanh(s);
+    }
+    float z = W[bout()];
+    if constexpr (PC2 > 0) {
+#pragma unroll
+      for (int j = 0; j < PC2; ++j) {
+        float s = W[b2() + j];
+#pragma unroll
+        for (int i = 0; i < PC1; ++i) s = fmaf(a1[i], W[w2() + i * PC2 + j], s);
+        a2[j] = dips_tanh(s);
+      }
+#pragma unroll
+      for (int i = 0; i < PC2; ++i) z = fmaf(a2[i], W[wout() + i], z);
+    } else {
+#pragma unroll
+      for (int i = 0; i < PC1; ++i) z = fmaf(a1[i], W[wout() + i], z);
+    }
+    sig = 1.0f / (1.0f + expf(-z));
+    return sig;
+  }
+
+  template <typename WP>
+  __device__ __forceinline__ static void deltas(WP W, float dz, const float (&a1)[PC1 > 0 ? PC1 : 1],
+                                                const float (&a2)[PC2 > 0 ? PC2 : 1], float (&d1)[PC1 > 0 ? PC1 : 1],
+                                                float (&d2)[PC2 > 0 ? PC2 : 1]) {
+    if constexpr (PC2 > 0) {
+#pragma unroll
+      for (int j = 0; j < PC2; ++j) d2[j] = dz * W[wout() + j] * (1.0f - a2[j] * a2[j]);
+#pragma unroll
+      for (int i = 0; i < PC1; ++i) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < PC2; ++j) acc = fmaf(W[w2() + i * PC2 + j], d2[j], acc);
+        d1[i] = acc * (1.0f - a1[i] * a1[i]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PC1; ++i) d1[i] = dz * W[wout() + i] * (1.0f - a1[i] * a1[i]);
+    }
+  }
+
+  // parameter index (relative to the preconditioner block) of element (i, j) of block blk, or -1
+  __device__ __forceinline__ static int param(int blk, int i, int j) {
+    if (blk == 0) return (j < PC1) ? w1() + i * PC1 + j : -1;
+    if (blk == 1) {
+      if (j >= PC1) return -1;
+      if (i < 10) return w1() + (16 + i) * PC1 + j;
+      return (i == 10) ? b1() + j : -1;
+    }
+    if constexpr (PC2 > 0) {
+      if (i < PC1 && j < PC2) return w2() + i * PC2 + j;
+      if (i == PC1 && j < PC2) return b2() + j;
+      if (i > PC1 && i <= PC1 + PC2 && j == PC2) return wout() + (i - PC1 - 1);
+      if (i == PC1 + PC2 + 1 && j == PC2) return bout();
+      return -1;
+    } else {
+      if (j != 0) return -1;
+      if (i < PC1) return wout() + i;
+      return (i == PC1) ? bout() : -1;
+    }
+  }
+};
+
+// Gradient accumulators of the preconditioner (three MFMA blocks).
+template <int PC1, int PC2, bool kGrad>
+struct PrecondGrad {
+  static constexpr bool kOn = kGrad && PC1 > 0;
+  using PN = Precond<(PC1 > 0 ? PC1 : 1), PC2>;
+  f4 acc[kOn ? 3 : 1];
+
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < (kOn ? 3 : 1); ++i) acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+
+  // One chunk of lanes: lanes with dz != 0 contribute (their c, a1, a2 finite).
+  // Wave-uniform call.
+  template <typename WP>
+  __device__ __forceinline__ void add(WP W, float* AT, float* BT, int lane, float dz, const float (&c)[26],
+                                      const float (&a1)[PC1 > 0 ? PC1 : 1], const float (&a2)[PC2 > 0 ? PC2 : 1]) {
+    if constexpr (kOn) {
+      float d1[PC1];
+      float d2[PC2 > 0 ? PC2 : 1];
+      PN::deltas(W, dz, a1, a2, d1, d2);
+      // block 0
+#pragma unroll
+      for (int i = 0; i < 16; ++i) AT[i * kLdsStride + lane] = c[i];
+#pragma unroll
+      for (int j = 0; j < PC1; ++j) BT[j * kLdsStride + lane] = d1[j];
+      __builtin_amdgcn_wave_barrier();
+      stage_mfma(AT, BT, lane, acc[0]);
+      __builtin_amdgcn_wave_barrier();
+      // block 1 (same B operand)
+#pragma unroll
+      for (int i = 0; i < 10; ++i) AT[i * kLdsStride + lane] = c[16 + i];
+      AT[10 * kLdsStride + lane] = 1.0f;
+      __builtin_amdgcn_wave_barrier();
+      stage_mfma(AT, BT, lane, acc[1]);
+      __builtin_amdgcn_wave_barrier();
+      // block 2
+#pragma unroll
+      for (int i = 0; i < PC1; ++i) AT[i * kLdsStride + lane] = a1[i];
+      AT[PC1 * kLdsStride + lane] = 1.0f;
+      if constexpr (PC2 > 0) {
+#pragma unroll
+        for (int i = 0; i < PC2; ++i) AT[(PC1 + 1 + i) * kLdsStride + lane] = a2[i];
+        AT[(PC1 + PC2 + 1) * kLdsStride + lane] = 1.0f;
+#pragma unroll
+        for (int j = 0; j < PC2; ++j) BT[j * kLdsStride + lane] = d2[j];
+        BT[PC2 * kLdsStride + lane] = dz;
+      } else {
+        BT[0 * kLdsStride + lane] = dz;
+      }
+      __builtin_amdgcn_wave_barrier();
+      stage_mfma(AT, BT, lane, acc[2]);
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+
+  __device__ __forceinline__ void flush(float* red, int off, int lane) {
+    if constexpr (kOn) {
+#pragma unroll
+      for (int blk = 0; blk < 3; ++blk)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int p = PN::param(blk, 4 * (lane >> 4) + r, lane & 15);
+          if (p >= 0) red[off + p] = acc[blk][r];
+        }
+    }
+  }
+};
+
 // Workgroup size per shape: 16 wavefronts (four per SIMD, 128 registers each)
 // unless both networks have two or more hidden layers on the matrix cores,
 // which need more registers per lane: then 8 wavefronts.
-template <int Lm, int Hm, int Lp, int Hp>
+template <int Lm, int Hm, int Lp, int Hp, int PC1 = 0>
 struct TileCfg {
-  static constexpr bool kWide = Lm >= 2 && Lp >= 2 && MfmaGrad<Hm>::value && MfmaGrad<Hp>::value;
+  static constexpr bool kWide = (Lm >= 2 && Lp >= 2 && MfmaGrad<Hm>::value && MfmaGrad<Hp>::value) || PC1 > 0;
   static constexpr int kThreads = kWide ? 512 : 1024;
   static constexpr int kWaves = kThreads / DIPS_WAVE;
   static constexpr size_t kLdsBytes = sizeof(float) * kStageFloats * kWaves;
@@ -328,16 +498,24 @@ struct TileCfg {
 // kOut: 0 = block-major partial rows (for dips_opt_step), 1 = parameter-major
 // rows (for tile_apply_kernel), 2 = parameter-major rows and the fused
 // last-workgroup reduction + optimizer.
-template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut>
-__global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step_kernel(
+//
+// With a preconditioner (PC1 > 0) the row's residual is r = P * r0 with
+// P = 0.5 + s*sigmoid(MLP(cell coefficients)) evaluated at the row's last
+// lane (trainer.py:845-847, discretization.py:418-419); the entries' seeds
+// become r * P * w and the preconditioner's parameters (after both networks)
+// receive r * r0 * dP/dtheta.
+template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1 = 0, int PC2 = 0>
+__global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile_step_kernel(
     TileStepArgs a, float* params, float* __restrict__ partial, FusedOpt fo) {
-  constexpr int kTileThreads = TileCfg<Lm, Hm, Lp, Hp>::kThreads;
-  constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp>::kWaves;
+  constexpr int kTileThreads = TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads;
+  constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
   using NP = NetLayout<Lp, Hp>;
   using NM = NetLayout<Lm, Hm>;
+  using PN = Precond<(PC1 > 0 ? PC1 : 1), PC2>;
   constexpr int PP = NP::size();
   constexpr int PM = NM::size();
-  constexpr int PT = PP + PM;
+  constexpr int PC = PC1 > 0 ? PN::size() : 0;
+  constexpr int PT = PP + PM + PC;
   extern __shared__ f4 tile_lds[];
   float* const lds = reinterpret_cast<float*>(tile_lds);
 
@@ -349,8 +527,10 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
 
   NetGrad<Lp, Hp, kGrad> gp;
   NetGrad<Lm, Hm, kGrad> gm;
+  PrecondGrad<PC1, PC2, kGrad> gc;
   gp.zero();
   gm.zero();
+  gc.zero();
   float lsum = 0.0f;
 
   const int nw = (int)gridDim.x * kTileWaves;
@@ -396,9 +576,33 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
       const float o = __shfl_up(v, d, DIPS_WAVE);
       if (lane - d >= head) v += o;
     }
-    const float r = __shfl(v, last, DIPS_WAVE);
+    float r = __shfl(v, last, DIPS_WAVE);
+    float rmul = r;  // d(0.5 r^2)/d(sum w u): r, times P when preconditioned
+    if constexpr (PC1 > 0) {
+      // the preconditioner at each row's last lane (other lanes: zero inputs,
+      // finite activations, zero seed)
+      const bool is_last = on && lane == last;
+      float cf[26] = {};
+      if (is_last) {
+        const int64_t row = clamp64(a.srow[slot_of(t)], 0, a.nrows - 1);
+        const float* cr = a.row_coeffs + row * 26;
+#pragma unroll
+        for (int i = 0; i < 26; ++i) cf[i] = cr[i];
+      }
+      dips_weight_t* Pc = Pp + PP + PM;
+      float a1[PC1], a2[PC2 > 0 ? PC2 : 1], sig;
+      PN::forward(Pc, cf, a1, a2, sig);
+      const float pre = __shfl(0.5f + a.pc_scale * sig, last, DIPS_WAVE);
+      const float r0 = r;
+      r = pre * r0;
+      rmul = r * pre;
+      if constexpr (kGrad) {
+        const float dz = is_last ? r * r0 * a.pc_scale * sig * (1.0f - sig) : 0.0f;
+        gc.add(Pc, AT, BT, lane, dz, cf, a1, a2);
+      }
+    }
     if (on && lane == last) lsum = fmaf(0.5f * r, r, lsum);
-    const float seed = r * w;
+    const float seed = rmul * w;
     gp.add(Pp, AT, BT, lane, on && plus, x, y, z, seed, actp);
     gm.add(Pm, AT, BT, lane, on && !plus, x, y, z, seed, actm);
   }
@@ -439,7 +643,28 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
           const float u = (code & 128u) ? mlp_forward<Lp, Hp>(Wp, x, y, z) : mlp_forward<Lm, Hm>(Wp + PP, x, y, z);
           acc = fmaf(a.ent_w[e], u, acc);
         }
-        const float r = dips_wave_sum(acc) + a.row_c[row];
+        float r = dips_wave_sum(acc) + a.row_c[row];
+        float rmul = r;
+        if constexpr (PC1 > 0) {
+          // every lane evaluates the row's preconditioner; lane 0 carries its seed
+          const float* Wraw = params;
+          asm volatile("" : "+s"(Wraw));
+          dips_weight_t* Pc = dips_weights(Wraw) + PP + PM;
+          float cf[26];
+          const float* cr = a.row_coeffs + row * 26;
+#pragma unroll
+          for (int i = 0; i < 26; ++i) cf[i] = cr[i];
+          float a1[PC1], a2[PC2 > 0 ? PC2 : 1], sig;
+          PN::forward(Pc, cf, a1, a2, sig);
+          const float pre = 0.5f + a.pc_scale * sig;
+          const float r0 = r;
+          r = pre * r0;
+          rmul = r * pre;
+          if constexpr (kGrad) {
+            const float dz = lane == 0 ? r * r0 * a.pc_scale * sig * (1.0f - sig) : 0.0f;
+            gc.add(Pc, AT, BT, lane, dz, cf, a1, a2);
+          }
+        }
         if (lane == 0) lsum = fmaf(0.5f * r, r, lsum);
         if (!kGrad) continue;
 #pragma unroll 1
@@ -460,7 +685,7 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
             x = px + (float)ox * a.hx;
             y = py + (float)oy * a.hy;
             z = pz + (float)oz * a.hz;
-            seed = r * a.ent_w[e];
+            seed = rmul * a.ent_w[e];
             plus = (code & 128u) != 0;
             if (plus)
               (void)mlp_forward_keep<Lp, Hp>(Wp, x, y, z, actp);
@@ -481,6 +706,7 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp>::kThreads)) void tile_step
   float* const red = lds;  // [kTileWaves][stride]
   gp.flush(red + wave * stride, 0, lane);
   gm.flush(red + wave * stride, PP, lane);
+  gc.flush(red + wave * stride, PP + PM, lane);
   {
     const float s = dips_wave_sum(lsum);
     if (lane == 0) red[wave * stride + PT] = s;
@@ -523,9 +749,14 @@ static int tile_blocks(int64_t ntiles, int waves) {
   return (int)(b < 1 ? 1 : (b > kMaxTileBlocks ? kMaxTileBlocks : b));
 }
 
+static bool tile_shape_is(const dips_net_t* n, int Lm, int Hm, int Lp, int Hp, int pc1, int pc2) {
+  const int w1 = n->pc_w1 > 0 ? n->pc_w1 : 0, w2 = n->pc_w1 > 0 ? n->pc_w2 : 0;
+  return n->Lm == Lm && n->Hm == Hm && n->Lp == Lp && n->Hp == Hp && w1 == pc1 && w2 == pc2;
+}
+
 int tile_supported(const dips_net_t* n) {
-#define X(A, B, C, D) \
-  if (n->Lm == A && n->Hm == B && n->Lp == C && n->Hp == D) return 1;
+#define X(A, B, C, D, E, F) \
+  if (tile_shape_is(n, A, B, C, D, E, F)) return 1;
   DIPS_TILE_SHAPES(X)
 #undef X
   return 0;
@@ -533,23 +764,25 @@ int tile_supported(const dips_net_t* n) {
 
 int64_t tile_partial_size(const dips_net_t* net) { return (int64_t)kMaxTileBlocks * partial_stride(net); }
 
-template <int Lm, int Hm, int Lp, int Hp>
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2>
 static void tile_launch_shape(const TileStepArgs& a, float* params, float* partial, int mode, const FusedOpt& fo,
                               int nblk, hipStream_t st) {
-  dim3 grid(nblk), block(TileCfg<Lm, Hm, Lp, Hp>::kThreads);
-  const size_t lds = TileCfg<Lm, Hm, Lp, Hp>::kLdsBytes;
+  dim3 grid(nblk), block(TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads);
+  const size_t lds = TileCfg<Lm, Hm, Lp, Hp, PC1>::kLdsBytes;
   // the dynamic LDS limit is raised once per kernel instantiation
 #define DIPS_TILE_LAUNCH(G, F)                                                                                   \
-  static const bool lds_set_##G##F = hipFuncSetAttribute((const void*)tile_step_kernel<Lm, Hm, Lp, Hp, G, F>,  \
-                                                         hipFuncAttributeMaxDynamicSharedMemorySize,            \
-                                                         (int)lds) == hipSuccess;                               \
+  static const bool lds_set_##G##F =                                                                             \
+      hipFuncSetAttribute((const void*)tile_step_kernel<Lm, Hm, Lp, Hp, G, F, PC1, PC2>,                         \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;                   \
   (void)lds_set_##G##F;                                                                                          \
-  hipLaunchKernelGGL((tile_step_kernel<Lm, Hm, Lp, Hp, G, F>), grid, block, lds, st, a, params, partial, fo)
+  hipLaunchKernelGGL((tile_step_kernel<Lm, Hm, Lp, Hp, G, F, PC1, PC2>), grid, block, lds, st, a, params, partial, \
+                     fo)
   if (mode == 2) {
     DIPS_TILE_LAUNCH(true, 2);
   } else if (mode == 3) {
     DIPS_TILE_LAUNCH(true, 1);
-    constexpr int kStride = NetLayout<Lp, Hp>::size() + NetLayout<Lm, Hm>::size() + 1;
+    constexpr int kStride =
+        NetLayout<Lp, Hp>::size() + NetLayout<Lm, Hm>::size() + (PC1 > 0 ? Precond<(PC1 > 0 ? PC1 : 1), PC2>::size() : 0) + 1;
     hipLaunchKernelGGL((tile_apply_kernel<kStride>), dim3(1), dim3(1024), 0, st, partial, nblk, fo, params);
   } else if (mode == 1) {
     DIPS_TILE_LAUNCH(true, 0);
@@ -575,25 +808,48 @@ int tile_step_launch(const dips_grid_t* g, const dips_records_t* rec, const dips
     set_error("tile step: grid or records too large for 32-bit point indices");
     return DIPS_EINVAL;
   }
+  const bool pc = net->pc_w1 > 0;
+  if (pc && (!tiles->slot_row || !rec->row_coeffs)) {
+    set_error("tile step: the preconditioner needs slot rows and per-row cell coefficients");
+    return DIPS_EINVAL;
+  }
   TileStepArgs a{point_set(g),
-             rec->hx,        rec->hy,        rec->hz,        rec->row_point, rec->row_ptr,  rec->row_c,
-             rec->nrows,     rec->ent_code,  rec->ent_w,     rec->nnz,       tiles->tile_row, tiles->slot_x,
-             tiles->slot_y,  tiles->slot_z,  tiles->slot_w,  tiles->slot_c,  tiles->slot_info, tile_begin,
-             (int)ntiles};
+                 rec->hx,
+                 rec->hy,
+                 rec->hz,
+                 rec->row_point,
+                 rec->row_ptr,
+                 rec->row_c,
+                 rec->nrows,
+                 rec->ent_code,
+                 rec->ent_w,
+                 rec->nnz,
+                 tiles->tile_row,
+                 tiles->slot_x,
+                 tiles->slot_y,
+                 tiles->slot_z,
+                 tiles->slot_w,
+                 tiles->slot_c,
+                 tiles->slot_info,
+                 pc ? tiles->slot_row : nullptr,
+                 pc ? rec->row_coeffs : nullptr,
+                 pc ? net->pc_scale : 0.0f,
+                 tile_begin,
+                 (int)ntiles};
   FusedOpt fo{};
   if (fused) fo = *fused;
   // with an optimizer: one fused launch when an arrival counter is given,
   // else the step kernel followed by tile_apply_kernel
   const int mode = fused ? (fused->sync ? 2 : 3) : (want_grad ? 1 : 0);
-#define X(A, B, C, D)                                                                                   \
-  if (net->Lm == A && net->Hm == B && net->Lp == C && net->Hp == D) {                                  \
-    const int nblk = tile_blocks(ntiles, TileCfg<A, B, C, D>::kWaves);                                 \
+#define X(A, B, C, D, E, F)                                                                             \
+  if (tile_shape_is(net, A, B, C, D, E, F)) {                                                          \
+    const int nblk = tile_blocks(ntiles, TileCfg<A, B, C, D, E>::kWaves);                              \
     if (partial_cap < (int64_t)nblk * partial_stride(net)) {                                           \
       set_error("tile step: partial buffer too small");                                                \
       return DIPS_EINVAL;                                                                              \
     }                                                                                                  \
     *nblk_out = nblk;                                                                                  \
-    tile_launch_shape<A, B, C, D>(a, const_cast<float*>(params), partial, mode, fo, nblk, stream);    \
+    tile_launch_shape<A, B, C, D, E, F>(a, const_cast<float*>(params), partial, mode, fo, nblk, stream); \
     return hip_status(hipGetLastError(), "tile_step_kernel launch");                                   \
   }
   DIPS_TILE_SHAPES(X)
@@ -638,6 +894,7 @@ struct SlotArgs {
   float* sw;
   float* sc;
   uint32_t* sinfo;
+  int32_t* srow;  // may be null
 };
 
 __global__ __launch_bounds__(kSlotWaves * DIPS_WAVE) void tile_slots_kernel(SlotArgs a) {
@@ -676,8 +933,10 @@ __global__ __launch_bounds__(kSlotWaves * DIPS_WAVE) void tile_slots_kernel(Slot
   const int tj = __shfl(tl, jj, DIPS_WAVE);
   uint32_t info = 0u;
   float x = 0.0f, y = 0.0f, z = 0.0f, w = 0.0f, c = 0.0f;
+  int32_t srow = -1;
   if (j >= 0) {
     const int64_t row = r0 + j;
+    srow = (int32_t)row;
     const int pt = a.row_point[row];
     if (point_xyz(a.ps, pt, x, y, z)) {
       const int head = sj, last = sj + lj - 1;
@@ -703,10 +962,12 @@ __global__ __launch_bounds__(kSlotWaves * DIPS_WAVE) void tile_slots_kernel(Slot
   a.sw[s] = w;
   a.sc[s] = c;
   a.sinfo[s] = info;
+  if (a.srow) a.srow[s] = srow;
 }
 
 int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
-                      float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, hipStream_t st) {
+                      float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, int32_t* srow,
+                      hipStream_t st) {
   if (rec->nrows >= INT_MAX || point_set(g).ngrid + point_set(g).nextra >= INT_MAX) {
     set_error("dips_tile_slots: grid or records too large for 32-bit indices");
     return DIPS_EINVAL;
@@ -714,7 +975,7 @@ int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int
   SlotArgs a{point_set(g),
              rec->hx,    rec->hy,       rec->hz,    rec->row_point, rec->row_ptr, rec->row_c,
              rec->nrows, rec->ent_code, rec->ent_w, rec->nnz, tile_row, nbounds,
-             sx,         sy,            sz,         sw,       sc,       sinfo};
+             sx,         sy,            sz,         sw,       sc,       sinfo,    srow};
   const int64_t nb = (nbounds + kSlotWaves - 1) / kSlotWaves;
   hipLaunchKernelGGL(tile_slots_kernel, dim3((unsigned)nb), dim3(kSlotWaves * DIPS_WAVE), 0, st, a);
   return hip_status(hipGetLastError(), "tile_slots_kernel launch");

@@ -27,11 +27,13 @@ class Grid(ctypes.Structure):
 class Records(ctypes.Structure):
     _fields_ = [("row_point", ctypes.c_void_p), ("row_ptr", ctypes.c_void_p), ("row_c", ctypes.c_void_p),
                 ("ent_code", ctypes.c_void_p), ("ent_w", ctypes.c_void_p), ("nrows", ctypes.c_int64),
-                ("nnz", ctypes.c_int64), ("hx", ctypes.c_float), ("hy", ctypes.c_float), ("hz", ctypes.c_float)]
+                ("nnz", ctypes.c_int64), ("hx", ctypes.c_float), ("hy", ctypes.c_float), ("hz", ctypes.c_float),
+                ("row_coeffs", ctypes.c_void_p)]
 
 
 class Net(ctypes.Structure):
-    _fields_ = [("Lm", ctypes.c_int32), ("Hm", ctypes.c_int32), ("Lp", ctypes.c_int32), ("Hp", ctypes.c_int32)]
+    _fields_ = [("Lm", ctypes.c_int32), ("Hm", ctypes.c_int32), ("Lp", ctypes.c_int32), ("Hp", ctypes.c_int32),
+                ("pc_w1", ctypes.c_int32), ("pc_w2", ctypes.c_int32), ("pc_scale", ctypes.c_float)]
 
 
 class Opt(ctypes.Structure):
@@ -49,7 +51,7 @@ class MfmaBatch(ctypes.Structure):
 class Tiles(ctypes.Structure):
     _fields_ = [("tile_row", ctypes.c_void_p), ("nbounds", ctypes.c_int64), ("slot_x", ctypes.c_void_p),
                 ("slot_y", ctypes.c_void_p), ("slot_z", ctypes.c_void_p), ("slot_w", ctypes.c_void_p),
-                ("slot_c", ctypes.c_void_p), ("slot_info", ctypes.c_void_p)]
+                ("slot_c", ctypes.c_void_p), ("slot_info", ctypes.c_void_p), ("slot_row", ctypes.c_void_p)]
 
 
 class Assemble(ctypes.Structure):
@@ -66,7 +68,8 @@ class Assemble(ctypes.Structure):
                 ("scratch", ctypes.c_void_p), ("ip_slot", ctypes.c_void_p), ("mode", ctypes.c_int32),
                 ("nnz", ctypes.c_void_p), ("ent_off", ctypes.c_void_p), ("row_of", ctypes.c_void_p),
                 ("ent_code", ctypes.c_void_p), ("ent_w", ctypes.c_void_p), ("row_c", ctypes.c_void_p),
-                ("npts", ctypes.c_int64), ("extra", ctypes.c_void_p), ("nextra", ctypes.c_int64)]
+                ("npts", ctypes.c_int64), ("extra", ctypes.c_void_p), ("nextra", ctypes.c_int64),
+                ("row_coeffs", ctypes.c_void_p)]
 
 
 # name -> (restype, argtypes); must match include/dips_hip.h
@@ -94,7 +97,7 @@ SIGNATURES = {
     "dips_eval": (ctypes.c_int, [ctypes.POINTER(Net), _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
     "dips_pack_tiles": (_I64, [_P, _I64, _P, _P, _I32, _P, _I64, _P]),
     "dips_tile_slots": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _I64, _P, _P, _P, _P, _P,
-                                       _P, _P]),
+                                       _P, _P, _P]),
     "dips_tile_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
     "dips_tile_partial_size": (_I64, [ctypes.POINTER(Net)]),
     "dips_tile_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(Tiles), _I64,

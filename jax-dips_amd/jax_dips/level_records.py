@@ -66,8 +66,12 @@ def _offsets(vals, device):
     return torch.tensor(vals, dtype=torch.float32, device=device)
 
 
-def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
+def build_level(grid, h, bounds, cbs, batch_size, device="cuda", coeffs=False):
     """DeviceRecords of one zoom level.
+
+    coeffs: also write the 26 cell coefficients of every row's point (the
+    learned preconditioner's input, rec.row_coeffs); the centre-cell geometry is
+    then computed for crossed box points too.
 
     grid: records.DeviceGrid (grid nodes and extra points, one row each); h: (3,) float32 stencil spacing; bounds: level-set
     grid bounds; cbs: dict of callbacks (phi, mu_m, mu_p, k_m, k_p, f_m, f_p,
@@ -104,7 +108,8 @@ def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
     cs_e = (cs_idx % 7).to(torch.int8).contiguous()
     cs_slot = torch.full((npts * 7,), -1, dtype=torch.int32, device=device)
     cs_slot[cs_idx] = torch.arange(ncs, dtype=torch.int32, device=device)
-    cc_pts = torch.nonzero(cut[:, 0], as_tuple=False).reshape(-1)
+    cc_mask = (region[:, 0] == 0) if coeffs else cut[:, 0]
+    cc_pts = torch.nonzero(cc_mask, as_tuple=False).reshape(-1)
     ncc = int(cc_pts.numel())
     cc_slot = torch.full((npts,), -1, dtype=torch.int32, device=device)
     cc_slot[cc_pts] = torch.arange(ncc, dtype=torch.int32, device=device)
@@ -194,7 +199,9 @@ def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
     ent_code = torch.empty(max(total, 1), dtype=torch.int32, device=device)
     ent_w = torch.empty(max(total, 1), dtype=torch.float32, device=device)
     row_c = torch.empty(npts, dtype=torch.float32, device=device)
+    row_coeffs = torch.empty((npts, 26), dtype=torch.float32, device=device) if coeffs else None
     A.mode = 1
+    A.row_coeffs = row_coeffs.data_ptr() if coeffs else None
     A.ent_off, A.row_of = ent_off.data_ptr(), row_of.data_ptr()
     A.ent_code, A.ent_w, A.row_c = ent_code.data_ptr(), ent_w.data_ptr(), row_c.data_ptr()
     N.check(L.dips_geom_assemble(ctypes.byref(A), st), "dips_geom_assemble(write)")
@@ -205,6 +212,6 @@ def build_level(grid, h, bounds, cbs, batch_size, device="cuda"):
     small = (nnz <= BULK_MAX_ENTRIES).to(torch.int64)
     nbulk = torch.zeros(nb, dtype=torch.int64, device=device).index_add_(0, pid // B, small).cpu().numpy()
     rec = DeviceRecords.from_device(order.to(torch.int32).contiguous(), row_ptr64.to(torch.int32).contiguous(), row_c,
-                                    ent_code, ent_w, total, h, B, begins, rows, nbulk)
+                                    ent_code, ent_w, total, h, B, begins, rows, nbulk, row_coeffs=row_coeffs)
     rec.stats = dict(npts=npts, ncs=ncs, ncc=ncc, nip=nip, nnz=total, boundary=int(bnd.sum().item()))
     return rec

@@ -143,11 +143,12 @@ class DeviceRecords:
         self.batch_begin = np.asarray(packed["batch_begin"], np.int64)
         self.batch_rows = np.asarray(packed["batch_rows"], np.int64)
         self.batch_nbulk = np.asarray(packed["batch_nbulk"], np.int64)
+        self.row_coeffs = None
         self._finish()
 
     @classmethod
     def from_device(cls, row_point, row_ptr, row_c, ent_code, ent_w, nnz, h, batch_size, batch_begin, batch_rows,
-                    batch_nbulk):
+                    batch_nbulk, row_coeffs=None):
         """Wrap records already in device memory (built by the geometry kernels)."""
         self = cls.__new__(cls)
         self.row_point, self.row_ptr, self.row_c = row_point, row_ptr, row_c
@@ -159,13 +160,26 @@ class DeviceRecords:
         self.batch_begin = np.asarray(batch_begin, np.int64)
         self.batch_rows = np.asarray(batch_rows, np.int64)
         self.batch_nbulk = np.asarray(batch_nbulk, np.int64)
+        self.row_coeffs = row_coeffs
         self._finish()
         return self
 
     def _finish(self):
         self.c = N.Records(self.row_point.data_ptr(), self.row_ptr.data_ptr(), self.row_c.data_ptr(),
                            self.ent_code.data_ptr(), self.ent_w.data_ptr(), self.nrows, self.nnz,
-                           float(self.h[0]), float(self.h[1]), float(self.h[2]))
+                           float(self.h[0]), float(self.h[1]), float(self.h[2]),
+                           self.row_coeffs.data_ptr() if self.row_coeffs is not None else None)
+
+    def set_point_coeffs(self, coeffs):
+        """Attach the 26 cell coefficients of every point ((N,26), point order)
+        as the per-row preconditioner input."""
+        import torch
+
+        C = torch.as_tensor(np.asarray(coeffs), device=self.row_point.device).to(torch.float32)
+        if C.shape != (self.nrows, 26):
+            raise ValueError(f"expected ({self.nrows}, 26) coefficients, got {tuple(C.shape)}")
+        self.row_coeffs = C[self.row_point.to(torch.int64)].contiguous()
+        self._finish()
 
     def mfma_batches(self):
         """Phase-sorted entry arrays (+ entries first inside each batch) and one
@@ -216,10 +230,11 @@ class DeviceRecords:
         nb = len(tile_row)
         slots = {k: torch.empty(nb * 64, dtype=torch.float32, device=dev) for k in ("x", "y", "z", "w", "c")}
         slots["info"] = torch.empty(nb * 64, dtype=torch.int32, device=dev)
+        slots["row"] = torch.empty(nb * 64, dtype=torch.int32, device=dev)
+        keys = ("x", "y", "z", "w", "c", "info", "row")
         N.check(L.dips_tile_slots(ctypes.byref(grid.c), ctypes.byref(self.c), N.ptr(tr), nb,
-                                  *(N.ptr(slots[k]) for k in ("x", "y", "z", "w", "c", "info")), N.current_stream()),
-                "dips_tile_slots")
-        c = N.Tiles(tr.data_ptr(), nb, *(slots[k].data_ptr() for k in ("x", "y", "z", "w", "c", "info")))
+                                  *(N.ptr(slots[k]) for k in keys), N.current_stream()), "dips_tile_slots")
+        c = N.Tiles(tr.data_ptr(), nb, *(slots[k].data_ptr() for k in keys))
         self._tiles = dict(tile_row=tr, batch_tile_begin=btb, slots=slots, c=c)
         return self._tiles
 
@@ -260,16 +275,30 @@ def make_opt(optimizer_dict):
 class Engine:
     """Parameters, optimizer state and scratch of one two-phase network on the GPU."""
 
-    def __init__(self, dims, params, optimizer_dict, max_rows, max_steps, device="cuda"):
+    def __init__(self, dims, params, optimizer_dict, max_rows, max_steps, device="cuda", precond=None):
+        """precond: None, or {"widths": [w1] or [w1, w2], "scale": s} for the
+        learned preconditioner; its parameters follow the two networks' in
+        params, and records then need per-row cell coefficients."""
         import torch
 
         self.dims = tuple(int(d) for d in dims)
-        self.net = N.Net(*self.dims)
+        self.precond = None
+        pc = (0, 0, 0.0)
+        if precond is not None:
+            widths = [int(w) for w in precond["widths"]]
+            if not 1 <= len(widths) <= 2 or min(widths) < 1:
+                raise ValueError(f"preconditioner widths must be one or two positive widths, got {widths}")
+            self.precond = dict(widths=widths, scale=float(precond.get("scale", 1.0)))
+            pc = (widths[0], widths[1] if len(widths) == 2 else 0, self.precond["scale"])
+        self.net = N.Net(*self.dims, *pc)
         L = N.lib()
         self.lane_path = bool(L.dips_net_supported(ctypes.byref(self.net)))
         self.tile_path = bool(L.dips_tile_supported(ctypes.byref(self.net)))
         self.mfma_path = bool(L.dips_mfma_supported(ctypes.byref(self.net)))
-        if not (self.lane_path or self.mfma_path):
+        if self.precond is not None and not self.tile_path:
+            raise N.DipsError(f"network shape {self.dims} with preconditioner widths {self.precond['widths']} "
+                              "has no compiled tile kernel")
+        if not (self.lane_path or self.mfma_path or self.tile_path):
             raise N.DipsError(f"network shape (Lm,Hm,Lp,Hp)={self.dims} has no compiled kernel")
         self.P = int(L.dips_param_count(ctypes.byref(self.net)))
         params = np.asarray(params, np.float32)
@@ -380,7 +409,7 @@ class Engine:
                                                int(self._mdwork.numel()), N.current_stream()), "dips_mfma_train_step")
 
     def train_level(self, grid, rec, nepochs):
-        if self.tile_path and self.use_tiles:
+        if self.tile_path and (self.use_tiles or self.precond is not None):
             return self.train_level_tiles(grid, rec, nepochs)
         if not self.lane_path:
             return self.train_level_mfma(grid, rec, nepochs)

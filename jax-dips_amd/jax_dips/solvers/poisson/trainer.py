@@ -23,8 +23,11 @@ What runs where (Trainer.solve):
     (trainer.py:960-977).
 A short last batch is padded with random points in the data's range, as the
 reference pads it (its draws come from jax's PRNGKey(0), these from numpy).
+The learned preconditioner (model_dict["preconditioner"], trainer.py:229-252)
+trains jointly with the networks inside the tile kernel; its parameters live
+under params["preconditioner"] and its initial draw comes from numpy.
 Unsupported reference options raise NotImplementedError: algorithm 1, lbfgs,
-the learned preconditioner, nonlinear operators and non-tanh activations.
+nonlinear operators and non-tanh activations.
 """
 import logging
 import os
@@ -39,6 +42,7 @@ from ...data import data_management
 from ...data.data_management import generate_random_points_like
 from ...level_records import build_level, sample
 from ...nn.configure import get_model
+from ...nn.preconditioner import Preconditioner
 from ...records import DeviceGrid, Engine
 from ..simulation_states import PoissonSimState, PoissonSimStateFn
 from ..._jaxmd_modules import dataclasses
@@ -81,8 +85,11 @@ class Trainer:
             raise NotImplementedError("algorithm 1 (network-based extrapolation) is not implemented in the reference either")
         if optimizer_dict.get("optimizer_name") == "lbfgs":
             raise NotImplementedError("lbfgs (jaxopt) has no device implementation")
-        if (model_dict.get("preconditioner") or {}).get("enable", False):
-            raise NotImplementedError("the learned preconditioner has no device implementation")
+        pc = model_dict.get("preconditioner") or {}
+        self.precond = None
+        if pc.get("enable", False):
+            self.precond = Preconditioner(Ds=pc["layer_widths"], out_dim=1,
+                                          scaling_coeff=pc.get("scaling_coeff", 1.0))
         if sim_state_fn.nonlinear_op_m is not None or sim_state_fn.nonlinear_op_p is not None:
             raise NotImplementedError("nonlinear operators have no device implementation")
         if multi_gpu:
@@ -112,11 +119,18 @@ class Trainer:
             self.epoch_start = int(state["epoch"])
             self.batch_size = int(state["batch_size"])
         elif init_params is not None:
-            self.params = init_params if isinstance(init_params, dict) else self.model.unflatten(init_params)
+            if isinstance(init_params, dict):
+                self.params = init_params
+            else:
+                flat = np.asarray(init_params, np.float32)
+                with_pc = self.precond is not None and flat.size > self._nn_size()
+                self.params = self.unflatten(flat) if with_pc else self.model.unflatten(flat)
         else:
             self.params = self.forward.init(42)
         self.params = dict(self.params)
         self.params.setdefault("preconditioner", {})
+        if self.precond is not None and not self.params["preconditioner"]:
+            self.params["preconditioner"] = self.precond.init(42)
         self.epoch_store = np.arange(self.epoch_start, self.num_epochs)
         self.loss_epochs = np.zeros(self.num_epochs - self.epoch_start, dtype=np.float32)
 
@@ -146,11 +160,27 @@ class Trainer:
             pickle.dump(state, f)
         return path
 
+    def flatten(self, tree):
+        """Flat device parameters: the two networks, then the preconditioner."""
+        flat = self.model.flatten(tree)
+        if self.precond is not None:
+            flat = np.concatenate([flat, self.precond.flatten(tree["preconditioner"])])
+        return flat
+
+    def _nn_size(self):
+        return sum(a * b + b for tag in ("p", "m") for a, b in self.model.shapes(tag))
+
+    def unflatten(self, flat):
+        flat = np.asarray(flat, np.float32)
+        nn = self._nn_size()
+        pre = self.precond.unflatten(flat[nn:]) if self.precond is not None else {}
+        return dict(self.model.unflatten(flat[:nn]), preconditioner=pre)
+
     def _state(self, engine, epoch):
-        return {"opt_state": {"mu": self.model.unflatten(engine.m.cpu().numpy()),
-                              "nu": self.model.unflatten(engine.v.cpu().numpy()),
+        return {"opt_state": {"mu": self.unflatten(engine.m.cpu().numpy()),
+                              "nu": self.unflatten(engine.v.cpu().numpy()),
                               "count": int(engine.count.item())},
-                "params": dict(self.model.unflatten(engine.params.cpu().numpy()), preconditioner={}),
+                "params": self.unflatten(engine.params.cpu().numpy()),
                 "epoch": int(epoch), "batch_size": self.batch_size,
                 "resolution": f"{self.train_dx}, {self.train_dy}, {self.train_dz}"}
 
@@ -172,11 +202,12 @@ class Trainer:
         bounds = (lv.xmin(), lv.xmax(), lv.ymin(), lv.ymax(), lv.zmin(), lv.zmax())
         E = self.num_epochs
         steps_total = (E - self.epoch_start) * nb
-        engine = Engine(self.model.dims, self.model.flatten(self.params), self.optimizer_dict, max_rows=B,
-                        max_steps=max(1, steps_total))
+        engine = Engine(self.model.dims, self.flatten(self.params), self.optimizer_dict, max_rows=B,
+                        max_steps=max(1, steps_total),
+                        precond=self.precond.engine_config() if self.precond is not None else None)
         if self.opt_restore is not None:
-            engine.m.copy_(torch.as_tensor(self.model.flatten(self.opt_restore["mu"]), device="cuda"))
-            engine.v.copy_(torch.as_tensor(self.model.flatten(self.opt_restore["nu"]), device="cuda"))
+            engine.m.copy_(torch.as_tensor(self.flatten(self.opt_restore["mu"]), device="cuda"))
+            engine.v.copy_(torch.as_tensor(self.flatten(self.opt_restore["nu"]), device="cuda"))
             engine.count.fill_(int(self.opt_restore["count"]))
         count0 = int(engine.count.item())
         prev = signal.signal(signal.SIGINT, signalHandler)
@@ -193,7 +224,7 @@ class Trainer:
                     last += 1
                 h = np.array(self.TD.alternate_res_sequentially(E, epoch, tr.dx, tr.dy, tr.dz), np.float32)
                 tr0 = time.time()
-                rec = build_level(grid, h, bounds, self.cbs, B)
+                rec = build_level(grid, h, bounds, self.cbs, B, coeffs=self.precond is not None)
                 torch.cuda.synchronize()
                 self.timing["records_s"] += time.time() - tr0
                 tt0 = time.time()
@@ -216,7 +247,7 @@ class Trainer:
             signal.signal(signal.SIGINT, prev)
         logger.info(f"solve took {time.time() - t0} (sec)")
         self.steps = int(engine.count.item()) - count0
-        self.params = dict(self.model.unflatten(engine.params.cpu().numpy()), preconditioner={})
+        self.params = self.unflatten(engine.params.cpu().numpy())
         self.save_checkpoint(self.checkpoint_dir, self._state(engine, done))
         self._plot()
         self.engine = engine

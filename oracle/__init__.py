@@ -14,6 +14,8 @@ repository root):
 * ``discretization`` — jax_dips/solvers/poisson/discretization.py:164-519
                        (per-point residual as an affine map of network outputs)
 * ``mlp``            — jax_dips/nn/mlp/MLP.py:93-139 (DoubleMLP forward + hand VJP)
+* ``precond``        — jax_dips/nn/preconditioner.py:10-35 and discretization.py:339, :418-419
+                       (learned per-point preconditioner P = 0.5 + s*sigmoid(MLP(26 coefficients)))
 * ``optim``          — jax_dips/solvers/optimizers.py:13-54 (optax 0.1.5 chain)
 * ``trainer``        — jax_dips/solvers/poisson/trainer.py:501-591, 783-789, 892-912
                        and jax_dips/data/data_management.py:70-160, 319-326

@@ -35,3 +35,55 @@ def test_host_only_queries():
     ps = lib.dips_partial_size(ctypes.byref(net), 256)
     assert ps > 0 and ps % 168 == 0  # whole partial rows of P + 1 floats
     assert lib.dips_version() >= 1
+
+
+STRUCTS = {"Grid": "dips_grid_t", "Records": "dips_records_t", "Net": "dips_net_t", "Opt": "dips_opt_t",
+           "Tiles": "dips_tiles_t", "MfmaBatch": "dips_mfma_batch_t", "Assemble": "dips_assemble_t"}
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """Size and every field offset of the ctypes mirrors equal the C header's
+    (a C program built with gcc prints offsetof/sizeof)."""
+    import shutil
+    import subprocess
+
+    import pytest
+    from jax_dips import _native
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dips_hip.h"', "int main(void) {"]
+    expect = []
+    for py, c in STRUCTS.items():
+        cls = getattr(_native, py)
+        lines.append(f'  printf("%zu\\n", sizeof({c}));')
+        expect.append(ctypes.sizeof(cls))
+        for name, _ in cls._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({c}, {name}));')
+            expect.append(getattr(cls, name).offset)
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == expect
+
+
+def test_precond_shapes_and_counts():
+    from jax_dips import _native
+
+    lib = _native.lib()
+    net = _native.Net(1, 1, 2, 10, 8, 4, 1.0)
+    # networks 167 + preconditioner 26*8+8 + 8*4+4 + 4+1
+    assert lib.dips_param_count(ctypes.byref(net)) == 167 + 216 + 36 + 5
+    assert lib.dips_tile_supported(ctypes.byref(net)) == 1
+    assert lib.dips_net_supported(ctypes.byref(net)) == 0  # only the tile kernel applies it
+    assert lib.dips_mfma_supported(ctypes.byref(net)) == 0
+    assert lib.dips_tile_supported(ctypes.byref(_native.Net(1, 1, 2, 10, 7, 4, 1.0))) == 0
+    one = _native.Net(1, 1, 2, 10, 8, 0, 0.5)
+    assert lib.dips_param_count(ctypes.byref(one)) == 167 + 216 + 9
+    assert lib.dips_tile_supported(ctypes.byref(one)) == 1
+    bad = _native.Net(1, 1, 2, 10, 8, 4, float("nan"))
+    assert lib.dips_param_count(ctypes.byref(bad)) < 0
