@@ -220,6 +220,41 @@ int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, co
                           int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
                           uint32_t* sync, void* stream);
 
+/* Batched training of many independent interface problems (one network shape,
+ * each problem with its own grid, records, tiles, parameters and optimizer
+ * state).  The reference trains one problem per process and runs several
+ * problems as a pool of processes (examples/biomolecules/main.py:400-432, each
+ * running the trainer of solvers/poisson/trainer.py); this entry point replaces
+ * that loop of trainer runs on one GPU.  Global step s launches one kernel over
+ * the tiles of batch (s mod nbatches) of every problem with
+ * s < nepochs * nbatches, then one reduction + optimizer workgroup per such
+ * problem, so every problem follows the trajectory dips_tile_train_level gives
+ * it alone (same kernels, block counts and reduction order). */
+typedef struct {
+  dips_grid_t grid;
+  dips_records_t rec;
+  dips_tiles_t tiles;
+  const int64_t* batch_tile_begin; /* host, nbatches + 1 (dips_pack_tiles) */
+  int32_t nbatches;
+  int64_t batch_size;
+  float* params;
+  float* m;
+  float* v;
+  int32_t* count;
+  float* loss_hist;
+  int32_t loss_cap;
+} dips_batch_problem_t;
+
+/* Bytes of device workspace dips_batch_train needs for these problems (< 0 on
+ * invalid input). */
+int64_t dips_batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net);
+
+/* nepochs epochs of every problem (probs: host array).  The problem table is
+ * copied into work (device, work_bytes >= dips_batch_workspace) on the stream,
+ * which is then synchronized once before the steps are enqueued. */
+int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                     const dips_opt_t* opt, void* work, int64_t work_bytes, void* stream);
+
 /* Network at m points pts (m,3): u (m), optionally grad_x u (m,3) and, with
  * normals (m,3), the normal derivative grad u . n (m).  phase (m) uint8:
  * 1 -> Omega+ network (phi >= 0), 0 -> Omega- network.  Replaces

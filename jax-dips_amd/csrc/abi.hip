@@ -249,6 +249,24 @@ int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, co
   return DIPS_OK;
 }
 
+int64_t dips_batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net) {
+  if (check_net(net)) return -1;
+  return batch_workspace(probs, nprob, net);
+}
+
+int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                     const dips_opt_t* opt, void* work, int64_t work_bytes, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if (!probs || nprob < 1) {
+    set_error("dips_batch_train: no problems");
+    return DIPS_EINVAL;
+  }
+  for (int p = 0; p < nprob; ++p)
+    if ((st = check_records(&probs[p].grid, &probs[p].rec))) return st;
+  return batch_train(probs, nprob, nepochs, net, opt, work, work_bytes, (hipStream_t)stream);
+}
+
 int dips_geom_regions(const double* phic, int64_t n7, int8_t* region, void* stream) {
   if (n7 < 0 || (n7 > 0 && (!phic || !region))) {
     set_error("dips_geom_regions: invalid arguments");

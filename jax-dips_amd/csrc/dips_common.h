@@ -17,6 +17,18 @@ struct PointSet {
   int64_t nextra;
 };
 
+// Loads and stores through explicitly global pointers: for pointers read from
+// memory the compiler would otherwise emit flat accesses (which also count
+// against the scalar-load wait counter).
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, int64_t i, T v) {
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
 // Coordinates of point pt; false if pt is not a point of the set.
 __device__ __forceinline__ bool point_xyz(const PointSet& s, int64_t pt, float& x, float& y, float& z) {
   if (pt < 0) return false;
@@ -25,16 +37,16 @@ __device__ __forceinline__ bool point_xyz(const PointSet& s, int64_t pt, float& 
     const int64_t ij = pt / s.nz;
     const int j = (int)(ij % s.ny);
     const int i = (int)(ij / s.ny);
-    x = s.xs[i];
-    y = s.ys[j];
-    z = s.zs[k];
+    x = gld(s.xs, i);
+    y = gld(s.ys, j);
+    z = gld(s.zs, k);
     return true;
   }
   const int64_t q = pt - s.ngrid;
   if (q >= s.nextra) return false;
-  x = s.extra[3 * q];
-  y = s.extra[3 * q + 1];
-  z = s.extra[3 * q + 2];
+  x = gld(s.extra, 3 * q);
+  y = gld(s.extra, 3 * q + 1);
+  z = gld(s.extra, 3 * q + 2);
   return true;
 }
 

@@ -39,6 +39,9 @@ int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_
                     float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, float* partial,
                     int64_t partial_cap, uint32_t* sync, hipStream_t stream);
 int64_t tile_partial_size(const dips_net_t* net);
+int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net);
+int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                const dips_opt_t* opt, void* work, int64_t work_bytes, hipStream_t stream);
 int tile_supported(const dips_net_t* net);
 int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
                       float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, int32_t* srow,

@@ -27,6 +27,9 @@
 // update (trainer.py:783-789), DoubleMLP (nn/mlp/MLP.py:93-139).
 #include <limits.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "dips_common.h"
 #include "dips_hip.h"
 #include "dips_mlp.h"
@@ -50,6 +53,7 @@ constexpr int kStageFloats = 2 * 16 * kLdsStride;     // A and B operand stacks 
   X(1, 3, 2, 10, 0, 0)       \
   X(1, 1, 1, 1, 0, 0)        \
   X(2, 10, 2, 10, 0, 0)      \
+  X(2, 16, 2, 16, 0, 0)      \
   X(1, 1, 2, 10, 8, 4)       \
   X(1, 1, 2, 10, 8, 0)       \
   X(2, 10, 2, 10, 8, 4)
@@ -109,11 +113,11 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
   for (int q = 0; q < kOwn; ++q) {
     const int p = threadIdx.x + q * kTileThreads;
     const int pc = p < PT ? p : PT - 1;
-    pv[q] = params[pc];
-    mv[q] = fo.m[pc];
-    vv[q] = fo.v[pc];
+    pv[q] = gld(params, pc);
+    mv[q] = gld(fo.m, pc);
+    vv[q] = gld(fo.v, pc);
   }
-  const int c = *fo.count;
+  const int c = gld(fo.count, 0);
   if (wave == 0) {
     // step-dependent scalars, computed once (powf is long) while the partial
     // rows are loaded by the other wavefronts
@@ -136,7 +140,7 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
         const int b = lane + i * DIPS_WAVE;
         const int pc = p < kStride ? p : kStride - 1;
         const int bc = b < nblk ? b : nblk - 1;
-        const float val = partial[(int64_t)pc * nblk + bc];
+        const float val = gld(partial, (int64_t)pc * nblk + bc);
         vals[qq][i] = b < nblk ? val : 0.0f;  // p >= kStride: a duplicate, never stored
       }
     }
@@ -167,14 +171,14 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
     const int p = threadIdx.x + q * kTileThreads;
     if (p < PT) {
       apply_update_value(fo.opt, lr_t, bc1, bc2, clip, gn, gsh[p], pv[q], mv[q], vv[q]);
-      params[p] = pv[q];
-      fo.m[p] = mv[q];
-      fo.v[p] = vv[q];
+      gst(params, p, pv[q]);
+      gst(fo.m, p, mv[q]);
+      gst(fo.v, p, vv[q]);
     }
   }
   if (threadIdx.x == 0) {
-    if (c >= 0 && c < fo.loss_cap) fo.loss_hist[c] = gsh[PT];
-    *fo.count = c + 1;
+    if (c >= 0 && c < fo.loss_cap) gst(fo.loss_hist, c, gsh[PT]);
+    gst(fo.count, 0, c + 1);
     if (fo.sync) *fo.sync = 0u;
   }
 }
@@ -197,10 +201,18 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 //             -> W0 (rows 0-2), b0 (row 3), Wout (col H), bout (row 3, col H)
 //   block l:  rows [a_{l-1}, 1]            cols [delta_l]
 //             -> W_l (rows 0..H-1), b_l (row H)
-// Smaller networks accumulate per lane in registers.
+// Smaller networks accumulate per lane in registers.  Width 13..16 leaves no
+// operand row for a bias: the weight matrices go to the matrix cores
+//   block 0:  rows [x, y, z, 1]   cols [delta_0]   -> W0, b0
+//   block l:  rows [a_{l-1}]      cols [delta_l]   -> W_l
+// and the biases b_l (l >= 1), Wout and bout accumulate per lane.
 template <int H>
 struct MfmaGrad {
   static constexpr bool value = H >= 8 && H <= 12;
+};
+template <int H>
+struct MfmaGradWide {
+  static constexpr bool value = H >= 13 && H <= 16;
 };
 
 
@@ -253,6 +265,43 @@ __device__ __forceinline__ void mfma_grad_chunk(float* AT, float* BT, int lane, 
   }
 }
 
+// The matrix-core part of one chunk for width 13..16 (MfmaGradWide).
+template <int L, int H>
+__device__ __forceinline__ void mfma_grad_chunk_wide(float* AT, float* BT, int lane, float x, float y, float z,
+                                                     const float (&act)[L][H], const float (&dl)[L][H],
+                                                     f4 (&acc)[L]) {
+  AT[0 * kLdsStride + lane] = x;
+  AT[1 * kLdsStride + lane] = y;
+  AT[2 * kLdsStride + lane] = z;
+  AT[3 * kLdsStride + lane] = 1.0f;
+#pragma unroll
+  for (int j = 0; j < H; ++j) BT[j * kLdsStride + lane] = dl[0][j];
+  __builtin_amdgcn_wave_barrier();
+  stage_mfma(AT, BT, lane, acc[0]);
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+#pragma unroll
+    for (int i = 0; i < H; ++i) AT[i * kLdsStride + lane] = act[l - 1][i];
+#pragma unroll
+    for (int j = 0; j < H; ++j) BT[j * kLdsStride + lane] = dl[l][j];
+    __builtin_amdgcn_wave_barrier();
+    stage_mfma(AT, BT, lane, acc[l]);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+template <int L, int H>
+__device__ __forceinline__ int mfma_grad_param_wide(int blk, int i, int j) {
+  using NL = NetLayout<L, H>;
+  if (j >= H) return -1;
+  if (blk == 0) {
+    if (i < 3) return NL::w(0) + i * H + j;
+    return (i == 3) ? NL::b(0) + j : -1;
+  }
+  return (i < H) ? NL::w(blk) + i * H + j : -1;
+}
+
 // Parameter index of element (i, j) of accumulator block blk, or -1 if unused.
 template <int L, int H>
 __device__ __forceinline__ int mfma_grad_param(int blk, int i, int j) {
@@ -272,14 +321,17 @@ __device__ __forceinline__ int mfma_grad_param(int blk, int i, int j) {
 // One phase network's gradient accumulators: MFMA blocks or per-lane registers.
 template <int L, int H, bool kGrad>
 struct NetGrad {
-  static constexpr bool kMfma = MfmaGrad<H>::value;
+  static constexpr bool kWide = MfmaGradWide<H>::value;
+  static constexpr bool kMfma = MfmaGrad<H>::value || kWide;
   static constexpr int P = NetLayout<L, H>::size();
-  float g[kGrad && !kMfma ? P : 1];
+  // per-lane sums of the wide layout: b_1 .. b_{L-1}, Wout, bout
+  static constexpr int kLane = (L - 1) * H + H + 1;
+  float g[kGrad && !kMfma ? P : (kGrad && kWide ? kLane : 1)];
   f4 acc[kGrad && kMfma ? L : 1];
 
   __device__ __forceinline__ void zero() {
 #pragma unroll
-    for (int i = 0; i < (kGrad && !kMfma ? P : 1); ++i) g[i] = 0.0f;
+    for (int i = 0; i < (kGrad && !kMfma ? P : (kGrad && kWide ? kLane : 1)); ++i) g[i] = 0.0f;
 #pragma unroll
     for (int i = 0; i < (kGrad && kMfma ? L : 1); ++i) acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
   }
@@ -293,7 +345,19 @@ struct NetGrad {
     // instead of keeping the forward pass's loads live in registers
     dips_weight_t* Q = P_;
     asm volatile("" : "+s"(Q));
-    if constexpr (kGrad && kMfma) {
+    if constexpr (kGrad && kWide) {
+      const float s = mine ? seed : 0.0f;
+      float dl[L][H];
+      mlp_deltas<L, H>(Q, s, act, dl);
+#pragma unroll
+      for (int l = 1; l < L; ++l)
+#pragma unroll
+        for (int j = 0; j < H; ++j) g[(l - 1) * H + j] += dl[l][j];
+#pragma unroll
+      for (int i = 0; i < H; ++i) g[(L - 1) * H + i] = fmaf(s, act[L - 1][i], g[(L - 1) * H + i]);
+      g[kLane - 1] += s;
+      mfma_grad_chunk_wide<L, H>(AT, BT, lane, x, y, z, act, dl, acc);
+    } else if constexpr (kGrad && kMfma) {
       const float s = mine ? seed : 0.0f;
       float dl[L][H];
       mlp_deltas<L, H>(Q, s, act, dl);
@@ -305,7 +369,22 @@ struct NetGrad {
 
   // Writes this wavefront's sums into red[off .. off + P).
   __device__ __forceinline__ void flush(float* red, int off, int lane) {
-    if constexpr (kGrad && kMfma) {
+    if constexpr (kGrad && kWide) {
+      using NL = NetLayout<L, H>;
+#pragma unroll
+      for (int blk = 0; blk < L; ++blk)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int p = mfma_grad_param_wide<L, H>(blk, 4 * (lane >> 4) + r, lane & 15);
+          if (p >= 0) red[off + p] = acc[blk][r];
+        }
+#pragma unroll
+      for (int q = 0; q < kLane; ++q) {
+        const float s = dips_wave_sum(g[q]);
+        const int p = q < (L - 1) * H ? NL::b(1 + q / H) + q % H : NL::wout() + (q - (L - 1) * H);
+        if (lane == 0) red[off + p] = s;
+      }
+    } else if constexpr (kGrad && kMfma) {
 #pragma unroll
       for (int blk = 0; blk < L; ++blk)
 #pragma unroll
@@ -489,7 +568,8 @@ struct PrecondGrad {
 // which need more registers per lane: then 8 wavefronts.
 template <int Lm, int Hm, int Lp, int Hp, int PC1 = 0>
 struct TileCfg {
-  static constexpr bool kWide = (Lm >= 2 && Lp >= 2 && MfmaGrad<Hm>::value && MfmaGrad<Hp>::value) || PC1 > 0;
+  static constexpr bool kWide = (Lm >= 2 && Lp >= 2 && MfmaGrad<Hm>::value && MfmaGrad<Hp>::value) || PC1 > 0 ||
+                                MfmaGradWide<Hm>::value || MfmaGradWide<Hp>::value;
   static constexpr int kThreads = kWide ? 512 : 1024;
   static constexpr int kWaves = kThreads / DIPS_WAVE;
   static constexpr size_t kLdsBytes = sizeof(float) * kStageFloats * kWaves;
@@ -504,9 +584,13 @@ struct TileCfg {
 // lane (trainer.py:845-847, discretization.py:418-419); the entries' seeds
 // become r * P * w and the preconditioner's parameters (after both networks)
 // receive r * r0 * dP/dtheta.
-template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1 = 0, int PC2 = 0>
-__global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile_step_kernel(
-    TileStepArgs a, float* params, float* __restrict__ partial, FusedOpt fo) {
+//
+// The body of one workgroup: block blk of nblk workgroups over the tiles of
+// a.  Writes the workgroup's partial row (kOut 0: partial[blk * stride + p],
+// else partial[p * nblk + blk]).
+template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1, int PC2>
+__device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* params, float* __restrict__ partial,
+                                               int blk, int nblk) {
   constexpr int kTileThreads = TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads;
   constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
   using NP = NetLayout<Lp, Hp>;
@@ -516,6 +600,7 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
   constexpr int PM = NM::size();
   constexpr int PC = PC1 > 0 ? PN::size() : 0;
   constexpr int PT = PP + PM + PC;
+  constexpr bool kSameShape = Lm == Lp && Hm == Hp;
   extern __shared__ f4 tile_lds[];
   float* const lds = reinterpret_cast<float*>(tile_lds);
 
@@ -533,14 +618,14 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
   gc.zero();
   float lsum = 0.0f;
 
-  const int nw = (int)gridDim.x * kTileWaves;
-  const int t0 = (int)blockIdx.x * kTileWaves + wave;
+  const int nw = nblk * kTileWaves;
+  const int t0 = blk * kTileWaves + wave;
   // slot data of the wavefront's next tile is loaded one tile ahead (the last
   // prefetch re-reads the final tile, unused)
   auto slot_of = [&](int t) { return (a.tile_begin + min(t, a.ntiles - 1)) * kTileCap + lane; };
   int64_t s0 = slot_of(t0);
-  uint32_t n_info = a.sinfo[s0];
-  float n_x = a.sx[s0], n_y = a.sy[s0], n_z = a.sz[s0], n_w = a.sw[s0], n_c = a.sc[s0];
+  uint32_t n_info = gld(a.sinfo, s0);
+  float n_x = gld(a.sx, s0), n_y = gld(a.sy, s0), n_z = gld(a.sz, s0), n_w = gld(a.sw, s0), n_c = gld(a.sc, s0);
   for (int t = t0; t < a.ntiles; t += nw) {
     // weights are re-read per tile (scalar loads that hit the scalar cache)
     // rather than held in vector registers across the loop
@@ -552,12 +637,12 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
     const float x = n_x, y = n_y, z = n_z, w = n_w, c = n_c;
     {
       const int64_t sn = slot_of(t + nw);
-      n_info = a.sinfo[sn];
-      n_x = a.sx[sn];
-      n_y = a.sy[sn];
-      n_z = a.sz[sn];
-      n_w = a.sw[sn];
-      n_c = a.sc[sn];
+      n_info = gld(a.sinfo, sn);
+      n_x = gld(a.sx, sn);
+      n_y = gld(a.sy, sn);
+      n_z = gld(a.sz, sn);
+      n_w = gld(a.sw, sn);
+      n_c = gld(a.sc, sn);
     }
     const bool on = (info & kSlotUsed) != 0;
     const bool plus = (info & kSlotPlus) != 0;
@@ -566,7 +651,15 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
     float actp[Lp][Hp] = {};
     float actm[Lm][Hm] = {};
     float v = 0.0f;
-    if (on) {
+    if constexpr (kSameShape) {
+      // the two phase networks in separate branches: merged into one call
+      // they would read the weights through a per-lane pointer
+      float up = 0.0f, um = 0.0f;
+      if (on && plus) up = mlp_forward_keep<Lp, Hp>(Pp, x, y, z, actp);
+      asm volatile("" : "+s"(Pm));
+      if (on && !plus) um = mlp_forward_keep<Lm, Hm>(Pm, x, y, z, actm);
+      if (on) v = fmaf(w, plus ? up : um, c);
+    } else if (on) {
       const float u = plus ? mlp_forward_keep<Lp, Hp>(Pp, x, y, z, actp) : mlp_forward_keep<Lm, Hm>(Pm, x, y, z, actm);
       v = fmaf(w, u, c);  // c is nonzero only in the row's last slot
     }
@@ -584,10 +677,10 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
       const bool is_last = on && lane == last;
       float cf[26] = {};
       if (is_last) {
-        const int64_t row = clamp64(a.srow[slot_of(t)], 0, a.nrows - 1);
+        const int64_t row = clamp64(gld(a.srow, slot_of(t)), 0, a.nrows - 1);
         const float* cr = a.row_coeffs + row * 26;
 #pragma unroll
-        for (int i = 0; i < 26; ++i) cf[i] = cr[i];
+        for (int i = 0; i < 26; ++i) cf[i] = gld(cr, i);
       }
       dips_weight_t* Pc = Pp + PP + PM;
       float a1[PC1], a2[PC2 > 0 ? PC2 : 1], sig;
@@ -610,40 +703,51 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
   // chunk; the gradient pass recomputes the forward activations.  A loop of
   // its own so that its register demand does not add to the packed path's.
 #pragma unroll 1
-  for (int t = (int)blockIdx.x * kTileWaves + wave; t < a.ntiles; t += nw) {
+  for (int t = t0; t < a.ntiles; t += nw) {
     const int64_t tb = a.tile_begin + t;
-    const int64_t r0 = clamp64(a.tile_row[tb], 0, a.nrows);
-    const int64_t r1 = clamp64(a.tile_row[tb + 1], r0, a.nrows);
+    const int64_t r0 = clamp64(gld(a.tile_row, tb), 0, a.nrows);
+    const int64_t r1 = clamp64(gld(a.tile_row, tb + 1), r0, a.nrows);
     // the tiles without slots: more than 64 rows or slots (tile_slots_kernel)
     if (r1 - r0 <= kTileCap) {
       int len = 0;
-      if (lane < r1 - r0) len = max(a.row_ptr[r0 + lane + 1] - a.row_ptr[r0 + lane], 1);
+      if (lane < r1 - r0) len = max(gld(a.row_ptr, r0 + lane + 1) - gld(a.row_ptr, r0 + lane), 1);
       if (dips_wave_sum_i(len) <= kTileCap) continue;
     }
     {
 #pragma unroll 1
       for (int64_t row = r0; row < r1; ++row) {
-        const int pt = a.row_point[row];
+        const int pt = gld(a.row_point, row);
         float px, py, pz;
         if (!point_xyz(a.ps, pt, px, py, pz)) continue;
-        const int64_t beg = clamp64(a.row_ptr[row], 0, a.nnz);
-        const int64_t end = clamp64(a.row_ptr[row + 1], beg, a.nnz);
+        const int64_t beg = clamp64(gld(a.row_ptr, row), 0, a.nnz);
+        const int64_t end = clamp64(gld(a.row_ptr, row + 1), beg, a.nnz);
         float acc = 0.0f;
 #pragma unroll 1
         for (int64_t e = beg + lane; e < end; e += DIPS_WAVE) {
           const float* Wraw = params;  // opaque per chunk: no weights hoisted into registers
           asm volatile("" : "+s"(Wraw));
           dips_weight_t* Wp = dips_weights(Wraw);
-          const uint32_t code = a.ent_code[e];
+          const uint32_t code = gld(a.ent_code, e);
           int ox, oy, oz;
           dips_decode_offset(code, ox, oy, oz);
           const float x = px + (float)ox * a.hx;
           const float y = py + (float)oy * a.hy;
           const float z = pz + (float)oz * a.hz;
-          const float u = (code & 128u) ? mlp_forward<Lp, Hp>(Wp, x, y, z) : mlp_forward<Lm, Hm>(Wp + PP, x, y, z);
-          acc = fmaf(a.ent_w[e], u, acc);
+          const bool pl = (code & 128u) != 0;
+          float u;
+          if constexpr (kSameShape) {
+            dips_weight_t* Wm = Wp + PP;
+            float up = 0.0f, um = 0.0f;
+            if (pl) up = mlp_forward<Lp, Hp>(Wp, x, y, z);
+            asm volatile("" : "+s"(Wm));
+            if (!pl) um = mlp_forward<Lm, Hm>(Wm, x, y, z);
+            u = pl ? up : um;
+          } else {
+            u = pl ? mlp_forward<Lp, Hp>(Wp, x, y, z) : mlp_forward<Lm, Hm>(Wp + PP, x, y, z);
+          }
+          acc = fmaf(gld(a.ent_w, e), u, acc);
         }
-        float r = dips_wave_sum(acc) + a.row_c[row];
+        float r = dips_wave_sum(acc) + gld(a.row_c, row);
         float rmul = r;
         if constexpr (PC1 > 0) {
           // every lane evaluates the row's preconditioner; lane 0 carries its seed
@@ -653,7 +757,7 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
           float cf[26];
           const float* cr = a.row_coeffs + row * 26;
 #pragma unroll
-          for (int i = 0; i < 26; ++i) cf[i] = cr[i];
+          for (int i = 0; i < 26; ++i) cf[i] = gld(cr, i);
           float a1[PC1], a2[PC2 > 0 ? PC2 : 1], sig;
           PN::forward(Pc, cf, a1, a2, sig);
           const float pre = 0.5f + a.pc_scale * sig;
@@ -679,21 +783,28 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
           float actp[Lp][Hp] = {};
           float actm[Lm][Hm] = {};
           if (on) {
-            const uint32_t code = a.ent_code[e];
+            const uint32_t code = gld(a.ent_code, e);
             int ox, oy, oz;
             dips_decode_offset(code, ox, oy, oz);
             x = px + (float)ox * a.hx;
             y = py + (float)oy * a.hy;
             z = pz + (float)oz * a.hz;
-            seed = rmul * a.ent_w[e];
+            seed = rmul * gld(a.ent_w, e);
             plus = (code & 128u) != 0;
+          }
+          dips_weight_t* Wm = Wp + PP;
+          if constexpr (kSameShape) {
+            if (on && plus) (void)mlp_forward_keep<Lp, Hp>(Wp, x, y, z, actp);
+            asm volatile("" : "+s"(Wm));
+            if (on && !plus) (void)mlp_forward_keep<Lm, Hm>(Wm, x, y, z, actm);
+          } else if (on) {
             if (plus)
               (void)mlp_forward_keep<Lp, Hp>(Wp, x, y, z, actp);
             else
-              (void)mlp_forward_keep<Lm, Hm>(Wp + PP, x, y, z, actm);
+              (void)mlp_forward_keep<Lm, Hm>(Wm, x, y, z, actm);
           }
           gp.add(Wp, AT, BT, lane, on && plus, x, y, z, seed, actp);
-          gm.add(Wp + PP, AT, BT, lane, on && !plus, x, y, z, seed, actm);
+          gm.add(Wm, AT, BT, lane, on && !plus, x, y, z, seed, actm);
         }
       }
     }
@@ -712,7 +823,6 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
     if (lane == 0) red[wave * stride + PT] = s;
   }
   __syncthreads();
-  const int nblk = (int)gridDim.x;
   for (int p = threadIdx.x; p < stride; p += kTileThreads) {
     float s = 0.0f;
     if (kGrad || p == PT) {
@@ -720,10 +830,25 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
       for (int w = 0; w < kTileWaves; ++w) s += red[w * stride + p];
     }
     if (kOut == 0)
-      partial[(int64_t)blockIdx.x * stride + p] = s;
+      gst(partial, (int64_t)blk * stride + p, s);
     else
-      partial[(int64_t)p * nblk + blockIdx.x] = s;
+      gst(partial, (int64_t)p * nblk + blk, s);
   }
+}
+
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2>
+constexpr int tile_stride() {
+  return NetLayout<Lp, Hp>::size() + NetLayout<Lm, Hm>::size() +
+         (PC1 > 0 ? Precond<(PC1 > 0 ? PC1 : 1), PC2>::size() : 0) + 1;
+}
+
+template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1 = 0, int PC2 = 0>
+__global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile_step_kernel(
+    TileStepArgs a, float* params, float* __restrict__ partial, FusedOpt fo) {
+  constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
+  constexpr int stride = tile_stride<Lm, Hm, Lp, Hp, PC1, PC2>();
+  const int nblk = (int)gridDim.x;
+  tile_step_body<Lm, Hm, Lp, Hp, kGrad, kOut, PC1, PC2>(a, params, partial, (int)blockIdx.x, nblk);
   if (kOut != 2) return;
 
   // One thread per workgroup fences and counts: the barrier completes the
@@ -744,7 +869,7 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
 }
 
 // workgroups of a launch over ntiles tiles with waves wavefronts per workgroup
-static int tile_blocks(int64_t ntiles, int waves) {
+__host__ __device__ static inline int tile_blocks(int64_t ntiles, int waves) {
   int64_t b = (ntiles + waves - 1) / waves;
   return (int)(b < 1 ? 1 : (b > kMaxTileBlocks ? kMaxTileBlocks : b));
 }
@@ -865,6 +990,237 @@ int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_
   FusedOpt fo{*opt, inv_batch, m, v, count, loss_hist, loss_cap, sync};
   int nb = 0;
   return tile_step_launch(g, rec, tiles, tile_begin, ntiles, net, params, partial, partial_cap, 1, &nb, stream, &fo);
+}
+
+// ---------------------------------------------------------------------------
+// Batched training of many problems.  Problem p owns the workgroups
+// [blk_begin, blk_begin + blk_count) of every step launch (blk_count = the
+// largest block count over its batches); at global step s it trains batch
+// s mod nbatches while s < steps.  Its partial rows and parameters are its own,
+// so one problem's step is exactly the single-problem two-launch step.
+struct BatchDesc {
+  TileStepArgs a;                   // tile_begin / ntiles are set per step
+  const int64_t* batch_tile_begin;  // device copy, nbatches + 1
+  int32_t nbatches;
+  int32_t steps;
+  int32_t blk_begin;
+  int32_t blk_count;
+  float* params;
+  float* partial;                   // blk_count * stride floats
+  FusedOpt fo;
+};
+
+// the step's tile range and block count of problem d at global step s
+__device__ __forceinline__ bool batch_step_range(const BatchDesc& d, int s, int waves, int64_t& tb, int& nt,
+                                                 int& nblk) {
+  if (s >= d.steps) return false;
+  const int b = s % d.nbatches;
+  tb = gld(d.batch_tile_begin, b);
+  nt = (int)(gld(d.batch_tile_begin, b + 1) - tb - 1);
+  nblk = tile_blocks(nt, waves);
+  return true;
+}
+
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2>
+__global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile_batch_step_kernel(
+    const BatchDesc* __restrict__ descs, int nprob, int step) {
+  constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
+  // owner problem of this workgroup: the last p with blk_begin <= blockIdx.x
+  const int bx = (int)blockIdx.x;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].blk_begin <= bx)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const BatchDesc& d = descs[lo];
+  int64_t tb;
+  int nt, nblk;
+  if (!batch_step_range(d, step, kTileWaves, tb, nt, nblk)) return;
+  const int k = bx - d.blk_begin;
+  if (k >= nblk) return;
+  TileStepArgs a = d.a;
+  a.tile_begin = tb;
+  a.ntiles = nt;
+  tile_step_body<Lm, Hm, Lp, Hp, true, 1, PC1, PC2>(a, d.params, d.partial, k, nblk);
+}
+
+template <int kStride, int kWaves>
+__global__ __launch_bounds__(1024) void tile_batch_apply_kernel(const BatchDesc* __restrict__ descs, int step) {
+  const BatchDesc& d = descs[blockIdx.x];
+  int64_t tb;
+  int nt, nblk;
+  if (!batch_step_range(d, step, kWaves, tb, nt, nblk)) return;
+  tile_reduce_apply<kStride, 16>(d.partial, nblk, d.fo, d.params);
+}
+
+namespace {
+struct BatchLayout {
+  int64_t desc_off, table_off, partial_off, bytes;
+  int32_t nblk_total;
+  std::vector<int32_t> blk_count;
+};
+
+constexpr int64_t kAlign = 256;
+int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+int tile_waves_of(const dips_net_t* net) {
+#define X(A, B, C, D, E, F) \
+  if (tile_shape_is(net, A, B, C, D, E, F)) return TileCfg<A, B, C, D, E>::kWaves;
+  DIPS_TILE_SHAPES(X)
+#undef X
+  return 0;
+}
+
+int batch_layout(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net, BatchLayout& L) {
+  const int waves = tile_waves_of(net);
+  if (!waves) {
+    set_error("batch train: network shape has no compiled tile kernel");
+    return DIPS_EUNSUPPORTED;
+  }
+  if (!probs || nprob < 1) {
+    set_error("batch train: no problems");
+    return DIPS_EINVAL;
+  }
+  const int64_t stride = partial_stride(net);
+  int64_t tables = 0, partials = 0, blocks = 0;
+  L.blk_count.assign(nprob, 0);
+  for (int p = 0; p < nprob; ++p) {
+    const dips_batch_problem_t& q = probs[p];
+    if (!q.batch_tile_begin || q.nbatches < 1 || q.batch_size < 1) {
+      set_error("batch train: problem " + std::to_string(p) + ": invalid batch table");
+      return DIPS_EINVAL;
+    }
+    int cnt = 1;
+    for (int b = 0; b < q.nbatches; ++b) {
+      const int64_t nt = q.batch_tile_begin[b + 1] - q.batch_tile_begin[b] - 1;
+      if (nt < 1 || q.batch_tile_begin[b] < 0 || q.batch_tile_begin[b + 1] > q.tiles.nbounds) {
+        set_error("batch train: problem " + std::to_string(p) + ": batch without tiles or outside the tile table");
+        return DIPS_EINVAL;
+      }
+      cnt = std::max(cnt, tile_blocks(nt, waves));
+    }
+    L.blk_count[p] = cnt;
+    blocks += cnt;
+    tables += q.nbatches + 1;
+    partials += (int64_t)cnt * stride;
+  }
+  if (blocks > INT_MAX / 2) {
+    set_error("batch train: too many workgroups");
+    return DIPS_EINVAL;
+  }
+  L.nblk_total = (int32_t)blocks;
+  L.desc_off = 0;
+  L.table_off = align_up((int64_t)sizeof(BatchDesc) * nprob);
+  L.partial_off = align_up(L.table_off + (int64_t)sizeof(int64_t) * tables);
+  L.bytes = align_up(L.partial_off + (int64_t)sizeof(float) * partials);
+  return DIPS_OK;
+}
+}  // namespace
+
+int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net) {
+  BatchLayout L;
+  if (batch_layout(probs, nprob, net, L)) return -1;
+  return L.bytes;
+}
+
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2>
+static void batch_launch_shape(const BatchDesc* descs, int nprob, int nblk_total, int step, hipStream_t st) {
+  constexpr int kWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
+  const size_t lds = TileCfg<Lm, Hm, Lp, Hp, PC1>::kLdsBytes;
+  static const bool lds_set =
+      hipFuncSetAttribute((const void*)tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
+  (void)lds_set;
+  hipLaunchKernelGGL((tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2>), dim3(nblk_total),
+                     dim3(TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads), lds, st, descs, nprob, step);
+  constexpr int kStride = tile_stride<Lm, Hm, Lp, Hp, PC1, PC2>();
+  hipLaunchKernelGGL((tile_batch_apply_kernel<kStride, kWaves>), dim3(nprob), dim3(1024), 0, st, descs, step);
+}
+
+int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                const dips_opt_t* opt, void* work, int64_t work_bytes, hipStream_t stream) {
+  BatchLayout L;
+  int st = batch_layout(probs, nprob, net, L);
+  if (st) return st;
+  if (!work || work_bytes < L.bytes) {
+    set_error("batch train: workspace too small (need " + std::to_string(L.bytes) + " bytes)");
+    return DIPS_EINVAL;
+  }
+  if (nepochs < 0 || !opt) {
+    set_error("batch train: invalid epochs or optimizer");
+    return DIPS_EINVAL;
+  }
+  const bool pc = net->pc_w1 > 0;
+  char* base = static_cast<char*>(work);
+  std::vector<BatchDesc> descs(nprob);
+  std::vector<int64_t> tables;
+  int32_t blk = 0;
+  int64_t maxsteps = 0;
+  int64_t poff = 0;
+  const int64_t stride = partial_stride(net);
+  for (int p = 0; p < nprob; ++p) {
+    const dips_batch_problem_t& q = probs[p];
+    if (q.rec.nrows >= INT_MAX || point_set(&q.grid).ngrid + point_set(&q.grid).nextra >= INT_MAX) {
+      set_error("batch train: problem " + std::to_string(p) + ": too large for 32-bit point indices");
+      return DIPS_EINVAL;
+    }
+    const dips_tiles_t& t = q.tiles;
+    if (!t.tile_row || !t.slot_x || !t.slot_y || !t.slot_z || !t.slot_w || !t.slot_c || !t.slot_info ||
+        (pc && (!t.slot_row || !q.rec.row_coeffs)) || !q.params || !q.m || !q.v || !q.count || !q.loss_hist) {
+      set_error("batch train: problem " + std::to_string(p) + ": null buffer");
+      return DIPS_EINVAL;
+    }
+    const int64_t steps = (int64_t)nepochs * q.nbatches;
+    if (steps > INT_MAX) {
+      set_error("batch train: too many steps");
+      return DIPS_EINVAL;
+    }
+    maxsteps = std::max(maxsteps, steps);
+    BatchDesc& d = descs[p];
+    d.a = TileStepArgs{point_set(&q.grid), q.rec.hx,  q.rec.hy,      q.rec.hz,      q.rec.row_point,
+                       q.rec.row_ptr,     q.rec.row_c, q.rec.nrows,  q.rec.ent_code, q.rec.ent_w,
+                       q.rec.nnz,         t.tile_row, t.slot_x,      t.slot_y,      t.slot_z,
+                       t.slot_w,          t.slot_c,   t.slot_info,   pc ? t.slot_row : nullptr,
+                       pc ? q.rec.row_coeffs : nullptr, pc ? net->pc_scale : 0.0f, 0, 0};
+    d.batch_tile_begin = reinterpret_cast<const int64_t*>(base + L.table_off) + tables.size();
+    tables.insert(tables.end(), q.batch_tile_begin, q.batch_tile_begin + q.nbatches + 1);
+    d.nbatches = q.nbatches;
+    d.steps = (int32_t)steps;
+    d.blk_begin = blk;
+    d.blk_count = L.blk_count[p];
+    blk += L.blk_count[p];
+    d.params = q.params;
+    d.partial = reinterpret_cast<float*>(base + L.partial_off) + poff;
+    poff += (int64_t)L.blk_count[p] * stride;
+    d.fo = FusedOpt{*opt, 1.0f / (float)q.batch_size, q.m, q.v, q.count, q.loss_hist, q.loss_cap, nullptr};
+  }
+  // the tables are read by the kernels: copy, then wait so the host vectors may go
+  hipError_t e = hipMemcpyAsync(base + L.desc_off, descs.data(), sizeof(BatchDesc) * nprob, hipMemcpyHostToDevice,
+                                stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(base + L.table_off, tables.data(), sizeof(int64_t) * tables.size(), hipMemcpyHostToDevice,
+                       stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_status(e, "batch train: table copy");
+  const BatchDesc* dd = reinterpret_cast<const BatchDesc*>(base + L.desc_off);
+  for (int64_t s = 0; s < maxsteps; ++s) {
+#define X(A, B, C, D, E, F)                                                         \
+  if (tile_shape_is(net, A, B, C, D, E, F)) {                                      \
+    batch_launch_shape<A, B, C, D, E, F>(dd, nprob, L.nblk_total, (int)s, stream); \
+  } else
+    DIPS_TILE_SHAPES(X)
+#undef X
+    {
+      set_error("batch train: network shape has no compiled tile kernel");
+      return DIPS_EUNSUPPORTED;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_status(e, "batch train: launch");
+  }
+  return DIPS_OK;
 }
 
 // Slot layout of every tile (once per zoom level): one wavefront per tile

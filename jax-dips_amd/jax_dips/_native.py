@@ -54,6 +54,13 @@ class Tiles(ctypes.Structure):
                 ("slot_c", ctypes.c_void_p), ("slot_info", ctypes.c_void_p), ("slot_row", ctypes.c_void_p)]
 
 
+class BatchProblem(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("rec", Records), ("tiles", Tiles), ("batch_tile_begin", ctypes.c_void_p),
+                ("nbatches", ctypes.c_int32), ("batch_size", ctypes.c_int64), ("params", ctypes.c_void_p),
+                ("m", ctypes.c_void_p), ("v", ctypes.c_void_p), ("count", ctypes.c_void_p),
+                ("loss_hist", ctypes.c_void_p), ("loss_cap", ctypes.c_int32)]
+
+
 class Assemble(ctypes.Structure):
     _fields_ = [("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("zs", ctypes.c_void_p),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
@@ -109,6 +116,9 @@ SIGNATURES = {
     "dips_tile_train_level": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(Tiles),
                                              ctypes.POINTER(_I64), _I32, _I64, _I32, ctypes.POINTER(Net),
                                              ctypes.POINTER(Opt), _P, _P, _P, _P, _P, _I32, _P, _I64, _P, _P]),
+    "dips_batch_workspace": (_I64, [ctypes.POINTER(BatchProblem), _I32, ctypes.POINTER(Net)]),
+    "dips_batch_train": (ctypes.c_int, [ctypes.POINTER(BatchProblem), _I32, _I32, ctypes.POINTER(Net),
+                                        ctypes.POINTER(Opt), _P, _I64, _P]),
     "dips_mfma_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
     "dips_mfma_workspace": (_I64, [ctypes.POINTER(Net), _I64, _I64]),
     "dips_mfma_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),

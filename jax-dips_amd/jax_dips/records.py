@@ -421,3 +421,61 @@ class Engine:
             int(self.loss_hist.numel()), N.ptr(self.partial), self.partial_cap,
             N.ptr(self.sync) if self.fused else None, N.current_stream())
         N.check(st, "dips_train_level")
+
+
+class BatchEngine:
+    """Parameters and optimizer state of many independent problems with one
+    network shape, trained together by dips_batch_train (one step launch over
+    the current batch of every problem, then one optimizer workgroup per
+    problem).  Each problem follows the trajectory Engine.train_level gives it
+    alone."""
+
+    def __init__(self, dims, params_list, optimizer_dict, max_steps, device="cuda", precond=None):
+        """max_steps: one optimizer-step capacity (loss history length) per problem."""
+        self.engines = [Engine(dims, p, optimizer_dict, max_rows=1, max_steps=ms, device=device, precond=precond)
+                        for p, ms in zip(params_list, max_steps)]
+        if len(self.engines) != len(params_list):
+            raise ValueError("one max_steps entry per problem")
+        if not self.engines[0].tile_path:
+            raise N.DipsError(f"network shape {tuple(dims)} has no compiled tile kernel for batched training")
+        self.net = self.engines[0].net
+        self.opt = self.engines[0].opt
+        self.device = device
+        self._work = None
+
+    def problems(self, grids, recs):
+        """The dips_batch_problem_t table (host) for these grids and records."""
+        n = len(self.engines)
+        if len(grids) != n or len(recs) != n:
+            raise ValueError("one grid and one record set per problem")
+        probs = (N.BatchProblem * n)()
+        keep = []
+        for i, (eng, g, rec) in enumerate(zip(self.engines, grids, recs)):
+            tl = rec.tiles(g)
+            btb = N.i64_array(tl["batch_tile_begin"])
+            keep.append(btb)
+            q = probs[i]
+            q.grid, q.rec, q.tiles = g.c, rec.c, tl["c"]
+            q.batch_tile_begin = ctypes.cast(btb, ctypes.c_void_p)
+            q.nbatches = len(rec.batch_begin)
+            q.batch_size = int(rec.batch_size)
+            q.params, q.m, q.v = eng.params.data_ptr(), eng.m.data_ptr(), eng.v.data_ptr()
+            q.count, q.loss_hist = eng.count.data_ptr(), eng.loss_hist.data_ptr()
+            q.loss_cap = int(eng.loss_hist.numel())
+        return probs, keep
+
+    def train_level(self, grids, recs, nepochs):
+        import torch
+
+        L = N.lib()
+        probs, keep = self.problems(grids, recs)
+        n = len(self.engines)
+        need = int(L.dips_batch_workspace(probs, n, ctypes.byref(self.net)))
+        if need < 0:
+            N.check(-1, "dips_batch_workspace")
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(need, dtype=torch.uint8, device=self.device)
+        N.check(L.dips_batch_train(probs, n, int(nepochs), ctypes.byref(self.net), ctypes.byref(self.opt),
+                                   N.ptr(self._work), int(self._work.numel()), N.current_stream()),
+                "dips_batch_train")
+        del keep

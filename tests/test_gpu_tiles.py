@@ -51,9 +51,10 @@ def test_tile_loss_grad_sphere16(level):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dims", [(2, 10, 2, 10), (1, 3, 2, 10), (1, 1, 1, 1)])
+@pytest.mark.parametrize("dims", [(2, 10, 2, 10), (1, 3, 2, 10), (1, 1, 1, 1), (2, 16, 2, 16)])
 def test_tile_loss_grad_shapes(dims):
-    """MFMA accumulation in both networks, and per-lane accumulation only."""
+    """MFMA accumulation in both networks (width 16: weight blocks on the matrix
+    cores, biases and output layer per lane), and per-lane accumulation only."""
     from jax_dips.records import Engine
 
     prob = problems.sphere(mu_m=2.0, mu_p=7.0, k_m=0.5, k_p=1.5)
