@@ -58,8 +58,11 @@ def main():
     ap.add_argument("--roofline-launches", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "star"],
-                    help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "sweep"],
+                    help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144; "
+                         "sweep: 64 problems of 32^3..96^3 with 2x16/2x16 MLPs, batched vs one at a time")
+    ap.add_argument("--problems", type=int, default=64, help="sweep: number of problems")
+    ap.add_argument("--epochs-per-level", type=int, default=3, help="sweep: timed epochs per zoom level")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy, 5 times (for scaling HBM counters)")
     ap.add_argument("--path", default="auto", choices=["auto", "tile", "lane", "mfma"],
@@ -83,6 +86,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}), flush=True)
+        return
+
+    if args.workload == "sweep":
+        run_sweep(args)
         return
 
     from jax_dips import _native as N
@@ -274,6 +281,110 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, args.cpu_seconds, n)
+    print(json.dumps(result), flush=True)
+
+
+def run_sweep(args):
+    """The batched sweep: problems of the sweep workload (problems.sweep_problem,
+    2x16/2x16 tanh MLPs, batch min(N, 65536), padding as the trainer pads),
+    trained level by level over the four zoom levels of a 1000-epoch run,
+    ``epochs_per_level`` epochs each; the same epochs run once batched
+    (dips_batch_train: one step launch over all problems' current batches, one
+    optimizer workgroup per problem) and once problem by problem
+    (dips_tile_train_level), both from the same parameters, HIP events around
+    each.  value = training points per second of the batched run (sum over
+    problems of steps x batch size / time)."""
+    import torch
+
+    from jax_dips import problems
+    from jax_dips.data.data_management import generate_random_points_like
+    from jax_dips.level_records import build_level
+    from jax_dips.nn.mlp.MLP import DoubleMLP
+    from jax_dips.records import BatchEngine, DeviceGrid, Engine
+
+    mlp = {"hidden_layers_m": 2, "hidden_dim_m": 16, "activation_m": "jnp.tanh",
+           "hidden_layers_p": 2, "hidden_dim_p": 16, "activation_p": "jnp.tanh"}
+    model = DoubleMLP(mlp=mlp)
+    dims = model.dims
+    opt = {"optimizer_name": "custom", "learning_rate": 1e-3, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
+    specs = [problems.sweep_problem(i) for i in range(args.problems)]
+    grids, info = [], []
+    for prob, n in specs:
+        x, y, z = problems.grid_axes(n)
+        N_ = n ** 3
+        B = min(N_, args.batch)
+        nb = -(-N_ // B)
+        P = np.stack(np.meshgrid(x, y, z, indexing="ij"), -1).reshape(-1, 3)
+        extra = generate_random_points_like(P, nb * B - N_) if nb * B > N_ else None
+        grids.append(DeviceGrid(x, y, z, extra=extra))
+        h0 = np.array([x[1] - x[0]] * 3, np.float32)
+        info.append((problems.oracle_callbacks(prob), h0, (x.min(), x.max()) * 3, B, nb, n))
+    thetas = [model.flatten(model.init(42 + i)) for i in range(len(specs))]
+    E = args.epochs_per_level
+    cap = [4 * (E + 1) * nb + 8 for (_, _, _, _, nb, _) in info]
+    beng = BatchEngine(dims, thetas, opt, cap)
+    sengs = [Engine(dims, th, opt, max_rows=1, max_steps=c) for th, c in zip(thetas, cap)]
+    t_rec = t_batch = t_seq = 0.0
+    points = flops = 0.0
+    steps_global = 0
+    for lvl in range(4):
+        t0 = time.time()
+        recs = []
+        for g, (cbs, h0, bnd, B, nb, n) in zip(grids, info):
+            recs.append(build_level(g, (h0 * np.float32(0.5) ** lvl).astype(np.float32), bnd, cbs, B))
+        for g, r in zip(grids, recs):
+            r.tiles(g)
+        torch.cuda.synchronize()
+        t_rec += time.time() - t0
+        if lvl == 0:  # warm up both paths (module load, LDS attributes)
+            beng.train_level(grids, recs, 1)
+            for e, g, r in zip(sengs, grids, recs):
+                e.train_level(g, r, 1)
+            torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        beng.train_level(grids, recs, E)
+        ev[1].record()
+        ev[2].record()
+        for e, g, r in zip(sengs, grids, recs):
+            e.train_level(g, r, E)
+        ev[3].record()
+        torch.cuda.synchronize()
+        t_batch += ev[0].elapsed_time(ev[1]) / 1e3
+        t_seq += ev[2].elapsed_time(ev[3]) / 1e3
+        steps_global += E * max(nb for (_, _, _, _, nb, _) in info)
+        for r, (_, _, _, B, nb, _) in zip(recs, info):
+            points += E * nb * B
+            codes = r.ent_code[: r.nnz].cpu().numpy()
+            flops += E * 3.0 * mlp_fwd_flops(2, 16) * codes.size  # both phases are 2x16
+        del recs
+    value = points / t_batch
+    achieved = flops / t_batch / 1e12
+    result = {
+        "metric": "batched sweep training points/s (64 problems, 2x16/2x16 MLP, batch min(N, 65536))",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": steps_global,
+        "warmup": 1,
+        "ms_per_step": t_batch * 1e3 / steps_global,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"sweep of {len(specs)} problems (sizes "
+                               + ",".join(f"{k}^3 x{sum(1 for _, n in specs if n == k)}" for k in problems.SWEEP_SIZES)
+                               + f"), {E} epochs per zoom level x 4 levels, tile kernels",
+                   "global_batch": int(sum(B for (_, _, _, B, _, _) in info)), "parallelism": "single GPU"},
+        "sequential_value": points / t_seq,
+        "speedup_vs_sequential": t_seq / t_batch,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "tile_batch_step_kernel + tile_batch_apply_kernel (whole batched step)"},
+        "records_build_s": t_rec,
+        "loss_tail": [float(e.loss_hist[int(e.count.item()) - 1]) for e in beng.engines[:4]],
+    }
     print(json.dumps(result), flush=True)
 
 

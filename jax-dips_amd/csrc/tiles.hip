@@ -345,6 +345,11 @@ struct NetGrad {
     // instead of keeping the forward pass's loads live in registers
     dips_weight_t* Q = P_;
     asm volatile("" : "+s"(Q));
+    // a chunk without entries of this network adds exact zeros: skip it (most
+    // tiles hold one phase only)
+    if constexpr (kGrad && kMfma) {
+      if (!__any(mine)) return;
+    }
     if constexpr (kGrad && kWide) {
       const float s = mine ? seed : 0.0f;
       float dl[L][H];

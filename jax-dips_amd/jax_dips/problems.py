@@ -142,6 +142,27 @@ def star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k
     return make_problem(phi, gphi, mu_m, mu_p, k_m, k_p, "star")
 
 
+SWEEP_SIZES = (32, 48, 64, 96)
+
+
+def sweep_problem(i):
+    """Problem i of the 64-problem sweep workload: numpy default_rng(seed=i)
+    draws N from {32^3, 48^3, 64^3, 96^3}, a sphere (radius in [0.3, 0.7]) or a
+    star (amplitude in [0.05, 0.15], lobes in {3, 5, 7}), mu+ log-uniform in
+    [2, 1000], mu- = 1, k = 0, and the manufactured solution pair of the sphere
+    workloads.  Returns (problem, n)."""
+    rng = np.random.default_rng(seed=i)
+    n = int(rng.choice(SWEEP_SIZES))
+    mu_p = float(np.exp(rng.uniform(np.log(2.0), np.log(1000.0))))
+    if rng.uniform() < 0.5:
+        prob = sphere(radius=float(rng.uniform(0.3, 0.7)), mu_m=1.0, mu_p=mu_p, k_m=0.0, k_p=0.0)
+    else:
+        lobes = int(rng.choice((3, 5, 7)))
+        prob = star(r0=0.5, amp=float(rng.uniform(0.05, 0.15)), lobes_t=lobes, lobes_p=lobes, mu_m=1.0, mu_p=mu_p,
+                    k_m=0.0, k_p=0.0)
+    return prob, n
+
+
 def grid_axes(n, lo=-1.0, hi=1.0):
     """float32 linspace as the reference builds grids (README quick example)."""
     a = np.linspace(lo, hi, n, dtype=np.float32)
