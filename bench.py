@@ -58,9 +58,10 @@ def main():
     ap.add_argument("--roofline-launches", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "sweep"],
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "sweep", "pb"],
                     help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144; "
-                         "sweep: 64 problems of 32^3..96^3 with 2x16/2x16 MLPs, batched vs one at a time")
+                         "sweep: 64 problems of 32^3..96^3 with 2x16/2x16 MLPs, batched vs one at a time; "
+                         "pb: Poisson-Boltzmann box of 2000 atoms at 256^3, 3x64/3x64 MLP, batch 2^20")
     ap.add_argument("--problems", type=int, default=64, help="sweep: number of problems")
     ap.add_argument("--epochs-per-level", type=int, default=3, help="sweep: timed epochs per zoom level")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -99,7 +100,17 @@ def main():
 
     dev = "cuda"
     n, B = args.n, args.batch
-    if args.workload == "star":
+    lo, hi = -1.0, 1.0
+    if args.workload == "pb":
+        prob = problems.pb_box(n_atoms=2000, half_width=20.0, seed=0)
+        lo, hi = -20.0, 20.0
+        if args.n == 128:
+            n = 256
+        if args.batch == 65536:
+            B = 1 << 20
+        mlp = {"hidden_layers_m": 3, "hidden_dim_m": 64, "activation_m": "jnp.tanh",
+               "hidden_layers_p": 3, "hidden_dim_p": 64, "activation_p": "jnp.tanh"}
+    elif args.workload == "star":
         prob = problems.star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0)
         if args.batch == 65536:
             B = 262144
@@ -110,7 +121,7 @@ def main():
         mlp = {"hidden_layers_m": 1, "hidden_dim_m": 1, "activation_m": "jnp.tanh",
                "hidden_layers_p": 2, "hidden_dim_p": 10, "activation_p": "jnp.tanh"}
     cbs = problems.oracle_callbacks(prob)
-    x, y, z = problems.grid_axes(n)
+    x, y, z = problems.grid_axes(n, lo, hi)
     grid = DeviceGrid(x, y, z)
     h0 = np.array([x[1] - x[0], y[1] - y[0], z[1] - z[0]], np.float32)
     bounds = (x.min(), x.max(), y.min(), y.max(), z.min(), z.max())
@@ -126,7 +137,8 @@ def main():
     for lvl in range(4):
         h = (h0 * np.float32(0.5) ** lvl).astype(np.float32)
         levels.append(build_level(grid, h, bounds, cbs, B))
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        print(f"# level {lvl} records built ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     t_records = time.time() - t0
     npts = n ** 3
     nb = levels[0].batch_begin.size
@@ -265,7 +277,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} {n}^3, MLP {dims[0]}x{dims[1]}/{dims[2]}x{dims[3]} tanh, batch {B}, "
+        "config": {"workload": f"{args.workload} {n}^3 on [{lo:g},{hi:g}]^3, MLP {dims[0]}x{dims[1]}/{dims[2]}x{dims[3]} tanh, batch {B}, "
                                f"4 zoom levels (2000-epoch schedule), {path} kernels",
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -280,7 +292,7 @@ def main():
         "loss_tail": loss_tail,
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, args.cpu_seconds, n)
+        result["cpu_baseline"] = cpu_baseline(args.workload, n, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 
@@ -388,28 +400,20 @@ def run_sweep(args):
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(prob, x, y, z, h0, bounds, dims, theta, seconds, n=128):
-    """The reference's per-step evaluation pattern in numpy on one host core,
-    on a bounded sample (consecutive batches of 2048 points of the same grid,
-    level-0 spacing) until ~``seconds`` of CPU work."""
-    from jax_dips import problems
-    from oracle import discretization as D
-    from oracle import refpattern
+def cpu_baseline(workload, n, seconds):
+    """The reference's per-step pattern (oracle.refpattern, numpy float32) on
+    the host cores: oracle.cpu_bench in a child interpreter, one
+    single-threaded process per core (at most 16), each on its own stretch of
+    batches through the middle of the grid, for ~``seconds``."""
+    import subprocess
 
-    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float32)
-    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
-    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
-    nb = 2048
-    mid = P.shape[0] // 2  # batches through the middle of the box cross the interface
-    done, t0 = 0, time.time()
-    while time.time() - t0 < seconds:
-        s = mid + done
-        refpattern.step(theta, dims, P[s:s + nb], h0, bounds, cb)
-        done += nb
-    dt = time.time() - t0
-    return {"value": done / dt, "unit": "points/s", "cores": 1, "kind": "port",
-            "sample": f"{done} points in batches of {nb} from the middle of the {n}^3 grid, level-0 spacing, "
-                      f"{dt:.1f} s, numpy float32 restatement of the reference's per-step pattern"}
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", "--workload", workload, "--n", str(n),
+                        "--seconds", str(seconds)], cwd=ROOT, env=env, capture_output=True, text=True,
+                       timeout=4 * seconds + 300)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed:\n{r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 if __name__ == "__main__":

@@ -127,3 +127,42 @@ def test_padded_batch_points(name, level):
     ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, orec, np.arange(b * B, (b + 1) * B))
     assert abs(loss - ol) <= 1e-5 * abs(ol)
     assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [1])
+def test_pb_box_records_match_oracle(level):
+    """The Poisson-Boltzmann-like box (union of 40 spheres, Gaussian charges,
+    mu 2 / 80, k+ = kappa^2) at 16^3 on [-6, 6]^3 Angstrom: device records equal
+    the oracle's, and the MFMA kernels (3x64 networks, the configuration of the
+    256^3 benchmark) give the oracle's loss and gradient on them."""
+    from jax_dips.level_records import build_level
+    from jax_dips.records import DeviceGrid, Engine
+    from oracle import discretization as D
+    from oracle import mlp as M
+    from oracle import trainer as T
+
+    prob = problems.pb_box(n_atoms=40, half_width=6.0, seed=3)
+    axes, P, h, bounds = grid_points(16, -6.0, 6.0)
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    hl = T.level_spacing(h, level)
+    orec = T.LevelRecords(P, hl, bounds, cb, np.float64)
+    grid = DeviceGrid(*axes)
+    n = P.shape[0]
+    drec = build_level(grid, hl, bounds, problems.oracle_callbacks(prob), n)
+    W, C = drec.to_dense()
+    scale = np.maximum(np.abs(orec.W).max(axis=1), 1.0)
+    err = np.abs(W - orec.W).max(axis=1) / scale
+    cerr = np.abs(C - orec.c) / np.maximum(np.abs(orec.c), scale)
+    print("pb", level, drec.stats, "max w err", err.max(), "max c err", cerr.max())
+    assert drec.stats["nip"] > 0.1 * n  # many interface points
+    assert err.max() <= 1e-6 and cerr.max() <= 1e-6
+    dims = (3, 64, 3, 64)
+    theta = M.init_params(dims, seed=12)
+    eng = Engine(dims, theta, {}, max_rows=n, max_steps=2)
+    assert eng.mfma_path
+    loss, grad = eng.mfma_loss_grad(grid, drec, 0, 1.0 / n)
+    ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, orec, np.arange(n))
+    print(f"loss gpu {loss:.9e} oracle {ol:.9e} grad rel {np.linalg.norm(grad - og) / np.linalg.norm(og):.2e}")
+    assert abs(loss - ol) <= 1e-5 * abs(ol)
+    assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)

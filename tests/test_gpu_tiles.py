@@ -191,3 +191,46 @@ def test_tile_fused_equals_two_kernel_and_row_kernel():
     assert np.array_equal(l1, l2)
     assert np.max(np.abs(p1 - p2)) <= 1e-7 * max(1.0, np.max(np.abs(p2)))
     assert np.all(np.abs(l1 - l3) <= 1e-5 * np.abs(l3))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("G", [2, 8])
+def test_fake_data_parallel_shards(G):
+    """One-GPU rehearsal of data parallelism: a batch's tiles split into G
+    contiguous shards (what G ranks would each take), the tile kernel run per
+    shard, the shard gradients summed (the all-reduce) equal the full batch's
+    gradient; the shards are disjoint, so the sums differ only by float32
+    summation order (1e-6 relative)."""
+    import ctypes
+
+    from jax_dips import _native as N
+    from jax_dips.records import Engine
+    import torch
+
+    prob = problems.sphere()
+    axes, P, h, bounds, cb, rec = oracle_level(prob, 20, 1)
+    n = P.shape[0]
+    theta = M.init_params(DIMS, seed=31)
+    grid, drec, _ = _device(axes, rec, n)
+    eng = Engine(DIMS, theta, {}, max_rows=n, max_steps=4)
+    full_l, full_g = eng.tile_loss_grad(grid, drec, 0, 1.0 / n)
+    tl = drec.tiles(grid)
+    tb = int(tl["batch_tile_begin"][0])
+    nt = int(tl["batch_tile_begin"][1]) - tb - 1
+    cuts = np.linspace(0, nt, G + 1).astype(int)
+    tot_l, tot_g = 0.0, np.zeros_like(full_g)
+    for k in range(G):
+        nblk = ctypes.c_int32(0)
+        N.check(N.lib().dips_tile_loss_grad(ctypes.byref(grid.c), ctypes.byref(drec.c), ctypes.byref(tl["c"]),
+                                            tb + int(cuts[k]), int(cuts[k + 1] - cuts[k]), ctypes.byref(eng.net),
+                                            N.ptr(eng.params), N.ptr(eng.partial), eng.partial_cap, 1,
+                                            ctypes.byref(nblk), N.current_stream()), "dips_tile_loss_grad")
+        torch.cuda.synchronize()
+        part = eng.partial[: nblk.value * (eng.P + 1)].view(nblk.value, eng.P + 1).double().cpu().numpy()
+        s = part.sum(axis=0) / n
+        tot_l += s[eng.P]
+        tot_g += s[: eng.P]
+    assert abs(tot_l - full_l) <= 1e-6 * abs(full_l)
+    assert np.linalg.norm(tot_g - full_g) <= 1e-6 * np.linalg.norm(full_g)
+    ol, og, _ = T.loss_and_grad(theta.astype(np.float64), DIMS, rec, np.arange(n))
+    _check(tot_l, tot_g, ol, og, f"{G} shards")
