@@ -95,6 +95,34 @@ __device__ __forceinline__ double dips_wave_sum_d(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+template <int Ctrl, int RowMask, bool BoundCtrl>
+__device__ __forceinline__ float dips_dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), Ctrl, RowMask, 0xF,
+                                                               BoundCtrl));
+}
+
+// Segmented inclusive scan over the wavefront: lane l ends with the sum of
+// lanes head..l, where head <= l is the first lane of l's segment (segments
+// are contiguous lane ranges; head == l starts one).  Row-local Hillis-Steele
+// steps (row_shr 1, 2, 4, 8) followed by the two row broadcasts (lane 15 of a
+// row to the next row, lane 31 to rows 2 and 3), each added only where it lies
+// inside the lane's segment: a fixed summation tree without LDS traffic.
+__device__ __forceinline__ float dips_segmented_scan(float v, int lane, int head) {
+  float t = dips_dpp_f<0x111, 0xF, true>(v);
+  if (lane - 1 >= head) v += t;
+  t = dips_dpp_f<0x112, 0xF, true>(v);
+  if (lane - 2 >= head) v += t;
+  t = dips_dpp_f<0x114, 0xF, true>(v);
+  if (lane - 4 >= head) v += t;
+  t = dips_dpp_f<0x118, 0xF, true>(v);
+  if (lane - 8 >= head) v += t;
+  t = dips_dpp_f<0x142, 0xA, false>(v);  // rows 1 and 3 receive lanes 15 and 47
+  if (head < (lane & ~15)) v += t;
+  t = dips_dpp_f<0x143, 0xC, false>(v);  // rows 2 and 3 receive lane 31
+  if (head < 32) v += t;
+  return v;
+}
+
 __device__ __forceinline__ int dips_wave_sum_i(int v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, DIPS_WAVE);

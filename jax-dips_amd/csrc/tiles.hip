@@ -669,11 +669,7 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
       v = fmaf(w, u, c);  // c is nonzero only in the row's last slot
     }
     // segmented inclusive scan: lane l ends with the sum of lanes head(l)..l
-#pragma unroll
-    for (int d = 1; d < DIPS_WAVE; d <<= 1) {
-      const float o = __shfl_up(v, d, DIPS_WAVE);
-      if (lane - d >= head) v += o;
-    }
+    v = dips_segmented_scan(v, lane, head);
     float r = __shfl(v, last, DIPS_WAVE);
     float rmul = r;  // d(0.5 r^2)/d(sum w u): r, times P when preconditioned
     if constexpr (PC1 > 0) {
