@@ -631,7 +631,11 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
   int64_t s0 = slot_of(t0);
   uint32_t n_info = gld(a.sinfo, s0);
   float n_x = gld(a.sx, s0), n_y = gld(a.sy, s0), n_z = gld(a.sz, s0), n_w = gld(a.sw, s0), n_c = gld(a.sc, s0);
-  for (int t = t0; t < a.ntiles; t += nw) {
+  // tiles without a used slot (a single row of more than 64 entries) are
+  // marked here, by the wavefront's iteration index, for the second loop
+  uint64_t longmask = 0;
+  int iter = 0;
+  for (int t = t0; t < a.ntiles; t += nw, ++iter) {
     // weights are re-read per tile (scalar loads that hit the scalar cache)
     // rather than held in vector registers across the loop
     const float* Praw = params;
@@ -650,6 +654,10 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
       n_c = gld(a.sc, sn);
     }
     const bool on = (info & kSlotUsed) != 0;
+    if (!__any(on)) {
+      if (iter < 64) longmask |= 1ull << iter;
+      continue;
+    }
     const bool plus = (info & kSlotPlus) != 0;
     const int head = on ? (int)(info & 63u) : lane;
     const int last = on ? (int)((info >> 6) & 63u) : lane;
@@ -703,8 +711,10 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
   // long rows (tiles skipped above): one wavefront per row, 64 entries per
   // chunk; the gradient pass recomputes the forward activations.  A loop of
   // its own so that its register demand does not add to the packed path's.
+  iter = 0;
 #pragma unroll 1
-  for (int t = t0; t < a.ntiles; t += nw) {
+  for (int t = t0; t < a.ntiles; t += nw, ++iter) {
+    if (iter < 64 && !((longmask >> iter) & 1)) continue;  // a packed tile, done above
     const int64_t tb = a.tile_begin + t;
     const int64_t r0 = clamp64(gld(a.tile_row, tb), 0, a.nrows);
     const int64_t r1 = clamp64(gld(a.tile_row, tb + 1), r0, a.nrows);
