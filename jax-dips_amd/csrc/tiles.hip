@@ -613,7 +613,8 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
   float* const AT = lds + wave * kStageFloats;
   float* const BT = AT + 16 * kLdsStride;
-  for (int i = lane; i < kStageFloats; i += DIPS_WAVE) AT[i] = 0.0f;
+  // the staging rows a block does not write hold stale values; they only feed
+  // accumulator entries that are never flushed (no zero fill needed)
 
   NetGrad<Lp, Hp, kGrad> gp;
   NetGrad<Lm, Hm, kGrad> gm;
