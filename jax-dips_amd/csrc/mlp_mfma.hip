@@ -33,7 +33,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMT = 256;  // threads per workgroup (4 wavefronts)
 constexpr int kTP = 17;   // padded row length of a transposed 16-position tile
-constexpr int kMfmaBwdBlocks = 512;  // workgroups of the backward kernel (both phases)
+constexpr int kMfmaBwdBlocks = 256;  // workgroups of the backward kernel (both phases), one per CU
 
 __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
@@ -64,7 +64,7 @@ template <int L, int NB>
 __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
   using PK = Packed<L, NB>;
   const int Hh = fn.Hh;
-  for (int idx = threadIdx.x; idx < PK::size(); idx += kMT) {
+  for (int idx = threadIdx.x; idx < PK::size(); idx += blockDim.x) {
     float v = 0.0f;
     if (idx < NB * 64) {
       const int ob = idx / 64, lane = idx % 64;
@@ -249,12 +249,27 @@ __global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restr
 }
 
 // ---- kernel 3: parameter-gradient partials ---------------------------------------
+// Cooperative backward: a workgroup of kBW wavefronts takes kBW tiles of one
+// phase per round.  Every wavefront recomputes its tile's forward pass and
+// back-propagates its deltas (registers); per layer, each wavefront stages its
+// tile's operands (activation and delta, feature-major rows of 16 positions)
+// in LDS, and after a barrier each wavefront contracts only the weight-gradient
+// blocks it owns over all kBW tiles (K = 16 kBW positions).  Owning 2 of the
+// 16 blocks of a 64-wide layer instead of all 16 keeps a lane's registers below
+// 256, so two workgroups' worth of wavefronts (8 per CU) run per SIMD pair and
+// the matrix cores see twice the wavefronts of the one-tile-per-wavefront form.
+// Bias, output-weight and output-bias sums stay per lane and are reduced over
+// the 16 positions with row-local DPP sums at the end.
+constexpr int kBW = 8;              // wavefronts per backward workgroup
+constexpr int kBT = 64 * kBW;       // its threads
+
 template <int L, int NB>
-struct GradAcc {
-  f4 dW[L > 1 ? L - 1 : 1][NB * NB];  // hidden layers 1..L-1, block (ib, ob)
-  f4 dW0[NB];                          // rows x,y,z,1 (padded to 16) x out block
-  float db[L][4 * NB];                 // per-lane partial bias sums (feature-major)
-  float dwo[4 * NB];                   // per-lane partial output-weight sums
+struct CoopAcc {
+  static constexpr int kW = (NB * NB + kBW - 1) / kBW;  // hidden-layer blocks owned per wavefront
+  f4 dW[L > 1 ? L - 1 : 1][kW];                         // block q = wave * kW + j: (q / NB, q % NB)
+  f4 dW0;                                               // layer 0, out block = wave (wave < NB)
+  float db[L][4 * NB];                                  // per-lane partial bias sums (feature-major)
+  float dwo[4 * NB];                                    // per-lane partial output-weight sums
   float dbo;
 };
 
@@ -269,14 +284,109 @@ __device__ __forceinline__ void to_lds(float* __restrict__ buf, const float (&m)
   }
 }
 
-template <int L, int NB>
-__device__ __forceinline__ void tile_backward(const float* __restrict__ W, float seed, float xin_t[4],
-                                              const float (&act)[L][4 * NB], float* __restrict__ tbuf,
-                                              GradAcc<L, NB>& G) {
-  using PK = Packed<L, NB>;
+// sum over the 16 lanes of each row (lane 16g + 15 receives it): row-local DPP steps
+__device__ __forceinline__ float row16_sum(float v) {
+  v += dips_dpp_f<0x111, 0xF, true>(v);
+  v += dips_dpp_f<0x112, 0xF, true>(v);
+  v += dips_dpp_f<0x114, 0xF, true>(v);
+  v += dips_dpp_f<0x118, 0xF, true>(v);
+  return v;
+}
+
+// D += A^T B over the kBW staged tiles of the workgroup: A rows arow.., B rows brow..
+__device__ __forceinline__ f4 coop_block(const float* __restrict__ stage, int stride_w, int arow, int brow, bool a_inputs,
+                                         f4 acc) {
   const int lane = threadIdx.x & 63;
-  float* bufA = tbuf;                 // activation operand (features x 16 positions)
-  float* bufD = tbuf + 16 * NB * kTP;  // delta operand
+  const int m = lane & 15, kq = lane / 16;
+#pragma unroll 2
+  for (int s = 0; s < kBW; ++s) {
+    const float* A = stage + s * stride_w;
+    const float* D = A + stride_w / 2;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int pos = 4 * st + kq;
+      const float a = a_inputs ? (m < 4 ? A[m * kTP + pos] : 0.0f) : A[(arow + m) * kTP + pos];
+      const float b = D[(brow + m) * kTP + pos];
+      acc = mfma4(a, b, acc);
+    }
+  }
+  return acc;
+}
+
+// writes the workgroup's gradient of one phase net into out[0 .. flat size)
+template <int L, int NB>
+__device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ red, float* __restrict__ out) {
+  using CA = CoopAcc<L, NB>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+  const int Hh = fn.Hh;
+  const int P = 4 * Hh + (L - 1) * (Hh * Hh + Hh) + Hh + 1;  // parameters of this net
+  const int base = fn.base;
+  for (int i = threadIdx.x; i < P; i += kBT) red[i] = 0.0f;
+  __syncthreads();
+  // row sums of the per-lane partials (every wavefront, no LDS)
+#pragma unroll
+  for (int q = 0; q < 4 * NB; ++q) {
+    G.dwo[q] = row16_sum(G.dwo[q]);
+#pragma unroll
+    for (int l = 1; l < L; ++l) G.db[l][q] = row16_sum(G.db[l][q]);
+  }
+  G.dbo = row16_sum(G.dbo);
+  // wavefronts add in order (fixed order: deterministic); weight blocks have one owner each
+  for (int w = 0; w < kBW; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int l = 1; l < L; ++l) {
+#pragma unroll
+        for (int j = 0; j < CA::kW; ++j) {
+          const int q = wave * CA::kW + j;
+          if (q < NB * NB) {
+            const int ib = q / NB, ob = q % NB;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int in = 16 * ib + 4 * (lane / 16) + r, o = 16 * ob + (lane & 15);
+              if (in < Hh && o < Hh) red[fn.w(l) - base + in * Hh + o] = G.dW[l - 1][j][r];
+            }
+          }
+        }
+      }
+      if (wave < NB) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 4 * (lane / 16) + r, o = 16 * wave + (lane & 15);
+          if (row < 3 && o < Hh) red[fn.w(0) - base + row * Hh + o] = G.dW0[r];
+          if (row == 3 && o < Hh) red[fn.b(0) - base + o] = G.dW0[r];
+        }
+      }
+      if ((lane & 15) == 15) {
+#pragma unroll
+        for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int h = 16 * fb + 4 * (lane / 16) + r;
+            if (h < Hh) {
+              red[fn.wout() - base + h] += G.dwo[fb * 4 + r];
+#pragma unroll
+              for (int l = 1; l < L; ++l) red[fn.b(l) - base + h] += G.db[l][fb * 4 + r];
+            }
+          }
+        }
+        if (lane == 15) red[fn.bout() - base] += G.dbo;
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < P; i += kBT) out[i] = red[i];
+}
+
+template <int L, int NB>
+__device__ __forceinline__ void coop_backward(const float* __restrict__ W, float seed, float xin,
+                                              const float (&act)[L][4 * NB], float* __restrict__ stage, int stride_w,
+                                              CoopAcc<L, NB>& G) {
+  using PK = Packed<L, NB>;
+  using CA = CoopAcc<L, NB>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+  float* bufA = stage + wave * stride_w;
+  float* bufD = bufA + stride_w / 2;
   // output layer
   float d[4 * NB];
 #pragma unroll
@@ -296,32 +406,20 @@ __device__ __forceinline__ void tile_backward(const float* __restrict__ W, float
 #pragma unroll
       for (int q = 0; q < 4 * NB; ++q) G.db[l][q] += d[q];
     }
-    // dW_l += A_{l-1} D_l^T over the 16 positions (K = positions, 4 steps)
+    // stage this tile's operands: delta rows, and activation rows (or the inputs x, y, z, 1)
     to_lds<NB>(bufD, d);
-    if (l > 0) to_lds<NB>(bufA, act[l - 1]);
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int pos = 4 * s + lane / 16;
-      float dB[NB];
-#pragma unroll
-      for (int ob = 0; ob < NB; ++ob) dB[ob] = bufD[(16 * ob + (lane & 15)) * kTP + pos];
-      if (l > 0) {
-#pragma unroll
-        for (int ib = 0; ib < NB; ++ib) {
-          const float aA = bufA[(16 * ib + (lane & 15)) * kTP + pos];
-#pragma unroll
-          for (int ob = 0; ob < NB; ++ob) G.dW[l - 1][ib * NB + ob] = mfma4(aA, dB[ob], G.dW[l - 1][ib * NB + ob]);
-        }
-      } else {
-        const float aA = ((lane & 15) < 4) ? xin_t[s] : 0.0f;
-#pragma unroll
-        for (int ob = 0; ob < NB; ++ob) G.dW0[ob] = mfma4(aA, dB[ob], G.dW0[ob]);
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
+    if (l > 0)
+      to_lds<NB>(bufA, act[l - 1]);
+    else
+      bufA[(lane / 16) * kTP + (lane & 15)] = xin;
+    __syncthreads();
     if (l > 0) {
-      // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2)
+#pragma unroll
+      for (int j = 0; j < CA::kW; ++j) {
+        const int q = wave * CA::kW + j;
+        if (q < NB * NB) G.dW[l - 1][j] = coop_block(stage, stride_w, 16 * (q / NB), 16 * (q % NB), false, G.dW[l - 1][j]);
+      }
+      // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2), from this tile's registers
       f4 c[NB];
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib) c[ib] = f4{0.f, 0.f, 0.f, 0.f};
@@ -342,77 +440,15 @@ __device__ __forceinline__ void tile_backward(const float* __restrict__ W, float
           d[ib * 4 + r] = c[ib][r] * (1.0f - a * a);
         }
       }
+    } else if (wave < NB) {
+      G.dW0 = coop_block(stage, stride_w, 0, 16 * wave, true, G.dW0);
     }
+    __syncthreads();  // operands consumed: the buffers may be restaged
   }
-}
-
-// writes the workgroup's gradient of one phase net into out[0 .. flat size)
-template <int L, int NB>
-__device__ void flush_grad(GradAcc<L, NB>& G, FlatNet fn, float* __restrict__ red, float* __restrict__ out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-  const int Hh = fn.Hh;
-  const int P = 4 * Hh + (L - 1) * (Hh * Hh + Hh) + Hh + 1;  // parameters of this net
-  const int base = fn.base;
-  // zero the LDS accumulator
-  for (int i = threadIdx.x; i < P; i += kMT) red[i] = 0.0f;
-  __syncthreads();
-  // each wave adds its part in wave order (fixed order -> deterministic)
-  for (int w = 0; w < 4; ++w) {
-    if (wave == w) {
-      // hidden dW blocks: lane holds dW[16ib + 4(lane/16) + r][16ob + lane%16]
-#pragma unroll
-      for (int l = 1; l < L; ++l) {
-#pragma unroll
-        for (int ib = 0; ib < NB; ++ib) {
-#pragma unroll
-          for (int ob = 0; ob < NB; ++ob) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int in = 16 * ib + 4 * (lane / 16) + r, o = 16 * ob + (lane & 15);
-              if (in < Hh && o < Hh) red[fn.w(l) - base + in * Hh + o] += G.dW[l - 1][ib * NB + ob][r];
-            }
-          }
-        }
-      }
-      // layer 0: rows 0..2 -> W0, row 3 -> b0
-#pragma unroll
-      for (int ob = 0; ob < NB; ++ob) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = 4 * (lane / 16) + r, o = 16 * ob + (lane & 15);
-          if (row < 3 && o < Hh) red[fn.w(0) - base + row * Hh + o] += G.dW0[ob][r];
-          if (row == 3 && o < Hh) red[fn.b(0) - base + o] += G.dW0[ob][r];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // per-lane partial sums over positions: reduce the 16 lanes of each row group with DPP-free LDS adds
-  for (int w = 0; w < 4; ++w) {
-    for (int g = 0; g < 16; ++g) {
-      if (wave == w && (lane & 15) == g) {
-#pragma unroll
-        for (int fb = 0; fb < NB; ++fb) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int h = 16 * fb + 4 * (lane / 16) + r;
-            if (h < Hh) {
-              red[fn.wout() - base + h] += G.dwo[fb * 4 + r];
-#pragma unroll
-              for (int l = 1; l < L; ++l) red[fn.b(l) - base + h] += G.db[l][fb * 4 + r];
-            }
-          }
-        }
-        if (lane < 16) red[fn.bout() - base] += G.dbo;
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = threadIdx.x; i < P; i += kMT) out[i] = red[i];
 }
 
 template <int Lm, int NBm, int Lp, int NBp>
-__global__ __launch_bounds__(kMT) void mfma_bwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
+__global__ __launch_bounds__(kBT) void mfma_bwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_, int64_t row_begin,
                                                        const float* __restrict__ r_row, float* __restrict__ gpart,
                                                        int nblk_p, int stride) {
@@ -420,11 +456,11 @@ __global__ __launch_bounds__(kMT) void mfma_bwd_kernel(MfmaTileArgs T, const flo
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool plus = (int)blockIdx.x < nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
-  const int PPm = 4 * Hm + (Lm - 1) * (Hm * Hm + Hm) + Hm + 1;
   constexpr int PKsz = (Packed<Lp, NBp>::size() > Packed<Lm, NBm>::size()) ? Packed<Lp, NBp>::size()
                                                                             : Packed<Lm, NBm>::size();
   constexpr int NBx = NBp > NBm ? NBp : NBm;
-  float* tbuf = lds + PKsz + wave * (2 * 16 * NBx * kTP);
+  constexpr int kStrideW = 2 * 16 * NBx * kTP;  // floats of one wavefront's operand pair
+  float* stage = lds + PKsz;
   const int64_t ntiles = plus ? (np_ + 15) / 16 : (nm_ + 15) / 16;
   const int nblk = plus ? nblk_p : (int)gridDim.x - nblk_p;
   const int bid = plus ? (int)blockIdx.x : (int)blockIdx.x - nblk_p;
@@ -434,15 +470,15 @@ __global__ __launch_bounds__(kMT) void mfma_bwd_kernel(MfmaTileArgs T, const flo
   auto run = [&](auto tagL, auto tagNB, FlatNet fn) {
     constexpr int L = decltype(tagL)::value;
     constexpr int NB = decltype(tagNB)::value;
+    using CA = CoopAcc<L, NB>;
     pack_weights<L, NB>(params, fn, lds);
     __syncthreads();
-    GradAcc<L, NB> G;
+    CA G;
 #pragma unroll
     for (int l = 0; l < (L > 1 ? L - 1 : 1); ++l)
 #pragma unroll
-      for (int q = 0; q < NB * NB; ++q) G.dW[l][q] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int q = 0; q < NB; ++q) G.dW0[q] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int q = 0; q < CA::kW; ++q) G.dW[l][q] = f4{0.f, 0.f, 0.f, 0.f};
+    G.dW0 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int l = 0; l < L; ++l)
 #pragma unroll
@@ -450,7 +486,12 @@ __global__ __launch_bounds__(kMT) void mfma_bwd_kernel(MfmaTileArgs T, const flo
 #pragma unroll
     for (int q = 0; q < 4 * NB; ++q) G.dwo[q] = 0.0f;
     G.dbo = 0.0f;
-    for (int64_t tile = (int64_t)bid * 4 + wave; tile < ntiles; tile += (int64_t)nblk * 4) {
+    // every wavefront runs the same number of rounds (the barriers inside);
+    // tiles past the end carry no entries (zero seeds, zero inputs)
+    const int64_t per_round = (int64_t)nblk * kBW;
+    const int64_t rounds = (ntiles + per_round - 1) / per_round;
+    for (int64_t rd = 0; rd < rounds; ++rd) {
+      const int64_t tile = (rd * nblk + bid) * kBW + wave;
       const int64_t ebeg = eb + tile * 16;
       float x, y, z;
       int row;
@@ -458,27 +499,18 @@ __global__ __launch_bounds__(kMT) void mfma_bwd_kernel(MfmaTileArgs T, const flo
       const float seed = valid ? r_row[row - row_begin] * T.t_w[ebeg + (lane & 15)] : 0.0f;
       const int kq = lane / 16;
       const float xin = (kq == 0) ? x : (kq == 1) ? y : (kq == 2) ? z : 1.0f;
-      // position-major copies of the tile inputs for the layer-0 weight gradient
-      float xin_t[4];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int src = 4 * s + lane / 16;  // tile position needed in step s
-        const float sx = __shfl(x, src, 64), sy = __shfl(y, src, 64), sz = __shfl(z, src, 64);
-        const int c = lane & 15;
-        xin_t[s] = (c == 0) ? sx : (c == 1) ? sy : (c == 2) ? sz : 1.0f;
-      }
       float act[L][4 * NB];
       tile_forward<L, NB>(lds, xin, act);
-      tile_backward<L, NB>(lds, seed, xin_t, act, tbuf, G);
+      coop_backward<L, NB>(lds, seed, xin, act, stage, kStrideW, G);
     }
     __syncthreads();  // all waves done with the packed weights: reuse LDS for the reduction
-    flush_grad<L, NB>(G, fn, lds, out);
+    flush_coop<L, NB>(G, fn, lds, out);
   };
   if (plus)
     run(std::integral_constant<int, Lp>{}, std::integral_constant<int, NBp>{}, FlatNet{Hp, Lp, 0});
   else
     run(std::integral_constant<int, Lm>{}, std::integral_constant<int, NBm>{}, FlatNet{Hm, Lm, PPp});
-  (void)PPm;
+  (void)lane;
 }
 
 // ---- kernel 4/5: reduction of gradient partials and optimizer -------------------
@@ -667,7 +699,7 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     constexpr int PK = Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size() : Packed<A, B>::size(); \
     constexpr int NBX = D > B ? D : B;                                                                             \
     const size_t lds_f = sizeof(float) * PK;                                                                       \
-    const size_t lds_b = sizeof(float) * (PK + 4 * 2 * 16 * NBX * kTP);                                           \
+    const size_t lds_b = sizeof(float) * (PK + kBW * 2 * 16 * NBX * kTP);                                         \
     const size_t red_b = sizeof(float) * std::max(PPp, PPm);                                                       \
     const size_t bwd_lds = std::max(lds_b, red_b);                                                                 \
     if (bwd_lds > 160 * 1024 || lds_f > 160 * 1024) {                                                              \
@@ -684,7 +716,7 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, rec->row_ptr, rec->row_c, bt->row_begin,   \
                        nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part);                              \
     if (nbp + nbm > 0)                                                                                             \
-      hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kMT), bwd_lds, st, T,                 \
+      hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kBT), bwd_lds, st, T,                 \
                          params, Hm, Hp, bt->t_begin, np_, nm_, bt->row_begin, r, gpart, nbp, stride);             \
   }
   DIPS_MFMA_SHAPES(X)
