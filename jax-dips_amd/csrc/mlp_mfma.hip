@@ -33,20 +33,24 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMT = 256;  // threads per workgroup (4 wavefronts)
 constexpr int kTP = 17;   // padded row length of a transposed 16-position tile
-constexpr int kMfmaBwdBlocks = 256;  // workgroups of the backward kernel (both phases), one per CU
+constexpr int kMfmaBwdBlocks = 256;   // workgroups of the backward kernel (both phases), one per CU
+constexpr int kMfmaFwdBlocks = 1024;  // workgroups of the forward kernel (both phases), four per CU
 
 __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
 // ---- packed weights of one phase net in LDS ---------------------------------
-// [L0: NB*64] [for hidden layer 1..L-1: fwd NB*NB*64*4, bwd NB*NB*64*4, bias NB*16] [wout NB*16] [bout 1]
-template <int L, int NB>
+// [L0: NB*64] [for hidden layer 1..L-1: fwd NB*NB*64*4, (bwd NB*NB*64*4,) bias NB*16] [wout NB*16] [bout 1]
+// The transposed (bwd) copies are packed only for the backward kernel.
+template <int L, int NB, bool kBwd = true>
 struct Packed {
   static constexpr int H = 16 * NB;
+  static constexpr int kMat = NB * NB * 256;
+  static constexpr int kPer = (kBwd ? 2 : 1) * kMat + H;
   static constexpr int l0() { return 0; }
-  static constexpr int fwd(int l) { return NB * 64 + (l - 1) * (2 * NB * NB * 256 + H); }
-  static constexpr int bwd(int l) { return fwd(l) + NB * NB * 256; }
-  static constexpr int bias(int l) { return bwd(l) + NB * NB * 256; }
-  static constexpr int wout() { return NB * 64 + (L - 1) * (2 * NB * NB * 256 + H); }
+  static constexpr int fwd(int l) { return NB * 64 + (l - 1) * kPer; }
+  static constexpr int bwd(int l) { return fwd(l) + kMat; }
+  static constexpr int bias(int l) { return fwd(l) + (kBwd ? 2 : 1) * kMat; }
+  static constexpr int wout() { return NB * 64 + (L - 1) * kPer; }
   static constexpr int bout() { return wout() + H; }
   static constexpr int size() { return bout() + 1; }
 };
@@ -60,9 +64,9 @@ struct FlatNet {
   __device__ int bout() const { return wout() + Hh; }
 };
 
-template <int L, int NB>
+template <int L, int NB, bool kBwd = true>
 __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
-  using PK = Packed<L, NB>;
+  using PK = Packed<L, NB, kBwd>;
   const int Hh = fn.Hh;
   for (int idx = threadIdx.x; idx < PK::size(); idx += blockDim.x) {
     float v = 0.0f;
@@ -79,19 +83,19 @@ __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float
       }
     } else {
       const int rel = idx - NB * 64;
-      const int per = 2 * NB * NB * 256 + PK::H;
+      const int per = PK::kPer;
       const int l = rel / per + 1;
       const int q = rel % per;
-      if (q < 2 * NB * NB * 256) {
-        const bool bw = q >= NB * NB * 256;
-        const int t = bw ? q - NB * NB * 256 : q;
+      if (q < (kBwd ? 2 : 1) * PK::kMat) {
+        const bool bw = q >= PK::kMat;
+        const int t = bw ? q - PK::kMat : q;
         const int r = t % 4, lane = (t / 4) % 64, fb = (t / 256) % NB, xb = t / (256 * NB);
         // fwd: W[16fb + 4(lane/16) + r][16xb + lane%16] ; bwd: W[16xb + lane%16][16fb + 4(lane/16) + r]
         const int a = 16 * fb + 4 * (lane / 16) + r, b = 16 * xb + lane % 16;
         const int in = bw ? b : a, out = bw ? a : b;
         if (in < Hh && out < Hh) v = params[fn.w(l) + in * Hh + out];
       } else {
-        const int h = q - 2 * NB * NB * 256;
+        const int h = q - (kBwd ? 2 : 1) * PK::kMat;
         if (h < Hh) v = params[fn.b(l) + h];
       }
     }
@@ -100,9 +104,9 @@ __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float
 }
 
 // forward of one tile: act[l][fb*4+r] feature-major; returns nothing (u by caller)
-template <int L, int NB>
+template <int L, int NB, bool kBwd = true>
 __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
-  using PK = Packed<L, NB>;
+  using PK = Packed<L, NB, kBwd>;
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int ob = 0; ob < NB; ++ob) {
@@ -137,9 +141,9 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
 }
 
 // output u of the tile position l%16 (identical in the 4 lanes of that position)
-template <int L, int NB>
+template <int L, int NB, bool kBwd = true>
 __device__ __forceinline__ float tile_output(const float* __restrict__ W, const float (&act)[L][4 * NB]) {
-  using PK = Packed<L, NB>;
+  using PK = Packed<L, NB, kBwd>;
   const int lane = threadIdx.x & 63;
   float s = 0.0f;
 #pragma unroll
@@ -197,9 +201,9 @@ __global__ __launch_bounds__(kMT) void mfma_fwd_kernel(MfmaTileArgs T, const flo
   const int bid = plus ? (int)blockIdx.x : (int)blockIdx.x - nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
   if (plus)
-    pack_weights<Lp, NBp>(params, FlatNet{Hp, Lp, 0}, lds);
+    pack_weights<Lp, NBp, false>(params, FlatNet{Hp, Lp, 0}, lds);
   else
-    pack_weights<Lm, NBm>(params, FlatNet{Hm, Lm, PPp}, lds);
+    pack_weights<Lm, NBm, false>(params, FlatNet{Hm, Lm, PPp}, lds);
   __syncthreads();
   const int64_t eb = plus ? e0 : e0 + np_;
   const int64_t ee = plus ? e0 + np_ : e0 + np_ + nm_;
@@ -213,12 +217,12 @@ __global__ __launch_bounds__(kMT) void mfma_fwd_kernel(MfmaTileArgs T, const flo
     float u;
     if (plus) {
       float act[Lp][4 * NBp];
-      tile_forward<Lp, NBp>(lds, xin, act);
-      u = tile_output<Lp, NBp>(lds, act);
+      tile_forward<Lp, NBp, false>(lds, xin, act);
+      u = tile_output<Lp, NBp, false>(lds, act);
     } else {
       float act[Lm][4 * NBm];
-      tile_forward<Lm, NBm>(lds, xin, act);
-      u = tile_output<Lm, NBm>(lds, act);
+      tile_forward<Lm, NBm, false>(lds, xin, act);
+      u = tile_output<Lm, NBm, false>(lds, act);
     }
     if (valid && lane < 16) u_out[T.t_csr[ebeg + lane]] = u;
   }
@@ -687,18 +691,24 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                  bt->t_csr};
   const int a = Lm, b = nb_of(Hm), c = Lp, d = nb_of(Hp);
   const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
-  // bwd workgroups split by phase in proportion to the tiles (each at least 1 if it has tiles)
-  int nbp = (ntp > 0) ? std::max<int64_t>(1, (int64_t)((double)kMfmaBwdBlocks * ntp / std::max<int64_t>(1, ntp + ntm)))
-                      : 0;
-  int nbm = (ntm > 0) ? std::max(1, kMfmaBwdBlocks - nbp) : 0;
-  if (nbp + nbm > kMfmaBwdBlocks) nbp = kMfmaBwdBlocks - nbm;
+  // workgroups split by phase in proportion to the tiles (each at least 1 if it has tiles)
+  auto split = [&](int total, int& p, int& m) {
+    p = (ntp > 0) ? (int)std::max<int64_t>(1, (int64_t)((double)total * ntp / std::max<int64_t>(1, ntp + ntm))) : 0;
+    m = (ntm > 0) ? std::max(1, total - p) : 0;
+    if (p + m > total) p = total - m;
+  };
+  int nbp, nbm, nfp, nfm;
+  split(kMfmaBwdBlocks, nbp, nbm);
+  split(kMfmaFwdBlocks, nfp, nfm);
   bool launched = false;
 #define X(A, B, C, D)                                                                                              \
   if (!launched && a == A && b == B && c == C && d == D) {                                                      \
     launched = true;                                                                                               \
     constexpr int PK = Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size() : Packed<A, B>::size(); \
+    constexpr int PKF = Packed<C, D, false>::size() > Packed<A, B, false>::size() ? Packed<C, D, false>::size()   \
+                                                                                  : Packed<A, B, false>::size();  \
     constexpr int NBX = D > B ? D : B;                                                                             \
-    const size_t lds_f = sizeof(float) * PK;                                                                       \
+    const size_t lds_f = sizeof(float) * PKF;                                                                      \
     const size_t lds_b = sizeof(float) * (PK + kBW * 2 * 16 * NBX * kTP);                                         \
     const size_t red_b = sizeof(float) * std::max(PPp, PPm);                                                       \
     const size_t bwd_lds = std::max(lds_b, red_b);                                                                 \
@@ -710,9 +720,9 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                         (int)lds_f);                                                                               \
     hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                         (int)bwd_lds);                                                                             \
-    if (nbp + nbm > 0)                                                                                             \
-      hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kMT), lds_f, st, T, params, Hm, Hp,   \
-                         bt->t_begin, np_, nm_, u - bt->ent_begin, nbp);                                           \
+    if (nfp + nfm > 0)                                                                                             \
+      hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kMT), lds_f, st, T, params, Hm, Hp,   \
+                         bt->t_begin, np_, nm_, u - bt->ent_begin, nfp);                                           \
     hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, rec->row_ptr, rec->row_c, bt->row_begin,   \
                        nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part);                              \
     if (nbp + nbm > 0)                                                                                             \
