@@ -80,11 +80,16 @@ typedef struct {
  * P = 0.5 + pc_scale * sigmoid(MLP(c)) on the 26 cell coefficients c of each
  * row (hidden widths pc_w1 and, if > 0, pc_w2, tanh), which multiplies the
  * row's residual; its parameters follow the two networks in the flat vector as
- * W1[26][pc_w1], b1, (W2[pc_w1][pc_w2], b2), Wout[.][1], bout. */
+ * W1[26][pc_w1], b1, (W2[pc_w1][pc_w2], b2), Wout[.][1], bout.
+ * act selects the hidden activation: 0 = tanh (DoubleMLP), 1 = CELU with
+ * alpha 1 (the coordinate MLPs of the hash network with the identity encoder,
+ * nn/hash_encoding/model.py:57-94 and blocks/nerfs_flax.py:142-170; same flat
+ * layout, kernel [in][out] then bias per layer). */
 typedef struct {
   int32_t Lm, Hm, Lp, Hp;
   int32_t pc_w1, pc_w2;
   float pc_scale;
+  int32_t act;
 } dips_net_t;
 
 /* Optimizer (solvers/optimizers.py:13-97).

@@ -29,6 +29,10 @@ static int check_net(const dips_net_t* net) {
     set_error("invalid preconditioner dimensions");
     return DIPS_EINVAL;
   }
+  if (net->act != 0 && net->act != 1) {
+    set_error("invalid activation (0 = tanh, 1 = celu)");
+    return DIPS_EINVAL;
+  }
   return DIPS_OK;
 }
 

@@ -32,6 +32,30 @@ __device__ __forceinline__ float dips_tanh(float x) {
   return ax < 0.55f ? small : copysignf(big, x);
 }
 
+// Hidden activations: 0 = tanh (DoubleMLP, nn/mlp/MLP.py:64-69), 1 = CELU with
+// alpha 1 (the coordinate MLPs of the hash network, nn/hash_encoding/blocks/
+// nerfs_flax.py:164: x above 0, expm1(x) below).  act_grad is the derivative
+// written in the activation value a: 1 - a^2 for tanh; 1 above 0 and a + 1
+// (= exp(x)) below for CELU.
+constexpr int kActTanh = 0;
+constexpr int kActCelu = 1;
+
+template <int Act>
+__device__ __forceinline__ float act_f(float s) {
+  if constexpr (Act == kActCelu)
+    return s > 0.0f ? s : expm1f(s);
+  else
+    return dips_tanh(s);
+}
+
+template <int Act>
+__device__ __forceinline__ float act_grad(float a) {
+  if constexpr (Act == kActCelu)
+    return a > 0.0f ? 1.0f : a + 1.0f;
+  else
+    return 1.0f - a * a;
+}
+
 // Network shapes with a compiled per-lane kernel.  (Lm, Hm, Lp, Hp)
 #define DIPS_SHAPES(X) \
   X(1, 1, 2, 10)       \
@@ -41,7 +65,7 @@ __device__ __forceinline__ float dips_tanh(float x) {
   X(2, 16, 2, 16)
 
 
-template <int L, int H, typename WP>
+template <int L, int H, int Act = kActTanh, typename WP>
 __device__ __forceinline__ float mlp_forward(WP P, float x, float y, float z) {
   using NL = NetLayout<L, H>;
   float a[H];
@@ -51,7 +75,7 @@ __device__ __forceinline__ float mlp_forward(WP P, float x, float y, float z) {
     s = fmaf(x, P[NL::w(0) + 0 * H + j], s);
     s = fmaf(y, P[NL::w(0) + 1 * H + j], s);
     s = fmaf(z, P[NL::w(0) + 2 * H + j], s);
-    a[j] = dips_tanh(s);
+    a[j] = act_f<Act>(s);
   }
 #pragma unroll
   for (int l = 1; l < L; ++l) {
@@ -61,7 +85,7 @@ __device__ __forceinline__ float mlp_forward(WP P, float x, float y, float z) {
       float s = P[NL::b(l) + j];
 #pragma unroll
       for (int i = 0; i < H; ++i) s = fmaf(a[i], P[NL::w(l) + i * H + j], s);
-      t[j] = dips_tanh(s);
+      t[j] = act_f<Act>(s);
     }
 #pragma unroll
     for (int j = 0; j < H; ++j) a[j] = t[j];
@@ -74,7 +98,7 @@ __device__ __forceinline__ float mlp_forward(WP P, float x, float y, float z) {
 
 // Forward pass that keeps the hidden activations (for a backward pass without
 // recomputation); returns the network output.
-template <int L, int H, typename WP>
+template <int L, int H, int Act = kActTanh, typename WP>
 __device__ __forceinline__ float mlp_forward_keep(WP P, float x, float y, float z, float (&act)[L][H]) {
   using NL = NetLayout<L, H>;
 #pragma unroll
@@ -83,7 +107,7 @@ __device__ __forceinline__ float mlp_forward_keep(WP P, float x, float y, float 
     s = fmaf(x, P[NL::w(0) + 0 * H + j], s);
     s = fmaf(y, P[NL::w(0) + 1 * H + j], s);
     s = fmaf(z, P[NL::w(0) + 2 * H + j], s);
-    act[0][j] = dips_tanh(s);
+    act[0][j] = act_f<Act>(s);
   }
 #pragma unroll
   for (int l = 1; l < L; ++l) {
@@ -92,7 +116,7 @@ __device__ __forceinline__ float mlp_forward_keep(WP P, float x, float y, float 
       float s = P[NL::b(l) + j];
 #pragma unroll
       for (int i = 0; i < H; ++i) s = fmaf(act[l - 1][i], P[NL::w(l) + i * H + j], s);
-      act[l][j] = dips_tanh(s);
+      act[l][j] = act_f<Act>(s);
     }
   }
   float o = P[NL::bout()];
@@ -103,7 +127,7 @@ __device__ __forceinline__ float mlp_forward_keep(WP P, float x, float y, float 
 
 // Adds seed * d NN(x,y,z) / d theta into G (register array of NL::size()),
 // given the activations of mlp_forward_keep at the same input.
-template <int L, int H, typename WP>
+template <int L, int H, int Act = kActTanh, typename WP>
 __device__ __forceinline__ void mlp_backward_kept(WP P, float x, float y, float z, float seed,
                                                   const float (&act)[L][H], float* G) {
   using NL = NetLayout<L, H>;
@@ -113,7 +137,7 @@ __device__ __forceinline__ void mlp_backward_kept(WP P, float x, float y, float 
   for (int i = 0; i < H; ++i) {
     G[NL::wout() + i] = fmaf(seed, act[L - 1][i], G[NL::wout() + i]);
     float a = act[L - 1][i];
-    d[i] = seed * P[NL::wout() + i] * (1.0f - a * a);
+    d[i] = seed * P[NL::wout() + i] * act_grad<Act>(a);
   }
 #pragma unroll
   for (int l = L - 1; l >= 1; --l) {
@@ -127,7 +151,7 @@ __device__ __forceinline__ void mlp_backward_kept(WP P, float x, float y, float 
         acc = fmaf(P[NL::w(l) + i * H + j], d[j], acc);
       }
       float a = act[l - 1][i];
-      dn[i] = acc * (1.0f - a * a);
+      dn[i] = acc * act_grad<Act>(a);
     }
 #pragma unroll
     for (int j = 0; j < H; ++j) G[NL::b(l) + j] += d[j];
@@ -145,13 +169,13 @@ __device__ __forceinline__ void mlp_backward_kept(WP P, float x, float y, float 
 
 // Pre-activation gradients of every hidden layer for the output seed:
 // dl[L-1] = seed * Wout * (1 - a^2), dl[l-1] = (W_l dl[l]) * (1 - a_{l-1}^2).
-template <int L, int H, typename WP>
+template <int L, int H, int Act = kActTanh, typename WP>
 __device__ __forceinline__ void mlp_deltas(WP P, float seed, const float (&act)[L][H], float (&dl)[L][H]) {
   using NL = NetLayout<L, H>;
 #pragma unroll
   for (int i = 0; i < H; ++i) {
     const float a = act[L - 1][i];
-    dl[L - 1][i] = seed * P[NL::wout() + i] * (1.0f - a * a);
+    dl[L - 1][i] = seed * P[NL::wout() + i] * act_grad<Act>(a);
   }
 #pragma unroll
   for (int l = L - 1; l >= 1; --l) {
@@ -161,22 +185,22 @@ __device__ __forceinline__ void mlp_deltas(WP P, float seed, const float (&act)[
 #pragma unroll
       for (int j = 0; j < H; ++j) acc = fmaf(P[NL::w(l) + i * H + j], dl[l][j], acc);
       const float a = act[l - 1][i];
-      dl[l - 1][i] = acc * (1.0f - a * a);
+      dl[l - 1][i] = acc * act_grad<Act>(a);
     }
   }
 }
 
 // Adds seed * d NN(x,y,z) / d theta into G, recomputing the forward activations.
-template <int L, int H, typename WP>
+template <int L, int H, int Act = kActTanh, typename WP>
 __device__ __forceinline__ void mlp_backward(WP P, float x, float y, float z, float seed, float* G) {
   float act[L][H];
-  (void)mlp_forward_keep<L, H>(P, x, y, z, act);
-  mlp_backward_kept<L, H>(P, x, y, z, seed, act, G);
+  (void)mlp_forward_keep<L, H, Act>(P, x, y, z, act);
+  mlp_backward_kept<L, H, Act>(P, x, y, z, seed, act, G);
 }
 
 
 // Forward value and input gradient d NN / d (x, y, z).
-template <int L, int H, typename WP>
+template <int L, int H, int Act = kActTanh, typename WP>
 __device__ __forceinline__ float mlp_value_grad(WP P, float x, float y, float z, float g[3]) {
   using NL = NetLayout<L, H>;
   float act[L][H];
@@ -186,7 +210,7 @@ __device__ __forceinline__ float mlp_value_grad(WP P, float x, float y, float z,
     s = fmaf(x, P[NL::w(0) + 0 * H + j], s);
     s = fmaf(y, P[NL::w(0) + 1 * H + j], s);
     s = fmaf(z, P[NL::w(0) + 2 * H + j], s);
-    act[0][j] = dips_tanh(s);
+    act[0][j] = act_f<Act>(s);
   }
 #pragma unroll
   for (int l = 1; l < L; ++l) {
@@ -195,7 +219,7 @@ __device__ __forceinline__ float mlp_value_grad(WP P, float x, float y, float z,
       float s = P[NL::b(l) + j];
 #pragma unroll
       for (int i = 0; i < H; ++i) s = fmaf(act[l - 1][i], P[NL::w(l) + i * H + j], s);
-      act[l][j] = dips_tanh(s);
+      act[l][j] = act_f<Act>(s);
     }
   }
   float o = P[NL::bout()];
@@ -204,7 +228,7 @@ __device__ __forceinline__ float mlp_value_grad(WP P, float x, float y, float z,
   for (int i = 0; i < H; ++i) {
     o = fmaf(act[L - 1][i], P[NL::wout() + i], o);
     const float a = act[L - 1][i];
-    d[i] = P[NL::wout() + i] * (1.0f - a * a);
+    d[i] = P[NL::wout() + i] * act_grad<Act>(a);
   }
 #pragma unroll
   for (int l = L - 1; l >= 1; --l) {
@@ -215,7 +239,7 @@ __device__ __forceinline__ float mlp_value_grad(WP P, float x, float y, float z,
 #pragma unroll
       for (int j = 0; j < H; ++j) acc = fmaf(P[NL::w(l) + i * H + j], d[j], acc);
       const float a = act[l - 1][i];
-      dn[i] = acc * (1.0f - a * a);
+      dn[i] = acc * act_grad<Act>(a);
     }
 #pragma unroll
     for (int i = 0; i < H; ++i) d[i] = dn[i];

@@ -11,7 +11,7 @@
 
 namespace dips {
 
-template <int Lm, int Hm, int Lp, int Hp>
+template <int Lm, int Hm, int Lp, int Hp, int Act>
 __global__ __launch_bounds__(256) void eval_kernel(const float* __restrict__ params, const float* __restrict__ pts,
                                                    const uint8_t* __restrict__ phase, const float* __restrict__ nrm,
                                                    int64_t m, float* __restrict__ u, float* __restrict__ grad,
@@ -22,7 +22,8 @@ __global__ __launch_bounds__(256) void eval_kernel(const float* __restrict__ par
   const float x = pts[t * 3 + 0], y = pts[t * 3 + 1], z = pts[t * 3 + 2];
   float g[3];
   dips_weight_t* W = dips_weights(params);
-  const float v = phase[t] ? mlp_value_grad<Lp, Hp>(W, x, y, z, g) : mlp_value_grad<Lm, Hm>(W + NP::size(), x, y, z, g);
+  const float v = phase[t] ? mlp_value_grad<Lp, Hp, Act>(W, x, y, z, g)
+                           : mlp_value_grad<Lm, Hm, Act>(W + NP::size(), x, y, z, g);
   u[t] = v;
   if (grad) {
     grad[t * 3 + 0] = g[0];
@@ -40,13 +41,17 @@ int eval_launch(const dips_net_t* net, const float* params, const float* pts, co
     set_error("dips_eval: too many points for one launch");
     return DIPS_EINVAL;
   }
-#define X(a, b, c, d)                                                                                           \
-  if (net->Lm == a && net->Hm == b && net->Lp == c && net->Hp == d) {                                          \
-    hipLaunchKernelGGL((eval_kernel<a, b, c, d>), dim3((unsigned)nb), dim3(256), 0, st, params, pts, phase, nrm, m, \
-                       u, grad, gradn);                                                                         \
+#define X(a, b, c, d, e)                                                                                        \
+  if (net->Lm == a && net->Hm == b && net->Lp == c && net->Hp == d && net->act == e) {                         \
+    hipLaunchKernelGGL((eval_kernel<a, b, c, d, e>), dim3((unsigned)nb), dim3(256), 0, st, params, pts, phase, nrm, \
+                       m, u, grad, gradn);                                                                      \
     return hip_status(hipGetLastError(), "eval_kernel launch");                                                 \
   }
-  DIPS_SHAPES(X)
+#define XT(a, b, c, d) X(a, b, c, d, 0)
+  DIPS_SHAPES(XT)
+  X(1, 16, 1, 16, 1)
+  X(2, 16, 2, 16, 1)
+#undef XT
 #undef X
   set_error("dips_eval: network shape has no compiled kernel");
   return DIPS_EUNSUPPORTED;

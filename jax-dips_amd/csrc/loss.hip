@@ -266,7 +266,7 @@ static void launch_shape(const dips_grid_t* g, const dips_records_t* rec, int64_
 }
 
 int net_supported(const dips_net_t* n) {
-  if (n->pc_w1 > 0) return 0;  // the preconditioner runs in the tile kernel only
+  if (n->pc_w1 > 0 || n->act != 0) return 0;  // the preconditioner and CELU run in the tile kernel only
 #define X(a, b, c, d) \
   if (n->Lm == a && n->Hm == b && n->Lp == c && n->Hp == d) return 1;
   DIPS_SHAPES(X)

@@ -633,7 +633,7 @@ static int nb_of(int H) { return (H + 15) / 16; }
   X(3, 2, 3, 2)
 
 int mfma_supported(const dips_net_t* n) {
-  if (n->Hm > 64 || n->Hp > 64 || n->pc_w1 > 0) return 0;
+  if (n->Hm > 64 || n->Hp > 64 || n->pc_w1 > 0 || n->act != 0) return 0;
   const int a = n->Lm, b = nb_of(n->Hm), c = n->Lp, d = nb_of(n->Hp);
 #define X(A, B, C, D) \
   if (a == A && b == B && c == C && d == D) return 1;
@@ -656,8 +656,8 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
               const dips_opt_t* opt, float inv_b, float* params, float* m, float* v, int32_t* count, float* loss_hist,
               int loss_cap, float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, float* grad_out,
               double* loss_out, hipStream_t st) {
-  if (net->pc_w1 > 0) {
-    set_error("dips_mfma_train_step: the preconditioner runs in the tile kernel only (dips_tile_*)");
+  if (net->pc_w1 > 0 || net->act != 0) {
+    set_error("dips_mfma_train_step: the preconditioner and CELU networks run in the tile kernel only (dips_tile_*)");
     return DIPS_EUNSUPPORTED;
   }
   const int Lm = net->Lm, Hm = net->Hm, Lp = net->Lp, Hp = net->Hp;

@@ -44,19 +44,23 @@ constexpr int kReduceChunk = 8;                       // parameters per wavefron
 constexpr int kLdsStride = 68;                        // floats per staged operand row: 64 entries + 4
 constexpr int kStageFloats = 2 * 16 * kLdsStride;     // A and B operand stacks of one wavefront
 
-// Network shapes with a compiled tile kernel (Lm, Hm, Lp, Hp) and
-// preconditioner hidden widths (PC1, PC2; 0 = no preconditioner, PC2 = 0 = one
-// hidden layer).  [8, 4] is the preconditioner of the reference's LPBE
+// Network shapes with a compiled tile kernel (Lm, Hm, Lp, Hp), preconditioner
+// hidden widths (PC1, PC2; 0 = no preconditioner, PC2 = 0 = one hidden layer)
+// and hidden activation (0 = tanh, 1 = CELU: the hash network's coordinate
+// MLPs with the identity encoder, 3 -> [16] -> 1 in the reference's LPBE
+// configuration, examples/benchmark_LPBE/conf/lpbe.yaml:48-60).  [8, 4] is the preconditioner of the reference's LPBE
 // configuration (examples/benchmark_LPBE/conf/lpbe.yaml:63-67).
 #define DIPS_TILE_SHAPES(X)  \
-  X(1, 1, 2, 10, 0, 0)       \
-  X(1, 3, 2, 10, 0, 0)       \
-  X(1, 1, 1, 1, 0, 0)        \
-  X(2, 10, 2, 10, 0, 0)      \
-  X(2, 16, 2, 16, 0, 0)      \
-  X(1, 1, 2, 10, 8, 4)       \
-  X(1, 1, 2, 10, 8, 0)       \
-  X(2, 10, 2, 10, 8, 4)
+  X(1, 1, 2, 10, 0, 0, 0)    \
+  X(1, 3, 2, 10, 0, 0, 0)    \
+  X(1, 1, 1, 1, 0, 0, 0)     \
+  X(2, 10, 2, 10, 0, 0, 0)   \
+  X(2, 16, 2, 16, 0, 0, 0)   \
+  X(1, 1, 2, 10, 8, 4, 0)    \
+  X(1, 1, 2, 10, 8, 0, 0)    \
+  X(2, 10, 2, 10, 8, 4, 0)   \
+  X(1, 16, 1, 16, 0, 0, 1)   \
+  X(2, 16, 2, 16, 0, 0, 1)
 
 struct TileStepArgs {
   PointSet ps;
@@ -319,7 +323,7 @@ __device__ __forceinline__ int mfma_grad_param(int blk, int i, int j) {
 }
 
 // One phase network's gradient accumulators: MFMA blocks or per-lane registers.
-template <int L, int H, bool kGrad>
+template <int L, int H, bool kGrad, int Act = kActTanh>
 struct NetGrad {
   static constexpr bool kWide = MfmaGradWide<H>::value;
   static constexpr bool kMfma = MfmaGrad<H>::value || kWide;
@@ -353,7 +357,7 @@ struct NetGrad {
     if constexpr (kGrad && kWide) {
       const float s = mine ? seed : 0.0f;
       float dl[L][H];
-      mlp_deltas<L, H>(Q, s, act, dl);
+      mlp_deltas<L, H, Act>(Q, s, act, dl);
 #pragma unroll
       for (int l = 1; l < L; ++l)
 #pragma unroll
@@ -365,10 +369,10 @@ struct NetGrad {
     } else if constexpr (kGrad && kMfma) {
       const float s = mine ? seed : 0.0f;
       float dl[L][H];
-      mlp_deltas<L, H>(Q, s, act, dl);
+      mlp_deltas<L, H, Act>(Q, s, act, dl);
       mfma_grad_chunk<L, H>(AT, BT, lane, x, y, z, act, dl, s, acc);
     } else if constexpr (kGrad) {
-      if (mine) mlp_backward_kept<L, H>(Q, x, y, z, seed, act, g);
+      if (mine) mlp_backward_kept<L, H, Act>(Q, x, y, z, seed, act, g);
     }
   }
 
@@ -593,7 +597,7 @@ struct TileCfg {
 // The body of one workgroup: block blk of nblk workgroups over the tiles of
 // a.  Writes the workgroup's partial row (kOut 0: partial[blk * stride + p],
 // else partial[p * nblk + blk]).
-template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1, int PC2>
+template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1, int PC2, int Act>
 __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* params, float* __restrict__ partial,
                                                int blk, int nblk) {
   constexpr int kTileThreads = TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads;
@@ -616,8 +620,8 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
   // the staging rows a block does not write hold stale values; they only feed
   // accumulator entries that are never flushed (no zero fill needed)
 
-  NetGrad<Lp, Hp, kGrad> gp;
-  NetGrad<Lm, Hm, kGrad> gm;
+  NetGrad<Lp, Hp, kGrad, Act> gp;
+  NetGrad<Lm, Hm, kGrad, Act> gm;
   PrecondGrad<PC1, PC2, kGrad> gc;
   gp.zero();
   gm.zero();
@@ -669,12 +673,12 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
       // the two phase networks in separate branches: merged into one call
       // they would read the weights through a per-lane pointer
       float up = 0.0f, um = 0.0f;
-      if (on && plus) up = mlp_forward_keep<Lp, Hp>(Pp, x, y, z, actp);
+      if (on && plus) up = mlp_forward_keep<Lp, Hp, Act>(Pp, x, y, z, actp);
       asm volatile("" : "+s"(Pm));
-      if (on && !plus) um = mlp_forward_keep<Lm, Hm>(Pm, x, y, z, actm);
+      if (on && !plus) um = mlp_forward_keep<Lm, Hm, Act>(Pm, x, y, z, actm);
       if (on) v = fmaf(w, plus ? up : um, c);
     } else if (on) {
-      const float u = plus ? mlp_forward_keep<Lp, Hp>(Pp, x, y, z, actp) : mlp_forward_keep<Lm, Hm>(Pm, x, y, z, actm);
+      const float u = plus ? mlp_forward_keep<Lp, Hp, Act>(Pp, x, y, z, actp) : mlp_forward_keep<Lm, Hm, Act>(Pm, x, y, z, actm);
       v = fmaf(w, u, c);  // c is nonzero only in the row's last slot
     }
     // segmented inclusive scan: lane l ends with the sum of lanes head(l)..l
@@ -750,12 +754,12 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
           if constexpr (kSameShape) {
             dips_weight_t* Wm = Wp + PP;
             float up = 0.0f, um = 0.0f;
-            if (pl) up = mlp_forward<Lp, Hp>(Wp, x, y, z);
+            if (pl) up = mlp_forward<Lp, Hp, Act>(Wp, x, y, z);
             asm volatile("" : "+s"(Wm));
-            if (!pl) um = mlp_forward<Lm, Hm>(Wm, x, y, z);
+            if (!pl) um = mlp_forward<Lm, Hm, Act>(Wm, x, y, z);
             u = pl ? up : um;
           } else {
-            u = pl ? mlp_forward<Lp, Hp>(Wp, x, y, z) : mlp_forward<Lm, Hm>(Wp + PP, x, y, z);
+            u = pl ? mlp_forward<Lp, Hp, Act>(Wp, x, y, z) : mlp_forward<Lm, Hm, Act>(Wp + PP, x, y, z);
           }
           acc = fmaf(gld(a.ent_w, e), u, acc);
         }
@@ -806,14 +810,14 @@ __device__ __forceinline__ void tile_step_body(const TileStepArgs& a, float* par
           }
           dips_weight_t* Wm = Wp + PP;
           if constexpr (kSameShape) {
-            if (on && plus) (void)mlp_forward_keep<Lp, Hp>(Wp, x, y, z, actp);
+            if (on && plus) (void)mlp_forward_keep<Lp, Hp, Act>(Wp, x, y, z, actp);
             asm volatile("" : "+s"(Wm));
-            if (on && !plus) (void)mlp_forward_keep<Lm, Hm>(Wm, x, y, z, actm);
+            if (on && !plus) (void)mlp_forward_keep<Lm, Hm, Act>(Wm, x, y, z, actm);
           } else if (on) {
             if (plus)
-              (void)mlp_forward_keep<Lp, Hp>(Wp, x, y, z, actp);
+              (void)mlp_forward_keep<Lp, Hp, Act>(Wp, x, y, z, actp);
             else
-              (void)mlp_forward_keep<Lm, Hm>(Wm, x, y, z, actm);
+              (void)mlp_forward_keep<Lm, Hm, Act>(Wm, x, y, z, actm);
           }
           gp.add(Wp, AT, BT, lane, on && plus, x, y, z, seed, actp);
           gm.add(Wm, AT, BT, lane, on && !plus, x, y, z, seed, actm);
@@ -854,13 +858,13 @@ constexpr int tile_stride() {
          (PC1 > 0 ? Precond<(PC1 > 0 ? PC1 : 1), PC2>::size() : 0) + 1;
 }
 
-template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1 = 0, int PC2 = 0>
+template <int Lm, int Hm, int Lp, int Hp, bool kGrad, int kOut, int PC1 = 0, int PC2 = 0, int Act = kActTanh>
 __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile_step_kernel(
     TileStepArgs a, float* params, float* __restrict__ partial, FusedOpt fo) {
   constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
   constexpr int stride = tile_stride<Lm, Hm, Lp, Hp, PC1, PC2>();
   const int nblk = (int)gridDim.x;
-  tile_step_body<Lm, Hm, Lp, Hp, kGrad, kOut, PC1, PC2>(a, params, partial, (int)blockIdx.x, nblk);
+  tile_step_body<Lm, Hm, Lp, Hp, kGrad, kOut, PC1, PC2, Act>(a, params, partial, (int)blockIdx.x, nblk);
   if (kOut != 2) return;
 
   // One thread per workgroup fences and counts: the barrier completes the
@@ -886,14 +890,14 @@ __host__ __device__ static inline int tile_blocks(int64_t ntiles, int waves) {
   return (int)(b < 1 ? 1 : (b > kMaxTileBlocks ? kMaxTileBlocks : b));
 }
 
-static bool tile_shape_is(const dips_net_t* n, int Lm, int Hm, int Lp, int Hp, int pc1, int pc2) {
+static bool tile_shape_is(const dips_net_t* n, int Lm, int Hm, int Lp, int Hp, int pc1, int pc2, int act) {
   const int w1 = n->pc_w1 > 0 ? n->pc_w1 : 0, w2 = n->pc_w1 > 0 ? n->pc_w2 : 0;
-  return n->Lm == Lm && n->Hm == Hm && n->Lp == Lp && n->Hp == Hp && w1 == pc1 && w2 == pc2;
+  return n->Lm == Lm && n->Hm == Hm && n->Lp == Lp && n->Hp == Hp && w1 == pc1 && w2 == pc2 && n->act == act;
 }
 
 int tile_supported(const dips_net_t* n) {
-#define X(A, B, C, D, E, F) \
-  if (tile_shape_is(n, A, B, C, D, E, F)) return 1;
+#define X(A, B, C, D, E, F, G) \
+  if (tile_shape_is(n, A, B, C, D, E, F, G)) return 1;
   DIPS_TILE_SHAPES(X)
 #undef X
   return 0;
@@ -901,7 +905,7 @@ int tile_supported(const dips_net_t* n) {
 
 int64_t tile_partial_size(const dips_net_t* net) { return (int64_t)kMaxTileBlocks * partial_stride(net); }
 
-template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2>
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2, int Act>
 static void tile_launch_shape(const TileStepArgs& a, float* params, float* partial, int mode, const FusedOpt& fo,
                               int nblk, hipStream_t st) {
   dim3 grid(nblk), block(TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads);
@@ -909,11 +913,11 @@ static void tile_launch_shape(const TileStepArgs& a, float* params, float* parti
   // the dynamic LDS limit is raised once per kernel instantiation
 #define DIPS_TILE_LAUNCH(G, F)                                                                                   \
   static const bool lds_set_##G##F =                                                                             \
-      hipFuncSetAttribute((const void*)tile_step_kernel<Lm, Hm, Lp, Hp, G, F, PC1, PC2>,                         \
+      hipFuncSetAttribute((const void*)tile_step_kernel<Lm, Hm, Lp, Hp, G, F, PC1, PC2, Act>,                    \
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;                   \
   (void)lds_set_##G##F;                                                                                          \
-  hipLaunchKernelGGL((tile_step_kernel<Lm, Hm, Lp, Hp, G, F, PC1, PC2>), grid, block, lds, st, a, params, partial, \
-                     fo)
+  hipLaunchKernelGGL((tile_step_kernel<Lm, Hm, Lp, Hp, G, F, PC1, PC2, Act>), grid, block, lds, st, a, params,      \
+                     partial, fo)
   if (mode == 2) {
     DIPS_TILE_LAUNCH(true, 2);
   } else if (mode == 3) {
@@ -978,15 +982,15 @@ int tile_step_launch(const dips_grid_t* g, const dips_records_t* rec, const dips
   // with an optimizer: one fused launch when an arrival counter is given,
   // else the step kernel followed by tile_apply_kernel
   const int mode = fused ? (fused->sync ? 2 : 3) : (want_grad ? 1 : 0);
-#define X(A, B, C, D, E, F)                                                                             \
-  if (tile_shape_is(net, A, B, C, D, E, F)) {                                                          \
+#define X(A, B, C, D, E, F, G)                                                                          \
+  if (tile_shape_is(net, A, B, C, D, E, F, G)) {                                                       \
     const int nblk = tile_blocks(ntiles, TileCfg<A, B, C, D, E>::kWaves);                              \
     if (partial_cap < (int64_t)nblk * partial_stride(net)) {                                           \
       set_error("tile step: partial buffer too small");                                                \
       return DIPS_EINVAL;                                                                              \
     }                                                                                                  \
     *nblk_out = nblk;                                                                                  \
-    tile_launch_shape<A, B, C, D, E, F>(a, const_cast<float*>(params), partial, mode, fo, nblk, stream); \
+    tile_launch_shape<A, B, C, D, E, F, G>(a, const_cast<float*>(params), partial, mode, fo, nblk, stream); \
     return hip_status(hipGetLastError(), "tile_step_kernel launch");                                   \
   }
   DIPS_TILE_SHAPES(X)
@@ -1033,7 +1037,7 @@ __device__ __forceinline__ bool batch_step_range(const BatchDesc& d, int s, int 
   return true;
 }
 
-template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2>
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2, int Act>
 __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile_batch_step_kernel(
     const BatchDesc* __restrict__ descs, int nprob, int step) {
   constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
@@ -1056,7 +1060,7 @@ __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile
   TileStepArgs a = d.a;
   a.tile_begin = tb;
   a.ntiles = nt;
-  tile_step_body<Lm, Hm, Lp, Hp, true, 1, PC1, PC2>(a, d.params, d.partial, k, nblk);
+  tile_step_body<Lm, Hm, Lp, Hp, true, 1, PC1, PC2, Act>(a, d.params, d.partial, k, nblk);
 }
 
 template <int kStride, int kWaves>
@@ -1079,8 +1083,8 @@ constexpr int64_t kAlign = 256;
 int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 int tile_waves_of(const dips_net_t* net) {
-#define X(A, B, C, D, E, F) \
-  if (tile_shape_is(net, A, B, C, D, E, F)) return TileCfg<A, B, C, D, E>::kWaves;
+#define X(A, B, C, D, E, F, G) \
+  if (tile_shape_is(net, A, B, C, D, E, F, G)) return TileCfg<A, B, C, D, E>::kWaves;
   DIPS_TILE_SHAPES(X)
 #undef X
   return 0;
@@ -1138,15 +1142,15 @@ int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const 
   return L.bytes;
 }
 
-template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2>
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2, int Act>
 static void batch_launch_shape(const BatchDesc* descs, int nprob, int nblk_total, int step, hipStream_t st) {
   constexpr int kWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
   const size_t lds = TileCfg<Lm, Hm, Lp, Hp, PC1>::kLdsBytes;
   static const bool lds_set =
-      hipFuncSetAttribute((const void*)tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2>,
+      hipFuncSetAttribute((const void*)tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2, Act>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
   (void)lds_set;
-  hipLaunchKernelGGL((tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2>), dim3(nblk_total),
+  hipLaunchKernelGGL((tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2, Act>), dim3(nblk_total),
                      dim3(TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads), lds, st, descs, nprob, step);
   constexpr int kStride = tile_stride<Lm, Hm, Lp, Hp, PC1, PC2>();
   hipLaunchKernelGGL((tile_batch_apply_kernel<kStride, kWaves>), dim3(nprob), dim3(1024), 0, st, descs, step);
@@ -1219,9 +1223,9 @@ int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepoch
   if (e != hipSuccess) return hip_status(e, "batch train: table copy");
   const BatchDesc* dd = reinterpret_cast<const BatchDesc*>(base + L.desc_off);
   for (int64_t s = 0; s < maxsteps; ++s) {
-#define X(A, B, C, D, E, F)                                                         \
-  if (tile_shape_is(net, A, B, C, D, E, F)) {                                      \
-    batch_launch_shape<A, B, C, D, E, F>(dd, nprob, L.nblk_total, (int)s, stream); \
+#define X(A, B, C, D, E, F, G)                                                         \
+  if (tile_shape_is(net, A, B, C, D, E, F, G)) {                                      \
+    batch_launch_shape<A, B, C, D, E, F, G>(dd, nprob, L.nblk_total, (int)s, stream); \
   } else
     DIPS_TILE_SHAPES(X)
 #undef X

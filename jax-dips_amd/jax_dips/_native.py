@@ -33,7 +33,8 @@ class Records(ctypes.Structure):
 
 class Net(ctypes.Structure):
     _fields_ = [("Lm", ctypes.c_int32), ("Hm", ctypes.c_int32), ("Lp", ctypes.c_int32), ("Hp", ctypes.c_int32),
-                ("pc_w1", ctypes.c_int32), ("pc_w2", ctypes.c_int32), ("pc_scale", ctypes.c_float)]
+                ("pc_w1", ctypes.c_int32), ("pc_w2", ctypes.c_int32), ("pc_scale", ctypes.c_float),
+                ("act", ctypes.c_int32)]
 
 
 class Opt(ctypes.Structure):

@@ -275,10 +275,11 @@ def make_opt(optimizer_dict):
 class Engine:
     """Parameters, optimizer state and scratch of one two-phase network on the GPU."""
 
-    def __init__(self, dims, params, optimizer_dict, max_rows, max_steps, device="cuda", precond=None):
+    def __init__(self, dims, params, optimizer_dict, max_rows, max_steps, device="cuda", precond=None, act=0):
         """precond: None, or {"widths": [w1] or [w1, w2], "scale": s} for the
         learned preconditioner; its parameters follow the two networks' in
-        params, and records then need per-row cell coefficients."""
+        params, and records then need per-row cell coefficients.  act: hidden
+        activation, 0 = tanh, 1 = CELU (tile kernel only)."""
         import torch
 
         self.dims = tuple(int(d) for d in dims)
@@ -290,14 +291,15 @@ class Engine:
                 raise ValueError(f"preconditioner widths must be one or two positive widths, got {widths}")
             self.precond = dict(widths=widths, scale=float(precond.get("scale", 1.0)))
             pc = (widths[0], widths[1] if len(widths) == 2 else 0, self.precond["scale"])
-        self.net = N.Net(*self.dims, *pc)
+        self.act = int(act)
+        self.net = N.Net(*self.dims, *pc, self.act)
         L = N.lib()
         self.lane_path = bool(L.dips_net_supported(ctypes.byref(self.net)))
         self.tile_path = bool(L.dips_tile_supported(ctypes.byref(self.net)))
         self.mfma_path = bool(L.dips_mfma_supported(ctypes.byref(self.net)))
-        if self.precond is not None and not self.tile_path:
-            raise N.DipsError(f"network shape {self.dims} with preconditioner widths {self.precond['widths']} "
-                              "has no compiled tile kernel")
+        if (self.precond is not None or self.act != 0) and not self.tile_path:
+            raise N.DipsError(f"network shape {self.dims} (activation {self.act}, preconditioner "
+                              f"{self.precond}) has no compiled tile kernel")
         if not (self.lane_path or self.mfma_path or self.tile_path):
             raise N.DipsError(f"network shape (Lm,Hm,Lp,Hp)={self.dims} has no compiled kernel")
         self.P = int(L.dips_param_count(ctypes.byref(self.net)))
@@ -409,7 +411,7 @@ class Engine:
                                                int(self._mdwork.numel()), N.current_stream()), "dips_mfma_train_step")
 
     def train_level(self, grid, rec, nepochs):
-        if self.tile_path and (self.use_tiles or self.precond is not None):
+        if self.tile_path and (self.use_tiles or self.precond is not None or self.act != 0):
             return self.train_level_tiles(grid, rec, nepochs)
         if not self.lane_path:
             return self.train_level_mfma(grid, rec, nepochs)
@@ -430,10 +432,10 @@ class BatchEngine:
     problem).  Each problem follows the trajectory Engine.train_level gives it
     alone."""
 
-    def __init__(self, dims, params_list, optimizer_dict, max_steps, device="cuda", precond=None):
+    def __init__(self, dims, params_list, optimizer_dict, max_steps, device="cuda", precond=None, act=0):
         """max_steps: one optimizer-step capacity (loss history length) per problem."""
-        self.engines = [Engine(dims, p, optimizer_dict, max_rows=1, max_steps=ms, device=device, precond=precond)
-                        for p, ms in zip(params_list, max_steps)]
+        self.engines = [Engine(dims, p, optimizer_dict, max_rows=1, max_steps=ms, device=device, precond=precond,
+                               act=act) for p, ms in zip(params_list, max_steps)]
         if len(self.engines) != len(params_list):
             raise ValueError("one max_steps entry per problem")
         if not self.engines[0].tile_path:

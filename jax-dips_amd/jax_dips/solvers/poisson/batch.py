@@ -93,7 +93,8 @@ def solve_batch(problems, num_epochs=1000, batch_size=65536, model_dict=DEFAULT_
             p = init_params[i]
             params0.append(_flatten(model, precond, p) if isinstance(p, dict) else np.asarray(p, np.float32))
     eng = BatchEngine(model.dims, params0, optimizer_dict, [E * nb for nb in nbs],
-                      precond=precond.engine_config() if precond is not None else None)
+                      precond=precond.engine_config() if precond is not None else None,
+                      act=getattr(model, "act", 0))
     epoch = 0
     while epoch < E:
         lvl = epoch // (E // 4)

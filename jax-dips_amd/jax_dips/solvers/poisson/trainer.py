@@ -26,8 +26,11 @@ reference pads it (its draws come from jax's PRNGKey(0), these from numpy).
 The learned preconditioner (model_dict["preconditioner"], trainer.py:229-252)
 trains jointly with the networks inside the tile kernel; its parameters live
 under params["preconditioner"] and its initial draw comes from numpy.
-Unsupported reference options raise NotImplementedError: algorithm 1, lbfgs,
-nonlinear operators and non-tanh activations.
+model_type "multiresolution_hash_network" runs with the identity position
+encoder (CELU coordinate MLPs, nn/hash_network.py).  Unsupported reference
+options raise NotImplementedError: algorithm 1, lbfgs, nonlinear operators,
+activations other than tanh in "mlp", and the hashgrid, frequency and
+spherical-harmonics encoders.
 """
 import logging
 import os
@@ -204,7 +207,8 @@ class Trainer:
         steps_total = (E - self.epoch_start) * nb
         engine = Engine(self.model.dims, self.flatten(self.params), self.optimizer_dict, max_rows=B,
                         max_steps=max(1, steps_total),
-                        precond=self.precond.engine_config() if self.precond is not None else None)
+                        precond=self.precond.engine_config() if self.precond is not None else None,
+                        act=getattr(self.model, "act", 0))
         if self.opt_restore is not None:
             engine.m.copy_(torch.as_tensor(self.flatten(self.opt_restore["mu"]), device="cuda"))
             engine.v.copy_(torch.as_tensor(self.flatten(self.opt_restore["nu"]), device="cuda"))

@@ -7,11 +7,31 @@ NN-(x); NN = [Linear(H), tanh] * L then Linear(1).  Parameters are handled as
 one flat vector laid out as (for net in (+, -)): for each layer W (fan_in x
 fan_out, row-major) then b.  ``dims`` = (L_m, H_m, L_p, H_p).
 
+act = "celu" restates the hidden activation of the coordinate MLPs of the
+multiresolution hash network (nn/hash_encoding/blocks/nerfs_flax.py:142-170,
+flax nn.celu with alpha = 1: x for x > 0, expm1(x) otherwise).
+
 Pinned by tests/test_oracle_mlp.py: forward against an independent torch
 (float64, CPU) implementation; the VJP against torch autograd and against
-float64 central finite differences (h = 1e-3 on 32 random parameters).
+float64 central finite differences (h = 1e-3 on 32 random parameters), for
+both activations; CELU against its closed form.
 """
 import numpy as np
+
+
+def activation(h, act):
+    if act == "tanh":
+        return np.tanh(h)
+    if act == "celu":
+        return np.where(h > 0, h, np.expm1(np.minimum(h, 0.0)))
+    raise ValueError(act)
+
+
+def activation_grad(a, act):
+    """Derivative from the activation value a (tanh: 1 - a^2; celu: 1 above 0, a + 1 below)."""
+    if act == "tanh":
+        return 1 - a * a
+    return np.where(a > 0, 1.0, a + 1.0)
 
 
 def net_shapes(L, H):
@@ -73,29 +93,29 @@ def init_params(dims, seed=42, dt=np.float32):
     return theta.astype(dt)
 
 
-def _forward_net(net, X, dt):
+def _forward_net(net, X, dt, act="tanh"):
     acts = [X]
     h = X
     for W, b in net[:-1]:
-        h = np.tanh(h @ W.astype(dt) + b.astype(dt))
+        h = activation(h @ W.astype(dt) + b.astype(dt), act)
         acts.append(h)
     W, b = net[-1]
     out = (h @ W.astype(dt) + b.astype(dt))[:, 0]
     return out, acts
 
 
-def forward(theta, dims, X, phase_p, dt=np.float64):
+def forward(theta, dims, X, phase_p, dt=np.float64, act="tanh"):
     """u at positions X (m,3); phase_p (m,) True selects the + network."""
     nets = unflatten(np.asarray(theta, dtype=dt), dims)
     X = np.asarray(X, dtype=dt)
     u = np.zeros(X.shape[0], dtype=dt)
     for tag, sel in (("p", phase_p), ("m", ~phase_p)):
         if sel.any():
-            u[sel], _ = _forward_net(nets[tag], X[sel], dt)
+            u[sel], _ = _forward_net(nets[tag], X[sel], dt, act)
     return u
 
 
-def vjp(theta, dims, X, phase_p, seed, dt=np.float64):
+def vjp(theta, dims, X, phase_p, seed, dt=np.float64, act="tanh"):
     """Gradient of sum_i seed_i * u(X_i) with respect to the flat parameters."""
     nets = unflatten(np.asarray(theta, dtype=dt), dims)
     lay = layout(dims)
@@ -106,7 +126,7 @@ def vjp(theta, dims, X, phase_p, seed, dt=np.float64):
         if not sel.any():
             continue
         net = nets[tag]
-        _, acts = _forward_net(net, X[sel], dt)
+        _, acts = _forward_net(net, X[sel], dt, act)
         d = seed[sel][:, None]  # d out, (m,1)
         for li in range(len(net) - 1, -1, -1):
             W, b = net[li]
@@ -115,7 +135,7 @@ def vjp(theta, dims, X, phase_p, seed, dt=np.float64):
             grad[off:off + fa * fb] += (a_in.T @ d).reshape(-1)
             grad[off + fa * fb:off + fa * fb + fb] += d.sum(axis=0)
             if li > 0:
-                d = (d @ W.astype(dt).T) * (1 - acts[li] * acts[li])
+                d = (d @ W.astype(dt).T) * activation_grad(acts[li], act)
     return grad
 
 

@@ -92,7 +92,7 @@ def residual(theta, dims, rec, rows, dt=np.float64):
     return r
 
 
-def loss_and_grad(theta, dims, rec, rows, dt=np.float64, precond=None):
+def loss_and_grad(theta, dims, rec, rows, dt=np.float64, precond=None, act="tanh"):
     """0.5 * mean(r^2) over the batch and its gradient (hand VJP).
 
     precond = dict(widths=..., scale=s): the last precond.size(widths) entries
@@ -102,27 +102,27 @@ def loss_and_grad(theta, dims, rec, rows, dt=np.float64, precond=None):
     W, nz, X, ph = rec.entries(rows)
     nn = M.layout(dims)["size"]
     th = np.asarray(theta)[:nn]
-    u = M.forward(th, dims, X, ph, dt)
+    u = M.forward(th, dims, X, ph, dt, act)
     r0 = rec.c[rows].astype(dt).copy()
     np.add.at(r0, nz[0], W[nz].astype(dt) * u)
     B = len(rows)
     if precond is None:
         loss = np.sum(dt(0.5) * r0 * r0) / dt(B)
         seed = r0[nz[0]] * W[nz].astype(dt) / dt(B)
-        return loss, M.vjp(th, dims, X, ph, seed, dt), r0
+        return loss, M.vjp(th, dims, X, ph, seed, dt, act), r0
     phi = np.asarray(theta)[nn:]
     C = rec.coeffs[rows]
     P, _ = PC.forward(phi, precond["widths"], C, precond["scale"], dt)
     r = P * r0
     loss = np.sum(dt(0.5) * r * r) / dt(B)
     seed = (P * P * r0)[nz[0]] * W[nz].astype(dt) / dt(B)
-    g_nn = M.vjp(th, dims, X, ph, seed, dt)
+    g_nn = M.vjp(th, dims, X, ph, seed, dt, act)
     g_pc = PC.vjp(phi, precond["widths"], C, precond["scale"], P * r0 * r0 / dt(B), dt)
     return loss, np.concatenate([g_nn, g_pc]), r
 
 
 def train(P, h_tr, bounds, cb, dims, theta0, num_epochs, batch_size, opt_cfg,
-          dt=np.float64, param_dt=np.float32, record_dt=np.float64, records=None, precond=None):
+          dt=np.float64, param_dt=np.float32, record_dt=np.float64, records=None, precond=None, act="tanh"):
     """Epoch loop of single_GPU_train with the zoom schedule.  Returns
     (theta, loss_epochs, per-step losses).  A short last batch is padded with
     padding_points (DatasetDict.gpu_padded_batches, data_management.py:103-173),
@@ -146,7 +146,7 @@ def train(P, h_tr, bounds, cb, dims, theta0, num_epochs, batch_size, opt_cfg,
         rec = recs[lvl]
         tot = 0.0
         for rows in bl:
-            loss, grad, _ = loss_and_grad(theta, dims, rec, rows, dt, precond)
+            loss, grad, _ = loss_and_grad(theta, dims, rec, rows, dt, precond, act)
             theta = O.step(theta, grad.astype(param_dt), state, opt_cfg, param_dt)
             tot += loss
             steps.append(loss)

@@ -1,0 +1,51 @@
+"""The multiresolution hash network model type with the identity encoder
+(nn/hash_network.py): flax-layout parameter tree, flat layout shared with the
+device kernels (the + network then the - network, kernel [in][out] then bias
+per layer), the C-ABI's shape table, and unsupported options (no GPU)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+LPBE = {"bound": 1.0, "pos_enc": "identity", "N_min": 16, "N_max": 2048, "F": 2, "T": 524288, "tv_scale": 0.0,
+        "pos_levels": 16, "layer_widths": [16], "sol_skip_in_layers": [], "sol_act": "linear", "highest_sh_order": 2}
+
+
+def test_model_tree_and_flat_layout():
+    from jax_dips import _native
+    from jax_dips.nn.configure import get_model
+    from oracle import mlp as M
+
+    forward, framework = get_model({"model_type": "multiresolution_hash_network",
+                                    "multiresolution_hash_network": LPBE}, "multiresolution_hash_network")
+    assert framework == "flax"
+    model = forward.model
+    assert model.dims == (1, 16, 1, 16) and model.act == 1
+    tree = forward.init(7)
+    assert set(tree["params"]) == {"sol_p", "sol_m"}
+    assert tree["params"]["sol_p"]["Dense_0"]["kernel"].shape == (3, 16)
+    assert tree["params"]["sol_m"]["Dense_1"]["kernel"].shape == (16, 1)
+    flat = model.flatten(tree)
+    lay = M.layout(model.dims)
+    assert flat.size == lay["size"]
+    # the + net's first kernel leads the flat vector, the - net's output bias ends it
+    assert np.array_equal(flat[:48], tree["params"]["sol_p"]["Dense_0"]["kernel"].reshape(-1))
+    assert flat[-1] == tree["params"]["sol_m"]["Dense_1"]["bias"][0]
+    assert np.array_equal(model.flatten(model.unflatten(flat)), flat)
+    lim = np.sqrt(6.0 / (3 + 16))
+    assert np.all(np.abs(tree["params"]["sol_p"]["Dense_0"]["kernel"]) <= lim)
+    net = _native.Net(*model.dims, 0, 0, 0.0, model.act)
+    lib = _native.lib()
+    assert lib.dips_tile_supported(ctypes.byref(net)) == 1
+    assert lib.dips_net_supported(ctypes.byref(net)) == 0 and lib.dips_mfma_supported(ctypes.byref(net)) == 0
+    assert lib.dips_param_count(ctypes.byref(net)) == flat.size
+    assert lib.dips_param_count(ctypes.byref(_native.Net(*model.dims, 0, 0, 0.0, 2))) < 0
+
+
+@pytest.mark.parametrize("change", [{"pos_enc": "hashgrid"}, {"pos_enc": "frequency"}, {"sol_act": "sigmoid"},
+                                    {"layer_widths": [32]}, {"sol_skip_in_layers": [1]}])
+def test_unsupported_options(change):
+    from jax_dips.nn.hash_network import HashNetwork
+
+    with pytest.raises(NotImplementedError):
+        HashNetwork(**dict(LPBE, **change))

@@ -32,15 +32,17 @@ def torch_net(theta, dims):
     return f, nets
 
 
+@pytest.mark.parametrize("act", ["tanh", "celu"])
 @pytest.mark.parametrize("dims", [(1, 1, 2, 10), (2, 16, 2, 16), (3, 8, 1, 5)])
-def test_forward_and_vjp(dims):
+def test_forward_and_vjp(dims, act):
     rng = np.random.default_rng(0)
     theta = M.init_params(dims, seed=4).astype(np.float64)
     X = rng.uniform(-1, 1, size=(300, 3))
     phase = rng.uniform(size=300) > 0.4
     seed = rng.normal(size=300)
-    u = M.forward(theta, dims, X, phase)
+    u = M.forward(theta, dims, X, phase, act=act)
     th = torch.tensor(theta, requires_grad=True)
+    tact = torch.tanh if act == "tanh" else torch.nn.functional.celu
     nets = M.unflatten(th, dims)
     Xt = torch.tensor(X)
     pt = torch.tensor(phase)
@@ -48,17 +50,21 @@ def test_forward_and_vjp(dims):
     for tag, sel in (("p", pt), ("m", ~pt)):
         h = Xt[sel]
         for W, b in nets[tag][:-1]:
-            h = torch.tanh(h @ W + b)
+            h = tact(h @ W + b)
         W, b = nets[tag][-1]
         out = out.index_put((torch.nonzero(sel)[:, 0],), (h @ W + b)[:, 0])
     assert np.allclose(u, out.detach().numpy(), rtol=1e-13, atol=1e-13)
     (out * torch.tensor(seed)).sum().backward()
-    g = M.vjp(theta, dims, X, phase, seed)
+    g = M.vjp(theta, dims, X, phase, seed, act=act)
     assert np.allclose(g, th.grad.numpy(), rtol=1e-11, atol=1e-12)
+    # CELU's second derivative jumps at 0: a smaller step keeps the central
+    # difference's O(h) error near the kink below the bar
+    h = 1e-3 if act == "tanh" else 1e-5
     for i in rng.choice(theta.size, 32, replace=False):
         e = np.zeros_like(theta)
-        e[i] = 1e-3
-        fd = (seed @ M.forward(theta + e, dims, X, phase) - seed @ M.forward(theta - e, dims, X, phase)) / 2e-3
+        e[i] = h
+        fd = (seed @ M.forward(theta + e, dims, X, phase, act=act)
+              - seed @ M.forward(theta - e, dims, X, phase, act=act)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g[i]))
 
 
@@ -102,3 +108,11 @@ def test_zoom_levels_and_batches():
     assert np.array_equal(h, np.array([0.25, 0.5, 0.125], np.float32) / 8)
     b = T.batches(10, 4)
     assert [list(x) for x in b] == [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, -1, -1]]
+
+
+def test_celu_closed_form():
+    """CELU (alpha 1): identity above 0, expm1 below, continuous with slope 1 at 0."""
+    x = np.array([-30.0, -2.0, -1e-8, 0.0, 1e-8, 3.0])
+    a = M.activation(x, "celu")
+    assert np.array_equal(a[3:], x[3:]) and np.allclose(a[:3], np.expm1(x[:3]), rtol=1e-15)
+    assert np.allclose(M.activation_grad(a, "celu"), np.where(x > 0, 1.0, np.exp(x)), rtol=1e-12)
