@@ -304,6 +304,49 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
                         float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Multiresolution hash network with the hashgrid encoder
+ * (nn/hash_encoding/model.py:53-161, blocks/encoders_flax.py:67-254): the
+ * encoding of a position is, per level l < levels, the trilinear interpolation
+ * of features-wide table rows at the 8 corners of its cell (dense index on
+ * levels with res^3 <= table_size, spatial hash otherwise, mod table_size, plus
+ * the level's first row); u = MLP_phase(encoding) with one CELU hidden layer of
+ * width 16 and a linear output (blocks/nerfs_flax.py:142-170).
+ * Flat parameters: the + MLP, the - MLP (W0[E][16], b0, Wout[16], bout with
+ * E = levels * features), then the table (rows x features).  Entries are the
+ * phase-sorted arrays of dips_mfma_batch_t.  The table gradient accumulates with
+ * atomic adds (summation order not fixed: results vary in the last bits).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int32_t levels;      /* L, 2..16 */
+  int32_t features;    /* F, must be 2 */
+  int64_t table_size;  /* T, rows per hashed level */
+  int32_t n_min;       /* coarsest resolution */
+  int32_t n_max;       /* finest resolution (the reference's int(N_max * bound)) */
+  float bound;         /* positions are mapped from [-bound, bound] to [0, 1] */
+} dips_hashgrid_t;
+
+/* Table rows of the encoder (sum over levels of min(res^3, table_size)); < 0 if invalid. */
+int64_t dips_hash_table_rows(const dips_hashgrid_t* hg);
+/* Parameters: 2 MLPs + table rows * features. */
+int64_t dips_hash_param_count(const dips_hashgrid_t* hg);
+/* float workspace for batches of up to max_rows rows and max_nnz entries, and
+ * the float64 workspace size (*doubles). */
+int64_t dips_hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles);
+/* One optimizer step (clip -> Adam -> schedule as dips_opt_step) over all
+ * parameters, table included. */
+int dips_hash_train_step(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                         const dips_hashgrid_t* hg, const dips_opt_t* opt, float inv_batch, float* params, float* m,
+                         float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* work,
+                         int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream);
+/* Loss (device double) and mean gradient (device grad[P]) of the batch. */
+int dips_hash_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                        const dips_hashgrid_t* hg, float inv_batch, const float* params, float* grad, double* loss,
+                        float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream);
+/* u (m) and, when grad != NULL, d u / d x (m,3) at points pts (m,3); phase 1 = + net. */
+int dips_hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, const uint8_t* phase, int64_t m,
+                   float* u, float* grad, void* stream);
+
+/* ---------------------------------------------------------------------------
  * K5: the grid-value operator and its matrix-free BiCGSTAB solve (judge and
  * fallback).  rec holds the zoom-0 records of the eval grid (spacing = grid
  * spacing); r(p) = c_p + sum_e w_e u(p + o_e) with u a grid vector (N floats,

@@ -408,4 +408,56 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
                    nullptr, 0, work, work_floats, dwork, dwork_doubles, grad, loss, (hipStream_t)stream);
 }
 
+int64_t dips_hash_table_rows(const dips_hashgrid_t* hg) { return hash_table_rows(hg); }
+
+int64_t dips_hash_param_count(const dips_hashgrid_t* hg) { return hash_param_count(hg); }
+
+int64_t dips_hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles) {
+  return hash_workspace(hg, max_rows, max_nnz, doubles);
+}
+
+static int check_hash_batch(const dips_records_t* rec, const dips_mfma_batch_t* batch) {
+  return batch && batch->nrows >= 1 && batch->row_begin >= 0 && batch->row_begin + batch->nrows <= rec->nrows &&
+         batch->ent_begin >= 0 && batch->ent_end <= rec->nnz && batch->ent_begin <= batch->ent_end &&
+         batch->t_begin >= 0 && batch->t_begin + batch->t_count <= rec->nnz && batch->n_plus >= 0 &&
+         batch->n_minus >= 0 &&
+         (batch->t_count == 0 || (batch->t_row && batch->t_code && batch->t_w && batch->t_csr));
+}
+
+int dips_hash_train_step(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                         const dips_hashgrid_t* hg, const dips_opt_t* opt, float inv_batch, float* params, float* m,
+                         float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* work,
+                         int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  int st = check_records(grid, rec);
+  if (st) return st;
+  if (!check_hash_batch(rec, batch) || !opt || !params || !m || !v || !count || !loss_hist || !work || !dwork) {
+    set_error("dips_hash_train_step: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return hash_step(grid, rec, batch, hg, opt, inv_batch, params, m, v, count, loss_hist, loss_cap, work, work_floats,
+                   dwork, dwork_doubles, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int dips_hash_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                        const dips_hashgrid_t* hg, float inv_batch, const float* params, float* grad, double* loss,
+                        float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  int st = check_records(grid, rec);
+  if (st) return st;
+  if (!check_hash_batch(rec, batch) || !params || !grad || !loss || !work || !dwork) {
+    set_error("dips_hash_loss_grad: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return hash_step(grid, rec, batch, hg, nullptr, inv_batch, const_cast<float*>(params), nullptr, nullptr, nullptr,
+                   nullptr, 0, work, work_floats, dwork, dwork_doubles, grad, loss, (hipStream_t)stream);
+}
+
+int dips_hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, const uint8_t* phase, int64_t m,
+                   float* u, float* grad, void* stream) {
+  if (m < 0 || (m > 0 && (!params || !pts || !phase || !u))) {
+    set_error("dips_hash_eval: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return hash_eval(hg, params, pts, phase, m, u, grad, (hipStream_t)stream);
+}
+
 }  // extern "C"

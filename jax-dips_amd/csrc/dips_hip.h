@@ -39,6 +39,13 @@ int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_
                     float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, float* partial,
                     int64_t partial_cap, uint32_t* sync, hipStream_t stream);
 int64_t tile_partial_size(const dips_net_t* net);
+void rowsum_launch(const int32_t* row_ptr, const float* row_c, int64_t row_begin, int64_t nrows, const float* ent_w,
+                   int64_t nnz, const float* u, float* r, double* loss_part, hipStream_t st);
+int rowsum_blocks(int64_t nrows);
+void apply_launch(const float* g, int64_t P, const double* sq_part, int nsq, const double* loss_part, int nloss,
+                  float inv_b, const dips_opt_t& opt, float* params, float* m, float* v, int32_t* count,
+                  float* loss_hist, int loss_cap, hipStream_t st);
+void loss_out_launch(const double* loss_part, int nloss, float inv_b, double* loss_out, hipStream_t st);
 int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net);
 int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
                 const dips_opt_t* opt, void* work, int64_t work_bytes, hipStream_t stream);
@@ -48,6 +55,17 @@ int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int
                       hipStream_t st);
 int64_t pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* batch_row_begin, const int64_t* batch_rows,
                    int32_t nbatches, int32_t* tile_row, int64_t tile_cap, int64_t* batch_tile_begin);
+
+// hashgrid.hip
+int64_t hash_table_rows(const dips_hashgrid_t* hg);
+int64_t hash_param_count(const dips_hashgrid_t* hg);
+int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles);
+int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, const dips_hashgrid_t* hg,
+              const dips_opt_t* opt, float inv_b, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+              int loss_cap, float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, float* grad_out,
+              double* loss_out, hipStream_t st);
+int hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, const uint8_t* phase, int64_t m,
+              float* u, float* grad, hipStream_t st);
 
 // optim.hip
 int opt_step_launch(const dips_net_t* net, const dips_opt_t* opt, const float* partial, int nblk, float inv_batch,

@@ -1,0 +1,621 @@
+// Multiresolution hash network with the hashgrid encoder: training step and
+// evaluation over record entries.
+//
+// Encoder (nn/hash_encoding/blocks/encoders_flax.py:67-254): pos = (x + bound)
+// / (2 bound); per level l the scale s_l = N_min b^l - 1 (host float64, stored
+// float32), pos_scaled = pos * s_l + 0.5 (two float32 roundings), corners
+// floor(pos_scaled) + {0,1}^3 in the order (0,0,0), (0,0,1), ..., (1,1,1),
+// table row = (dense ? x + y res + z res^2 : x ^ y*2654435761 ^ z*805459861)
+// mod T + the level's first row (uint32 arithmetic), trilinear weights; the
+// encoding is level-major (l * 2 + f).  Network (model.py:53-94,
+// blocks/nerfs_flax.py:142-170): per phase W0[E][16], b0, CELU, Wout[16], bout.
+//
+// Step: forward (one lane per phase-sorted entry: encoding + MLP) -> row sums
+// (the MFMA path's kernel) -> backward (one wavefront per 64 entries: the MLP
+// weight gradient W0 = enc^T delta is a K = 64 contraction on the matrix cores
+// through LDS staging, biases and output weights per lane; the table gradient
+// d enc = W0 delta is scattered to the 8 corners of every level with atomic
+// adds) -> reduction (fixed order for the MLPs; table gradient scaled;
+// float64 norm partials) -> norm -> Adam over every parameter.
+#include <math.h>
+
+#include <algorithm>
+
+#include "dips_common.h"
+#include "dips_hip.h"
+#include "dips_mlp.h"
+
+namespace dips {
+
+constexpr int kHL = 16;               // levels (maximum)
+constexpr int kHF = 2;                // features per table row
+constexpr int kHE = kHL * kHF;        // encoding width (maximum)
+constexpr int kHW = 16;               // hidden width
+constexpr int kHashWaves = 8;
+constexpr int kHashThreads = kHashWaves * DIPS_WAVE;
+constexpr int kHashBlocks = 256;      // backward workgroups (persistent)
+constexpr int kHStride = 68;          // staged operand row: 64 entries + 4
+constexpr int kHStage = 2 * 16 * kHStride;
+
+struct HashLevels {
+  int L;
+  uint32_t T;
+  int64_t rows;
+  float bound, two_bound;
+  float scale[kHL];
+  uint32_t res[kHL];
+  int dense[kHL];
+  int64_t offset[kHL];
+};
+
+__host__ __device__ inline int hash_mlp_size(int E) { return E * kHW + kHW + kHW + 1; }
+
+static int hash_levels(const dips_hashgrid_t* hg, HashLevels& H) {
+  if (!hg || hg->levels < 2 || hg->levels > kHL || hg->features != kHF || hg->table_size < 1 ||
+      hg->table_size > (int64_t)1 << 31 || hg->n_min < 1 || hg->n_max < hg->n_min || !(hg->bound > 0.0f)) {
+    set_error("hashgrid: invalid encoder description (levels 2..16, features 2, table_size 1..2^31, n_min <= n_max)");
+    return DIPS_EINVAL;
+  }
+  H.L = hg->levels;
+  H.T = (uint32_t)hg->table_size;
+  H.bound = hg->bound;
+  H.two_bound = 2.0f * hg->bound;
+  const double b = exp((log((double)hg->n_max) - log((double)hg->n_min)) / (double)(hg->levels - 1));
+  int64_t off = 0;
+  for (int l = 0; l < kHL; ++l) {
+    H.scale[l] = 0.0f;
+    H.res[l] = 0;
+    H.dense[l] = 0;
+    H.offset[l] = 0;
+  }
+  for (int l = 0; l < H.L; ++l) {
+    const double scale = (double)hg->n_min * pow(b, (double)l) - 1.0;
+    const int64_t res = (int64_t)ceil(scale) + 1;
+    const double n3 = (double)res * (double)res * (double)res;
+    const bool dense = n3 <= (double)hg->table_size;
+    H.scale[l] = (float)scale;
+    H.res[l] = (uint32_t)res;
+    H.dense[l] = dense ? 1 : 0;
+    H.offset[l] = off;
+    off += dense ? res * res * res : hg->table_size;
+  }
+  H.rows = off;
+  return DIPS_OK;
+}
+
+// position in the unit cube, pos = (x + bound) / (2 bound), correctly rounded
+__device__ __forceinline__ void hash_pos(const HashLevels& H, float x, float y, float z, float (&p)[3]) {
+  p[0] = (x + H.bound) / H.two_bound;
+  p[1] = (y + H.bound) / H.two_bound;
+  p[2] = (z + H.bound) / H.two_bound;
+}
+
+// cell of position p on level l: the corner coordinates (lower and upper per
+// axis, float -> uint32 saturating at 0) and the fractions modf(pos_scaled)
+// with the reference's weight clip folded in: a negative pos_scaled gives the
+// lower corner weight 1 and the upper weight 0, i.e. fraction 0; live[d] is
+// false there (the weights do not depend on that coordinate).
+__device__ __forceinline__ void hash_cell(const HashLevels& H, int l, const float (&p)[3], uint32_t (&lo)[3],
+                                          uint32_t (&hi)[3], float (&frac)[3], bool (&live)[3]) {
+  // pos * scale + 0.5 rounds twice, as written in the reference: the operators
+  // stand in this scope so that contraction into a fused multiply-add is off
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float ps = p[d] * H.scale[l] + 0.5f;
+    const float fl = floorf(ps);
+    const float fh = fl + 1.0f;
+    live[d] = ps >= 0.0f;
+    frac[d] = live[d] ? ps - fl : 0.0f;
+    lo[d] = fl > 0.0f ? (uint32_t)fl : 0u;
+    hi[d] = fh > 0.0f ? (uint32_t)fh : 0u;
+  }
+}
+
+__device__ __forceinline__ int64_t hash_row(const HashLevels& H, int l, const uint32_t (&lo)[3],
+                                            const uint32_t (&hi)[3], int c) {
+  const uint32_t vx = ((c >> 2) & 1) ? hi[0] : lo[0];
+  const uint32_t vy = ((c >> 1) & 1) ? hi[1] : lo[1];
+  const uint32_t vz = (c & 1) ? hi[2] : lo[2];
+  uint32_t idx;
+  if (H.dense[l]) {
+    const uint32_t r = H.res[l];
+    idx = vx + vy * r + vz * (r * r);
+  } else {
+    idx = vx ^ (vy * 2654435761u) ^ (vz * 805459861u);
+  }
+  const int64_t row = H.offset[l] + (int64_t)(idx % H.T);
+  return row < H.rows ? row : H.rows - 1;
+}
+
+__device__ __forceinline__ float hash_weight(const float (&frac)[3], int c) {
+  const float wx = ((c >> 2) & 1) ? frac[0] : 1.0f - frac[0];
+  const float wy = ((c >> 1) & 1) ? frac[1] : 1.0f - frac[1];
+  const float wz = (c & 1) ? frac[2] : 1.0f - frac[2];
+  return wx * wy * wz;
+}
+
+__device__ __forceinline__ void hash_encode(const HashLevels& H, const float* __restrict__ table, const float (&p)[3],
+                                            float (&enc)[kHE]) {
+#pragma unroll
+  for (int l = 0; l < kHL; ++l) {
+    enc[2 * l] = 0.0f;
+    enc[2 * l + 1] = 0.0f;
+    if (l < H.L) {
+      uint32_t lo[3], hi[3];
+      float frac[3];
+      bool live[3];
+      hash_cell(H, l, p, lo, hi, frac, live);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int64_t row = hash_row(H, l, lo, hi, c);
+        const float w = hash_weight(frac, c);
+        const float2 t = reinterpret_cast<const float2*>(table)[row];
+        enc[2 * l] = fmaf(w, t.x, enc[2 * l]);
+        enc[2 * l + 1] = fmaf(w, t.y, enc[2 * l + 1]);
+      }
+    }
+  }
+}
+
+// MLP of one phase: act (CELU hidden), returns u; W = flat MLP (W0[E][16], b0, Wout, bout)
+template <typename WP>
+__device__ __forceinline__ float hash_mlp(WP W, int E, const float (&enc)[kHE], float (&act)[kHW]) {
+#pragma unroll
+  for (int j = 0; j < kHW; ++j) {
+    float s = W[E * kHW + j];
+#pragma unroll
+    for (int k = 0; k < kHE; ++k)
+      if (k < E) s = fmaf(enc[k], W[k * kHW + j], s);
+    act[j] = act_f<kActCelu>(s);
+  }
+  float u = W[E * kHW + 2 * kHW];
+#pragma unroll
+  for (int j = 0; j < kHW; ++j) u = fmaf(act[j], W[E * kHW + kHW + j], u);
+  return u;
+}
+
+// hidden deltas dl_j = seed Wout_j celu'(a_j) and the encoding gradient W0 dl
+template <typename WP>
+__device__ __forceinline__ void hash_delta(WP W, int E, float seed, const float (&act)[kHW], float (&dl)[kHW],
+                                           float (&denc)[kHE]) {
+#pragma unroll
+  for (int j = 0; j < kHW; ++j) dl[j] = seed * W[E * kHW + kHW + j] * act_grad<kActCelu>(act[j]);
+#pragma unroll
+  for (int k = 0; k < kHE; ++k) {
+    float s = 0.0f;
+    if (k < E) {
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) s = fmaf(W[k * kHW + j], dl[j], s);
+    }
+    denc[k] = s;
+  }
+}
+
+struct HashArgs {
+  PointSet ps;
+  float hx, hy, hz;
+  const int32_t* row_point;
+  const int32_t* t_row;
+  const uint32_t* t_code;
+  const float* t_w;
+  const int32_t* t_csr;
+  int64_t t_begin, t_count, n_plus;
+};
+
+__device__ __forceinline__ bool hash_entry(const HashArgs& A, int64_t t, float& x, float& y, float& z, int& row) {
+  x = y = z = 0.0f;
+  row = -1;
+  if (t < 0 || t >= A.t_count) return false;
+  const int64_t e = A.t_begin + t;
+  row = A.t_row[e];
+  float px, py, pz;
+  if (!point_xyz(A.ps, A.row_point[row], px, py, pz)) return false;
+  int ox, oy, oz;
+  dips_decode_offset(A.t_code[e], ox, oy, oz);
+  x = px + (float)ox * A.hx;
+  y = py + (float)oy * A.hy;
+  z = pz + (float)oz * A.hz;
+  return true;
+}
+
+// ---- forward: u per entry ------------------------------------------------------
+__global__ __launch_bounds__(256) void hash_fwd_kernel(HashArgs A, HashLevels H, const float* __restrict__ params,
+                                                       float* __restrict__ u_out) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  float x, y, z;
+  int row;
+  const bool valid = hash_entry(A, t, x, y, z, row);
+  const int E = 2 * H.L, PM = hash_mlp_size(E);
+  float enc[kHE];
+  float p[3];
+  hash_pos(H, x, y, z, p);
+  hash_encode(H, params + 2 * (int64_t)PM, p, enc);
+  const bool plus = t < A.n_plus;
+  float act[kHW];
+  const float* Praw = params;
+  asm volatile("" : "+s"(Praw));
+  dips_weight_t* Wp = dips_weights(Praw);
+  dips_weight_t* Wm = Wp + PM;
+  float up = 0.0f, um = 0.0f;
+  if (plus) up = hash_mlp(Wp, E, enc, act);
+  asm volatile("" : "+s"(Wm));
+  if (!plus) um = hash_mlp(Wm, E, enc, act);
+  if (valid) u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
+}
+
+// ---- backward -----------------------------------------------------------------
+typedef float hf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void hash_stage_mfma(const float* AT, const float* BT, int lane, hf4& acc) {
+  const int m = lane & 15, kq = lane >> 4;
+  const hf4* ap = reinterpret_cast<const hf4*>(AT + m * kHStride + 16 * kq);
+  const hf4* bp = reinterpret_cast<const hf4*>(BT + m * kHStride + 16 * kq);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const hf4 av = ap[q], bv = bp[q];
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc, 0, 0, 0);
+  }
+}
+
+// gradient accumulators of one phase MLP within a wavefront
+struct HashGrad {
+  hf4 acc[2];       // W0 rows 0..15 and 16..31 (x the 16 hidden units)
+  float gb0[kHW];   // per-lane partial sums of delta
+  float gwo[kHW];   // per-lane partial sums of seed * act
+  float gbo;
+
+  __device__ __forceinline__ void zero() {
+    acc[0] = acc[1] = hf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < kHW; ++j) gb0[j] = gwo[j] = 0.0f;
+    gbo = 0.0f;
+  }
+
+  // lanes that are not of this phase carry seed 0 and zero deltas; wave-uniform call
+  __device__ __forceinline__ void add(float* AT, float* BT, int lane, int E, float seed, const float (&enc)[kHE],
+                                      const float (&act)[kHW], const float (&dl)[kHW]) {
+#pragma unroll
+    for (int j = 0; j < kHW; ++j) BT[j * kHStride + lane] = dl[j];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) AT[i * kHStride + lane] = enc[i];
+    __builtin_amdgcn_wave_barrier();
+    hash_stage_mfma(AT, BT, lane, acc[0]);
+    __builtin_amdgcn_wave_barrier();
+    if (E > 16) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) AT[i * kHStride + lane] = enc[16 + i];
+      __builtin_amdgcn_wave_barrier();
+      hash_stage_mfma(AT, BT, lane, acc[1]);
+      __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int j = 0; j < kHW; ++j) {
+      gb0[j] += dl[j];
+      gwo[j] = fmaf(seed, act[j], gwo[j]);
+    }
+    gbo += seed;
+  }
+
+  // this wavefront's sums into red[0 .. PM)
+  __device__ __forceinline__ void flush(float* red, int lane, int E) {
+#pragma unroll
+    for (int blk = 0; blk < 2; ++blk)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * blk + 4 * (lane >> 4) + r, j = lane & 15;
+        if (i < E) red[i * kHW + j] = acc[blk][r];
+      }
+#pragma unroll
+    for (int j = 0; j < kHW; ++j) {
+      const float a = dips_wave_sum(gb0[j]);
+      const float b = dips_wave_sum(gwo[j]);
+      if (lane == 0) {
+        red[E * kHW + j] = a;
+        red[E * kHW + kHW + j] = b;
+      }
+    }
+    const float c = dips_wave_sum(gbo);
+    if (lane == 0) red[E * kHW + 2 * kHW] = c;
+  }
+};
+
+__global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, HashLevels H,
+                                                                const float* __restrict__ params, int64_t row_begin,
+                                                                const float* __restrict__ r_row,
+                                                                float* __restrict__ gtab,
+                                                                float* __restrict__ partial) {
+  extern __shared__ float hash_lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
+  float* const AT = hash_lds + wave * kHStage;
+  float* const BT = AT + 16 * kHStride;
+  const int E = 2 * H.L, PM = hash_mlp_size(E);
+  const float* table = params + 2 * (int64_t)PM;
+  HashGrad gp, gm;
+  gp.zero();
+  gm.zero();
+  const int64_t nchunks = (A.t_count + 63) / 64;
+  const int64_t nw = (int64_t)gridDim.x * kHashWaves;
+  for (int64_t ch = (int64_t)blockIdx.x * kHashWaves + wave; ch < nchunks; ch += nw) {
+    const int64_t t = ch * 64 + lane;
+    float x, y, z;
+    int row;
+    const bool valid = hash_entry(A, t, x, y, z, row);
+    const float seed = valid ? r_row[row - row_begin] * A.t_w[A.t_begin + t] : 0.0f;
+    const bool plus = t < A.n_plus;
+    float enc[kHE];
+    float p[3];
+    hash_pos(H, x, y, z, p);
+    hash_encode(H, table, p, enc);
+    const float* Praw = params;
+    asm volatile("" : "+s"(Praw));
+    dips_weight_t* Wp = dips_weights(Praw);
+    dips_weight_t* Wm = Wp + PM;
+    float act[kHW] = {};
+    float dl[kHW];
+    float denc[kHE];
+    if (plus) {
+      (void)hash_mlp(Wp, E, enc, act);
+      hash_delta(Wp, E, seed, act, dl, denc);
+    }
+    asm volatile("" : "+s"(Wm));
+    if (!plus) {
+      (void)hash_mlp(Wm, E, enc, act);
+      hash_delta(Wm, E, seed, act, dl, denc);
+    }
+    // MLP weight gradients, one network at a time (the other's lanes contribute zeros)
+    if (__any(plus && valid)) {
+      float d2[kHW];
+      const float s2 = plus ? seed : 0.0f;
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) d2[j] = plus ? dl[j] : 0.0f;
+      gp.add(AT, BT, lane, E, s2, enc, act, d2);
+    }
+    if (__any(!plus && valid)) {
+      float d2[kHW];
+      const float s2 = plus ? 0.0f : seed;
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) d2[j] = plus ? 0.0f : dl[j];
+      gm.add(AT, BT, lane, E, s2, enc, act, d2);
+    }
+    // table gradient: scatter d enc to the corners
+    if (valid && seed != 0.0f) {
+#pragma unroll
+      for (int l = 0; l < kHL; ++l) {
+        if (l < H.L) {
+          uint32_t lo[3], hi[3];
+          float frac[3];
+          bool live[3];
+          hash_cell(H, l, p, lo, hi, frac, live);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const int64_t trow = hash_row(H, l, lo, hi, c);
+            const float w = hash_weight(frac, c);
+            unsafeAtomicAdd(gtab + 2 * trow, w * denc[2 * l]);
+            unsafeAtomicAdd(gtab + 2 * trow + 1, w * denc[2 * l + 1]);
+          }
+        }
+      }
+    }
+  }
+  // workgroup reduction: per-wavefront rows in LDS, then the wavefronts in order
+  const int stride = 2 * PM;
+  __syncthreads();
+  float* red = hash_lds + wave * stride;
+  gp.flush(red, lane, E);
+  gm.flush(red + PM, lane, E);
+  __syncthreads();
+  for (int p = threadIdx.x; p < stride; p += kHashThreads) {
+    float s = 0.0f;
+    for (int w = 0; w < kHashWaves; ++w) s += hash_lds[w * stride + p];
+    partial[(int64_t)blockIdx.x * stride + p] = s;
+  }
+}
+
+// g = inv_b * (fixed-order sum of the MLP partial rows | the table gradient); float64 norm partials
+__global__ __launch_bounds__(256) void hash_reduce_kernel(const float* __restrict__ partial, int nblk, int stride,
+                                                          const float* __restrict__ gtab, int64_t P, float inv_b,
+                                                          float* __restrict__ g, double* __restrict__ sq_part) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (p < P) {
+    float gg;
+    if (p < stride) {
+      double acc = 0.0;
+      for (int b = 0; b < nblk; ++b) acc += (double)partial[(int64_t)b * stride + p];
+      gg = (float)(acc * (double)inv_b);
+    } else {
+      gg = gtab[p - stride] * inv_b;
+    }
+    g[p] = gg;
+    v = (double)gg * (double)gg;
+  }
+  v = dips_wave_sum_d(v);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) sq_part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ __launch_bounds__(256) void hash_norm_kernel(const double* __restrict__ sq_part, int64_t n,
+                                                        double* __restrict__ out) {
+  __shared__ double red[4];
+  double t = 0.0;
+  for (int64_t b = threadIdx.x; b < n; b += 256) t += sq_part[b];
+  t = dips_wave_sum_d(t);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ __launch_bounds__(256) void hash_zero_kernel(float* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+// ---- evaluation: u and d u / d x ----------------------------------------------
+__global__ __launch_bounds__(256) void hash_eval_kernel(HashLevels H, const float* __restrict__ params,
+                                                        const float* __restrict__ pts,
+                                                        const uint8_t* __restrict__ phase, int64_t m,
+                                                        float* __restrict__ u, float* __restrict__ grad) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= m) return;
+  const float x = pts[3 * t], y = pts[3 * t + 1], z = pts[3 * t + 2];
+  const int E = 2 * H.L, PM = hash_mlp_size(E);
+  const float* table = params + 2 * (int64_t)PM;
+  float enc[kHE], act[kHW];
+  float p[3];
+  hash_pos(H, x, y, z, p);
+  hash_encode(H, table, p, enc);
+  const float* W = params + (phase[t] ? 0 : PM);
+  const float v = hash_mlp(W, E, enc, act);
+  u[t] = v;
+  if (!grad) return;
+  // d u / d enc_k = sum_j W0[k][j] Wout_j celu'(a_j)
+  float dj[kHW];
+#pragma unroll
+  for (int j = 0; j < kHW; ++j) dj[j] = W[E * kHW + kHW + j] * act_grad<kActCelu>(act[j]);
+  float g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int l = 0; l < kHL; ++l) {
+    if (l < H.L) {
+      float du0 = 0.0f, du1 = 0.0f;
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) {
+        du0 = fmaf(W[(2 * l) * kHW + j], dj[j], du0);
+        du1 = fmaf(W[(2 * l + 1) * kHW + j], dj[j], du1);
+      }
+      uint32_t lo[3], hi[3];
+      float frac[3];
+      bool live[3];
+      hash_cell(H, l, p, lo, hi, frac, live);
+      const float dps = H.scale[l] / H.two_bound;  // d pos_scaled / d x
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int64_t row = hash_row(H, l, lo, hi, c);
+        const float2 tv = reinterpret_cast<const float2*>(table)[row];
+        const float val = du0 * tv.x + du1 * tv.y;
+        const float wx = ((c >> 2) & 1) ? frac[0] : 1.0f - frac[0];
+        const float wy = ((c >> 1) & 1) ? frac[1] : 1.0f - frac[1];
+        const float wz = (c & 1) ? frac[2] : 1.0f - frac[2];
+        // d weight / d pos_scaled per axis: +-1 where the fraction follows the position
+        const float sx = live[0] ? (((c >> 2) & 1) ? 1.0f : -1.0f) : 0.0f;
+        const float sy = live[1] ? (((c >> 1) & 1) ? 1.0f : -1.0f) : 0.0f;
+        const float sz = live[2] ? ((c & 1) ? 1.0f : -1.0f) : 0.0f;
+        g[0] = fmaf(val * sx * wy * wz, dps, g[0]);
+        g[1] = fmaf(val * wx * sy * wz, dps, g[1]);
+        g[2] = fmaf(val * wx * wy * sz, dps, g[2]);
+      }
+    }
+  }
+  grad[3 * t] = g[0];
+  grad[3 * t + 1] = g[1];
+  grad[3 * t + 2] = g[2];
+}
+
+// ---- host side ------------------------------------------------------------------
+int64_t hash_table_rows(const dips_hashgrid_t* hg) {
+  HashLevels H;
+  if (hash_levels(hg, H)) return -1;
+  return H.rows;
+}
+
+int64_t hash_param_count(const dips_hashgrid_t* hg) {
+  HashLevels H;
+  if (hash_levels(hg, H)) return -1;
+  return 2 * (int64_t)hash_mlp_size(2 * H.L) + H.rows * kHF;
+}
+
+// float workspace: u (max_nnz) | r (max_rows) | partial rows | table gradient | g
+int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles) {
+  HashLevels H;
+  if (hash_levels(hg, H) || max_rows < 1 || max_nnz < 0) return -1;
+  const int64_t PM2 = 2 * (int64_t)hash_mlp_size(2 * H.L);
+  const int64_t P = PM2 + H.rows * kHF;
+  if (doubles) *doubles = rowsum_blocks(max_rows) + (P + 255) / 256 + 2;
+  return max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
+}
+
+int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, const dips_hashgrid_t* hg,
+              const dips_opt_t* opt, float inv_b, float* params, float* m, float* v, int32_t* count, float* loss_hist,
+              int loss_cap, float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, float* grad_out,
+              double* loss_out, hipStream_t st) {
+  HashLevels H;
+  int s = hash_levels(hg, H);
+  if (s) return s;
+  if (!bt || bt->t_count < 0 || bt->n_plus + bt->n_minus != bt->t_count) {
+    set_error("hash step: invalid batch");
+    return DIPS_EINVAL;
+  }
+  const int PM = hash_mlp_size(2 * H.L);
+  const int64_t stride = 2 * (int64_t)PM;
+  const int64_t P = stride + H.rows * kHF;
+  const int64_t nrows = bt->nrows;
+  const int64_t nent = bt->ent_end - bt->ent_begin;
+  int64_t need_d = 0;
+  const int64_t need = hash_workspace(hg, std::max<int64_t>(nrows, 1), nent, &need_d);
+  if (need < 0 || work_floats < need || dwork_doubles < need_d || !work || !dwork) {
+    set_error("hash step: workspace too small");
+    return DIPS_EINVAL;
+  }
+  if (P > INT_MAX) {
+    set_error("hash step: too many parameters");
+    return DIPS_EINVAL;
+  }
+  float* u = work;
+  float* r = u + nent;
+  float* partial = r + nrows;
+  float* gtab = partial + (int64_t)kHashBlocks * stride;
+  float* gvec = gtab + H.rows * kHF;
+  const int nrb = rowsum_blocks(nrows);
+  double* loss_part = dwork;
+  double* sq_part = dwork + nrb;
+  const int64_t nsq = (P + 255) / 256;
+  double* sq_total = sq_part + nsq;
+  HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
+             bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
+  if (bt->t_count > 0) {
+    const int64_t nb = (bt->t_count + 255) / 256;
+    hipLaunchKernelGGL(hash_fwd_kernel, dim3((unsigned)nb), dim3(256), 0, st, A, H, params, u - bt->ent_begin);
+  }
+  rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
+                st);
+  hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
+                     H.rows * kHF);
+  const size_t lds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage, (int64_t)kHashWaves * stride);
+  static const bool lds_set = hipFuncSetAttribute((const void*)hash_bwd_kernel,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+  (void)lds_set;
+  if (lds > 160 * 1024) {
+    set_error("hash step: encoder too wide for the workgroup's LDS");
+    return DIPS_EUNSUPPORTED;
+  }
+  hipLaunchKernelGGL(hash_bwd_kernel, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, bt->row_begin, r,
+                     gtab, partial);
+  hipLaunchKernelGGL(hash_reduce_kernel, dim3((unsigned)nsq), dim3(256), 0, st, partial, kHashBlocks, (int)stride, gtab,
+                     P, inv_b, grad_out ? grad_out : gvec, sq_part);
+  if (!opt) {
+    loss_out_launch(loss_part, nrb, inv_b, loss_out, st);
+    return hip_status(hipGetLastError(), "hash step launch");
+  }
+  hipLaunchKernelGGL(hash_norm_kernel, dim3(1), dim3(256), 0, st, sq_part, nsq, sq_total);
+  apply_launch(gvec, P, sq_total, 1, loss_part, nrb, inv_b, *opt, params, m, v, count, loss_hist, loss_cap, st);
+  return hip_status(hipGetLastError(), "hash step launch");
+}
+
+int hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, const uint8_t* phase, int64_t m,
+              float* u, float* grad, hipStream_t st) {
+  HashLevels H;
+  int s = hash_levels(hg, H);
+  if (s) return s;
+  if (m <= 0) return DIPS_OK;
+  hipLaunchKernelGGL(hash_eval_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, H, params, pts, phase, m,
+                     u, grad);
+  return hip_status(hipGetLastError(), "hash_eval_kernel launch");
+}
+
+}  // namespace dips

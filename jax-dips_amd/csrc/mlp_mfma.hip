@@ -622,6 +622,30 @@ __global__ __launch_bounds__(256) void mfma_loss_out_kernel(const double* __rest
 }
 
 // ---- host side ----------------------------------------------------------------
+// launchers of the row-sum, optimizer and counter kernels for other entry-parallel
+// networks (the hashgrid network)
+void rowsum_launch(const int32_t* row_ptr, const float* row_c, int64_t row_begin, int64_t nrows, const float* ent_w,
+                   int64_t nnz, const float* u, float* r, double* loss_part, hipStream_t st) {
+  const int nrb = (int)((nrows + kMT - 1) / kMT);
+  hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, row_ptr, row_c, row_begin, nrows, ent_w, nnz, u,
+                     r, loss_part);
+}
+
+int rowsum_blocks(int64_t nrows) { return (int)((nrows + kMT - 1) / kMT); }
+
+void apply_launch(const float* g, int64_t P, const double* sq_part, int nsq, const double* loss_part, int nloss,
+                  float inv_b, const dips_opt_t& opt, float* params, float* m, float* v, int32_t* count,
+                  float* loss_hist, int loss_cap, hipStream_t st) {
+  const int64_t nb = (P + 255) / 256;
+  hipLaunchKernelGGL(mfma_apply_kernel, dim3((unsigned)nb), dim3(256), 0, st, g, (int)P, sq_part, nsq, loss_part, nloss,
+                     inv_b, opt, params, m, v, count, loss_hist, loss_cap);
+  hipLaunchKernelGGL(count_inc_kernel, dim3(1), dim3(64), 0, st, count);
+}
+
+void loss_out_launch(const double* loss_part, int nloss, float inv_b, double* loss_out, hipStream_t st) {
+  hipLaunchKernelGGL(mfma_loss_out_kernel, dim3(1), dim3(256), 0, st, loss_part, nloss, inv_b, loss_out);
+}
+
 static int nb_of(int H) { return (H + 15) / 16; }
 
 #define DIPS_MFMA_SHAPES(X) \

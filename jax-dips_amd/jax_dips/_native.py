@@ -62,6 +62,11 @@ class BatchProblem(ctypes.Structure):
                 ("loss_hist", ctypes.c_void_p), ("loss_cap", ctypes.c_int32)]
 
 
+class HashGrid(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("features", ctypes.c_int32), ("table_size", ctypes.c_int64),
+                ("n_min", ctypes.c_int32), ("n_max", ctypes.c_int32), ("bound", ctypes.c_float)]
+
+
 class Assemble(ctypes.Structure):
     _fields_ = [("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("zs", ctypes.c_void_p),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
@@ -127,6 +132,16 @@ SIGNATURES = {
                                             _P, _I32, _P, _I64, _P, _I64, _P]),
     "dips_mfma_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
                                            ctypes.POINTER(Net), ctypes.c_float, _P, _P, _P, _P, _I64, _P, _I64, _P]),
+    "dips_hash_table_rows": (_I64, [ctypes.POINTER(HashGrid)]),
+    "dips_hash_param_count": (_I64, [ctypes.POINTER(HashGrid)]),
+    "dips_hash_workspace": (_I64, [ctypes.POINTER(HashGrid), _I64, _I64, ctypes.POINTER(_I64)]),
+    "dips_hash_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
+                                            ctypes.POINTER(HashGrid), ctypes.POINTER(Opt), ctypes.c_float, _P, _P,
+                                            _P, _P, _P, _I32, _P, _I64, _P, _I64, _P]),
+    "dips_hash_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
+                                           ctypes.POINTER(HashGrid), ctypes.c_float, _P, _P, _P, _P, _I64, _P, _I64,
+                                           _P]),
+    "dips_hash_eval": (ctypes.c_int, [ctypes.POINTER(HashGrid), _P, _P, _P, _I64, _P, _P, _P]),
     "dips_grid_apply": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _P, ctypes.c_int, _P]),
     "dips_bicgstab": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _P, _P, _I32, ctypes.c_double,
                                      ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double), _P]),

@@ -14,6 +14,8 @@ def evaluate_points(model, flat_params, X, phase_p, normals=None, want_grad=Fals
     """u (m,) [and grad (m,3), grad.n (m,)] of the two-phase network at X (m,3)."""
     import torch
 
+    if getattr(model, "encoder", "identity") == "hashgrid":
+        return _evaluate_hash(model, flat_params, X, phase_p, normals, want_grad, device)
     net = N.Net(*model.dims, 0, 0, 0.0, int(getattr(model, "act", 0)))
     X = torch.as_tensor(np.asarray(X, np.float32), device=device).contiguous()
     m = X.shape[0]
@@ -31,6 +33,29 @@ def evaluate_points(model, flat_params, X, phase_p, normals=None, want_grad=Fals
     out_u = u.cpu().numpy()
     if normals is not None:
         return out_u, g.cpu().numpy(), gn.cpu().numpy()
+    if want_grad:
+        return out_u, g.cpu().numpy()
+    return out_u, None
+
+
+def _evaluate_hash(model, flat_params, X, phase_p, normals, want_grad, device):
+    """evaluate_points for the hashgrid network (dips_hash_eval); the normal
+    derivative is grad u . n."""
+    import torch
+
+    hg = N.HashGrid(*model.hashgrid)
+    X = torch.as_tensor(np.asarray(X, np.float32), device=device).contiguous()
+    m = X.shape[0]
+    ph = torch.as_tensor(np.asarray(phase_p, dtype=np.uint8), device=device).contiguous()
+    prm = torch.as_tensor(np.asarray(flat_params, np.float32), device=device).contiguous()
+    u = torch.empty(m, dtype=torch.float32, device=device)
+    g = torch.empty((m, 3), dtype=torch.float32, device=device) if (want_grad or normals is not None) else None
+    N.check(N.lib().dips_hash_eval(ctypes.byref(hg), N.ptr(prm), N.ptr(X), N.ptr(ph), m, N.ptr(u), N.ptr(g),
+                                   N.current_stream()), "dips_hash_eval")
+    out_u = u.cpu().numpy()
+    if normals is not None:
+        nr = torch.as_tensor(np.asarray(normals, np.float32), device=device)
+        return out_u, g.cpu().numpy(), (g * nr).sum(dim=1).cpu().numpy()
     if want_grad:
         return out_u, g.cpu().numpy()
     return out_u, None

@@ -425,6 +425,76 @@ class Engine:
         N.check(st, "dips_train_level")
 
 
+class HashEngine:
+    """Parameters, optimizer state and workspace of the hashgrid network on the
+    GPU (dips_hash_*): the two CELU MLPs and the encoder's table, trained over
+    the phase-sorted entries of each batch (DeviceRecords.mfma_batches)."""
+
+    def __init__(self, hashgrid, params, optimizer_dict, max_steps, device="cuda"):
+        """hashgrid: (levels, features, table_size, n_min, n_max, bound) of
+        dips_hashgrid_t; params: flat float32 (sol_p, sol_m, table)."""
+        import torch
+
+        self.hg = N.HashGrid(*[t(v) for t, v in zip((int, int, int, int, int, float), hashgrid)])
+        L = N.lib()
+        self.P = int(L.dips_hash_param_count(ctypes.byref(self.hg)))
+        if self.P < 0:
+            raise N.DipsError(f"hashgrid {tuple(hashgrid)}: {N.lib().dips_last_error().decode()}")
+        params = np.asarray(params, np.float32)
+        if params.size != self.P:
+            raise ValueError(f"expected {self.P} parameters, got {params.size}")
+        self.device = device
+        self.params = torch.as_tensor(params, device=device).clone()
+        self.m = torch.zeros(self.P, dtype=torch.float32, device=device)
+        self.v = torch.zeros(self.P, dtype=torch.float32, device=device)
+        self.count = torch.zeros(1, dtype=torch.int32, device=device)
+        self.loss_hist = torch.zeros(max(1, int(max_steps)), dtype=torch.float32, device=device)
+        self.opt = make_opt(optimizer_dict)
+        self._work = None
+
+    def workspace(self, rec):
+        import torch
+
+        mb = rec.mfma_batches()
+        dbl = ctypes.c_int64(0)
+        need = int(N.lib().dips_hash_workspace(ctypes.byref(self.hg), int(rec.batch_size), mb["max_nnz"],
+                                               ctypes.byref(dbl)))
+        if need < 0:
+            raise N.DipsError(N.lib().dips_last_error().decode())
+        if self._work is None or self._work.numel() < need or self._dwork.numel() < dbl.value:
+            self._work = torch.empty(need, dtype=torch.float32, device=self.device)
+            self._dwork = torch.empty(dbl.value, dtype=torch.float64, device=self.device)
+        return mb
+
+    def loss_grad(self, grid, rec, b, inv_batch):
+        """(loss, grad) of batch b, float64 numpy."""
+        import torch
+
+        mb = self.workspace(rec)
+        g = torch.empty(self.P, dtype=torch.float32, device=self.device)
+        loss = torch.empty(1, dtype=torch.float64, device=self.device)
+        N.check(N.lib().dips_hash_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb["batches"][b]),
+                                            ctypes.byref(self.hg), ctypes.c_float(inv_batch), N.ptr(self.params),
+                                            N.ptr(g), N.ptr(loss), N.ptr(self._work), int(self._work.numel()),
+                                            N.ptr(self._dwork), int(self._dwork.numel()), N.current_stream()),
+                "dips_hash_loss_grad")
+        torch.cuda.synchronize()
+        return float(loss.item()), g.double().cpu().numpy()
+
+    def train_level(self, grid, rec, nepochs):
+        mb = self.workspace(rec)
+        L = N.lib()
+        inv_b = ctypes.c_float(1.0 / rec.batch_size)
+        for _ in range(int(nepochs)):
+            for bt in mb["batches"]:
+                N.check(L.dips_hash_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(bt),
+                                               ctypes.byref(self.hg), ctypes.byref(self.opt), inv_b,
+                                               N.ptr(self.params), N.ptr(self.m), N.ptr(self.v), N.ptr(self.count),
+                                               N.ptr(self.loss_hist), int(self.loss_hist.numel()), N.ptr(self._work),
+                                               int(self._work.numel()), N.ptr(self._dwork),
+                                               int(self._dwork.numel()), N.current_stream()), "dips_hash_train_step")
+
+
 class BatchEngine:
     """Parameters and optimizer state of many independent problems with one
     network shape, trained together by dips_batch_train (one step launch over

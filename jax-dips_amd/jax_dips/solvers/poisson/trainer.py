@@ -27,10 +27,12 @@ The learned preconditioner (model_dict["preconditioner"], trainer.py:229-252)
 trains jointly with the networks inside the tile kernel; its parameters live
 under params["preconditioner"] and its initial draw comes from numpy.
 model_type "multiresolution_hash_network" runs with the identity position
-encoder (CELU coordinate MLPs, nn/hash_network.py).  Unsupported reference
-options raise NotImplementedError: algorithm 1, lbfgs, nonlinear operators,
-activations other than tanh in "mlp", and the hashgrid, frequency and
-spherical-harmonics encoders.
+encoder (CELU coordinate MLPs in the tile kernel) or the hashgrid encoder
+(dips_hash_train_step per batch: the MLPs and the encoder's table train
+together), nn/hash_network.py.  Unsupported reference options raise
+NotImplementedError: algorithm 1, lbfgs, nonlinear operators, activations
+other than tanh in "mlp", the preconditioner with the hashgrid encoder, and
+the frequency and spherical-harmonics encoders.
 """
 import logging
 import os
@@ -46,7 +48,7 @@ from ...data.data_management import generate_random_points_like
 from ...level_records import build_level, sample
 from ...nn.configure import get_model
 from ...nn.preconditioner import Preconditioner
-from ...records import DeviceGrid, Engine
+from ...records import DeviceGrid, Engine, HashEngine
 from ..simulation_states import PoissonSimState, PoissonSimStateFn
 from ..._jaxmd_modules import dataclasses
 
@@ -110,6 +112,8 @@ class Trainer:
                                                        np.float32(tr_gstate.dz))
         self.forward, self.framework = get_model(model_dict, model_type=model_dict.get("model_type", "mlp"))
         self.model = self.forward.model
+        if self.precond is not None and getattr(self.model, "encoder", "identity") == "hashgrid":
+            raise NotImplementedError("the preconditioner with the hashgrid network has no device implementation")
         self.cbs = _callbacks(sim_state_fn)
         self.epoch_start = 0
         self.opt_restore = None
@@ -171,6 +175,8 @@ class Trainer:
         return flat
 
     def _nn_size(self):
+        if hasattr(self.model, "n_params"):
+            return self.model.n_params
         return sum(a * b + b for tag in ("p", "m") for a, b in self.model.shapes(tag))
 
     def unflatten(self, flat):
@@ -205,10 +211,14 @@ class Trainer:
         bounds = (lv.xmin(), lv.xmax(), lv.ymin(), lv.ymax(), lv.zmin(), lv.zmax())
         E = self.num_epochs
         steps_total = (E - self.epoch_start) * nb
-        engine = Engine(self.model.dims, self.flatten(self.params), self.optimizer_dict, max_rows=B,
-                        max_steps=max(1, steps_total),
-                        precond=self.precond.engine_config() if self.precond is not None else None,
-                        act=getattr(self.model, "act", 0))
+        if getattr(self.model, "encoder", "identity") == "hashgrid":
+            engine = HashEngine(self.model.hashgrid, self.flatten(self.params), self.optimizer_dict,
+                                max_steps=max(1, steps_total))
+        else:
+            engine = Engine(self.model.dims, self.flatten(self.params), self.optimizer_dict, max_rows=B,
+                            max_steps=max(1, steps_total),
+                            precond=self.precond.engine_config() if self.precond is not None else None,
+                            act=getattr(self.model, "act", 0))
         if self.opt_restore is not None:
             engine.m.copy_(torch.as_tensor(self.flatten(self.opt_restore["mu"]), device="cuda"))
             engine.v.copy_(torch.as_tensor(self.flatten(self.opt_restore["nu"]), device="cuda"))

@@ -1,0 +1,204 @@
+"""Multiresolution hash network (hashgrid encoder + CELU coordinate MLPs) —
+CPU restatement.
+
+Test infrastructure only (see oracle/__init__.py).
+
+Restates HashGridEncoder.__call__ (jax_dips/nn/hash_encoding/blocks/
+encoders_flax.py:67-254) and SolutionMLP / CoordinateBasedMLP
+(nn/hash_encoding/model.py:53-94, blocks/nerfs_flax.py:142-170):
+
+* pos = (x + bound) / (2 bound); level i has scale s_i = N_min * b^i - 1 with
+  b = exp((ln N_max - ln N_min) / (L - 1)) (float64 host arithmetic, stored
+  float32), resolution res_i = ceil(s_i) + 1; levels with res_i^3 <= T are
+  dense (index x + y res + z res^2), the others hashed (x ^ y*2654435761 ^
+  z*805459861 in uint32); every index is taken mod T and offset by the level's
+  first table row (encoders_flax.py:98-112, :145-197);
+* pos_scaled = pos * s_i + 0.5 (float32), corners floor(pos_scaled) + {0,1}^3
+  in the order (0,0,0), (0,0,1), ..., (1,1,1); weights prod_d clip((1-o) +
+  (2o-1) frac, 0, 1) with frac = modf(pos_scaled) (:199-224);
+* encoding[l*F + f] = sum_corners table[idx, f] * weight (:229-235); total
+  variation does not enter the solution (model.py:82-94);
+* u = MLP_phase(encoding), MLP = [Dense(D), celu] then Dense(1).
+
+Flat parameters: the + MLP, the - MLP (per layer kernel [in][out] row-major,
+then bias), then the table (rows x F, row-major).  Table rows are clamped to
+the table (jax's default gather clamps out-of-range indices).  Left of the box
+(pos_scaled < 0) the reference's corner coordinates go through a float ->
+uint32 conversion of negative values (encoders_flax.py:128-132); this
+restatement takes that conversion as saturating (negative -> 0, the semantics
+of XLA's convert and of the GPU conversion instructions), and modf's negative
+fraction puts the whole weight of that axis on the lower corner (parity
+unpinned for such positions: the reference's own tests hold no vectors there).
+
+Pinned by tests/test_oracle_hashgrid.py: dense-level indices and a linear table
+reproduce a linear function exactly (trilinear interpolation), a constant
+table gives a constant encoding (weights sum to 1), and the VJP against central
+finite differences for table rows and MLP weights.
+"""
+import math
+
+import numpy as np
+
+from . import mlp as M
+
+PRIMES = (1, 2654435761, 805459861)
+
+
+class HashGrid:
+    def __init__(self, L=16, T=2 ** 19, F=2, N_min=16, N_max=2048, bound=1.0):
+        self.L, self.T, self.F, self.N_min = int(L), int(T), int(F), int(N_min)
+        self.N_max = int(N_max * bound)
+        self.bound = float(bound)
+        b = math.exp((math.log(self.N_max) - math.log(self.N_min)) / (self.L - 1))
+        self.scales, self.res, self.dense, offs = [], [], [], [0]
+        for i in range(self.L):
+            scale = self.N_min * (b ** i) - 1
+            res = math.ceil(scale) + 1
+            n = res ** 3
+            dense = n <= self.T
+            self.scales.append(scale)
+            self.res.append(res)
+            self.dense.append(dense)
+            offs.append(offs[-1] + (n if dense else self.T))
+        self.scales32 = np.asarray(self.scales, np.float32)
+        self.offsets = np.asarray(offs, np.int64)
+        self.rows = int(offs[-1])
+
+    @property
+    def n_features(self):
+        return self.L * self.F
+
+    def corners(self, X):
+        """(L, n, 8) table rows and trilinear weights of positions X (n,3)."""
+        X = np.asarray(X, np.float32)
+        pos = (X + np.float32(self.bound)) / np.float32(2 * self.bound)
+        n = X.shape[0]
+        rows = np.zeros((self.L, n, 8), np.int64)
+        wts = np.zeros((self.L, n, 8), np.float64)
+        for l in range(self.L):
+            ps = pos * self.scales32[l] + np.float32(0.5)  # float32
+            fl = np.floor(ps)
+            frac, _ = np.modf(ps)  # negative below 0 (jnp.modf)
+            for c in range(8):
+                o = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1])
+                # float -> uint32 conversion of the corner coordinates, saturating below 0
+                v = np.maximum(fl + o.astype(np.float32), 0).astype(np.uint64)
+                if self.dense[l]:
+                    r = np.uint64(self.res[l])
+                    idx = (v[:, 0] + v[:, 1] * r + v[:, 2] * r * r) & np.uint64(0xFFFFFFFF)
+                else:
+                    idx = (v[:, 0] ^ ((v[:, 1] * np.uint64(PRIMES[1])) & np.uint64(0xFFFFFFFF))
+                           ^ ((v[:, 2] * np.uint64(PRIMES[2])) & np.uint64(0xFFFFFFFF)))
+                idx = (idx % np.uint64(self.T)).astype(np.int64) + self.offsets[l]
+                rows[l, :, c] = np.minimum(idx, self.rows - 1)
+                w = np.clip((1 - o)[None, :] + (2 * o - 1)[None, :] * frac.astype(np.float64), 0, 1)
+                wts[l, :, c] = w.prod(axis=1)
+        return rows, wts
+
+    def encode(self, table, X, dt=np.float64):
+        rows, wts = self.corners(X)
+        tab = np.asarray(table, dt).reshape(self.rows, self.F)
+        enc = (tab[rows] * wts[..., None]).sum(axis=2)  # (L, n, F)
+        return enc.transpose(1, 0, 2).reshape(X.shape[0], self.L * self.F), (rows, wts)
+
+
+def mlp_size(E, widths):
+    dims = [E] + list(widths) + [1]
+    return sum(dims[i] * dims[i + 1] + dims[i + 1] for i in range(len(dims) - 1))
+
+
+def _unflatten_mlp(flat, E, widths):
+    dims = [E] + list(widths) + [1]
+    out, off = [], 0
+    for i in range(len(dims) - 1):
+        a, b = dims[i], dims[i + 1]
+        out.append((np.asarray(flat[off:off + a * b]).reshape(a, b), np.asarray(flat[off + a * b:off + a * b + b])))
+        off += a * b + b
+    return out
+
+
+def split(theta, grid, widths):
+    E = grid.n_features
+    P = mlp_size(E, widths)
+    return theta[:P], theta[P:2 * P], theta[2 * P:]
+
+
+def forward(theta, grid, widths, X, phase_p, dt=np.float64):
+    """u at X (n,3); phase_p True selects the + network."""
+    tp, tm, tab = split(np.asarray(theta, dt), grid, widths)
+    enc, _ = grid.encode(tab, X, dt)
+    u = np.zeros(X.shape[0], dt)
+    for flat, sel in ((tp, phase_p), (tm, ~phase_p)):
+        if sel.any():
+            u[sel], _ = M._forward_net(_unflatten_mlp(flat, grid.n_features, widths), enc[sel], dt, "celu")
+    return u
+
+
+def vjp(theta, grid, widths, X, phase_p, seed, dt=np.float64):
+    """Gradient of sum_i seed_i u(X_i) with respect to the flat parameters."""
+    theta = np.asarray(theta, dt)
+    E = grid.n_features
+    P = mlp_size(E, widths)
+    tp, tm, tab = split(theta, grid, widths)
+    enc, (rows, wts) = grid.encode(tab, X, dt)
+    grad = np.zeros(theta.size, dt)
+    denc = np.zeros_like(enc)
+    seed = np.asarray(seed, dt)
+    for k, (flat, sel) in enumerate(((tp, phase_p), (tm, ~phase_p))):
+        if not sel.any():
+            continue
+        net = _unflatten_mlp(flat, E, widths)
+        _, acts = M._forward_net(net, enc[sel], dt, "celu")
+        d = seed[sel][:, None]
+        g = np.zeros(P, dt)
+        offs = []
+        off = 0
+        for W, b in net:
+            offs.append(off)
+            off += W.size + b.size
+        for li in range(len(net) - 1, -1, -1):
+            W, b = net[li]
+            g[offs[li]:offs[li] + W.size] += (acts[li].T @ d).reshape(-1)
+            g[offs[li] + W.size:offs[li] + W.size + b.size] += d.sum(axis=0)
+            d = d @ W.T
+            if li > 0:
+                d = d * M.activation_grad(acts[li], "celu")
+        denc[sel] = d
+        grad[k * P:(k + 1) * P] = g
+    # table: encoding (n, L, F) <- sum_c table[rows[l, :, c]] * wts[l, :, c]
+    gt = np.zeros((grid.rows, grid.F), dt)
+    de = denc.reshape(X.shape[0], grid.L, grid.F)
+    for l in range(grid.L):
+        for c in range(8):
+            np.add.at(gt, rows[l, :, c], de[:, l, :] * wts[l, :, c][:, None])
+    grad[2 * P:] = gt.reshape(-1)
+    return grad
+
+
+def init_params(grid, widths, seed=0, dt=np.float32):
+    """Glorot-uniform MLP kernels, zero biases, table U(-1e-4, 1e-4)
+    (encoders_flax.py:118-123); numpy's generator."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    dims = [grid.n_features] + list(widths) + [1]
+    for _ in range(2):
+        for i in range(len(dims) - 1):
+            a, b = dims[i], dims[i + 1]
+            lim = np.sqrt(6.0 / (a + b))
+            parts += [rng.uniform(-lim, lim, a * b), np.zeros(b)]
+    parts.append(rng.uniform(-1e-4, 1e-4, grid.rows * grid.F))
+    return np.concatenate(parts).astype(dt)
+
+
+def loss_and_grad(theta, grid, widths, rec, rows, dt=np.float64):
+    """0.5 * mean(r^2) over the batch rows of oracle.trainer.LevelRecords rec and
+    its gradient, with the hashgrid network (the trainer's loss,
+    trainer.py:892-912)."""
+    W, nz, X, ph = rec.entries(rows)
+    u = forward(theta, grid, widths, X, ph, dt)
+    r = rec.c[rows].astype(dt).copy()
+    np.add.at(r, nz[0], W[nz].astype(dt) * u)
+    B = len(rows)
+    loss = np.sum(dt(0.5) * r * r) / dt(B)
+    seed = r[nz[0]] * W[nz].astype(dt) / dt(B)
+    return loss, vjp(theta, grid, widths, X, ph, seed, dt), r

@@ -122,11 +122,14 @@ def loss_and_grad(theta, dims, rec, rows, dt=np.float64, precond=None, act="tanh
 
 
 def train(P, h_tr, bounds, cb, dims, theta0, num_epochs, batch_size, opt_cfg,
-          dt=np.float64, param_dt=np.float32, record_dt=np.float64, records=None, precond=None, act="tanh"):
+          dt=np.float64, param_dt=np.float32, record_dt=np.float64, records=None, precond=None, act="tanh",
+          loss_fn=None):
     """Epoch loop of single_GPU_train with the zoom schedule.  Returns
     (theta, loss_epochs, per-step losses).  A short last batch is padded with
     padding_points (DatasetDict.gpu_padded_batches, data_management.py:103-173),
-    which become training points after the grid points."""
+    which become training points after the grid points.  loss_fn(theta, rec,
+    rows) -> (loss, grad, r) replaces the two-phase MLP's loss_and_grad (the
+    hashgrid network, oracle/hashgrid.py)."""
     n = P.shape[0]
     bl = batches(n, batch_size)
     npad = int(sum((b < 0).sum() for b in bl))
@@ -146,7 +149,10 @@ def train(P, h_tr, bounds, cb, dims, theta0, num_epochs, batch_size, opt_cfg,
         rec = recs[lvl]
         tot = 0.0
         for rows in bl:
-            loss, grad, _ = loss_and_grad(theta, dims, rec, rows, dt, precond, act)
+            if loss_fn is not None:
+                loss, grad, _ = loss_fn(theta, rec, rows)
+            else:
+                loss, grad, _ = loss_and_grad(theta, dims, rec, rows, dt, precond, act)
             theta = O.step(theta, grad.astype(param_dt), state, opt_cfg, param_dt)
             tot += loss
             steps.append(loss)

@@ -39,7 +39,7 @@ def test_host_only_queries():
 
 STRUCTS = {"Grid": "dips_grid_t", "Records": "dips_records_t", "Net": "dips_net_t", "Opt": "dips_opt_t",
            "Tiles": "dips_tiles_t", "MfmaBatch": "dips_mfma_batch_t", "Assemble": "dips_assemble_t",
-           "BatchProblem": "dips_batch_problem_t"}
+           "BatchProblem": "dips_batch_problem_t", "HashGrid": "dips_hashgrid_t"}
 
 
 def test_ctypes_structs_match_header(tmp_path):
@@ -88,3 +88,25 @@ def test_precond_shapes_and_counts():
     assert lib.dips_tile_supported(ctypes.byref(one)) == 1
     bad = _native.Net(1, 1, 2, 10, 8, 4, float("nan"))
     assert lib.dips_param_count(ctypes.byref(bad)) < 0
+
+
+def test_hashgrid_table_and_param_counts():
+    """Table rows and parameter count of the encoder equal the oracle's level
+    layout (dense levels res^3 rows, hashed levels table_size rows)."""
+    from jax_dips import _native
+    from oracle.hashgrid import HashGrid, mlp_size
+
+    lib = _native.lib()
+    for L, T, nmin, nmax, bound in ((16, 2 ** 19, 16, 2048, 1.0), (4, 2 ** 10, 4, 32, 1.5), (8, 2 ** 14, 16, 512, 1.0)):
+        hg = _native.HashGrid(L, 2, T, nmin, int(nmax * bound), bound)
+        ref = HashGrid(L, T, 2, nmin, nmax, bound)
+        assert lib.dips_hash_table_rows(ctypes.byref(hg)) == ref.rows
+        assert lib.dips_hash_param_count(ctypes.byref(hg)) == 2 * mlp_size(2 * L, [16]) + 2 * ref.rows
+        dbl = ctypes.c_int64(0)
+        assert lib.dips_hash_workspace(ctypes.byref(hg), 1000, 5000, ctypes.byref(dbl)) > 5000
+        assert dbl.value > 0
+    for bad in (_native.HashGrid(1, 2, 1024, 16, 64, 1.0), _native.HashGrid(17, 2, 1024, 16, 64, 1.0),
+                _native.HashGrid(4, 4, 1024, 16, 64, 1.0), _native.HashGrid(4, 2, 1024, 16, 8, 1.0),
+                _native.HashGrid(4, 2, 1024, 16, 64, 0.0)):
+        assert lib.dips_hash_table_rows(ctypes.byref(bad)) < 0
+        assert lib.dips_hash_param_count(ctypes.byref(bad)) < 0

@@ -1,7 +1,7 @@
-"""The multiresolution hash network model type with the identity encoder
-(nn/hash_network.py): flax-layout parameter tree, flat layout shared with the
-device kernels (the + network then the - network, kernel [in][out] then bias
-per layer), the C-ABI's shape table, and unsupported options (no GPU)."""
+"""The multiresolution hash network model type (nn/hash_network.py):
+flax-layout parameter tree, flat layout shared with the device kernels (the +
+network then the - network, kernel [in][out] then bias per layer, then the
+hashgrid table), the C-ABI's shape tables, and unsupported options (no GPU)."""
 import ctypes
 
 import numpy as np
@@ -42,10 +42,43 @@ def test_model_tree_and_flat_layout():
     assert lib.dips_param_count(ctypes.byref(_native.Net(*model.dims, 0, 0, 0.0, 2))) < 0
 
 
-@pytest.mark.parametrize("change", [{"pos_enc": "hashgrid"}, {"pos_enc": "frequency"}, {"sol_act": "sigmoid"},
-                                    {"layer_widths": [32]}, {"sol_skip_in_layers": [1]}])
+@pytest.mark.parametrize("change", [{"pos_enc": "frequency"}, {"pos_enc": "spherical_harmonics"},
+                                    {"sol_act": "sigmoid"}, {"layer_widths": [32]}, {"sol_skip_in_layers": [1]},
+                                    {"pos_enc": "hashgrid", "layer_widths": [16, 16]},
+                                    {"pos_enc": "hashgrid", "F": 4}, {"pos_enc": "hashgrid", "pos_levels": 17}])
 def test_unsupported_options(change):
     from jax_dips.nn.hash_network import HashNetwork
 
     with pytest.raises(NotImplementedError):
         HashNetwork(**dict(LPBE, **change))
+
+
+def test_hashgrid_tree_and_flat_layout():
+    """pos_enc="hashgrid": the table under position_encoder with the reference's
+    parameter name and (rows, F) shape, N_max scaled by the bound
+    (model.py:124-134), the flat vector sol_p | sol_m | table, and the C-ABI's
+    parameter count."""
+    from jax_dips import _native
+    from jax_dips.nn.configure import get_model
+    from jax_dips.nn.hash_network import TABLE_KEY
+    from oracle.hashgrid import HashGrid, mlp_size
+
+    cfg = dict(LPBE, pos_enc="hashgrid", bound=1.5, T=2 ** 12, pos_levels=6, N_min=4, N_max=40)
+    forward, _ = get_model({"model_type": "multiresolution_hash_network", "multiresolution_hash_network": cfg},
+                           "multiresolution_hash_network")
+    model = forward.model
+    assert model.hashgrid == (6, 2, 2 ** 12, 4, 60, 1.5) and model.in_dim == 12
+    ref = HashGrid(6, 2 ** 12, 2, 4, 40, 1.5)
+    tree = forward.init(3)
+    tab = tree["params"]["position_encoder"][TABLE_KEY]
+    assert tab.shape == (ref.rows, 2) and np.all(np.abs(tab) <= 1e-4)
+    assert tree["params"]["sol_p"]["Dense_0"]["kernel"].shape == (12, 16)
+    flat = model.flatten(tree)
+    n_mlp = 2 * mlp_size(12, [16])
+    assert flat.size == model.n_params == n_mlp + 2 * ref.rows
+    assert np.array_equal(flat[n_mlp:], tab.reshape(-1))
+    assert np.array_equal(model.flatten(model.unflatten(flat)), flat)
+    hg = _native.HashGrid(*model.hashgrid)
+    assert _native.lib().dips_hash_param_count(ctypes.byref(hg)) == flat.size
+    with pytest.raises(AttributeError):
+        model.dims
