@@ -58,10 +58,11 @@ def main():
     ap.add_argument("--roofline-launches", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "sweep", "pb"],
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "sweep", "pb", "hashgrid"],
                     help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144; "
                          "sweep: 64 problems of 32^3..96^3 with 2x16/2x16 MLPs, batched vs one at a time; "
-                         "pb: Poisson-Boltzmann box of 2000 atoms at 256^3, 3x64/3x64 MLP, batch 2^20")
+                         "pb: Poisson-Boltzmann box of 2000 atoms at 256^3, 3x64/3x64 MLP, batch 2^20; "
+                         "hashgrid: sphere 128^3 with the hash network (hashgrid encoder, 16 levels, T 2^19)")
     ap.add_argument("--problems", type=int, default=64, help="sweep: number of problems")
     ap.add_argument("--epochs-per-level", type=int, default=3, help="sweep: timed epochs per zoom level")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -91,6 +92,9 @@ def main():
 
     if args.workload == "sweep":
         run_sweep(args)
+        return
+    if args.workload == "hashgrid":
+        run_hashgrid(args)
         return
 
     from jax_dips import _native as N
@@ -396,6 +400,113 @@ def run_sweep(args):
                      "kernel": "tile_batch_step_kernel + tile_batch_apply_kernel (whole batched step)"},
         "records_build_s": t_rec,
         "loss_tail": [float(e.loss_hist[int(e.count.item()) - 1]) for e in beng.engines[:4]],
+    }
+    print(json.dumps(result), flush=True)
+
+
+def run_hashgrid(args):
+    """The hash network with the hashgrid encoder at the reference's defaults
+    (16 levels of 2 features, T = 2^19, N_min 16, N_max 2048, one CELU layer of
+    16; nn/hash_encoding/model.py:96-117) on the sphere problem at n^3, batch
+    65536, the four zoom levels of a 2000-epoch run.  A step is one
+    dips_hash_train_step: forward, row sums, backward (MLP gradients on the
+    matrix cores, table gradient by atomic adds), reduction and Adam over every
+    parameter, the table's 16.8 M entries included.
+
+    Roofline (HBM): the step's compulsory bytes, P x 48 B (parameters, m and v
+    read and written: 24; table gradient zeroed 4, scattered into once 8 and
+    read 4; gradient written 4 and read 4) + 24 B per entry (phase-sorted entry
+    arrays and u), over the step's time; table gathers that stay in the caches
+    are not counted."""
+    import ctypes
+
+    import torch
+
+    from jax_dips import _native as N
+    from jax_dips import problems
+    from jax_dips.level_records import build_level
+    from jax_dips.nn.hash_network import HashNetwork
+    from jax_dips.records import DeviceGrid, HashEngine
+
+    n, B = args.n, args.batch
+    prob = problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0)
+    cbs = problems.oracle_callbacks(prob)
+    x, y, z = problems.grid_axes(n)
+    grid = DeviceGrid(x, y, z)
+    h0 = np.array([x[1] - x[0]] * 3, np.float32)
+    bounds = (x.min(), x.max(), y.min(), y.max(), z.min(), z.max())
+    model = HashNetwork(bound=1.0, pos_enc="hashgrid", layer_widths=[16], pos_levels=16, T=2 ** 19, N_min=16,
+                        N_max=2048, F=2)
+    theta = model.flatten(model.init(42))
+    opt = {"optimizer_name": "custom", "learning_rate": 1e-3, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
+    t0 = time.time()
+    levels = []
+    for lvl in range(4):
+        levels.append(build_level(grid, (h0 * np.float32(0.5) ** lvl).astype(np.float32), bounds, cbs, B))
+        torch.cuda.synchronize()
+        print(f"# level {lvl} records built ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+    t_records = time.time() - t0
+    nb = levels[0].batch_begin.size
+    K = max(4, args.steps - args.steps % 4)
+    W = max(4, args.warmup - args.warmup % 4)
+    eng = HashEngine(model.hashgrid, theta, opt, max_steps=W + K + 8)
+    for rec in levels:
+        eng.workspace(rec)
+    L = N.lib()
+    stream = N.current_stream()
+    inv_b = ctypes.c_float(1.0 / B)
+
+    def run_steps(k):
+        per = k // 4
+        ents = 0
+        for rec in levels:
+            mb = rec.mfma_batches()["batches"]
+            for i in range(per):
+                bt = mb[i % nb]
+                ents += bt.t_count
+                N.check(L.dips_hash_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(bt),
+                                               ctypes.byref(eng.hg), ctypes.byref(eng.opt), inv_b, N.ptr(eng.params),
+                                               N.ptr(eng.m), N.ptr(eng.v), N.ptr(eng.count), N.ptr(eng.loss_hist),
+                                               int(eng.loss_hist.numel()), N.ptr(eng._work), int(eng._work.numel()),
+                                               N.ptr(eng._dwork), int(eng._dwork.numel()), stream),
+                        "dips_hash_train_step")
+        return ents
+
+    run_steps(W)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ents = run_steps(K)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    value = K * B / (ms / 1e3)
+    step_s = ms / 1e3 / K
+    bytes_step = eng.P * 48.0 + 24.0 * ents / K
+    achieved = bytes_step / step_s / 1e9
+    result = {
+        "metric": "hash network (hashgrid) training points/s",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"sphere {n}^3 on [-1,1]^3, hash network (hashgrid 16 levels x 2 features, T 2^19, "
+                               f"N_min 16, N_max 2048, CELU 16), batch {B}, 4 zoom levels, "
+                               f"{eng.P} parameters",
+                   "global_batch": B, "grid": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "whole dips_hash_train_step (7 launches), compulsory bytes",
+                     "bytes_per_step": bytes_step, "entries_per_step": ents / K},
+        "records_build_s": t_records,
+        "loss_tail": eng.loss_hist[: int(eng.count.item())][-4:].cpu().numpy().tolist(),
     }
     print(json.dumps(result), flush=True)
 

@@ -42,9 +42,11 @@ int64_t tile_partial_size(const dips_net_t* net);
 void rowsum_launch(const int32_t* row_ptr, const float* row_c, int64_t row_begin, int64_t nrows, const float* ent_w,
                    int64_t nnz, const float* u, float* r, double* loss_part, hipStream_t st);
 int rowsum_blocks(int64_t nrows);
-void apply_launch(const float* g, int64_t P, const double* sq_part, int nsq, const double* loss_part, int nloss,
-                  float inv_b, const dips_opt_t& opt, float* params, float* m, float* v, int32_t* count,
-                  float* loss_hist, int loss_cap, hipStream_t st);
+// optimizer over P parameters: gradient g[p] for p < split, g2[p - split] * g2_scale above
+void apply_launch(const float* g, int64_t P, const float* g2, int64_t split, float g2_scale, const double* sq_part,
+                  int nsq, const double* loss_part, int nloss, float inv_b, const dips_opt_t& opt, float* params,
+                  float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, hipStream_t st);
+constexpr int kApplyBlocks = 2048;  // grid cap of the optimizer kernel (it strides over the parameters)
 void loss_out_launch(const double* loss_part, int nloss, float inv_b, double* loss_out, hipStream_t st);
 int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net);
 int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,

@@ -33,7 +33,8 @@ constexpr int kHE = kHL * kHF;        // encoding width (maximum)
 constexpr int kHW = 16;               // hidden width
 constexpr int kHashWaves = 8;
 constexpr int kHashThreads = kHashWaves * DIPS_WAVE;
-constexpr int kHashBlocks = 256;      // backward workgroups (persistent)
+constexpr int kHashBlocks = 256;      // backward workgroups (persistent), a multiple of kXcds
+constexpr int kXcds = 8;              // accelerator dies (L2 domains) of the GPU
 constexpr int kHStride = 68;          // staged operand row: 64 entries + 4
 constexpr int kHStage = 2 * 16 * kHStride;
 
@@ -43,9 +44,8 @@ struct HashLevels {
   int64_t rows;
   float bound, two_bound;
   float scale[kHL];
-  uint32_t res[kHL];
-  int dense[kHL];
-  int64_t offset[kHL];
+  uint32_t res[kHL];     // resolution of a dense level, 0 for a hashed level
+  uint32_t offset[kHL];  // first table row of the level (tables of < 2^31 parameters)
 };
 
 __host__ __device__ inline int hash_mlp_size(int E) { return E * kHW + kHW + kHW + 1; }
@@ -65,7 +65,6 @@ static int hash_levels(const dips_hashgrid_t* hg, HashLevels& H) {
   for (int l = 0; l < kHL; ++l) {
     H.scale[l] = 0.0f;
     H.res[l] = 0;
-    H.dense[l] = 0;
     H.offset[l] = 0;
   }
   for (int l = 0; l < H.L; ++l) {
@@ -74,9 +73,8 @@ static int hash_levels(const dips_hashgrid_t* hg, HashLevels& H) {
     const double n3 = (double)res * (double)res * (double)res;
     const bool dense = n3 <= (double)hg->table_size;
     H.scale[l] = (float)scale;
-    H.res[l] = (uint32_t)res;
-    H.dense[l] = dense ? 1 : 0;
-    H.offset[l] = off;
+    H.res[l] = dense ? (uint32_t)res : 0u;
+    H.offset[l] = (uint32_t)off;
     off += dense ? res * res * res : hg->table_size;
   }
   H.rows = off;
@@ -118,8 +116,8 @@ __device__ __forceinline__ int64_t hash_row(const HashLevels& H, int l, const ui
   const uint32_t vy = ((c >> 1) & 1) ? hi[1] : lo[1];
   const uint32_t vz = (c & 1) ? hi[2] : lo[2];
   uint32_t idx;
-  if (H.dense[l]) {
-    const uint32_t r = H.res[l];
+  const uint32_t r = H.res[l];
+  if (r) {
     idx = vx + vy * r + vz * (r * r);
   } else {
     idx = vx ^ (vy * 2654435761u) ^ (vz * 805459861u);
@@ -323,6 +321,36 @@ struct HashGrad {
   }
 };
 
+// Adds (v0, v1) of every lane in live to table row `row` of gtab.  Lanes that
+// share a row are summed over the wavefront first (up to kAggIters rows of at
+// least kAggMin lanes each: the coarse levels, where a wavefront's entries fall
+// into a few cells), the others issue their own atomic adds.  Wave-uniform call.
+constexpr int kAggIters = 8;
+constexpr int kAggMin = 4;
+__device__ __forceinline__ void hash_scatter(float* __restrict__ gtab, int32_t row, float v0, float v1, uint64_t live,
+                                             int lane) {
+  uint64_t rem = live;
+#pragma unroll 1
+  for (int it = 0; it < kAggIters && rem != 0; ++it) {
+    const int leader = __builtin_ctzll(rem);
+    const int32_t key = __builtin_amdgcn_readlane(row, leader);
+    const uint64_t same = rem & __ballot(row == key);
+    if (__builtin_popcountll(same) < kAggMin) break;
+    const bool in = (same >> lane) & 1;
+    const float s0 = dips_wave_sum(in ? v0 : 0.0f);
+    const float s1 = dips_wave_sum(in ? v1 : 0.0f);
+    if (lane == leader) {
+      unsafeAtomicAdd(gtab + 2 * (int64_t)key, s0);
+      unsafeAtomicAdd(gtab + 2 * (int64_t)key + 1, s1);
+    }
+    rem &= ~same;
+  }
+  if ((rem >> lane) & 1) {
+    unsafeAtomicAdd(gtab + 2 * (int64_t)row, v0);
+    unsafeAtomicAdd(gtab + 2 * (int64_t)row + 1, v1);
+  }
+}
+
 __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params, int64_t row_begin,
                                                                 const float* __restrict__ r_row,
@@ -338,9 +366,13 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   HashGrad gp, gm;
   gp.zero();
   gm.zero();
+  // each XCD (workgroups blockIdx % 8, dispatched round-robin) takes a contiguous
+  // eighth of the entries, so that the table rows of a region stay in one L2
   const int64_t nchunks = (A.t_count + 63) / 64;
-  const int64_t nw = (int64_t)gridDim.x * kHashWaves;
-  for (int64_t ch = (int64_t)blockIdx.x * kHashWaves + wave; ch < nchunks; ch += nw) {
+  const int xcd = blockIdx.x % kXcds, xblk = blockIdx.x / kXcds;
+  const int64_t c_end = nchunks * (xcd + 1) / kXcds;
+  const int64_t nw = (int64_t)(gridDim.x / kXcds) * kHashWaves;
+  for (int64_t ch = nchunks * xcd / kXcds + (int64_t)xblk * kHashWaves + wave; ch < c_end; ch += nw) {
     const int64_t t = ch * 64 + lane;
     float x, y, z;
     int row;
@@ -382,8 +414,9 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
       for (int j = 0; j < kHW; ++j) d2[j] = plus ? 0.0f : dl[j];
       gm.add(AT, BT, lane, E, s2, enc, act, d2);
     }
-    // table gradient: scatter d enc to the corners
-    if (valid && seed != 0.0f) {
+    // table gradient: scatter d enc to the corners (wave-uniform from here on)
+    const uint64_t live_lanes = __ballot(valid && seed != 0.0f);
+    if (live_lanes) {
 #pragma unroll
       for (int l = 0; l < kHL; ++l) {
         if (l < H.L) {
@@ -393,10 +426,9 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
           hash_cell(H, l, p, lo, hi, frac, live);
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
-            const int64_t trow = hash_row(H, l, lo, hi, c);
+            const int32_t trow = (int32_t)hash_row(H, l, lo, hi, c);
             const float w = hash_weight(frac, c);
-            unsafeAtomicAdd(gtab + 2 * trow, w * denc[2 * l]);
-            unsafeAtomicAdd(gtab + 2 * trow + 1, w * denc[2 * l + 1]);
+            hash_scatter(gtab, trow, w * denc[2 * l], w * denc[2 * l + 1], live_lanes, lane);
           }
         }
       }
@@ -416,40 +448,35 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   }
 }
 
-// g = inv_b * (fixed-order sum of the MLP partial rows | the table gradient); float64 norm partials
+// Gradient of the MLPs, g[p] = inv_b * (fixed-order sum of the partial rows)
+// for p < stride; the table's is inv_b * gtab, written to g only when
+// write_table (the loss/gradient entry point; the optimizer reads gtab in
+// place).  float64 sums of squares per workgroup (fixed grid, grid-stride
+// order) for the clipping norm.
+constexpr int kRedBlocks = 1024;
 __global__ __launch_bounds__(256) void hash_reduce_kernel(const float* __restrict__ partial, int nblk, int stride,
                                                           const float* __restrict__ gtab, int64_t P, float inv_b,
-                                                          float* __restrict__ g, double* __restrict__ sq_part) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+                                                          int write_table, float* __restrict__ g,
+                                                          double* __restrict__ sq_part) {
   double v = 0.0;
-  if (p < P) {
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < P; p += (int64_t)gridDim.x * 256) {
     float gg;
     if (p < stride) {
       double acc = 0.0;
       for (int b = 0; b < nblk; ++b) acc += (double)partial[(int64_t)b * stride + p];
       gg = (float)(acc * (double)inv_b);
+      g[p] = gg;
     } else {
       gg = gtab[p - stride] * inv_b;
+      if (write_table) g[p] = gg;
     }
-    g[p] = gg;
-    v = (double)gg * (double)gg;
+    v += (double)gg * (double)gg;
   }
   v = dips_wave_sum_d(v);
   __shared__ double red[4];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
   __syncthreads();
   if (threadIdx.x == 0) sq_part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
-}
-
-__global__ __launch_bounds__(256) void hash_norm_kernel(const double* __restrict__ sq_part, int64_t n,
-                                                        double* __restrict__ out) {
-  __shared__ double red[4];
-  double t = 0.0;
-  for (int64_t b = threadIdx.x; b < n; b += 256) t += sq_part[b];
-  t = dips_wave_sum_d(t);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = t;
-  __syncthreads();
-  if (threadIdx.x == 0) *out = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 __global__ __launch_bounds__(256) void hash_zero_kernel(float* __restrict__ p, int64_t n) {
@@ -536,7 +563,7 @@ int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_
   if (hash_levels(hg, H) || max_rows < 1 || max_nnz < 0) return -1;
   const int64_t PM2 = 2 * (int64_t)hash_mlp_size(2 * H.L);
   const int64_t P = PM2 + H.rows * kHF;
-  if (doubles) *doubles = rowsum_blocks(max_rows) + (P + 255) / 256 + 2;
+  if (doubles) *doubles = rowsum_blocks(max_rows) + kRedBlocks + 2;
   return max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
 }
 
@@ -574,8 +601,7 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const int nrb = rowsum_blocks(nrows);
   double* loss_part = dwork;
   double* sq_part = dwork + nrb;
-  const int64_t nsq = (P + 255) / 256;
-  double* sq_total = sq_part + nsq;
+  const int nsq = (int)std::min<int64_t>((P + 255) / 256, kRedBlocks);
   HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
   if (bt->t_count > 0) {
@@ -596,14 +622,14 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   }
   hipLaunchKernelGGL(hash_bwd_kernel, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, bt->row_begin, r,
                      gtab, partial);
-  hipLaunchKernelGGL(hash_reduce_kernel, dim3((unsigned)nsq), dim3(256), 0, st, partial, kHashBlocks, (int)stride, gtab,
-                     P, inv_b, grad_out ? grad_out : gvec, sq_part);
+  hipLaunchKernelGGL(hash_reduce_kernel, dim3(nsq), dim3(256), 0, st, partial, kHashBlocks, (int)stride, gtab, P, inv_b,
+                     grad_out ? 1 : 0, grad_out ? grad_out : gvec, sq_part);
   if (!opt) {
     loss_out_launch(loss_part, nrb, inv_b, loss_out, st);
     return hip_status(hipGetLastError(), "hash step launch");
   }
-  hipLaunchKernelGGL(hash_norm_kernel, dim3(1), dim3(256), 0, st, sq_part, nsq, sq_total);
-  apply_launch(gvec, P, sq_total, 1, loss_part, nrb, inv_b, *opt, params, m, v, count, loss_hist, loss_cap, st);
+  apply_launch(gvec, P, gtab, stride, inv_b, sq_part, nsq, loss_part, nrb, inv_b, *opt, params, m, v, count, loss_hist,
+               loss_cap, st);
   return hip_status(hipGetLastError(), "hash step launch");
 }
 
@@ -612,6 +638,10 @@ int hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, 
   HashLevels H;
   int s = hash_levels(hg, H);
   if (s) return s;
+  if (2 * (int64_t)hash_mlp_size(2 * H.L) + H.rows * kHF > INT_MAX) {
+    set_error("hash eval: too many parameters");
+    return DIPS_EINVAL;
+  }
   if (m <= 0) return DIPS_OK;
   hipLaunchKernelGGL(hash_eval_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, H, params, pts, phase, m,
                      u, grad);

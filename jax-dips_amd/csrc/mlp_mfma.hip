@@ -542,7 +542,11 @@ __global__ __launch_bounds__(256) void mfma_reduce_kernel(const float* __restric
   if (threadIdx.x == 0) sq_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Parameters [0, split) take their gradient from g, [split, P) from
+// g2[p - split] * g2_scale (the hashgrid table's accumulated gradient, read in
+// place); the grid strides over P.
 __global__ __launch_bounds__(256) void mfma_apply_kernel(const float* __restrict__ g, int P,
+                                                         const float* __restrict__ g2, int split, float g2_scale,
                                                          const double* __restrict__ sq_part, int nsq,
                                                          const double* __restrict__ loss_part, int nloss, float inv_b,
                                                          dips_opt_t opt, float* __restrict__ params,
@@ -573,9 +577,8 @@ __global__ __launch_bounds__(256) void mfma_apply_kernel(const float* __restrict
   }
   const float t1 = (float)(c + 1);
   const float bc1 = 1.0f - powf(0.9f, t1), bc2 = 1.0f - powf(0.999f, t1);
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p < P) {
-    float gg = g[p];
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < P; p += gridDim.x * 256) {
+    float gg = p < split ? g[p] : g2[p - split] * g2_scale;
     if (clip) gg = (gg / gn) * opt.max_norm;
     float u;
     if (opt.kind == 2) {
@@ -633,12 +636,12 @@ void rowsum_launch(const int32_t* row_ptr, const float* row_c, int64_t row_begin
 
 int rowsum_blocks(int64_t nrows) { return (int)((nrows + kMT - 1) / kMT); }
 
-void apply_launch(const float* g, int64_t P, const double* sq_part, int nsq, const double* loss_part, int nloss,
-                  float inv_b, const dips_opt_t& opt, float* params, float* m, float* v, int32_t* count,
-                  float* loss_hist, int loss_cap, hipStream_t st) {
-  const int64_t nb = (P + 255) / 256;
-  hipLaunchKernelGGL(mfma_apply_kernel, dim3((unsigned)nb), dim3(256), 0, st, g, (int)P, sq_part, nsq, loss_part, nloss,
-                     inv_b, opt, params, m, v, count, loss_hist, loss_cap);
+void apply_launch(const float* g, int64_t P, const float* g2, int64_t split, float g2_scale, const double* sq_part,
+                  int nsq, const double* loss_part, int nloss, float inv_b, const dips_opt_t& opt, float* params,
+                  float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, hipStream_t st) {
+  const int64_t nb = std::min<int64_t>((P + 255) / 256, kApplyBlocks);
+  hipLaunchKernelGGL(mfma_apply_kernel, dim3((unsigned)nb), dim3(256), 0, st, g, (int)P, g2, (int)split, g2_scale,
+                     sq_part, nsq, loss_part, nloss, inv_b, opt, params, m, v, count, loss_hist, loss_cap);
   hipLaunchKernelGGL(count_inc_kernel, dim3(1), dim3(64), 0, st, count);
 }
 
@@ -769,8 +772,8 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     hipLaunchKernelGGL(mfma_loss_out_kernel, dim3(1), dim3(256), 0, st, loss_part, nrb, inv_b, loss_out);
     return hip_status(hipGetLastError(), "mfma reduce launch");
   }
-  hipLaunchKernelGGL(mfma_apply_kernel, dim3(nred), dim3(256), 0, st, gvec, P, sq_part, nred, loss_part, nrb, inv_b,
-                     *opt, params, m, v, count, loss_hist, loss_cap);
+  hipLaunchKernelGGL(mfma_apply_kernel, dim3(nred), dim3(256), 0, st, gvec, P, nullptr, P, 1.0f, sq_part, nred,
+                     loss_part, nrb, inv_b, *opt, params, m, v, count, loss_hist, loss_cap);
   hipLaunchKernelGGL(count_inc_kernel, dim3(1), dim3(64), 0, st, count);
   return hip_status(hipGetLastError(), "mfma reduce/apply launch");
 }
