@@ -211,9 +211,11 @@ __device__ __forceinline__ bool hash_entry(const HashArgs& A, int64_t t, float& 
   if (!point_xyz(A.ps, A.row_point[row], px, py, pz)) return false;
   int ox, oy, oz;
   dips_decode_offset(A.t_code[e], ox, oy, oz);
-  x = px + (float)ox * A.hx;
-  y = py + (float)oy * A.hy;
-  z = pz + (float)oz * A.hz;
+  // fl32(p + o h): o h is exact for o in [-2, 2], so the fused form rounds once, as the
+  // separate product and sum do; written fused so that every kernel computes it alike
+  x = fmaf((float)ox, A.hx, px);
+  y = fmaf((float)oy, A.hy, py);
+  z = fmaf((float)oz, A.hz, pz);
   return true;
 }
 
@@ -321,44 +323,241 @@ struct HashGrad {
   }
 };
 
-// Adds (v0, v1) of every lane in live to table row `row` of gtab.  Lanes that
-// share a row are summed over the wavefront first (up to kAggIters rows of at
-// least kAggMin lanes each: the coarse levels, where a wavefront's entries fall
-// into a few cells), the others issue their own atomic adds.  Wave-uniform call.
+// Table-gradient sinks of hash_scatter.  Atomic: adds into the table gradient.
+// Buckets: appends (row, v0, v1) to the row's bucket of 2^kBShift rows at the
+// workgroup's LDS cursor (bucket-major record array, positions counted
+// beforehand).  Count: counts the records the bucket sink will append.
+constexpr int kBShift = 13;
+constexpr int kBRows = 1 << kBShift;
+constexpr int kMaxBuckets = 4096;  // tables of up to 2^25 rows use the buckets
+
+struct AtomicSink {
+  float* gtab;
+  __device__ __forceinline__ void put(int32_t row, float v0, float v1) const {
+    unsafeAtomicAdd(gtab + 2 * (int64_t)row, v0);
+    unsafeAtomicAdd(gtab + 2 * (int64_t)row + 1, v1);
+  }
+};
+
+struct BucketSink {
+  int* cursor;  // LDS, per bucket
+  uint32_t* keys;
+  float2* vals;
+  int cap;  // records the arrays hold
+  __device__ __forceinline__ void put(int32_t row, float v0, float v1) const {
+    const int pos = atomicAdd(cursor + (row >> kBShift), 1);
+    if (pos < cap) {
+      keys[pos] = (uint32_t)row;
+      vals[pos] = make_float2(v0, v1);
+    }
+  }
+};
+
+struct CountSink {
+  int* cnt;  // LDS, per bucket
+  __device__ __forceinline__ void put(int32_t row, float, float) const { atomicAdd(cnt + (row >> kBShift), 1); }
+};
+
+// Sends (v0, v1) of every lane in live for table row `row` to the sink.  Lanes
+// that share a row are summed over the wavefront first: on a dense level
+// (`full`: the coarse levels, where a wavefront's entries fall into a few
+// cells) every row of the wavefront becomes one record; elsewhere up to
+// kAggIters rows of at least kAggMin lanes each, the other lanes sending their
+// own values.  Wave-uniform call; the grouping depends on the rows and the
+// mask only, so that a counting pass (kValues false) sees the same records.
 constexpr int kAggIters = 8;
 constexpr int kAggMin = 4;
-__device__ __forceinline__ void hash_scatter(float* __restrict__ gtab, int32_t row, float v0, float v1, uint64_t live,
-                                             int lane) {
+template <bool kValues, typename Sink>
+__device__ __forceinline__ void hash_scatter(const Sink& sink, int32_t row, float v0, float v1, uint64_t live,
+                                             int lane, bool full) {
   uint64_t rem = live;
+  if (full) {
+#pragma unroll 1
+    while (rem != 0) {
+      const int leader = __builtin_ctzll(rem);
+      const int32_t key = __builtin_amdgcn_readlane(row, leader);
+      const uint64_t same = rem & __ballot(row == key);
+      float s0 = v0, s1 = v1;
+      if constexpr (kValues) {
+        if (__builtin_popcountll(same) > 1) {
+          const bool in = (same >> lane) & 1;
+          s0 = dips_wave_sum(in ? v0 : 0.0f);
+          s1 = dips_wave_sum(in ? v1 : 0.0f);
+        }
+      }
+      if (lane == leader) sink.put(key, s0, s1);
+      rem &= ~same;
+    }
+    return;
+  }
 #pragma unroll 1
   for (int it = 0; it < kAggIters && rem != 0; ++it) {
     const int leader = __builtin_ctzll(rem);
     const int32_t key = __builtin_amdgcn_readlane(row, leader);
     const uint64_t same = rem & __ballot(row == key);
     if (__builtin_popcountll(same) < kAggMin) break;
-    const bool in = (same >> lane) & 1;
-    const float s0 = dips_wave_sum(in ? v0 : 0.0f);
-    const float s1 = dips_wave_sum(in ? v1 : 0.0f);
-    if (lane == leader) {
-      unsafeAtomicAdd(gtab + 2 * (int64_t)key, s0);
-      unsafeAtomicAdd(gtab + 2 * (int64_t)key + 1, s1);
+    float s0 = 0.0f, s1 = 0.0f;
+    if constexpr (kValues) {
+      const bool in = (same >> lane) & 1;
+      s0 = dips_wave_sum(in ? v0 : 0.0f);
+      s1 = dips_wave_sum(in ? v1 : 0.0f);
     }
+    if (lane == leader) sink.put(key, s0, s1);
     rem &= ~same;
   }
-  if ((rem >> lane) & 1) {
-    unsafeAtomicAdd(gtab + 2 * (int64_t)row, v0);
-    unsafeAtomicAdd(gtab + 2 * (int64_t)row + 1, v1);
-  }
+  if ((rem >> lane) & 1) sink.put(row, v0, v1);
 }
 
+// chunk c of the backward's persistent grid: each XCD (workgroups blockIdx % 8,
+// dispatched round-robin) takes a contiguous eighth of the entries, so that the
+// table rows of a region stay in one L2
+struct ChunkRange {
+  int64_t first, end, step;
+  __device__ __forceinline__ ChunkRange(int64_t t_count, int wave) {
+    const int64_t nchunks = (t_count + 63) / 64;
+    const int xcd = blockIdx.x % kXcds, xblk = blockIdx.x / kXcds;
+    end = nchunks * (xcd + 1) / kXcds;
+    step = (int64_t)(gridDim.x / kXcds) * kHashWaves;
+    first = nchunks * xcd / kXcds + (int64_t)xblk * kHashWaves + wave;
+  }
+};
+
+// Records per (bucket, workgroup) of the bucketed backward: the same chunks,
+// rows and grouping as hash_bwd_kernel<true>; cnt_out is bucket-major
+// [bucket][workgroup].
+__global__ __launch_bounds__(kHashThreads) void hash_count_kernel(HashArgs A, HashLevels H, int nb,
+                                                                  int* __restrict__ cnt_out) {
+  extern __shared__ int hash_cnt[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
+  for (int b = threadIdx.x; b < nb; b += kHashThreads) hash_cnt[b] = 0;
+  __syncthreads();
+  const CountSink sink{hash_cnt};
+  const ChunkRange cr(A.t_count, wave);
+  for (int64_t ch = cr.first; ch < cr.end; ch += cr.step) {
+    const int64_t t = ch * 64 + lane;
+    float x, y, z;
+    int row;
+    const bool valid = hash_entry(A, t, x, y, z, row);
+    const uint64_t live_lanes = __ballot(valid);
+    if (!live_lanes) continue;
+    float p[3];
+    hash_pos(H, x, y, z, p);
+#pragma unroll
+    for (int l = 0; l < kHL; ++l) {
+      if (l < H.L) {
+        uint32_t lo[3], hi[3];
+        float frac[3];
+        bool live[3];
+        hash_cell(H, l, p, lo, hi, frac, live);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          hash_scatter<false>(sink, (int32_t)hash_row(H, l, lo, hi, c), 0.0f, 0.0f, live_lanes, lane, H.res[l] != 0);
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += kHashThreads) cnt_out[(int64_t)b * kHashBlocks + blockIdx.x] = hash_cnt[b];
+}
+
+// exclusive scan of each bucket's kHashBlocks counts (one workgroup per bucket)
+__global__ __launch_bounds__(kHashBlocks) void hash_scan_local_kernel(const int* __restrict__ cnt,
+                                                                      int* __restrict__ loff,
+                                                                      int* __restrict__ total) {
+  __shared__ int wsum[kHashBlocks / 64];
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t / 64;
+  const int v = cnt[(int64_t)b * kHashBlocks + t];
+  int incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wave; ++w) base += wsum[w];
+  loff[(int64_t)b * kHashBlocks + t] = base + incl - v;
+  if (t == kHashBlocks - 1) total[b] = base + incl;
+}
+
+// exclusive scan of the bucket totals: bstart[0..nb] (one workgroup)
+__global__ __launch_bounds__(1024) void hash_scan_total_kernel(const int* __restrict__ total, int nb,
+                                                               int* __restrict__ bstart) {
+  __shared__ int part[1024];
+  constexpr int kPer = kMaxBuckets / 1024;
+  const int t = threadIdx.x;
+  int v[kPer], s = 0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int b = t * kPer + i;
+    v[i] = b < nb ? total[b] : 0;
+    s += v[i];
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int o = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += o;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int b = t * kPer + i;
+    if (b < nb) bstart[b] = run;
+    run += v[i];
+  }
+  if (t == 1023) bstart[nb] = part[1023];
+}
+
+// Table gradient rows of one bucket (one workgroup per bucket): the bucket's
+// records summed in LDS, then every row of the bucket written (rows no record
+// touches get 0).
+constexpr int kBThreads = 1024;
+__global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint32_t* __restrict__ keys,
+                                                                const float2* __restrict__ vals,
+                                                                const int* __restrict__ bstart, int64_t rows,
+                                                                float* __restrict__ gtab) {
+  __shared__ float acc[2 * kBRows];
+  const int b = blockIdx.x;
+  for (int i = threadIdx.x; i < 2 * kBRows; i += kBThreads) acc[i] = 0.0f;
+  __syncthreads();
+  const int e = bstart[b + 1];
+  for (int i = bstart[b] + threadIdx.x; i < e; i += kBThreads) {
+    const uint32_t k = keys[i] & (kBRows - 1);
+    const float2 v = vals[i];
+    atomicAdd(acc + 2 * k, v.x);
+    atomicAdd(acc + 2 * k + 1, v.y);
+  }
+  __syncthreads();
+  const int64_t r0 = (int64_t)b * kBRows;
+  const int64_t nr = rows - r0 < kBRows ? rows - r0 : kBRows;
+  for (int i = threadIdx.x; i < 2 * nr; i += kBThreads) gtab[2 * r0 + i] = acc[i];
+}
+
+// kBuckets: the table gradient goes to bucket records (cursors from the scans
+// of hash_count_kernel's counts), else by atomic adds into gtab
+template <bool kBuckets>
 __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params, int64_t row_begin,
                                                                 const float* __restrict__ r_row,
                                                                 float* __restrict__ gtab,
-                                                                float* __restrict__ partial) {
+                                                                float* __restrict__ partial, int nb,
+                                                                const int* __restrict__ bstart,
+                                                                const int* __restrict__ loff,
+                                                                uint32_t* __restrict__ keys,
+                                                                float2* __restrict__ vals, int cap) {
   extern __shared__ float hash_lds[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
+  int* const cursor = reinterpret_cast<int*>(hash_lds + kHashWaves * kHStage);
+  if constexpr (kBuckets) {
+    for (int b = threadIdx.x; b < nb; b += kHashThreads)
+      cursor[b] = bstart[b] + loff[(int64_t)b * kHashBlocks + blockIdx.x];
+    __syncthreads();
+  }
   float* const AT = hash_lds + wave * kHStage;
   float* const BT = AT + 16 * kHStride;
   const int E = 2 * H.L, PM = hash_mlp_size(E);
@@ -366,13 +565,8 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   HashGrad gp, gm;
   gp.zero();
   gm.zero();
-  // each XCD (workgroups blockIdx % 8, dispatched round-robin) takes a contiguous
-  // eighth of the entries, so that the table rows of a region stay in one L2
-  const int64_t nchunks = (A.t_count + 63) / 64;
-  const int xcd = blockIdx.x % kXcds, xblk = blockIdx.x / kXcds;
-  const int64_t c_end = nchunks * (xcd + 1) / kXcds;
-  const int64_t nw = (int64_t)(gridDim.x / kXcds) * kHashWaves;
-  for (int64_t ch = nchunks * xcd / kXcds + (int64_t)xblk * kHashWaves + wave; ch < c_end; ch += nw) {
+  const ChunkRange cr(A.t_count, wave);
+  for (int64_t ch = cr.first; ch < cr.end; ch += cr.step) {
     const int64_t t = ch * 64 + lane;
     float x, y, z;
     int row;
@@ -414,8 +608,9 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
       for (int j = 0; j < kHW; ++j) d2[j] = plus ? 0.0f : dl[j];
       gm.add(AT, BT, lane, E, s2, enc, act, d2);
     }
-    // table gradient: scatter d enc to the corners (wave-uniform from here on)
-    const uint64_t live_lanes = __ballot(valid && seed != 0.0f);
+    // table gradient: scatter d enc to the corners (wave-uniform from here on;
+    // the bucket records of every valid entry were counted, zero seeds included)
+    const uint64_t live_lanes = __ballot(kBuckets ? valid : (valid && seed != 0.0f));
     if (live_lanes) {
 #pragma unroll
       for (int l = 0; l < kHL; ++l) {
@@ -428,7 +623,12 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
           for (int c = 0; c < 8; ++c) {
             const int32_t trow = (int32_t)hash_row(H, l, lo, hi, c);
             const float w = hash_weight(frac, c);
-            hash_scatter(gtab, trow, w * denc[2 * l], w * denc[2 * l + 1], live_lanes, lane);
+            if constexpr (kBuckets)
+              hash_scatter<true>(BucketSink{cursor, keys, vals, cap}, trow, w * denc[2 * l], w * denc[2 * l + 1],
+                                 live_lanes, lane, H.res[l] != 0);
+            else
+              hash_scatter<true>(AtomicSink{gtab}, trow, w * denc[2 * l], w * denc[2 * l + 1], live_lanes, lane,
+                                 H.res[l] != 0);
           }
         }
       }
@@ -558,13 +758,26 @@ int64_t hash_param_count(const dips_hashgrid_t* hg) {
 }
 
 // float workspace: u (max_nnz) | r (max_rows) | partial rows | table gradient | g
+// table-gradient records of a batch of nent entries (one per entry, level and corner)
+static int64_t hash_records(const HashLevels& H, int64_t nent) { return nent * H.L * 8; }
+
+// the bucketed table gradient serves tables of up to kMaxBuckets buckets and
+// batches of fewer than 2^31 records; others use atomic adds
+static bool hash_bucketed(const HashLevels& H, int64_t nent, int* nb) {
+  *nb = (int)((H.rows + kBRows - 1) / kBRows);
+  return H.rows <= (int64_t)kMaxBuckets * kBRows && hash_records(H, nent) < INT_MAX;
+}
+
 int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles) {
   HashLevels H;
   if (hash_levels(hg, H) || max_rows < 1 || max_nnz < 0) return -1;
   const int64_t PM2 = 2 * (int64_t)hash_mlp_size(2 * H.L);
   const int64_t P = PM2 + H.rows * kHF;
   if (doubles) *doubles = rowsum_blocks(max_rows) + kRedBlocks + 2;
-  return max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
+  int64_t n = max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
+  int nb;
+  if (hash_bucketed(H, max_nnz, &nb)) n += 3 * hash_records(H, max_nnz) + 2 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
+  return n;
 }
 
 int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, const dips_hashgrid_t* hg,
@@ -574,7 +787,8 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   HashLevels H;
   int s = hash_levels(hg, H);
   if (s) return s;
-  if (!bt || bt->t_count < 0 || bt->n_plus + bt->n_minus != bt->t_count) {
+  if (!bt || bt->t_count < 0 || bt->n_plus + bt->n_minus != bt->t_count ||
+      bt->t_count > bt->ent_end - bt->ent_begin) {
     set_error("hash step: invalid batch");
     return DIPS_EINVAL;
   }
@@ -593,11 +807,24 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("hash step: too many parameters");
     return DIPS_EINVAL;
   }
+  // float workspace: u | r | MLP partial rows | table gradient | g | [records: keys | vals | counts | offsets |
+  // bucket totals | bucket starts]
   float* u = work;
   float* r = u + nent;
   float* partial = r + nrows;
   float* gtab = partial + (int64_t)kHashBlocks * stride;
   float* gvec = gtab + H.rows * kHF;
+  int nb = 0;
+  const bool buckets = hash_bucketed(H, nent, &nb);
+  const int64_t nrec = hash_records(H, nent);
+  uint32_t* keys = reinterpret_cast<uint32_t*>(gvec + P);
+  float* vbase = reinterpret_cast<float*>(keys + nrec);
+  vbase += (reinterpret_cast<uintptr_t>(vbase) / sizeof(float)) & 1;  // 8-byte alignment
+  float2* vals = reinterpret_cast<float2*>(vbase);
+  int* cnt = reinterpret_cast<int*>(vbase + 2 * nrec);
+  int* loff = cnt + (int64_t)nb * kHashBlocks;
+  int* total = loff + (int64_t)nb * kHashBlocks;
+  int* bstart = total + nb;
   const int nrb = rowsum_blocks(nrows);
   double* loss_part = dwork;
   double* sq_part = dwork + nrb;
@@ -605,23 +832,36 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
   if (bt->t_count > 0) {
-    const int64_t nb = (bt->t_count + 255) / 256;
-    hipLaunchKernelGGL(hash_fwd_kernel, dim3((unsigned)nb), dim3(256), 0, st, A, H, params, u - bt->ent_begin);
+    const int64_t nfb = (bt->t_count + 255) / 256;
+    hipLaunchKernelGGL(hash_fwd_kernel, dim3((unsigned)nfb), dim3(256), 0, st, A, H, params, u - bt->ent_begin);
   }
   rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
                 st);
-  hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
-                     H.rows * kHF);
-  const size_t lds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage, (int64_t)kHashWaves * stride);
-  static const bool lds_set = hipFuncSetAttribute((const void*)hash_bwd_kernel,
-                                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+  const size_t lds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage + (buckets ? nb : 0),
+                                                       (int64_t)kHashWaves * stride);
+  static const bool lds_set =
+      hipFuncSetAttribute((const void*)hash_bwd_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          160 * 1024) == hipSuccess &&
+      hipFuncSetAttribute((const void*)hash_bwd_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          160 * 1024) == hipSuccess;
   (void)lds_set;
   if (lds > 160 * 1024) {
     set_error("hash step: encoder too wide for the workgroup's LDS");
     return DIPS_EUNSUPPORTED;
   }
-  hipLaunchKernelGGL(hash_bwd_kernel, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, bt->row_begin, r,
-                     gtab, partial);
+  if (buckets) {
+    hipLaunchKernelGGL(hash_count_kernel, dim3(kHashBlocks), dim3(kHashThreads), sizeof(int) * nb, st, A, H, nb, cnt);
+    hipLaunchKernelGGL(hash_scan_local_kernel, dim3(nb), dim3(kHashBlocks), 0, st, cnt, loff, total);
+    hipLaunchKernelGGL(hash_scan_total_kernel, dim3(1), dim3(1024), 0, st, total, nb, bstart);
+    hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params,
+                       bt->row_begin, r, gtab, partial, nb, bstart, loff, keys, vals, (int)nrec);
+    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, keys, vals, bstart, H.rows, gtab);
+  } else {
+    hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
+                       H.rows * kHF);
+    hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params,
+                       bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, nullptr, 0);
+  }
   hipLaunchKernelGGL(hash_reduce_kernel, dim3(nsq), dim3(256), 0, st, partial, kHashBlocks, (int)stride, gtab, P, inv_b,
                      grad_out ? 1 : 0, grad_out ? grad_out : gvec, sq_part);
   if (!opt) {
