@@ -313,8 +313,9 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
  * width 16 and a linear output (blocks/nerfs_flax.py:142-170).
  * Flat parameters: the + MLP, the - MLP (W0[E][16], b0, Wout[16], bout with
  * E = levels * features), then the table (rows x features).  Entries are the
- * phase-sorted arrays of dips_mfma_batch_t.  The table gradient accumulates with
- * atomic adds (summation order not fixed: results vary in the last bits).
+ * phase-sorted arrays of dips_mfma_batch_t.  The table gradient is summed per
+ * bucket of table rows in an order that is not fixed (results vary in the last
+ * bits); tables above 2^25 rows use atomic adds.
  * ------------------------------------------------------------------------- */
 typedef struct {
   int32_t levels;      /* L, 2..16 */
