@@ -508,10 +508,12 @@ def run_hashgrid(args):
         "records_build_s": t_records,
         "loss_tail": eng.loss_hist[: int(eng.count.item())][-4:].cpu().numpy().tolist(),
     }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline("hashgrid", n, args.cpu_seconds, batch=256)
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(workload, n, seconds):
+def cpu_baseline(workload, n, seconds, batch=2048):
     """The reference's per-step pattern (oracle.refpattern, numpy float32) on
     the host cores: oracle.cpu_bench in a child interpreter, one
     single-threaded process per core (at most 16), each on its own stretch of
@@ -520,7 +522,8 @@ def cpu_baseline(workload, n, seconds):
 
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
     r = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", "--workload", workload, "--n", str(n),
-                        "--seconds", str(seconds)], cwd=ROOT, env=env, capture_output=True, text=True,
+                        "--seconds", str(seconds), "--batch", str(batch)], cwd=ROOT, env=env, capture_output=True,
+                       text=True,
                        timeout=4 * seconds + 300)
     if r.returncode != 0:
         raise RuntimeError(f"cpu baseline failed:\n{r.stderr[-2000:]}")

@@ -27,6 +27,8 @@ for _p in (ROOT, os.path.join(ROOT, "jax-dips_amd")):
 def _problem(workload):
     from jax_dips import problems
 
+    if workload == "hashgrid":
+        return problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0), None
     if workload == "pb":
         return problems.pb_box(n_atoms=2000, half_width=20.0, seed=0), (3, 64, 3, 64)
     if workload == "star":
@@ -43,9 +45,19 @@ def _worker(args):
     from oracle import refpattern
 
     prob, dims = _problem(workload)
-    mlp = {"hidden_layers_m": dims[0], "hidden_dim_m": dims[1], "activation_m": "jnp.tanh",
-           "hidden_layers_p": dims[2], "hidden_dim_p": dims[3], "activation_p": "jnp.tanh"}
-    model = DoubleMLP(mlp=mlp)
+    net = None
+    if workload == "hashgrid":
+        # the hash network at the reference's defaults (model.py:96-117)
+        from jax_dips.nn.hash_network import HashNetwork
+        from oracle import hashgrid as HG
+
+        model = HashNetwork(bound=1.0, pos_enc="hashgrid", layer_widths=[16], pos_levels=16, T=2 ** 19, N_min=16,
+                            N_max=2048, F=2)
+        net = HG.Network(HG.HashGrid(16, 2 ** 19, 2, 16, 2048, 1.0), [16])
+    else:
+        mlp = {"hidden_layers_m": dims[0], "hidden_dim_m": dims[1], "activation_m": "jnp.tanh",
+               "hidden_layers_p": dims[2], "hidden_dim_p": dims[3], "activation_p": "jnp.tanh"}
+        model = DoubleMLP(mlp=mlp)
     theta = model.flatten(model.init(42))
     cb = D.Callbacks(problems.oracle_callbacks(prob), np.float32)
     x, y, z = problems.grid_axes(n, -20.0, 20.0) if workload == "pb" else problems.grid_axes(n)
@@ -58,14 +70,14 @@ def _worker(args):
     done, t0 = 0, time.time()
     while time.time() - t0 < seconds:
         s = (start + done) % (P.shape[0] - batch)
-        refpattern.step(theta, dims, P[s:s + batch], h0, bounds, cb)
+        refpattern.step(theta, dims, P[s:s + batch], h0, bounds, cb, net=net)
         done += batch
     return done, time.time() - t0
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "pb"])
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "pb", "hashgrid"])
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--seconds", type=float, default=15.0)
     ap.add_argument("--batch", type=int, default=2048)
@@ -81,7 +93,9 @@ def main():
     print(json.dumps({"value": done / per, "unit": "points/s", "cores": workers, "kind": "port",
                       "sample": f"{done} points in batches of {a.batch} through the middle of the {a.n}^3 grid, "
                                 f"level-0 spacing, {workers} processes x {per:.1f} s (wall {wall:.1f} s), numpy "
-                                f"float32 restatement of the reference's per-step pattern, one thread each"}),
+                                f"float32 restatement of the reference's per-step pattern"
+                                + (" with the hashgrid network" if a.workload == "hashgrid" else "")
+                                + ", one thread each"}),
           flush=True)
 
 

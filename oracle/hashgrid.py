@@ -202,3 +202,17 @@ def loss_and_grad(theta, grid, widths, rec, rows, dt=np.float64):
     loss = np.sum(dt(0.5) * r * r) / dt(B)
     seed = r[nz[0]] * W[nz].astype(dt) / dt(B)
     return loss, vjp(theta, grid, widths, X, ph, seed, dt), r
+
+
+class Network:
+    """forward / vjp of the hashgrid network with a fixed encoder and widths, in
+    the call form of oracle.refpattern.step's ``net``."""
+
+    def __init__(self, grid, widths):
+        self.grid, self.widths = grid, list(widths)
+
+    def forward(self, theta, X, phase_p, dt=np.float64):
+        return forward(theta, self.grid, self.widths, X, phase_p, dt)
+
+    def vjp(self, theta, X, phase_p, seed, dt=np.float64):
+        return vjp(theta, self.grid, self.widths, X, phase_p, seed, dt)

@@ -18,15 +18,26 @@ from . import discretization as D
 from . import mlp as M
 
 
-def step(theta, dims, P, h, bounds, cb, dt=np.float32):
-    """Loss and gradient of one batch P (n,3), recomputing everything."""
+def step(theta, dims, P, h, bounds, cb, dt=np.float32, net=None):
+    """Loss and gradient of one batch P (n,3), recomputing everything.  net:
+    None for the two-phase MLP of ``dims``, else an object with
+    ``forward(theta, X, phase_p, dt)`` and ``vjp(theta, X, phase_p, seed, dt)``
+    (the hashgrid network, oracle/hashgrid.py)."""
+    if net is not None:
+        fwd, vjp = net.forward, net.vjp
+    else:
+        def fwd(th, X, ph, d):
+            return M.forward(th, dims, X, ph, d)
+
+        def vjp(th, X, ph, sd, d):
+            return M.vjp(th, dims, X, ph, sd, d)
     slots = D.point_slots(P, h, bounds, cb, dt)
     W = slots["W"].reshape(P.shape[0], -1)  # (n, 189)
     pos = slots["pos"].reshape(-1, 3)
     phase = cb("phi", pos) >= 0
     # both networks at every slot, then the phase selection (jnp.where)
-    up = M.forward(theta, dims, pos, np.ones(pos.shape[0], bool), dt)
-    um = M.forward(theta, dims, pos, np.zeros(pos.shape[0], bool), dt)
+    up = fwd(theta, pos, np.ones(pos.shape[0], bool), dt)
+    um = fwd(theta, pos, np.zeros(pos.shape[0], bool), dt)
     u = np.where(phase, up, um).reshape(W.shape)
     r = (W * u).sum(axis=1) + slots["c"]
     B = P.shape[0]
@@ -34,5 +45,5 @@ def step(theta, dims, P, h, bounds, cb, dt=np.float32):
     seed = (r[:, None] * W / dt(B)).reshape(-1)
     # the VJP of jnp.where runs both branches (the unselected one with a zero cotangent)
     ones = np.ones(pos.shape[0], bool)
-    grad = M.vjp(theta, dims, pos, ones, seed * phase, dt) + M.vjp(theta, dims, pos, ~ones, seed * ~phase, dt)
+    grad = vjp(theta, pos, ones, seed * phase, dt) + vjp(theta, pos, ~ones, seed * ~phase, dt)
     return loss, grad

@@ -127,6 +127,24 @@ def test_reference_pattern_equals_records_and_gradient_fd():
         assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(grad[i])) + 1e-3 * abs(grad[i]), (i, fd, grad[i])
 
 
+def test_reference_pattern_with_hashgrid_equals_records():
+    """The per-step pattern with the hashgrid network (the hashgrid workload's
+    CPU baseline) gives the loss and gradient of the merged records."""
+    from oracle import hashgrid as HG
+
+    prob = problems.sphere(radius=0.45, mu_m=1.0, mu_p=4.0)
+    axes, P, h, bounds = grid_points(8)
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    grid = HG.HashGrid(4, 2 ** 10, 2, 4, 32, 1.0)
+    theta = HG.init_params(grid, [16], seed=2).astype(np.float64)
+    theta[2 * HG.mlp_size(8, [16]):] *= 1e4
+    rec = T.LevelRecords(P, h, bounds, cb)
+    loss, grad, _ = HG.loss_and_grad(theta, grid, [16], rec, np.arange(P.shape[0]))
+    loss_ref, grad_ref = refpattern.step(theta, None, P, h, bounds, cb, dt=np.float64, net=HG.Network(grid, [16]))
+    assert abs(loss - loss_ref) <= 1e-6 * loss
+    assert np.linalg.norm(grad - grad_ref) <= 1e-5 * np.linalg.norm(grad)
+
+
 def test_beta_not_finite_at_origin_zeroes_crossed_rhs():
     """Padding triangles sit at the origin: a beta that is NaN there makes the
     interface integral NaN and nan_to_num(rhs/diag) = 0 on every crossed cell."""
