@@ -503,7 +503,7 @@ def run_hashgrid(args):
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "whole dips_hash_train_step (7 launches), compulsory bytes",
+                     "kernel": "whole dips_hash_train_step (9 launches), compulsory bytes",
                      "bytes_per_step": bytes_step, "entries_per_step": ents / K},
         "records_build_s": t_records,
         "loss_tail": eng.loss_hist[: int(eng.count.item())][-4:].cpu().numpy().tolist(),
