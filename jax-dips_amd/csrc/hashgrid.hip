@@ -10,13 +10,17 @@
 // encoding is level-major (l * 2 + f).  Network (model.py:53-94,
 // blocks/nerfs_flax.py:142-170): per phase W0[E][16], b0, CELU, Wout[16], bout.
 //
-// Step: forward (one lane per phase-sorted entry: encoding + MLP) -> row sums
-// (the MFMA path's kernel) -> backward (one wavefront per 64 entries: the MLP
-// weight gradient W0 = enc^T delta is a K = 64 contraction on the matrix cores
-// through LDS staging, biases and output weights per lane; the table gradient
-// d enc = W0 delta is scattered to the 8 corners of every level with atomic
-// adds) -> reduction (fixed order for the MLPs; table gradient scaled;
-// float64 norm partials) -> norm -> Adam over every parameter.
+// Step: forward (one lane per phase-sorted entry: encoding + MLP; it also
+// counts the backward's table-gradient records per bucket and workgroup) ->
+// row sums (the MFMA path's kernel) -> two scans of the counts -> backward (one
+// wavefront per 64 entries: the MLP weight gradient W0 = enc^T delta is a
+// K = 64 contraction on the matrix cores through LDS staging, biases and
+// output weights per lane; the table gradient d enc = W0 delta goes to the 8
+// corners of every level as (row, value) records in buckets of 8192 rows) ->
+// bucket sums (every table row written) -> reduction (fixed order for the
+// MLPs; float64 norm partials) -> Adam over every parameter, the table's
+// gradient read in place.  Tables above 2^25 rows use atomic adds instead of
+// the buckets.
 #include <math.h>
 
 #include <algorithm>
@@ -219,31 +223,6 @@ __device__ __forceinline__ bool hash_entry(const HashArgs& A, int64_t t, float& 
   return true;
 }
 
-// ---- forward: u per entry ------------------------------------------------------
-__global__ __launch_bounds__(256) void hash_fwd_kernel(HashArgs A, HashLevels H, const float* __restrict__ params,
-                                                       float* __restrict__ u_out) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  float x, y, z;
-  int row;
-  const bool valid = hash_entry(A, t, x, y, z, row);
-  const int E = 2 * H.L, PM = hash_mlp_size(E);
-  float enc[kHE];
-  float p[3];
-  hash_pos(H, x, y, z, p);
-  hash_encode(H, params + 2 * (int64_t)PM, p, enc);
-  const bool plus = t < A.n_plus;
-  float act[kHW];
-  const float* Praw = params;
-  asm volatile("" : "+s"(Praw));
-  dips_weight_t* Wp = dips_weights(Praw);
-  dips_weight_t* Wm = Wp + PM;
-  float up = 0.0f, um = 0.0f;
-  if (plus) up = hash_mlp(Wp, E, enc, act);
-  asm volatile("" : "+s"(Wm));
-  if (!plus) um = hash_mlp(Wm, E, enc, act);
-  if (valid) u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
-}
-
 // ---- backward -----------------------------------------------------------------
 typedef float hf4 __attribute__((ext_vector_type(4)));
 
@@ -359,37 +338,16 @@ struct CountSink {
 };
 
 // Sends (v0, v1) of every lane in live for table row `row` to the sink.  Lanes
-// that share a row are summed over the wavefront first: on a dense level
-// (`full`: the coarse levels, where a wavefront's entries fall into a few
-// cells) every row of the wavefront becomes one record; elsewhere up to
-// kAggIters rows of at least kAggMin lanes each, the other lanes sending their
-// own values.  Wave-uniform call; the grouping depends on the rows and the
-// mask only, so that a counting pass (kValues false) sees the same records.
+// that share a row are summed over the wavefront first: up to kAggIters rows of
+// at least kAggMin lanes each, the other lanes sending their own values.
+// Wave-uniform call; the grouping depends on the rows and the mask only, so
+// that a counting pass (kValues false) sees the same records.
 constexpr int kAggIters = 8;
 constexpr int kAggMin = 4;
 template <bool kValues, typename Sink>
 __device__ __forceinline__ void hash_scatter(const Sink& sink, int32_t row, float v0, float v1, uint64_t live,
-                                             int lane, bool full) {
+                                             int lane) {
   uint64_t rem = live;
-  if (full) {
-#pragma unroll 1
-    while (rem != 0) {
-      const int leader = __builtin_ctzll(rem);
-      const int32_t key = __builtin_amdgcn_readlane(row, leader);
-      const uint64_t same = rem & __ballot(row == key);
-      float s0 = v0, s1 = v1;
-      if constexpr (kValues) {
-        if (__builtin_popcountll(same) > 1) {
-          const bool in = (same >> lane) & 1;
-          s0 = dips_wave_sum(in ? v0 : 0.0f);
-          s1 = dips_wave_sum(in ? v1 : 0.0f);
-        }
-      }
-      if (lane == leader) sink.put(key, s0, s1);
-      rem &= ~same;
-    }
-    return;
-  }
 #pragma unroll 1
   for (int it = 0; it < kAggIters && rem != 0; ++it) {
     const int leader = __builtin_ctzll(rem);
@@ -408,6 +366,42 @@ __device__ __forceinline__ void hash_scatter(const Sink& sink, int32_t row, floa
   if ((rem >> lane) & 1) sink.put(row, v0, v1);
 }
 
+// The 8 corner records of every lane in live on a dense (coarse) level, where a
+// wavefront's entries fall into a few cells: lanes whose 8 corner rows all
+// equal are summed over the wavefront and their leader sends the 8 records.
+// Wave-uniform call; the grouping depends on the rows and the mask only.
+template <bool kValues, typename Sink>
+__device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_t (&rows)[8], const float (&v0)[8],
+                                                  const float (&v1)[8], uint64_t live, int lane) {
+  uint64_t rem = live;
+#pragma unroll 1
+  while (rem != 0) {
+    const int leader = __builtin_ctzll(rem);
+    bool match = true;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) match = match && rows[c] == __builtin_amdgcn_readlane(rows[c], leader);
+    const uint64_t same = rem & __ballot(match);
+    if (__builtin_popcountll(same) == 1) {
+      if (lane == leader) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) sink.put(rows[c], v0[c], v1[c]);
+      }
+    } else {
+      const bool in = (same >> lane) & 1;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        float s0 = 0.0f, s1 = 0.0f;
+        if constexpr (kValues) {
+          s0 = dips_wave_sum(in ? v0[c] : 0.0f);
+          s1 = dips_wave_sum(in ? v1[c] : 0.0f);
+        }
+        if (lane == leader) sink.put(rows[c], s0, s1);
+      }
+    }
+    rem &= ~same;
+  }
+}
+
 // chunk c of the backward's persistent grid: each XCD (workgroups blockIdx % 8,
 // dispatched round-robin) takes a contiguous eighth of the entries, so that the
 // table rows of a region stay in one L2
@@ -422,17 +416,26 @@ struct ChunkRange {
   }
 };
 
-// Records per (bucket, workgroup) of the bucketed backward: the same chunks,
-// rows and grouping as hash_bwd_kernel<true>; cnt_out is bucket-major
-// [bucket][workgroup].
-__global__ __launch_bounds__(kHashThreads) void hash_count_kernel(HashArgs A, HashLevels H, int nb,
-                                                                  int* __restrict__ cnt_out) {
+// ---- forward: u per entry --------------------------------------------------------
+// Persistent, on the backward's chunks (ChunkRange).  kCount: also counts the
+// table-gradient records of the bucketed backward per (bucket, workgroup) — the
+// same chunks, rows and grouping as hash_bwd_kernel<true> — into cnt_out,
+// bucket-major [bucket][workgroup].
+template <bool kCount>
+__global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, HashLevels H,
+                                                                const float* __restrict__ params,
+                                                                float* __restrict__ u_out, int nb,
+                                                                int* __restrict__ cnt_out) {
   extern __shared__ int hash_cnt[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
-  for (int b = threadIdx.x; b < nb; b += kHashThreads) hash_cnt[b] = 0;
-  __syncthreads();
+  if constexpr (kCount) {
+    for (int b = threadIdx.x; b < nb; b += kHashThreads) hash_cnt[b] = 0;
+    __syncthreads();
+  }
   const CountSink sink{hash_cnt};
+  const int E = 2 * H.L, PM = hash_mlp_size(E);
+  const float* table = params + 2 * (int64_t)PM;
   const ChunkRange cr(A.t_count, wave);
   for (int64_t ch = cr.first; ch < cr.end; ch += cr.step) {
     const int64_t t = ch * 64 + lane;
@@ -440,24 +443,54 @@ __global__ __launch_bounds__(kHashThreads) void hash_count_kernel(HashArgs A, Ha
     int row;
     const bool valid = hash_entry(A, t, x, y, z, row);
     const uint64_t live_lanes = __ballot(valid);
-    if (!live_lanes) continue;
     float p[3];
     hash_pos(H, x, y, z, p);
+    float enc[kHE];
 #pragma unroll
     for (int l = 0; l < kHL; ++l) {
+      enc[2 * l] = 0.0f;
+      enc[2 * l + 1] = 0.0f;
       if (l < H.L) {
         uint32_t lo[3], hi[3];
         float frac[3];
         bool live[3];
         hash_cell(H, l, p, lo, hi, frac, live);
+        int32_t rows[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-          hash_scatter<false>(sink, (int32_t)hash_row(H, l, lo, hi, c), 0.0f, 0.0f, live_lanes, lane, H.res[l] != 0);
+        for (int c = 0; c < 8; ++c) {
+          rows[c] = (int32_t)hash_row(H, l, lo, hi, c);
+          const float w = hash_weight(frac, c);
+          const float2 tv = reinterpret_cast<const float2*>(table)[rows[c]];
+          enc[2 * l] = fmaf(w, tv.x, enc[2 * l]);
+          enc[2 * l + 1] = fmaf(w, tv.y, enc[2 * l + 1]);
+        }
+        if constexpr (kCount) {
+          if (H.res[l] != 0) {
+            const float zero[8] = {};
+            hash_scatter_cell<false>(sink, rows, zero, zero, live_lanes, lane);
+          } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) hash_scatter<false>(sink, rows[c], 0.0f, 0.0f, live_lanes, lane);
+          }
+        }
       }
     }
+    const bool plus = t < A.n_plus;
+    float act[kHW];
+    const float* Praw = params;
+    asm volatile("" : "+s"(Praw));
+    dips_weight_t* Wp = dips_weights(Praw);
+    dips_weight_t* Wm = Wp + PM;
+    float up = 0.0f, um = 0.0f;
+    if (plus) up = hash_mlp(Wp, E, enc, act);
+    asm volatile("" : "+s"(Wm));
+    if (!plus) um = hash_mlp(Wm, E, enc, act);
+    if (valid) u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
   }
-  __syncthreads();
-  for (int b = threadIdx.x; b < nb; b += kHashThreads) cnt_out[(int64_t)b * kHashBlocks + blockIdx.x] = hash_cnt[b];
+  if constexpr (kCount) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kHashThreads) cnt_out[(int64_t)b * kHashBlocks + blockIdx.x] = hash_cnt[b];
+  }
 }
 
 // exclusive scan of each bucket's kHashBlocks counts (one workgroup per bucket)
@@ -538,7 +571,7 @@ __global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint32_t* 
 }
 
 // kBuckets: the table gradient goes to bucket records (cursors from the scans
-// of hash_count_kernel's counts), else by atomic adds into gtab
+// of hash_fwd_kernel<true>'s counts), else by atomic adds into gtab
 template <bool kBuckets>
 __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params, int64_t row_begin,
@@ -619,16 +652,32 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
           float frac[3];
           bool live[3];
           hash_cell(H, l, p, lo, hi, frac, live);
+          int32_t rows[8];
+          float v0[8], v1[8];
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
-            const int32_t trow = (int32_t)hash_row(H, l, lo, hi, c);
+            rows[c] = (int32_t)hash_row(H, l, lo, hi, c);
             const float w = hash_weight(frac, c);
-            if constexpr (kBuckets)
-              hash_scatter<true>(BucketSink{cursor, keys, vals, cap}, trow, w * denc[2 * l], w * denc[2 * l + 1],
-                                 live_lanes, lane, H.res[l] != 0);
-            else
-              hash_scatter<true>(AtomicSink{gtab}, trow, w * denc[2 * l], w * denc[2 * l + 1], live_lanes, lane,
-                                 H.res[l] != 0);
+            v0[c] = w * denc[2 * l];
+            v1[c] = w * denc[2 * l + 1];
+          }
+          const bool dense = H.res[l] != 0;
+          if constexpr (kBuckets) {
+            const BucketSink sink{cursor, keys, vals, cap};
+            if (dense) {
+              hash_scatter_cell<true>(sink, rows, v0, v1, live_lanes, lane);
+            } else {
+#pragma unroll
+              for (int c = 0; c < 8; ++c) hash_scatter<true>(sink, rows[c], v0[c], v1[c], live_lanes, lane);
+            }
+          } else {
+            const AtomicSink sink{gtab};
+            if (dense) {
+              hash_scatter_cell<true>(sink, rows, v0, v1, live_lanes, lane);
+            } else {
+#pragma unroll
+              for (int c = 0; c < 8; ++c) hash_scatter<true>(sink, rows[c], v0[c], v1[c], live_lanes, lane);
+            }
           }
         }
       }
@@ -831,10 +880,12 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const int nsq = (int)std::min<int64_t>((P + 255) / 256, kRedBlocks);
   HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
-  if (bt->t_count > 0) {
-    const int64_t nfb = (bt->t_count + 255) / 256;
-    hipLaunchKernelGGL(hash_fwd_kernel, dim3((unsigned)nfb), dim3(256), 0, st, A, H, params, u - bt->ent_begin);
-  }
+  if (buckets)
+    hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), sizeof(int) * nb, st, A, H,
+                       params, u - bt->ent_begin, nb, cnt);
+  else if (bt->t_count > 0)
+    hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), 0, st, A, H, params,
+                       u - bt->ent_begin, 0, nullptr);
   rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
                 st);
   const size_t lds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage + (buckets ? nb : 0),
@@ -850,7 +901,6 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     return DIPS_EUNSUPPORTED;
   }
   if (buckets) {
-    hipLaunchKernelGGL(hash_count_kernel, dim3(kHashBlocks), dim3(kHashThreads), sizeof(int) * nb, st, A, H, nb, cnt);
     hipLaunchKernelGGL(hash_scan_local_kernel, dim3(nb), dim3(kHashBlocks), 0, st, cnt, loff, total);
     hipLaunchKernelGGL(hash_scan_total_kernel, dim3(1), dim3(1024), 0, st, total, nb, bstart);
     hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params,
