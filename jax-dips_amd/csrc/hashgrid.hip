@@ -303,12 +303,25 @@ struct HashGrad {
 };
 
 // Table-gradient sinks of hash_scatter.  Atomic: adds into the table gradient.
-// Buckets: appends (row, v0, v1) to the row's bucket of 2^kBShift rows at the
-// workgroup's LDS cursor (bucket-major record array, positions counted
-// beforehand).  Count: counts the records the bucket sink will append.
-constexpr int kBShift = 13;
-constexpr int kBRows = 1 << kBShift;
+// Buckets: appends (row, v0, v1) to the row's bucket at the workgroup's LDS
+// cursor (bucket-major record array, positions counted beforehand).  Count:
+// counts the records the bucket sink will append.
+//
+// Buckets are block-cyclic over groups of 64 consecutive rows: group G = row >>
+// 6 belongs to bucket G mod nb (nb a power of two), at local rows ((G / nb) <<
+// 6) + (row & 63) < kBRows.  The hot rows of the small dense levels thus spread
+// over many buckets, and a bucket still writes whole 512-byte row groups.
+constexpr int kGShift = 6;
+constexpr int kBRows = 8192;       // local rows of a bucket (LDS accumulator)
 constexpr int kMaxBuckets = 4096;  // tables of up to 2^25 rows use the buckets
+
+struct BucketMap {
+  int shift;  // log2(nb)
+  __device__ __forceinline__ int bucket(int32_t row) const { return (row >> kGShift) & ((1 << shift) - 1); }
+  __device__ __forceinline__ int local(uint32_t row) const {
+    return (int)(((row >> kGShift) >> shift) << kGShift) | (int)(row & ((1u << kGShift) - 1));
+  }
+};
 
 struct AtomicSink {
   float* gtab;
@@ -323,8 +336,9 @@ struct BucketSink {
   uint32_t* keys;
   float2* vals;
   int cap;  // records the arrays hold
+  BucketMap map;
   __device__ __forceinline__ void put(int32_t row, float v0, float v1) const {
-    const int pos = atomicAdd(cursor + (row >> kBShift), 1);
+    const int pos = atomicAdd(cursor + map.bucket(row), 1);
     if (pos < cap) {
       keys[pos] = (uint32_t)row;
       vals[pos] = make_float2(v0, v1);
@@ -334,7 +348,8 @@ struct BucketSink {
 
 struct CountSink {
   int* cnt;  // LDS, per bucket
-  __device__ __forceinline__ void put(int32_t row, float, float) const { atomicAdd(cnt + (row >> kBShift), 1); }
+  BucketMap map;
+  __device__ __forceinline__ void put(int32_t row, float, float) const { atomicAdd(cnt + map.bucket(row), 1); }
 };
 
 // Sends (v0, v1) of every lane in live for table row `row` to the sink.  Lanes
@@ -424,7 +439,7 @@ struct ChunkRange {
 template <bool kCount>
 __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params,
-                                                                float* __restrict__ u_out, int nb,
+                                                                float* __restrict__ u_out, int nb, BucketMap map,
                                                                 int* __restrict__ cnt_out) {
   extern __shared__ int hash_cnt[];
   const int lane = threadIdx.x & 63;
@@ -433,7 +448,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, Hash
     for (int b = threadIdx.x; b < nb; b += kHashThreads) hash_cnt[b] = 0;
     __syncthreads();
   }
-  const CountSink sink{hash_cnt};
+  const CountSink sink{hash_cnt, map};
   const int E = 2 * H.L, PM = hash_mlp_size(E);
   const float* table = params + 2 * (int64_t)PM;
   const ChunkRange cr(A.t_count, wave);
@@ -551,23 +566,27 @@ __global__ __launch_bounds__(1024) void hash_scan_total_kernel(const int* __rest
 constexpr int kBThreads = 1024;
 __global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint32_t* __restrict__ keys,
                                                                 const float2* __restrict__ vals,
-                                                                const int* __restrict__ bstart, int64_t rows,
-                                                                float* __restrict__ gtab) {
+                                                                const int* __restrict__ bstart, BucketMap map,
+                                                                int64_t rows, float* __restrict__ gtab) {
   __shared__ float acc[2 * kBRows];
   const int b = blockIdx.x;
   for (int i = threadIdx.x; i < 2 * kBRows; i += kBThreads) acc[i] = 0.0f;
   __syncthreads();
   const int e = bstart[b + 1];
   for (int i = bstart[b] + threadIdx.x; i < e; i += kBThreads) {
-    const uint32_t k = keys[i] & (kBRows - 1);
+    const int k = map.local(keys[i]);
     const float2 v = vals[i];
     atomicAdd(acc + 2 * k, v.x);
     atomicAdd(acc + 2 * k + 1, v.y);
   }
   __syncthreads();
-  const int64_t r0 = (int64_t)b * kBRows;
-  const int64_t nr = rows - r0 < kBRows ? rows - r0 : kBRows;
-  for (int i = threadIdx.x; i < 2 * nr; i += kBThreads) gtab[2 * r0 + i] = acc[i];
+  // local row k of group lg is table row ((lg * nb + b) << 6) + (k & 63)
+  for (int i = threadIdx.x; i < 2 * kBRows; i += kBThreads) {
+    const int k = i >> 1;
+    const int64_t g = ((int64_t)(k >> kGShift) << map.shift) + b;
+    const int64_t row = (g << kGShift) + (k & ((1 << kGShift) - 1));
+    if (row < rows) gtab[2 * row + (i & 1)] = acc[i];
+  }
 }
 
 // kBuckets: the table gradient goes to bucket records (cursors from the scans
@@ -581,7 +600,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
                                                                 const int* __restrict__ bstart,
                                                                 const int* __restrict__ loff,
                                                                 uint32_t* __restrict__ keys,
-                                                                float2* __restrict__ vals, int cap) {
+                                                                float2* __restrict__ vals, int cap, BucketMap map) {
   extern __shared__ float hash_lds[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
@@ -663,7 +682,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
           }
           const bool dense = H.res[l] != 0;
           if constexpr (kBuckets) {
-            const BucketSink sink{cursor, keys, vals, cap};
+            const BucketSink sink{cursor, keys, vals, cap, map};
             if (dense) {
               hash_scatter_cell<true>(sink, rows, v0, v1, live_lanes, lane);
             } else {
@@ -697,24 +716,33 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   }
 }
 
-// Gradient of the MLPs, g[p] = inv_b * (fixed-order sum of the partial rows)
-// for p < stride; the table's is inv_b * gtab, written to g only when
-// write_table (the loss/gradient entry point; the optimizer reads gtab in
-// place).  float64 sums of squares per workgroup (fixed grid, grid-stride
-// order) for the clipping norm.
+// Gradient of the MLPs, g[p] = inv_b * (sum of the backward's partial rows),
+// one wavefront per parameter: lanes take workgroups lane, lane + 64, ... in
+// order, then the DPP tree (float64, fixed order).
+__global__ __launch_bounds__(256) void hash_mlp_reduce_kernel(const float* __restrict__ partial, int nblk, int stride,
+                                                              float inv_b, float* __restrict__ g) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + threadIdx.x / 64;
+  if (p >= stride) return;  // whole wavefronts
+  double acc = 0.0;
+  for (int b = lane; b < nblk; b += 64) acc += (double)partial[(int64_t)b * stride + p];
+  acc = dips_wave_sum_d(acc);
+  if (lane == 0) g[p] = (float)(acc * (double)inv_b);
+}
+
+// Table gradient inv_b * gtab, written to g only when write_table (the
+// loss/gradient entry point; the optimizer reads gtab in place), and float64
+// sums of squares of the whole gradient per workgroup (fixed grid, grid-stride
+// order) for the clipping norm; g[0, stride) holds the MLPs' already.
 constexpr int kRedBlocks = 1024;
-__global__ __launch_bounds__(256) void hash_reduce_kernel(const float* __restrict__ partial, int nblk, int stride,
-                                                          const float* __restrict__ gtab, int64_t P, float inv_b,
-                                                          int write_table, float* __restrict__ g,
+__global__ __launch_bounds__(256) void hash_reduce_kernel(int stride, const float* __restrict__ gtab, int64_t P,
+                                                          float inv_b, int write_table, float* __restrict__ g,
                                                           double* __restrict__ sq_part) {
   double v = 0.0;
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < P; p += (int64_t)gridDim.x * 256) {
     float gg;
     if (p < stride) {
-      double acc = 0.0;
-      for (int b = 0; b < nblk; ++b) acc += (double)partial[(int64_t)b * stride + p];
-      gg = (float)(acc * (double)inv_b);
-      g[p] = gg;
+      gg = g[p];
     } else {
       gg = gtab[p - stride] * inv_b;
       if (write_table) g[p] = gg;
@@ -812,9 +840,14 @@ static int64_t hash_records(const HashLevels& H, int64_t nent) { return nent * H
 
 // the bucketed table gradient serves tables of up to kMaxBuckets buckets and
 // batches of fewer than 2^31 records; others use atomic adds
-static bool hash_bucketed(const HashLevels& H, int64_t nent, int* nb) {
-  *nb = (int)((H.rows + kBRows - 1) / kBRows);
-  return H.rows <= (int64_t)kMaxBuckets * kBRows && hash_records(H, nent) < INT_MAX;
+static bool hash_bucketed(const HashLevels& H, int64_t nent, int* nb, int* shift = nullptr) {
+  // smallest power of two nb with nb * (kBRows / 64) row groups covering the table
+  const int64_t groups = (H.rows + (1 << kGShift) - 1) >> kGShift;
+  int sh = 0;
+  while (((int64_t)kBRows >> kGShift << sh) < groups) ++sh;
+  *nb = 1 << sh;
+  if (shift) *shift = sh;
+  return *nb <= kMaxBuckets && hash_records(H, nent) < INT_MAX;
 }
 
 int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles) {
@@ -863,8 +896,9 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   float* partial = r + nrows;
   float* gtab = partial + (int64_t)kHashBlocks * stride;
   float* gvec = gtab + H.rows * kHF;
-  int nb = 0;
-  const bool buckets = hash_bucketed(H, nent, &nb);
+  int nb = 0, nb_shift = 0;
+  const bool buckets = hash_bucketed(H, nent, &nb, &nb_shift);
+  const BucketMap bmap{nb_shift};
   const int64_t nrec = hash_records(H, nent);
   uint32_t* keys = reinterpret_cast<uint32_t*>(gvec + P);
   float* vbase = reinterpret_cast<float*>(keys + nrec);
@@ -882,10 +916,10 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
   if (buckets)
     hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), sizeof(int) * nb, st, A, H,
-                       params, u - bt->ent_begin, nb, cnt);
+                       params, u - bt->ent_begin, nb, bmap, cnt);
   else if (bt->t_count > 0)
     hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), 0, st, A, H, params,
-                       u - bt->ent_begin, 0, nullptr);
+                       u - bt->ent_begin, 0, bmap, nullptr);
   rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
                 st);
   const size_t lds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage + (buckets ? nb : 0),
@@ -904,16 +938,19 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     hipLaunchKernelGGL(hash_scan_local_kernel, dim3(nb), dim3(kHashBlocks), 0, st, cnt, loff, total);
     hipLaunchKernelGGL(hash_scan_total_kernel, dim3(1), dim3(1024), 0, st, total, nb, bstart);
     hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params,
-                       bt->row_begin, r, gtab, partial, nb, bstart, loff, keys, vals, (int)nrec);
-    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, keys, vals, bstart, H.rows, gtab);
+                       bt->row_begin, r, gtab, partial, nb, bstart, loff, keys, vals, (int)nrec, bmap);
+    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, keys, vals, bstart, bmap, H.rows, gtab);
   } else {
     hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
                        H.rows * kHF);
     hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params,
-                       bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, nullptr, 0);
+                       bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, nullptr, 0, bmap);
   }
-  hipLaunchKernelGGL(hash_reduce_kernel, dim3(nsq), dim3(256), 0, st, partial, kHashBlocks, (int)stride, gtab, P, inv_b,
-                     grad_out ? 1 : 0, grad_out ? grad_out : gvec, sq_part);
+  float* const gout = grad_out ? grad_out : gvec;
+  hipLaunchKernelGGL(hash_mlp_reduce_kernel, dim3((unsigned)((stride + 3) / 4)), dim3(256), 0, st, partial,
+                     kHashBlocks, (int)stride, inv_b, gout);
+  hipLaunchKernelGGL(hash_reduce_kernel, dim3(nsq), dim3(256), 0, st, (int)stride, gtab, P, inv_b, grad_out ? 1 : 0,
+                     gout, sq_part);
   if (!opt) {
     loss_out_launch(loss_part, nrb, inv_b, loss_out, st);
     return hip_status(hipGetLastError(), "hash step launch");
