@@ -439,7 +439,8 @@ struct ChunkRange {
 template <bool kCount>
 __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params,
-                                                                float* __restrict__ u_out, int nb, BucketMap map,
+                                                                float* __restrict__ u_out,
+                                                                float* __restrict__ encbuf, int nb, BucketMap map,
                                                                 int* __restrict__ cnt_out) {
   extern __shared__ int hash_cnt[];
   const int lane = threadIdx.x & 63;
@@ -500,7 +501,13 @@ __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, Hash
     if (plus) up = hash_mlp(Wp, E, enc, act);
     asm volatile("" : "+s"(Wm));
     if (!plus) um = hash_mlp(Wm, E, enc, act);
-    if (valid) u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
+    if (valid) {
+      u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
+      // the encoding, feature-major, for the backward
+#pragma unroll
+      for (int k = 0; k < kHE; ++k)
+        if (k < E) encbuf[(int64_t)k * A.t_count + t] = enc[k];
+    }
   }
   if constexpr (kCount) {
     __syncthreads();
@@ -593,7 +600,8 @@ __global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint32_t* 
 // of hash_fwd_kernel<true>'s counts), else by atomic adds into gtab
 template <bool kBuckets>
 __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, HashLevels H,
-                                                                const float* __restrict__ params, int64_t row_begin,
+                                                                const float* __restrict__ params,
+                                                                const float* __restrict__ encbuf, int64_t row_begin,
                                                                 const float* __restrict__ r_row,
                                                                 float* __restrict__ gtab,
                                                                 float* __restrict__ partial, int nb,
@@ -613,7 +621,6 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   float* const AT = hash_lds + wave * kHStage;
   float* const BT = AT + 16 * kHStride;
   const int E = 2 * H.L, PM = hash_mlp_size(E);
-  const float* table = params + 2 * (int64_t)PM;
   HashGrad gp, gm;
   gp.zero();
   gm.zero();
@@ -625,10 +632,12 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
     const bool valid = hash_entry(A, t, x, y, z, row);
     const float seed = valid ? r_row[row - row_begin] * A.t_w[A.t_begin + t] : 0.0f;
     const bool plus = t < A.n_plus;
+    // the forward's encoding of this entry (0 for lanes past the batch)
     float enc[kHE];
+#pragma unroll
+    for (int k = 0; k < kHE; ++k) enc[k] = (valid && k < E) ? encbuf[(int64_t)k * A.t_count + t] : 0.0f;
     float p[3];
     hash_pos(H, x, y, z, p);
-    hash_encode(H, table, p, enc);
     const float* Praw = params;
     asm volatile("" : "+s"(Praw));
     dips_weight_t* Wp = dips_weights(Praw);
@@ -858,6 +867,7 @@ int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_
   if (doubles) *doubles = rowsum_blocks(max_rows) + kRedBlocks + 2;
   int64_t n = max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
   int nb;
+  n += max_nnz * 2 * H.L;  // the forward's encodings
   if (hash_bucketed(H, max_nnz, &nb)) n += 3 * hash_records(H, max_nnz) + 2 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
   return n;
 }
@@ -900,7 +910,8 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const bool buckets = hash_bucketed(H, nent, &nb, &nb_shift);
   const BucketMap bmap{nb_shift};
   const int64_t nrec = hash_records(H, nent);
-  uint32_t* keys = reinterpret_cast<uint32_t*>(gvec + P);
+  float* encbuf = gvec + P;
+  uint32_t* keys = reinterpret_cast<uint32_t*>(encbuf + nent * 2 * H.L);
   float* vbase = reinterpret_cast<float*>(keys + nrec);
   vbase += (reinterpret_cast<uintptr_t>(vbase) / sizeof(float)) & 1;  // 8-byte alignment
   float2* vals = reinterpret_cast<float2*>(vbase);
@@ -916,10 +927,10 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
   if (buckets)
     hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), sizeof(int) * nb, st, A, H,
-                       params, u - bt->ent_begin, nb, bmap, cnt);
+                       params, u - bt->ent_begin, encbuf, nb, bmap, cnt);
   else if (bt->t_count > 0)
     hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), 0, st, A, H, params,
-                       u - bt->ent_begin, 0, bmap, nullptr);
+                       u - bt->ent_begin, encbuf, 0, bmap, nullptr);
   rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
                 st);
   const size_t lds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage + (buckets ? nb : 0),
@@ -937,13 +948,13 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   if (buckets) {
     hipLaunchKernelGGL(hash_scan_local_kernel, dim3(nb), dim3(kHashBlocks), 0, st, cnt, loff, total);
     hipLaunchKernelGGL(hash_scan_total_kernel, dim3(1), dim3(1024), 0, st, total, nb, bstart);
-    hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params,
+    hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
                        bt->row_begin, r, gtab, partial, nb, bstart, loff, keys, vals, (int)nrec, bmap);
     hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, keys, vals, bstart, bmap, H.rows, gtab);
   } else {
     hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
                        H.rows * kHF);
-    hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params,
+    hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
                        bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, nullptr, 0, bmap);
   }
   float* const gout = grad_out ? grad_out : gvec;
