@@ -442,9 +442,12 @@ __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, Hash
                                                                 float* __restrict__ u_out,
                                                                 float* __restrict__ encbuf, int nb, BucketMap map,
                                                                 int* __restrict__ cnt_out) {
+  // LDS: the per-bucket counts (kCount), then each wavefront's encodings
+  // (level loop not unrolled: an unrolled one overflows the instruction cache)
   extern __shared__ int hash_cnt[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
+  float* const lenc = reinterpret_cast<float*>(hash_cnt + kMaxBuckets) + wave * (kHE * DIPS_WAVE);
   if constexpr (kCount) {
     for (int b = threadIdx.x; b < nb; b += kHashThreads) hash_cnt[b] = 0;
     __syncthreads();
@@ -461,36 +464,44 @@ __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, Hash
     const uint64_t live_lanes = __ballot(valid);
     float p[3];
     hash_pos(H, x, y, z, p);
-    float enc[kHE];
+#pragma unroll 1
+    for (int l = 0; l < H.L; ++l) {
+      uint32_t lo[3], hi[3];
+      float frac[3];
+      bool live[3];
+      hash_cell(H, l, p, lo, hi, frac, live);
+      int32_t rows[8];
+      float e0 = 0.0f, e1 = 0.0f;
 #pragma unroll
-    for (int l = 0; l < kHL; ++l) {
-      enc[2 * l] = 0.0f;
-      enc[2 * l + 1] = 0.0f;
-      if (l < H.L) {
-        uint32_t lo[3], hi[3];
-        float frac[3];
-        bool live[3];
-        hash_cell(H, l, p, lo, hi, frac, live);
-        int32_t rows[8];
+      for (int c = 0; c < 8; ++c) {
+        rows[c] = (int32_t)hash_row(H, l, lo, hi, c);
+        const float w = hash_weight(frac, c);
+        const float2 tv = reinterpret_cast<const float2*>(table)[rows[c]];
+        e0 = fmaf(w, tv.x, e0);
+        e1 = fmaf(w, tv.y, e1);
+      }
+      lenc[(2 * l) * DIPS_WAVE + lane] = e0;
+      lenc[(2 * l + 1) * DIPS_WAVE + lane] = e1;
+      if (valid) {
+        // the encoding, feature-major, for the backward
+        encbuf[(int64_t)(2 * l) * A.t_count + t] = e0;
+        encbuf[(int64_t)(2 * l + 1) * A.t_count + t] = e1;
+      }
+      if constexpr (kCount) {
+        if (H.res[l] != 0) {
+          const float zero[8] = {};
+          hash_scatter_cell<false>(sink, rows, zero, zero, live_lanes, lane);
+        } else {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          rows[c] = (int32_t)hash_row(H, l, lo, hi, c);
-          const float w = hash_weight(frac, c);
-          const float2 tv = reinterpret_cast<const float2*>(table)[rows[c]];
-          enc[2 * l] = fmaf(w, tv.x, enc[2 * l]);
-          enc[2 * l + 1] = fmaf(w, tv.y, enc[2 * l + 1]);
-        }
-        if constexpr (kCount) {
-          if (H.res[l] != 0) {
-            const float zero[8] = {};
-            hash_scatter_cell<false>(sink, rows, zero, zero, live_lanes, lane);
-          } else {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) hash_scatter<false>(sink, rows[c], 0.0f, 0.0f, live_lanes, lane);
-          }
+          for (int c = 0; c < 8; ++c) hash_scatter<false>(sink, rows[c], 0.0f, 0.0f, live_lanes, lane);
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
+    float enc[kHE];
+#pragma unroll
+    for (int k = 0; k < kHE; ++k) enc[k] = k < E ? lenc[k * DIPS_WAVE + lane] : 0.0f;
+    __builtin_amdgcn_wave_barrier();
     const bool plus = t < A.n_plus;
     float act[kHW];
     const float* Praw = params;
@@ -501,13 +512,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, Hash
     if (plus) up = hash_mlp(Wp, E, enc, act);
     asm volatile("" : "+s"(Wm));
     if (!plus) um = hash_mlp(Wm, E, enc, act);
-    if (valid) {
-      u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
-      // the encoding, feature-major, for the backward
-#pragma unroll
-      for (int k = 0; k < kHE; ++k)
-        if (k < E) encbuf[(int64_t)k * A.t_count + t] = enc[k];
-    }
+    if (valid) u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
   }
   if constexpr (kCount) {
     __syncthreads();
@@ -673,21 +678,28 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
     // the bucket records of every valid entry were counted, zero seeds included)
     const uint64_t live_lanes = __ballot(kBuckets ? valid : (valid && seed != 0.0f));
     if (live_lanes) {
+      // d enc through the wavefront's (now free) staging area, so that the level
+      // loop need not be unrolled (an unrolled loop overflows the instruction cache)
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
-      for (int l = 0; l < kHL; ++l) {
-        if (l < H.L) {
+      for (int k = 0; k < kHE; ++k) AT[k * DIPS_WAVE + lane] = denc[k];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+      for (int l = 0; l < H.L; ++l) {
+        {
           uint32_t lo[3], hi[3];
           float frac[3];
           bool live[3];
           hash_cell(H, l, p, lo, hi, frac, live);
+          const float d0 = AT[(2 * l) * DIPS_WAVE + lane], d1 = AT[(2 * l + 1) * DIPS_WAVE + lane];
           int32_t rows[8];
           float v0[8], v1[8];
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
             rows[c] = (int32_t)hash_row(H, l, lo, hi, c);
             const float w = hash_weight(frac, c);
-            v0[c] = w * denc[2 * l];
-            v1[c] = w * denc[2 * l + 1];
+            v0[c] = w * d0;
+            v1[c] = w * d1;
           }
           const bool dense = H.res[l] != 0;
           if constexpr (kBuckets) {
@@ -925,11 +937,18 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const int nsq = (int)std::min<int64_t>((P + 255) / 256, kRedBlocks);
   HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
+  constexpr size_t fwd_lds = sizeof(float) * (kMaxBuckets + kHashWaves * kHE * DIPS_WAVE);
+  static const bool fwd_set =
+      hipFuncSetAttribute((const void*)hash_fwd_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)fwd_lds) == hipSuccess &&
+      hipFuncSetAttribute((const void*)hash_fwd_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)fwd_lds) == hipSuccess;
+  (void)fwd_set;
   if (buckets)
-    hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), sizeof(int) * nb, st, A, H,
-                       params, u - bt->ent_begin, encbuf, nb, bmap, cnt);
+    hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), fwd_lds, st, A, H, params,
+                       u - bt->ent_begin, encbuf, nb, bmap, cnt);
   else if (bt->t_count > 0)
-    hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), 0, st, A, H, params,
+    hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), fwd_lds, st, A, H, params,
                        u - bt->ent_begin, encbuf, 0, bmap, nullptr);
   rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
                 st);
