@@ -37,7 +37,7 @@ constexpr int kHE = kHL * kHF;        // encoding width (maximum)
 constexpr int kHW = 16;               // hidden width
 constexpr int kHashWaves = 8;
 constexpr int kHashThreads = kHashWaves * DIPS_WAVE;
-constexpr int kHashBlocks = 256;      // backward workgroups (persistent), a multiple of kXcds
+constexpr int kHashBlocks = 256;      // forward / backward workgroups (persistent), a multiple of kXcds
 constexpr int kXcds = 8;              // accelerator dies (L2 domains) of the GPU
 constexpr int kHStride = 68;          // staged operand row: 64 entries + 4
 constexpr int kHStage = 2 * 16 * kHStride;
@@ -417,9 +417,11 @@ __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_
   }
 }
 
-// chunk c of the backward's persistent grid: each XCD (workgroups blockIdx % 8,
-// dispatched round-robin) takes a contiguous eighth of the entries, so that the
-// table rows of a region stay in one L2
+// The 64-entry chunks a wavefront of the persistent forward / backward grid
+// takes: each XCD (workgroups blockIdx % 8, dispatched round-robin) takes a
+// contiguous eighth of the entries, so that the table rows of a region stay in
+// one L2; both kernels use the same assignment (the forward counts the
+// backward's records per workgroup).
 struct ChunkRange {
   int64_t first, end, step;
   __device__ __forceinline__ ChunkRange(int64_t t_count, int wave) {
@@ -432,7 +434,7 @@ struct ChunkRange {
 };
 
 // ---- forward: u per entry --------------------------------------------------------
-// Persistent, on the backward's chunks (ChunkRange).  kCount: also counts the
+// Persistent, on the chunks of ChunkRange.  kCount: also counts the
 // table-gradient records of the bucketed backward per (bucket, workgroup) — the
 // same chunks, rows and grouping as hash_bwd_kernel<true> — into cnt_out,
 // bucket-major [bucket][workgroup].
