@@ -82,3 +82,12 @@ def test_hashgrid_tree_and_flat_layout():
     assert _native.lib().dips_hash_param_count(ctypes.byref(hg)) == flat.size
     with pytest.raises(AttributeError):
         model.dims
+
+
+def test_solve_batch_rejects_hashgrid():
+    from jax_dips.solvers.poisson.batch import solve_batch
+
+    cfg = dict(LPBE, pos_enc="hashgrid", T=2 ** 10, pos_levels=4, N_min=4, N_max=32)
+    model_dict = {"model_type": "multiresolution_hash_network", "multiresolution_hash_network": cfg}
+    with pytest.raises(NotImplementedError):
+        solve_batch([{}], num_epochs=4, model_dict=model_dict)
