@@ -318,12 +318,15 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
  * bits); tables above 2^25 rows use atomic adds.
  * ------------------------------------------------------------------------- */
 typedef struct {
-  int32_t levels;      /* L, 2..16 */
-  int32_t features;    /* F, must be 2 */
-  int64_t table_size;  /* T, rows per hashed level */
-  int32_t n_min;       /* coarsest resolution */
-  int32_t n_max;       /* finest resolution (the reference's int(N_max * bound)) */
+  int32_t levels;      /* L, 2..16; with encoder 1 the number of frequencies, 10 */
+  int32_t features;    /* F, must be 2 (encoder 0) */
+  int64_t table_size;  /* T, rows per hashed level (encoder 0) */
+  int32_t n_min;       /* coarsest resolution (encoder 0) */
+  int32_t n_max;       /* finest resolution (the reference's int(N_max * bound)) (encoder 0) */
   float bound;         /* positions are mapped from [-bound, bound] to [0, 1] */
+  int32_t encoder;     /* 0 = hashgrid; 1 = frequency encoding (FrequencyEncoder, encoders_flax.py:269-307):
+                          per axis d and frequency l < levels, sin and cos of 2^l pi pos_d, feature
+                          (d levels + l) 2 + {0: sin, 1: cos}; no table */
 } dips_hashgrid_t;
 
 /* Table rows of the encoder (sum over levels of min(res^3, table_size)); < 0 if invalid. */

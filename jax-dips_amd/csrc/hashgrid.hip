@@ -43,7 +43,8 @@ constexpr int kHStride = 68;          // staged operand row: 64 entries + 4
 constexpr int kHStage = 2 * 16 * kHStride;
 
 struct HashLevels {
-  int L;
+  int L;    // levels (hashgrid) or frequencies (frequency encoding)
+  int enc;  // 0 = hashgrid, 1 = frequency encoding
   uint32_t T;
   int64_t rows;
   float bound, two_bound;
@@ -54,13 +55,32 @@ struct HashLevels {
 
 __host__ __device__ inline int hash_mlp_size(int E) { return E * kHW + kHW + kHW + 1; }
 
+constexpr int kFL = 10;          // frequencies of the frequency encoding (FrequencyEncoder(L=10), model.py:122-123)
+constexpr int kFE = 6 * kFL;     // its encoding width
+constexpr int kFEPad = 64;       // ... padded to whole 16-row MFMA blocks
+
 static int hash_levels(const dips_hashgrid_t* hg, HashLevels& H) {
-  if (!hg || hg->levels < 2 || hg->levels > kHL || hg->features != kHF || hg->table_size < 1 ||
+  if (hg && hg->encoder == 1) {
+    if (hg->levels != kFL || !(hg->bound > 0.0f)) {
+      set_error("frequency encoding: invalid description (levels = 10 frequencies, bound > 0)");
+      return DIPS_EINVAL;
+    }
+    H = HashLevels{};
+    H.L = hg->levels;
+    H.enc = 1;
+    H.T = 1;
+    H.rows = 0;
+    H.bound = hg->bound;
+    H.two_bound = 2.0f * hg->bound;
+    return DIPS_OK;
+  }
+  if (!hg || hg->encoder != 0 || hg->levels < 2 || hg->levels > kHL || hg->features != kHF || hg->table_size < 1 ||
       hg->table_size > (int64_t)1 << 31 || hg->n_min < 1 || hg->n_max < hg->n_min || !(hg->bound > 0.0f)) {
     set_error("hashgrid: invalid encoder description (levels 2..16, features 2, table_size 1..2^31, n_min <= n_max)");
     return DIPS_EINVAL;
   }
   H.L = hg->levels;
+  H.enc = 0;
   H.T = (uint32_t)hg->table_size;
   H.bound = hg->bound;
   H.two_bound = 2.0f * hg->bound;
@@ -161,13 +181,13 @@ __device__ __forceinline__ void hash_encode(const HashLevels& H, const float* __
 }
 
 // MLP of one phase: act (CELU hidden), returns u; W = flat MLP (W0[E][16], b0, Wout, bout)
-template <typename WP>
-__device__ __forceinline__ float hash_mlp(WP W, int E, const float (&enc)[kHE], float (&act)[kHW]) {
+template <int N, typename WP>
+__device__ __forceinline__ float hash_mlp(WP W, int E, const float (&enc)[N], float (&act)[kHW]) {
 #pragma unroll
   for (int j = 0; j < kHW; ++j) {
     float s = W[E * kHW + j];
 #pragma unroll
-    for (int k = 0; k < kHE; ++k)
+    for (int k = 0; k < N; ++k)
       if (k < E) s = fmaf(enc[k], W[k * kHW + j], s);
     act[j] = act_f<kActCelu>(s);
   }
@@ -240,36 +260,38 @@ __device__ __forceinline__ void hash_stage_mfma(const float* AT, const float* BT
   }
 }
 
-// gradient accumulators of one phase MLP within a wavefront
+// gradient accumulators of one phase MLP within a wavefront (encodings of up
+// to 16 NB features)
+template <int NB>
 struct HashGrad {
-  hf4 acc[2];       // W0 rows 0..15 and 16..31 (x the 16 hidden units)
+  hf4 acc[NB];      // W0 rows 16 b .. 16 b + 15 (x the 16 hidden units)
   float gb0[kHW];   // per-lane partial sums of delta
   float gwo[kHW];   // per-lane partial sums of seed * act
   float gbo;
 
   __device__ __forceinline__ void zero() {
-    acc[0] = acc[1] = hf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = hf4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < kHW; ++j) gb0[j] = gwo[j] = 0.0f;
     gbo = 0.0f;
   }
 
   // lanes that are not of this phase carry seed 0 and zero deltas; wave-uniform call
-  __device__ __forceinline__ void add(float* AT, float* BT, int lane, int E, float seed, const float (&enc)[kHE],
-                                      const float (&act)[kHW], const float (&dl)[kHW]) {
+  __device__ __forceinline__ void add(float* AT, float* BT, int lane, int E, float seed,
+                                      const float (&enc)[16 * NB], const float (&act)[kHW],
+                                      const float (&dl)[kHW]) {
 #pragma unroll
     for (int j = 0; j < kHW; ++j) BT[j * kHStride + lane] = dl[j];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) AT[i * kHStride + lane] = enc[i];
-    __builtin_amdgcn_wave_barrier();
-    hash_stage_mfma(AT, BT, lane, acc[0]);
-    __builtin_amdgcn_wave_barrier();
-    if (E > 16) {
+    for (int b = 0; b < NB; ++b) {
+      if (E > 16 * b) {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) AT[i * kHStride + lane] = enc[16 + i];
-      __builtin_amdgcn_wave_barrier();
-      hash_stage_mfma(AT, BT, lane, acc[1]);
-      __builtin_amdgcn_wave_barrier();
+        for (int i = 0; i < 16; ++i) AT[i * kHStride + lane] = enc[16 * b + i];
+        __builtin_amdgcn_wave_barrier();
+        hash_stage_mfma(AT, BT, lane, acc[b]);
+        __builtin_amdgcn_wave_barrier();
+      }
     }
 #pragma unroll
     for (int j = 0; j < kHW; ++j) {
@@ -282,7 +304,7 @@ struct HashGrad {
   // this wavefront's sums into red[0 .. PM)
   __device__ __forceinline__ void flush(float* red, int lane, int E) {
 #pragma unroll
-    for (int blk = 0; blk < 2; ++blk)
+    for (int blk = 0; blk < NB; ++blk)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * blk + 4 * (lane >> 4) + r, j = lane & 15;
@@ -628,7 +650,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   float* const AT = hash_lds + wave * kHStage;
   float* const BT = AT + 16 * kHStride;
   const int E = 2 * H.L, PM = hash_mlp_size(E);
-  HashGrad gp, gm;
+  HashGrad<2> gp, gm;
   gp.zero();
   gm.zero();
   const ChunkRange cr(A.t_count, wave);
@@ -844,7 +866,168 @@ __global__ __launch_bounds__(256) void hash_eval_kernel(HashLevels H, const floa
   grad[3 * t + 2] = g[2];
 }
 
+// ---- frequency encoding ---------------------------------------------------------
+// FrequencyEncoder (encoders_flax.py:269-307): pos = (x + bound) / (2 bound);
+// A[d][l] = (2^l pi) pos_d (the power-of-two multiple of pi is exact, then one
+// rounding), feature (d L + l) 2 = sin A, + 1 = cos A, L = 10.  Width 60,
+// padded with zeros to 64.
+__device__ __forceinline__ void freq_encode(const HashLevels& H, const float (&p)[3], float (&enc)[kFEPad]) {
+#pragma unroll
+  for (int k = 0; k < kFEPad; ++k) enc[k] = 0.0f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int l = 0; l < kFL; ++l) {
+      const float a = ldexpf(3.14159265358979323846f, l) * p[d];
+      float sn, cs;
+      sincosf(a, &sn, &cs);
+      enc[(d * kFL + l) * 2] = sn;
+      enc[(d * kFL + l) * 2 + 1] = cs;
+    }
+}
+
+// forward: u per entry (one lane per phase-sorted entry) and the encoding for the backward
+__global__ __launch_bounds__(256) void freq_fwd_kernel(HashArgs A, HashLevels H, const float* __restrict__ params,
+                                                       float* __restrict__ u_out, float* __restrict__ encbuf) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  float x, y, z;
+  int row;
+  const bool valid = hash_entry(A, t, x, y, z, row);
+  const int E = 6 * H.L, PM = hash_mlp_size(E);
+  float p[3];
+  hash_pos(H, x, y, z, p);
+  float enc[kFEPad];
+  freq_encode(H, p, enc);
+  const bool plus = t < A.n_plus;
+  float act[kHW];
+  const float* Praw = params;
+  asm volatile("" : "+s"(Praw));
+  dips_weight_t* Wp = dips_weights(Praw);
+  dips_weight_t* Wm = Wp + PM;
+  float up = 0.0f, um = 0.0f;
+  if (plus) up = hash_mlp(Wp, E, enc, act);
+  asm volatile("" : "+s"(Wm));
+  if (!plus) um = hash_mlp(Wm, E, enc, act);
+  if (valid) {
+    u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
+#pragma unroll
+    for (int k = 0; k < kFEPad; ++k)
+      if (k < E) encbuf[(int64_t)k * A.t_count + t] = enc[k];
+  }
+}
+
+// backward: the MLP weight gradients (no table), partial rows per workgroup as
+// in hash_bwd_kernel
+__global__ __launch_bounds__(kHashThreads) void freq_bwd_kernel(HashArgs A, HashLevels H,
+                                                                const float* __restrict__ params,
+                                                                const float* __restrict__ encbuf, int64_t row_begin,
+                                                                const float* __restrict__ r_row,
+                                                                float* __restrict__ partial) {
+  extern __shared__ float hash_lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
+  float* const AT = hash_lds + wave * kHStage;
+  float* const BT = AT + 16 * kHStride;
+  const int E = 6 * H.L, PM = hash_mlp_size(E);
+  HashGrad<kFEPad / 16> gp, gm;
+  gp.zero();
+  gm.zero();
+  const ChunkRange cr(A.t_count, wave);
+  for (int64_t ch = cr.first; ch < cr.end; ch += cr.step) {
+    const int64_t t = ch * 64 + lane;
+    float x, y, z;
+    int row;
+    const bool valid = hash_entry(A, t, x, y, z, row);
+    const float seed = valid ? r_row[row - row_begin] * A.t_w[A.t_begin + t] : 0.0f;
+    const bool plus = t < A.n_plus;
+    float enc[kFEPad];
+#pragma unroll
+    for (int k = 0; k < kFEPad; ++k) enc[k] = (valid && k < E) ? encbuf[(int64_t)k * A.t_count + t] : 0.0f;
+    const float* Praw = params;
+    asm volatile("" : "+s"(Praw));
+    dips_weight_t* Wp = dips_weights(Praw);
+    dips_weight_t* Wm = Wp + PM;
+    float act[kHW] = {};
+    float dl[kHW];
+    if (plus) {
+      (void)hash_mlp(Wp, E, enc, act);
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) dl[j] = seed * Wp[E * kHW + kHW + j] * act_grad<kActCelu>(act[j]);
+    }
+    asm volatile("" : "+s"(Wm));
+    if (!plus) {
+      (void)hash_mlp(Wm, E, enc, act);
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) dl[j] = seed * Wm[E * kHW + kHW + j] * act_grad<kActCelu>(act[j]);
+    }
+    if (__any(plus && valid)) {
+      float d2[kHW];
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) d2[j] = plus ? dl[j] : 0.0f;
+      gp.add(AT, BT, lane, E, plus ? seed : 0.0f, enc, act, d2);
+    }
+    if (__any(!plus && valid)) {
+      float d2[kHW];
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) d2[j] = plus ? 0.0f : dl[j];
+      gm.add(AT, BT, lane, E, plus ? 0.0f : seed, enc, act, d2);
+    }
+  }
+  const int stride = 2 * PM;
+  __syncthreads();
+  float* red = hash_lds + wave * stride;
+  gp.flush(red, lane, E);
+  gm.flush(red + PM, lane, E);
+  __syncthreads();
+  for (int q = threadIdx.x; q < stride; q += kHashThreads) {
+    float sum = 0.0f;
+    for (int w = 0; w < kHashWaves; ++w) sum += hash_lds[w * stride + q];
+    partial[(int64_t)blockIdx.x * stride + q] = sum;
+  }
+}
+
+// evaluation: u and d u / d x (d enc / d x through sin' = cos, cos' = -sin)
+__global__ __launch_bounds__(256) void freq_eval_kernel(HashLevels H, const float* __restrict__ params,
+                                                        const float* __restrict__ pts,
+                                                        const uint8_t* __restrict__ phase, int64_t m,
+                                                        float* __restrict__ u, float* __restrict__ grad) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= m) return;
+  const float x = pts[3 * t], y = pts[3 * t + 1], z = pts[3 * t + 2];
+  const int E = 6 * H.L, PM = hash_mlp_size(E);
+  float p[3];
+  hash_pos(H, x, y, z, p);
+  float enc[kFEPad], act[kHW];
+  freq_encode(H, p, enc);
+  const float* W = params + (phase[t] ? 0 : PM);
+  u[t] = hash_mlp(W, E, enc, act);
+  if (!grad) return;
+  float dj[kHW];
+#pragma unroll
+  for (int j = 0; j < kHW; ++j) dj[j] = W[E * kHW + kHW + j] * act_grad<kActCelu>(act[j]);
+  float g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int l = 0; l < kFL; ++l) {
+      const int ks = (d * kFL + l) * 2;
+      float du_s = 0.0f, du_c = 0.0f;
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) {
+        du_s = fmaf(W[ks * kHW + j], dj[j], du_s);
+        du_c = fmaf(W[(ks + 1) * kHW + j], dj[j], du_c);
+      }
+      const float da = ldexpf(3.14159265358979323846f, l) / H.two_bound;  // d A / d x_d
+      g[d] = fmaf(du_s * enc[ks + 1] - du_c * enc[ks], da, g[d]);
+    }
+  grad[3 * t] = g[0];
+  grad[3 * t + 1] = g[1];
+  grad[3 * t + 2] = g[2];
+}
+
 // ---- host side ------------------------------------------------------------------
+static int hash_width(const HashLevels& H) { return H.enc == 1 ? 6 * H.L : 2 * H.L; }
+
 int64_t hash_table_rows(const dips_hashgrid_t* hg) {
   HashLevels H;
   if (hash_levels(hg, H)) return -1;
@@ -854,7 +1037,7 @@ int64_t hash_table_rows(const dips_hashgrid_t* hg) {
 int64_t hash_param_count(const dips_hashgrid_t* hg) {
   HashLevels H;
   if (hash_levels(hg, H)) return -1;
-  return 2 * (int64_t)hash_mlp_size(2 * H.L) + H.rows * kHF;
+  return 2 * (int64_t)hash_mlp_size(hash_width(H)) + H.rows * kHF;
 }
 
 // float workspace: u (max_nnz) | r (max_rows) | partial rows | table gradient | g
@@ -876,13 +1059,13 @@ static bool hash_bucketed(const HashLevels& H, int64_t nent, int* nb, int* shift
 int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles) {
   HashLevels H;
   if (hash_levels(hg, H) || max_rows < 1 || max_nnz < 0) return -1;
-  const int64_t PM2 = 2 * (int64_t)hash_mlp_size(2 * H.L);
+  const int64_t PM2 = 2 * (int64_t)hash_mlp_size(hash_width(H));
   const int64_t P = PM2 + H.rows * kHF;
   if (doubles) *doubles = rowsum_blocks(max_rows) + kRedBlocks + 2;
   int64_t n = max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
   int nb;
-  n += max_nnz * 2 * H.L;  // the forward's encodings
-  if (hash_bucketed(H, max_nnz, &nb)) n += 3 * hash_records(H, max_nnz) + 2 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
+  n += max_nnz * hash_width(H);  // the forward's encodings
+  if (H.enc == 0 && hash_bucketed(H, max_nnz, &nb)) n += 3 * hash_records(H, max_nnz) + 2 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
   return n;
 }
 
@@ -898,7 +1081,7 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("hash step: invalid batch");
     return DIPS_EINVAL;
   }
-  const int PM = hash_mlp_size(2 * H.L);
+  const int PM = hash_mlp_size(hash_width(H));
   const int64_t stride = 2 * (int64_t)PM;
   const int64_t P = stride + H.rows * kHF;
   const int64_t nrows = bt->nrows;
@@ -921,11 +1104,11 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   float* gtab = partial + (int64_t)kHashBlocks * stride;
   float* gvec = gtab + H.rows * kHF;
   int nb = 0, nb_shift = 0;
-  const bool buckets = hash_bucketed(H, nent, &nb, &nb_shift);
+  const bool buckets = H.enc == 0 && hash_bucketed(H, nent, &nb, &nb_shift);
   const BucketMap bmap{nb_shift};
-  const int64_t nrec = hash_records(H, nent);
+  const int64_t nrec = buckets ? hash_records(H, nent) : 0;
   float* encbuf = gvec + P;
-  uint32_t* keys = reinterpret_cast<uint32_t*>(encbuf + nent * 2 * H.L);
+  uint32_t* keys = reinterpret_cast<uint32_t*>(encbuf + nent * hash_width(H));
   float* vbase = reinterpret_cast<float*>(keys + nrec);
   vbase += (reinterpret_cast<uintptr_t>(vbase) / sizeof(float)) & 1;  // 8-byte alignment
   float2* vals = reinterpret_cast<float2*>(vbase);
@@ -939,6 +1122,25 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const int nsq = (int)std::min<int64_t>((P + 255) / 256, kRedBlocks);
   HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
+  if (H.enc == 1) {
+    // frequency encoding: no table, so no records and no table gradient
+    if (bt->t_count > 0)
+      hipLaunchKernelGGL(freq_fwd_kernel, dim3((unsigned)((bt->t_count + 255) / 256)), dim3(256), 0, st, A, H,
+                         params, u - bt->ent_begin, encbuf);
+    rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r,
+                  loss_part, st);
+    const size_t flds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage, (int64_t)kHashWaves * stride);
+    static const bool freq_set = hipFuncSetAttribute((const void*)freq_bwd_kernel,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     160 * 1024) == hipSuccess;
+    (void)freq_set;
+    if (flds > 160 * 1024) {
+      set_error("hash step: encoder too wide for the workgroup's LDS");
+      return DIPS_EUNSUPPORTED;
+    }
+    hipLaunchKernelGGL(freq_bwd_kernel, dim3(kHashBlocks), dim3(kHashThreads), flds, st, A, H, params, encbuf,
+                       bt->row_begin, r, partial);
+  } else {
   constexpr size_t fwd_lds = sizeof(float) * (kMaxBuckets + kHashWaves * kHE * DIPS_WAVE);
   static const bool fwd_set =
       hipFuncSetAttribute((const void*)hash_fwd_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -978,6 +1180,7 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
                        bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, nullptr, 0, bmap);
   }
+  }
   float* const gout = grad_out ? grad_out : gvec;
   hipLaunchKernelGGL(hash_mlp_reduce_kernel, dim3((unsigned)((stride + 3) / 4)), dim3(256), 0, st, partial,
                      kHashBlocks, (int)stride, inv_b, gout);
@@ -997,11 +1200,16 @@ int hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, 
   HashLevels H;
   int s = hash_levels(hg, H);
   if (s) return s;
-  if (2 * (int64_t)hash_mlp_size(2 * H.L) + H.rows * kHF > INT_MAX) {
+  if (2 * (int64_t)hash_mlp_size(hash_width(H)) + H.rows * kHF > INT_MAX) {
     set_error("hash eval: too many parameters");
     return DIPS_EINVAL;
   }
   if (m <= 0) return DIPS_OK;
+  if (H.enc == 1) {
+    hipLaunchKernelGGL(freq_eval_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, H, params, pts, phase,
+                       m, u, grad);
+    return hip_status(hipGetLastError(), "freq_eval_kernel launch");
+  }
   hipLaunchKernelGGL(hash_eval_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, H, params, pts, phase, m,
                      u, grad);
   return hip_status(hipGetLastError(), "hash_eval_kernel launch");

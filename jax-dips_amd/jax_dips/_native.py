@@ -64,7 +64,8 @@ class BatchProblem(ctypes.Structure):
 
 class HashGrid(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("features", ctypes.c_int32), ("table_size", ctypes.c_int64),
-                ("n_min", ctypes.c_int32), ("n_max", ctypes.c_int32), ("bound", ctypes.c_float)]
+                ("n_min", ctypes.c_int32), ("n_max", ctypes.c_int32), ("bound", ctypes.c_float),
+                ("encoder", ctypes.c_int32)]
 
 
 class Assemble(ctypes.Structure):

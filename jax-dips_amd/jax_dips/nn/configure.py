@@ -1,6 +1,6 @@
 """Model factory (jax_dips/nn/configure.py:28-50): "mlp" (DoubleMLP, tanh) and
-"multiresolution_hash_network" with the identity encoder (CELU coordinate
-MLPs, nn/hash_network.py).
+"multiresolution_hash_network" with the identity, hashgrid or frequency
+encoder (CELU coordinate MLPs, nn/hash_network.py).
 
 ``get_model(model_dict, model_type)`` returns ``(forward, framework)``;
 ``forward.init(rng, x, phi_x)`` builds the parameter tree and

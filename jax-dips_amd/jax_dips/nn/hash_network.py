@@ -25,15 +25,20 @@ Parameter tree (flax): {"params": {"sol_p": {"Dense_i": {"kernel", "bias"}},
 "sol_m": {...}[, "position_encoder": {"latent codes stored on grid vertices":
 (rows, F)}]}}.
 
-Not implemented (NotImplementedError): the frequency and spherical harmonics
-encoders, skip connections, output activations other than linear, other hidden
-widths, and the total variation term (it does not enter the solution,
-model.py:82-94).
+* ``pos_enc="frequency"`` (model.py:122-123, FrequencyEncoder(L=10),
+  blocks/encoders_flax.py:269-307): sin and cos of 2^l pi pos per axis and
+  l < 10, 60 features; the same device kernels (dips_hashgrid_t.encoder = 1),
+  no table, one hidden layer of 16.
+
+Not implemented (NotImplementedError): the spherical harmonics encoder, skip
+connections, output activations other than linear, other hidden widths, and
+the total variation term (it does not enter the solution, model.py:82-94).
 """
 import numpy as np
 
 SUPPORTED_WIDTHS = ([16], [16, 16])
 HASHGRID_WIDTHS = ([16],)
+FREQUENCIES = 10
 TABLE_KEY = "latent codes stored on grid vertices"
 
 
@@ -43,15 +48,15 @@ class HashNetwork:
     def __init__(self, bound=1.0, pos_enc="identity", tv_scale=0.0, pos_levels=16, layer_widths=(64,),
                  sol_skip_in_layers=(), sol_act="linear", highest_sh_order=8, N_min=16, N_max=2048, F=2,
                  T=2 ** 19, **kwargs):
-        if pos_enc not in ("identity", "hashgrid"):
-            raise NotImplementedError(f"pos_enc={pos_enc!r}: only the identity and hashgrid encoders have device "
-                                      "kernels")
+        if pos_enc not in ("identity", "hashgrid", "frequency"):
+            raise NotImplementedError(f"pos_enc={pos_enc!r}: only the identity, hashgrid and frequency encoders have "
+                                      "device kernels")
         if list(sol_skip_in_layers):
             raise NotImplementedError("skip connections have no device implementation")
         if sol_act != "linear":
             raise NotImplementedError(f"sol_act={sol_act!r}: only 'linear' has a device implementation")
         widths = [int(w) for w in layer_widths]
-        allowed = HASHGRID_WIDTHS if pos_enc == "hashgrid" else SUPPORTED_WIDTHS
+        allowed = HASHGRID_WIDTHS if pos_enc in ("hashgrid", "frequency") else SUPPORTED_WIDTHS
         if widths not in allowed:
             raise NotImplementedError(f"layer_widths={widths}: compiled for {allowed} with pos_enc={pos_enc!r}")
         self.bound = float(bound)
@@ -65,17 +70,21 @@ class HashNetwork:
                 raise NotImplementedError(f"F={F}: the hashgrid kernels use 2 features per table row")
             if not 2 <= int(pos_levels) <= 16:
                 raise NotImplementedError(f"pos_levels={pos_levels}: the hashgrid kernels take 2..16 levels")
-            # (levels, features, table_size, n_min, n_max, bound) of dips_hashgrid_t; the
-            # encoder's finest resolution is int(N_max * bound) (model.py:129)
-            self.hashgrid = (int(pos_levels), int(F), int(T), int(N_min), int(N_max * bound), self.bound)
+            # (levels, features, table_size, n_min, n_max, bound, encoder) of dips_hashgrid_t;
+            # the encoder's finest resolution is int(N_max * bound) (model.py:129)
+            self.hashgrid = (int(pos_levels), int(F), int(T), int(N_min), int(N_max * bound), self.bound, 0)
             self.in_dim = int(pos_levels) * int(F)
             self.table_rows = self._table_rows()
+        elif pos_enc == "frequency":
+            # FrequencyEncoder(L=10) (model.py:122-123): sin and cos of 2^l pi pos per axis
+            self.hashgrid = (FREQUENCIES, 2, 1, 1, 1, self.bound, 1)
+            self.in_dim = 6 * FREQUENCIES
 
     def _table_rows(self):
         """Rows of the table: per level min(res^3, T) (encoders_flax.py:98-112)."""
         import math
 
-        L, _, T, nmin, nmax, _ = self.hashgrid
+        L, _, T, nmin, nmax, _, _ = self.hashgrid
         b = math.exp((math.log(nmax) - math.log(nmin)) / (L - 1))
         rows = 0
         for i in range(L):
@@ -85,7 +94,7 @@ class HashNetwork:
 
     @property
     def dims(self):
-        if self.encoder == "hashgrid":
+        if self.encoder in ("hashgrid", "frequency"):
             raise AttributeError("the hashgrid network has no (Lm, Hm, Lp, Hp) shape; it trains through dips_hash_*")
         return (self.L, self.H, self.L, self.H)
 

@@ -431,11 +431,12 @@ class HashEngine:
     the phase-sorted entries of each batch (DeviceRecords.mfma_batches)."""
 
     def __init__(self, hashgrid, params, optimizer_dict, max_steps, device="cuda"):
-        """hashgrid: (levels, features, table_size, n_min, n_max, bound) of
-        dips_hashgrid_t; params: flat float32 (sol_p, sol_m, table)."""
+        """hashgrid: (levels, features, table_size, n_min, n_max, bound, encoder)
+        of dips_hashgrid_t; params: flat float32 (sol_p, sol_m, then the table
+        of the hashgrid encoder)."""
         import torch
 
-        self.hg = N.HashGrid(*[t(v) for t, v in zip((int, int, int, int, int, float), hashgrid)])
+        self.hg = N.HashGrid(*[t(v) for t, v in zip((int, int, int, int, int, float, int), hashgrid)])
         L = N.lib()
         self.P = int(L.dips_hash_param_count(ctypes.byref(self.hg)))
         if self.P < 0:

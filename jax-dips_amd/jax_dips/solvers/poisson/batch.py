@@ -68,9 +68,9 @@ def solve_batch(problems, num_epochs=1000, batch_size=65536, model_dict=DEFAULT_
         raise ValueError("num_epochs must be at least 4 (four zoom levels of E // 4 epochs)")
     forward, _ = get_model(model_dict, model_type=model_dict.get("model_type", "mlp"))
     model = forward.model
-    if getattr(model, "encoder", "identity") == "hashgrid":
-        raise NotImplementedError("batched training of the hashgrid network has no device implementation; "
-                                  "train each problem with trainer.setup")
+    if getattr(model, "encoder", "identity") in ("hashgrid", "frequency"):
+        raise NotImplementedError("batched training of the hashgrid / frequency encoders has no device "
+                                  "implementation; train each problem with trainer.setup")
     pc = model_dict.get("preconditioner") or {}
     precond = Preconditioner(pc["layer_widths"], 1, pc.get("scaling_coeff", 1.0)) if pc.get("enable", False) else None
 

@@ -27,12 +27,13 @@ The learned preconditioner (model_dict["preconditioner"], trainer.py:229-252)
 trains jointly with the networks inside the tile kernel; its parameters live
 under params["preconditioner"] and its initial draw comes from numpy.
 model_type "multiresolution_hash_network" runs with the identity position
-encoder (CELU coordinate MLPs in the tile kernel) or the hashgrid encoder
+encoder (CELU coordinate MLPs in the tile kernel), the hashgrid encoder
 (dips_hash_train_step per batch: the MLPs and the encoder's table train
-together), nn/hash_network.py.  Unsupported reference options raise
+together) or the frequency encoder (the same entry points, no table),
+nn/hash_network.py.  Unsupported reference options raise
 NotImplementedError: algorithm 1, lbfgs, nonlinear operators, activations
-other than tanh in "mlp", the preconditioner with the hashgrid encoder, and
-the frequency and spherical-harmonics encoders.
+other than tanh in "mlp", the preconditioner with the hashgrid or frequency
+encoders, and the spherical-harmonics encoder.
 """
 import logging
 import os
@@ -112,8 +113,9 @@ class Trainer:
                                                        np.float32(tr_gstate.dz))
         self.forward, self.framework = get_model(model_dict, model_type=model_dict.get("model_type", "mlp"))
         self.model = self.forward.model
-        if self.precond is not None and getattr(self.model, "encoder", "identity") == "hashgrid":
-            raise NotImplementedError("the preconditioner with the hashgrid network has no device implementation")
+        if self.precond is not None and getattr(self.model, "encoder", "identity") in ("hashgrid", "frequency"):
+            raise NotImplementedError("the preconditioner with the hashgrid / frequency encoders has no device "
+                                      "implementation")
         self.cbs = _callbacks(sim_state_fn)
         self.epoch_start = 0
         self.opt_restore = None
@@ -211,7 +213,7 @@ class Trainer:
         bounds = (lv.xmin(), lv.xmax(), lv.ymin(), lv.ymax(), lv.zmin(), lv.zmax())
         E = self.num_epochs
         steps_total = (E - self.epoch_start) * nb
-        if getattr(self.model, "encoder", "identity") == "hashgrid":
+        if getattr(self.model, "encoder", "identity") in ("hashgrid", "frequency"):
             engine = HashEngine(self.model.hashgrid, self.flatten(self.params), self.optimizer_dict,
                                 max_steps=max(1, steps_total))
         else:

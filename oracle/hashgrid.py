@@ -30,10 +30,15 @@ of XLA's convert and of the GPU conversion instructions), and modf's negative
 fraction puts the whole weight of that axis on the lower corner (parity
 unpinned for such positions: the reference's own tests hold no vectors there).
 
+``Frequency`` restates the frequency encoder of the same model type
+(encoders_flax.py:269-307); forward / vjp take either encoder.
+
 Pinned by tests/test_oracle_hashgrid.py: dense-level indices and a linear table
 reproduce a linear function exactly (trilinear interpolation), a constant
 table gives a constant encoding (weights sum to 1), and the VJP against central
-finite differences for table rows and MLP weights.
+finite differences for table rows and MLP weights; the frequency encoding's
+closed forms (pos = 0 gives sin 0 = 0 and cos 0 = 1 on every frequency,
+pos = 1/2 gives sin(2^l pi / 2)) and its VJP against finite differences.
 """
 import math
 
@@ -102,6 +107,29 @@ class HashGrid:
         return enc.transpose(1, 0, 2).reshape(X.shape[0], self.L * self.F), (rows, wts)
 
 
+class Frequency:
+    """FrequencyEncoder (encoders_flax.py:269-307) with L frequencies (the hash
+    network uses L = 10, model.py:122-123): pos = (x + bound) / (2 bound),
+    A[d][l] = (2^l pi) pos_d in float32 (2^l pi is exact), features
+    (d L + l) 2 = sin A and (d L + l) 2 + 1 = cos A.  No table (rows = 0)."""
+
+    def __init__(self, L=10, bound=1.0):
+        self.L, self.bound = int(L), float(bound)
+        self.rows, self.F = 0, 2
+
+    @property
+    def n_features(self):
+        return 6 * self.L
+
+    def encode(self, table, X, dt=np.float64):
+        X = np.asarray(X, np.float32)
+        pos = (X + np.float32(self.bound)) / np.float32(2 * self.bound)
+        c = np.exp2(np.arange(self.L)).astype(np.float32) * np.float32(np.pi)
+        A = (c[None, None, :] * pos[:, :, None]).astype(np.float32)  # (n, 3, L)
+        enc = np.stack([np.sin(A.astype(dt)), np.cos(A.astype(dt))], axis=-1)
+        return enc.reshape(X.shape[0], 6 * self.L), None
+
+
 def mlp_size(E, widths):
     dims = [E] + list(widths) + [1]
     return sum(dims[i] * dims[i + 1] + dims[i + 1] for i in range(len(dims) - 1))
@@ -140,7 +168,7 @@ def vjp(theta, grid, widths, X, phase_p, seed, dt=np.float64):
     E = grid.n_features
     P = mlp_size(E, widths)
     tp, tm, tab = split(theta, grid, widths)
-    enc, (rows, wts) = grid.encode(tab, X, dt)
+    enc, aux = grid.encode(tab, X, dt)
     grad = np.zeros(theta.size, dt)
     denc = np.zeros_like(enc)
     seed = np.asarray(seed, dt)
@@ -165,6 +193,9 @@ def vjp(theta, grid, widths, X, phase_p, seed, dt=np.float64):
                 d = d * M.activation_grad(acts[li], "celu")
         denc[sel] = d
         grad[k * P:(k + 1) * P] = g
+    if not grid.rows:
+        return grad
+    rows, wts = aux
     # table: encoding (n, L, F) <- sum_c table[rows[l, :, c]] * wts[l, :, c]
     gt = np.zeros((grid.rows, grid.F), dt)
     de = denc.reshape(X.shape[0], grid.L, grid.F)
@@ -177,7 +208,8 @@ def vjp(theta, grid, widths, X, phase_p, seed, dt=np.float64):
 
 def init_params(grid, widths, seed=0, dt=np.float32):
     """Glorot-uniform MLP kernels, zero biases, table U(-1e-4, 1e-4)
-    (encoders_flax.py:118-123); numpy's generator."""
+    (encoders_flax.py:118-123; none for the frequency encoder); numpy's
+    generator."""
     rng = np.random.default_rng(seed)
     parts = []
     dims = [grid.n_features] + list(widths) + [1]
@@ -186,7 +218,8 @@ def init_params(grid, widths, seed=0, dt=np.float32):
             a, b = dims[i], dims[i + 1]
             lim = np.sqrt(6.0 / (a + b))
             parts += [rng.uniform(-lim, lim, a * b), np.zeros(b)]
-    parts.append(rng.uniform(-1e-4, 1e-4, grid.rows * grid.F))
+    if grid.rows:
+        parts.append(rng.uniform(-1e-4, 1e-4, grid.rows * grid.F))
     return np.concatenate(parts).astype(dt)
 
 

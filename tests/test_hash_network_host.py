@@ -42,7 +42,7 @@ def test_model_tree_and_flat_layout():
     assert lib.dips_param_count(ctypes.byref(_native.Net(*model.dims, 0, 0, 0.0, 2))) < 0
 
 
-@pytest.mark.parametrize("change", [{"pos_enc": "frequency"}, {"pos_enc": "spherical_harmonics"},
+@pytest.mark.parametrize("change", [{"pos_enc": "spherical_harmonics"}, {"pos_enc": "frequency", "layer_widths": [32]},
                                     {"sol_act": "sigmoid"}, {"layer_widths": [32]}, {"sol_skip_in_layers": [1]},
                                     {"pos_enc": "hashgrid", "layer_widths": [16, 16]},
                                     {"pos_enc": "hashgrid", "F": 4}, {"pos_enc": "hashgrid", "pos_levels": 17}])
@@ -67,7 +67,7 @@ def test_hashgrid_tree_and_flat_layout():
     forward, _ = get_model({"model_type": "multiresolution_hash_network", "multiresolution_hash_network": cfg},
                            "multiresolution_hash_network")
     model = forward.model
-    assert model.hashgrid == (6, 2, 2 ** 12, 4, 60, 1.5) and model.in_dim == 12
+    assert model.hashgrid == (6, 2, 2 ** 12, 4, 60, 1.5, 0) and model.in_dim == 12
     ref = HashGrid(6, 2 ** 12, 2, 4, 40, 1.5)
     tree = forward.init(3)
     tab = tree["params"]["position_encoder"][TABLE_KEY]
@@ -91,3 +91,29 @@ def test_solve_batch_rejects_hashgrid():
     model_dict = {"model_type": "multiresolution_hash_network", "multiresolution_hash_network": cfg}
     with pytest.raises(NotImplementedError):
         solve_batch([{}], num_epochs=4, model_dict=model_dict)
+
+
+def test_frequency_tree_and_flat_layout():
+    """pos_enc="frequency": FrequencyEncoder(L=10) (model.py:122-123) gives 60
+    MLP inputs and no table; the C-ABI counts the two MLPs only."""
+    from jax_dips import _native
+    from jax_dips.nn.configure import get_model
+    from oracle.hashgrid import mlp_size
+
+    cfg = dict(LPBE, pos_enc="frequency")
+    forward, _ = get_model({"model_type": "multiresolution_hash_network", "multiresolution_hash_network": cfg},
+                           "multiresolution_hash_network")
+    model = forward.model
+    assert model.hashgrid[-1] == 1 and model.in_dim == 60 and model.table_rows == 0
+    tree = forward.init(5)
+    assert set(tree["params"]) == {"sol_p", "sol_m"}
+    assert tree["params"]["sol_p"]["Dense_0"]["kernel"].shape == (60, 16)
+    flat = model.flatten(tree)
+    assert flat.size == model.n_params == 2 * mlp_size(60, [16])
+    assert np.array_equal(model.flatten(model.unflatten(flat)), flat)
+    lib = _native.lib()
+    hg = _native.HashGrid(*model.hashgrid)
+    assert lib.dips_hash_param_count(ctypes.byref(hg)) == flat.size
+    assert lib.dips_hash_table_rows(ctypes.byref(hg)) == 0
+    assert lib.dips_hash_param_count(ctypes.byref(_native.HashGrid(9, 2, 1, 1, 1, 1.0, 1))) < 0
+    assert lib.dips_hash_param_count(ctypes.byref(_native.HashGrid(10, 2, 1, 1, 1, 1.0, 2))) < 0

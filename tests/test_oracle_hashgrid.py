@@ -98,3 +98,28 @@ def test_left_of_box_saturates_to_the_first_vertex():
         assert np.allclose(enc[:, 2 * l + 1], want, rtol=1e-9, atol=1e-9)
         # the upper x corners carry no weight
         assert np.all(wts[l][:, 4:] == 0)
+
+
+def test_frequency_encoding_closed_forms_and_vjp():
+    f = HG.Frequency(10, bound=1.0)
+    assert f.n_features == 60
+    X = np.array([[-1.0, 0.0, -1.0]], np.float32)  # pos = (0, 1/2, 0)
+    enc, _ = f.encode(None, X)
+    e = enc.reshape(3, 10, 2)
+    assert np.allclose(e[0, :, 0], 0.0) and np.allclose(e[0, :, 1], 1.0)
+    want = np.sin(np.exp2(np.arange(10)) * np.float64(np.float32(np.pi)) * 0.5)
+    assert np.allclose(e[1, :, 0], want, atol=1e-6)
+    widths = [16]
+    theta = HG.init_params(f, widths, seed=3).astype(np.float64)
+    assert theta.size == 2 * HG.mlp_size(60, widths)
+    rng = np.random.default_rng(6)
+    X = rng.uniform(-0.9, 0.9, (40, 3)).astype(np.float32)
+    phase = rng.uniform(size=40) > 0.5
+    seed = rng.normal(size=40)
+    gr = HG.vjp(theta, f, widths, X, phase, seed)
+    h = 1e-5
+    for i in rng.choice(theta.size, 24, replace=False):
+        e = np.zeros_like(theta)
+        e[i] = h
+        fd = (seed @ HG.forward(theta + e, f, widths, X, phase) - seed @ HG.forward(theta - e, f, widths, X, phase)) / (2 * h)
+        assert abs(fd - gr[i]) <= 1e-5 * max(1.0, abs(gr[i])), (i, fd, gr[i])
