@@ -355,16 +355,12 @@ struct AtomicSink {
 
 struct BucketSink {
   int* cursor;  // LDS, per bucket
-  uint32_t* keys;
-  float2* vals;
-  int cap;  // records the arrays hold
+  uint4* recs;  // records (row, v0 bits, v1 bits, 0): one 16-byte store each
+  int cap;      // records the array holds
   BucketMap map;
   __device__ __forceinline__ void put(int32_t row, float v0, float v1) const {
     const int pos = atomicAdd(cursor + map.bucket(row), 1);
-    if (pos < cap) {
-      keys[pos] = (uint32_t)row;
-      vals[pos] = make_float2(v0, v1);
-    }
+    if (pos < cap) recs[pos] = make_uint4((uint32_t)row, __float_as_uint(v0), __float_as_uint(v1), 0u);
   }
 };
 
@@ -600,8 +596,7 @@ __global__ __launch_bounds__(1024) void hash_scan_total_kernel(const int* __rest
 // records summed in LDS, then every row of the bucket written (rows no record
 // touches get 0).
 constexpr int kBThreads = 1024;
-__global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint32_t* __restrict__ keys,
-                                                                const float2* __restrict__ vals,
+__global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint4* __restrict__ recs,
                                                                 const int* __restrict__ bstart, BucketMap map,
                                                                 int64_t rows, float* __restrict__ gtab) {
   __shared__ float acc[2 * kBRows];
@@ -610,8 +605,9 @@ __global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint32_t* 
   __syncthreads();
   const int e = bstart[b + 1];
   for (int i = bstart[b] + threadIdx.x; i < e; i += kBThreads) {
-    const int k = map.local(keys[i]);
-    const float2 v = vals[i];
+    const uint4 rc = recs[i];
+    const int k = map.local(rc.x);
+    const float2 v = make_float2(__uint_as_float(rc.y), __uint_as_float(rc.z));
     atomicAdd(acc + 2 * k, v.x);
     atomicAdd(acc + 2 * k + 1, v.y);
   }
@@ -636,8 +632,8 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
                                                                 float* __restrict__ partial, int nb,
                                                                 const int* __restrict__ bstart,
                                                                 const int* __restrict__ loff,
-                                                                uint32_t* __restrict__ keys,
-                                                                float2* __restrict__ vals, int cap, BucketMap map) {
+                                                                uint4* __restrict__ recs, int cap,
+                                                                BucketMap map) {
   extern __shared__ float hash_lds[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
@@ -727,7 +723,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
           }
           const bool dense = H.res[l] != 0;
           if constexpr (kBuckets) {
-            const BucketSink sink{cursor, keys, vals, cap, map};
+            const BucketSink sink{cursor, recs, cap, map};
             if (dense) {
               hash_scatter_cell<true>(sink, rows, v0, v1, live_lanes, lane);
             } else {
@@ -1065,7 +1061,8 @@ int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_
   int64_t n = max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
   int nb;
   n += max_nnz * hash_width(H);  // the forward's encodings
-  if (H.enc == 0 && hash_bucketed(H, max_nnz, &nb)) n += 3 * hash_records(H, max_nnz) + 2 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
+  if (H.enc == 0 && hash_bucketed(H, max_nnz, &nb))
+    n += 4 * hash_records(H, max_nnz) + 4 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
   return n;
 }
 
@@ -1096,8 +1093,8 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("hash step: too many parameters");
     return DIPS_EINVAL;
   }
-  // float workspace: u | r | MLP partial rows | table gradient | g | [records: keys | vals | counts | offsets |
-  // bucket totals | bucket starts]
+  // float workspace: u | r | MLP partial rows | table gradient | g | encodings | [records (16 B) | counts |
+  // offsets | bucket totals | bucket starts]
   float* u = work;
   float* r = u + nent;
   float* partial = r + nrows;
@@ -1108,11 +1105,10 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const BucketMap bmap{nb_shift};
   const int64_t nrec = buckets ? hash_records(H, nent) : 0;
   float* encbuf = gvec + P;
-  uint32_t* keys = reinterpret_cast<uint32_t*>(encbuf + nent * hash_width(H));
-  float* vbase = reinterpret_cast<float*>(keys + nrec);
-  vbase += (reinterpret_cast<uintptr_t>(vbase) / sizeof(float)) & 1;  // 8-byte alignment
-  float2* vals = reinterpret_cast<float2*>(vbase);
-  int* cnt = reinterpret_cast<int*>(vbase + 2 * nrec);
+  float* rbase = encbuf + nent * hash_width(H);
+  rbase += (4 - (reinterpret_cast<uintptr_t>(rbase) / sizeof(float)) % 4) % 4;  // 16-byte alignment
+  uint4* recs = reinterpret_cast<uint4*>(rbase);
+  int* cnt = reinterpret_cast<int*>(rbase + 4 * nrec);
   int* loff = cnt + (int64_t)nb * kHashBlocks;
   int* total = loff + (int64_t)nb * kHashBlocks;
   int* bstart = total + nb;
@@ -1172,13 +1168,13 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     hipLaunchKernelGGL(hash_scan_local_kernel, dim3(nb), dim3(kHashBlocks), 0, st, cnt, loff, total);
     hipLaunchKernelGGL(hash_scan_total_kernel, dim3(1), dim3(1024), 0, st, total, nb, bstart);
     hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
-                       bt->row_begin, r, gtab, partial, nb, bstart, loff, keys, vals, (int)nrec, bmap);
-    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, keys, vals, bstart, bmap, H.rows, gtab);
+                       bt->row_begin, r, gtab, partial, nb, bstart, loff, recs, (int)nrec, bmap);
+    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, recs, bstart, bmap, H.rows, gtab);
   } else {
     hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
                        H.rows * kHF);
     hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
-                       bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, nullptr, 0, bmap);
+                       bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, 0, bmap);
   }
   }
   float* const gout = grad_out ? grad_out : gvec;
