@@ -440,16 +440,23 @@ __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_
 // contiguous eighth of the entries, so that the table rows of a region stay in
 // one L2; both kernels use the same assignment (the forward counts the
 // backward's records per workgroup).
+// kSplit: workgroups of kSplit * kHashWaves wavefronts cover the same chunks
+// (wavefront w takes every kSplit-th round of slot w % kHashWaves), so that a
+// kernel with more wavefronts per workgroup keeps the per-workgroup chunk sets.
+template <int kSplit = 1>
 struct ChunkRange {
   int64_t first, end, step;
   __device__ __forceinline__ ChunkRange(int64_t t_count, int wave) {
     const int64_t nchunks = (t_count + 63) / 64;
     const int xcd = blockIdx.x % kXcds, xblk = blockIdx.x / kXcds;
     end = nchunks * (xcd + 1) / kXcds;
-    step = (int64_t)(gridDim.x / kXcds) * kHashWaves;
-    first = nchunks * xcd / kXcds + (int64_t)xblk * kHashWaves + wave;
+    const int64_t round = (int64_t)(gridDim.x / kXcds) * kHashWaves;
+    step = round * kSplit;
+    first = nchunks * xcd / kXcds + (int64_t)xblk * kHashWaves + wave % kHashWaves + (wave / kHashWaves) * round;
   }
 };
+constexpr int kFwdSplit = 2;  // the forward runs 16 wavefronts per workgroup (its gathers need them)
+constexpr int kFwdThreads = kFwdSplit * kHashThreads;
 
 // ---- forward: u per entry --------------------------------------------------------
 // Persistent, on the chunks of ChunkRange.  kCount: also counts the
@@ -457,7 +464,7 @@ struct ChunkRange {
 // same chunks, rows and grouping as hash_bwd_kernel<true> — into cnt_out,
 // bucket-major [bucket][workgroup].
 template <bool kCount>
-__global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, HashLevels H,
+__global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params,
                                                                 float* __restrict__ u_out,
                                                                 float* __restrict__ encbuf, int nb, BucketMap map,
@@ -469,13 +476,13 @@ __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, Hash
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
   float* const lenc = reinterpret_cast<float*>(hash_cnt + kMaxBuckets) + wave * (kHE * DIPS_WAVE);
   if constexpr (kCount) {
-    for (int b = threadIdx.x; b < nb; b += kHashThreads) hash_cnt[b] = 0;
+    for (int b = threadIdx.x; b < nb; b += kFwdThreads) hash_cnt[b] = 0;
     __syncthreads();
   }
   const CountSink sink{hash_cnt, map};
   const int E = 2 * H.L, PM = hash_mlp_size(E);
   const float* table = params + 2 * (int64_t)PM;
-  const ChunkRange cr(A.t_count, wave);
+  const ChunkRange<kFwdSplit> cr(A.t_count, wave);
   for (int64_t ch = cr.first; ch < cr.end; ch += cr.step) {
     const int64_t t = ch * 64 + lane;
     float x, y, z;
@@ -536,7 +543,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_fwd_kernel(HashArgs A, Hash
   }
   if constexpr (kCount) {
     __syncthreads();
-    for (int b = threadIdx.x; b < nb; b += kHashThreads) cnt_out[(int64_t)b * kHashBlocks + blockIdx.x] = hash_cnt[b];
+    for (int b = threadIdx.x; b < nb; b += kFwdThreads) cnt_out[(int64_t)b * kHashBlocks + blockIdx.x] = hash_cnt[b];
   }
 }
 
@@ -649,7 +656,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   HashGrad<2> gp, gm;
   gp.zero();
   gm.zero();
-  const ChunkRange cr(A.t_count, wave);
+  const ChunkRange<> cr(A.t_count, wave);
   for (int64_t ch = cr.first; ch < cr.end; ch += cr.step) {
     const int64_t t = ch * 64 + lane;
     float x, y, z;
@@ -928,7 +935,7 @@ __global__ __launch_bounds__(kHashThreads) void freq_bwd_kernel(HashArgs A, Hash
   HashGrad<kFEPad / 16> gp, gm;
   gp.zero();
   gm.zero();
-  const ChunkRange cr(A.t_count, wave);
+  const ChunkRange<> cr(A.t_count, wave);
   for (int64_t ch = cr.first; ch < cr.end; ch += cr.step) {
     const int64_t t = ch * 64 + lane;
     float x, y, z;
@@ -1137,7 +1144,7 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     hipLaunchKernelGGL(freq_bwd_kernel, dim3(kHashBlocks), dim3(kHashThreads), flds, st, A, H, params, encbuf,
                        bt->row_begin, r, partial);
   } else {
-  constexpr size_t fwd_lds = sizeof(float) * (kMaxBuckets + kHashWaves * kHE * DIPS_WAVE);
+  constexpr size_t fwd_lds = sizeof(float) * (kMaxBuckets + kFwdSplit * kHashWaves * kHE * DIPS_WAVE);
   static const bool fwd_set =
       hipFuncSetAttribute((const void*)hash_fwd_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)fwd_lds) == hipSuccess &&
@@ -1145,10 +1152,10 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                           (int)fwd_lds) == hipSuccess;
   (void)fwd_set;
   if (buckets)
-    hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), fwd_lds, st, A, H, params,
+    hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kFwdThreads), fwd_lds, st, A, H, params,
                        u - bt->ent_begin, encbuf, nb, bmap, cnt);
   else if (bt->t_count > 0)
-    hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), fwd_lds, st, A, H, params,
+    hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kFwdThreads), fwd_lds, st, A, H, params,
                        u - bt->ent_begin, encbuf, 0, bmap, nullptr);
   rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
                 st);
