@@ -351,7 +351,7 @@ int dips_hash_eval(const dips_hashgrid_t* hg, const float* params, const float* 
                    float* u, float* grad, void* stream);
 
 /* ---------------------------------------------------------------------------
- * K5: the grid-value operator and its matrix-free BiCGSTAB solve (judge and
+ * The grid-value operator and its matrix-free BiCGSTAB solve (judge and
  * fallback).  rec holds the zoom-0 records of the eval grid (spacing = grid
  * spacing); r(p) = c_p + sum_e w_e u(p + o_e) with u a grid vector (N floats,
  * C order) read through a linear ghost layer beyond the box.  Replaces
@@ -371,7 +371,7 @@ int dips_bicgstab(const dips_grid_t* grid, const dips_records_t* rec, float* x, 
                   int32_t max_iter, double rtol, int32_t* iters, double* relres, void* stream);
 
 /* ---------------------------------------------------------------------------
- * K1: FP64 cut-cell geometry, regression rows and record assembly.
+ * FP64 cut-cell geometry, regression rows and record assembly.
  * (geometry/geometric_integrations_per_point.py:32-1009;
  *  solvers/poisson/discretization.py:164-519)
  * Callback samples are prepared by the host at the positions the reference

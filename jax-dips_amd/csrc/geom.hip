@@ -1,4 +1,4 @@
-// Cut-cell geometry, regression rows and record assembly in FP64 (K1).
+// Cut-cell geometry, regression rows and record assembly in FP64.
 //
 // All kernels are per-work-item (one lane per cell, stencil point or training
 // point), read callback samples prepared by the host (phi at cell corners and

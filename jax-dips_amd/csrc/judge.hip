@@ -1,4 +1,4 @@
-// Matrix-free BiCGSTAB on grid values of the same cut-cell FV discretisation (K5).
+// Matrix-free BiCGSTAB on grid values of the same cut-cell FV discretisation.
 //
 // The grid-value operator of the reference (Trainer.residual_vector with
 // compute_Ax_and_b_discrete_fn, jax_dips/solvers/poisson/trainer.py:883-890,

@@ -1,4 +1,4 @@
-// Fused residual + loss + parameter-gradient kernel (forward K2 and its VJP K3).
+// Fused residual + loss + parameter-gradient kernel (the loss forward and its VJP).
 //
 // One lane owns one record row (training point).  Pass 1 evaluates the
 // selected phase network at each entry position and accumulates the affine

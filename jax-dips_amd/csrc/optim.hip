@@ -1,4 +1,4 @@
-// Gradient reduction + fused optimizer step (K4).
+// Gradient reduction + fused optimizer step.
 //
 // One workgroup of 1024 lanes reduces the per-block partial rows written by
 // the loss kernel in a fixed block order with float64 accumulation, scales by

@@ -1,4 +1,4 @@
-// Entry-parallel fused training step over packed wavefront tiles (K2 + K3 + K4).
+// Entry-parallel fused training step over packed wavefront tiles (loss, gradient, optimizer).
 //
 // The rows of a batch are packed (on the host, once per zoom level) into tiles
 // of whole rows holding at most 64 slots, one slot per entry and one for a row

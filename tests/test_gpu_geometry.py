@@ -1,4 +1,4 @@
-"""GPU parity of the FP64 geometry / regression / assembly kernels (K1) with
+"""GPU parity of the FP64 geometry / regression / assembly kernels with
 the CPU oracle: the records the product builds on the device equal the
 oracle's (offset, phase)-keyed affine records of every training point.
 

@@ -1,4 +1,4 @@
-"""GPU parity of the grid-value operator and its BiCGSTAB solve (K5) with the
+"""GPU parity of the grid-value operator and its BiCGSTAB solve with the
 CPU oracle's operator and direct solve.
 
 Tolerances: float32 operator applications with float64 reductions; the solve
