@@ -539,7 +539,12 @@ __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashL
     if (plus) up = hash_mlp(Wp, E, enc, act);
     asm volatile("" : "+s"(Wm));
     if (!plus) um = hash_mlp(Wm, E, enc, act);
-    if (valid) u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
+    if (valid) {
+      u_out[A.t_csr[A.t_begin + t]] = plus ? up : um;
+      // the hidden activations of the entry's network, after the encoding, for the backward
+#pragma unroll
+      for (int j = 0; j < kHW; ++j) encbuf[(int64_t)(E + j) * A.t_count + t] = act[j];
+    }
   }
   if constexpr (kCount) {
     __syncthreads();
@@ -674,18 +679,15 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
     asm volatile("" : "+s"(Praw));
     dips_weight_t* Wp = dips_weights(Praw);
     dips_weight_t* Wm = Wp + PM;
-    float act[kHW] = {};
+    // the forward's hidden activations of the entry's own network
+    float act[kHW];
+#pragma unroll
+    for (int j = 0; j < kHW; ++j) act[j] = valid ? encbuf[(int64_t)(E + j) * A.t_count + t] : 0.0f;
     float dl[kHW];
     float denc[kHE];
-    if (plus) {
-      (void)hash_mlp(Wp, E, enc, act);
-      hash_delta(Wp, E, seed, act, dl, denc);
-    }
+    if (plus) hash_delta(Wp, E, seed, act, dl, denc);
     asm volatile("" : "+s"(Wm));
-    if (!plus) {
-      (void)hash_mlp(Wm, E, enc, act);
-      hash_delta(Wm, E, seed, act, dl, denc);
-    }
+    if (!plus) hash_delta(Wm, E, seed, act, dl, denc);
     // MLP weight gradients, one network at a time (the other's lanes contribute zeros)
     if (__any(plus && valid)) {
       float d2[kHW];
@@ -1067,7 +1069,7 @@ int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_
   if (doubles) *doubles = rowsum_blocks(max_rows) + kRedBlocks + 2;
   int64_t n = max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
   int nb;
-  n += max_nnz * hash_width(H);  // the forward's encodings
+  n += max_nnz * (hash_width(H) + (H.enc == 0 ? kHW : 0));  // the forward's encodings (and activations)
   if (H.enc == 0 && hash_bucketed(H, max_nnz, &nb))
     n += 4 * hash_records(H, max_nnz) + 4 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
   return n;
@@ -1112,7 +1114,7 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const BucketMap bmap{nb_shift};
   const int64_t nrec = buckets ? hash_records(H, nent) : 0;
   float* encbuf = gvec + P;
-  float* rbase = encbuf + nent * hash_width(H);
+  float* rbase = encbuf + nent * (hash_width(H) + (H.enc == 0 ? kHW : 0));
   rbase += (4 - (reinterpret_cast<uintptr_t>(rbase) / sizeof(float)) % 4) % 4;  // 16-byte alignment
   uint4* recs = reinterpret_cast<uint4*>(rbase);
   int* cnt = reinterpret_cast<int*>(rbase + 4 * nrec);
