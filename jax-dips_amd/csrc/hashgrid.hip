@@ -41,6 +41,8 @@ constexpr int kHashBlocks = 256;      // forward / backward workgroups (persiste
 constexpr int kXcds = 8;              // accelerator dies (L2 domains) of the GPU
 constexpr int kHStride = 68;          // staged operand row: 64 entries + 4
 constexpr int kHStage = 2 * 16 * kHStride;
+// the backward's staging area holds d enc (kHE rows of a wavefront) and the duplicate-position slots
+static_assert(kHE * DIPS_WAVE + DIPS_WAVE <= kHStage, "staging area too small");
 
 struct HashLevels {
   int L;    // levels (hashgrid) or frequencies (frequency encoding)
@@ -435,6 +437,34 @@ __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_
   }
 }
 
+// Lanes of live at bit-identical positions (a stencil neighbour of one point is
+// often the centre of the next) share every corner row and weight on every
+// level, so their table-gradient values can be summed before the scatter.
+// Each live lane hashes its position to one of 64 slots of `slot` (64 ints of
+// the wavefront's LDS); the slot keeps the lowest lane (LDS atomic min, so the
+// grouping is the same in the counting pass and the backward).  A lane folds
+// into that lane, rep, when rep's position is bit-identical to its own.
+// Returns the mask of folding lanes; rep is the lane itself for every other
+// lane.  Wave-uniform call.
+__device__ __forceinline__ uint64_t hash_dups(int* slot, float x, float y, float z, uint64_t live, int lane,
+                                              int& rep) {
+  slot[lane] = DIPS_WAVE;
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t xb = __float_as_uint(x), yb = __float_as_uint(y), zb = __float_as_uint(z);
+  const int key = (int)(((xb * 0x9E3779B1u) ^ (yb * 0x85EBCA77u) ^ (zb * 0xC2B2AE3Du)) >> 26);
+  const bool in = (live >> lane) & 1;
+  if (in) atomicMin(slot + key, lane);
+  __builtin_amdgcn_wave_barrier();
+  const int r = slot[key];
+  rep = (in && r < DIPS_WAVE) ? r : lane;
+  const uint32_t rx = __float_as_uint(__shfl(x, rep)), ry = __float_as_uint(__shfl(y, rep)),
+                 rz = __float_as_uint(__shfl(z, rep));
+  const bool fold = in && rep != lane && rx == xb && ry == yb && rz == zb;
+  if (!fold) rep = lane;
+  __builtin_amdgcn_wave_barrier();
+  return __ballot(fold);
+}
+
 // The 64-entry chunks a wavefront of the persistent forward / backward grid
 // takes: each XCD (workgroups blockIdx % 8, dispatched round-robin) takes a
 // contiguous eighth of the entries, so that the table rows of a region stay in
@@ -488,7 +518,14 @@ __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashL
     float x, y, z;
     int row;
     const bool valid = hash_entry(A, t, x, y, z, row);
-    const uint64_t live_lanes = __ballot(valid);
+    uint64_t live_lanes = __ballot(valid);
+    if constexpr (kCount) {
+      // the backward's records: lanes at a position another lane holds send none
+      if (live_lanes) {
+        int rep;
+        live_lanes &= ~hash_dups(reinterpret_cast<int*>(lenc), x, y, z, live_lanes, lane, rep);
+      }
+    }
     float p[3];
     hash_pos(H, x, y, z, p);
 #pragma unroll 1
@@ -705,14 +742,26 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
     }
     // table gradient: scatter d enc to the corners (wave-uniform from here on;
     // the bucket records of every valid entry were counted, zero seeds included)
-    const uint64_t live_lanes = __ballot(kBuckets ? valid : (valid && seed != 0.0f));
+    uint64_t live_lanes = __ballot(kBuckets ? valid : (valid && seed != 0.0f));
     if (live_lanes) {
       // d enc through the wavefront's (now free) staging area, so that the level
       // loop need not be unrolled (an unrolled loop overflows the instruction cache)
       __builtin_amdgcn_wave_barrier();
+      int rep;
+      const uint64_t folded = hash_dups(reinterpret_cast<int*>(AT + kHE * DIPS_WAVE), x, y, z, live_lanes, lane, rep);
 #pragma unroll
       for (int k = 0; k < kHE; ++k) AT[k * DIPS_WAVE + lane] = denc[k];
       __builtin_amdgcn_wave_barrier();
+      if (folded) {
+        // lanes at a position another lane holds add their d enc to that lane's
+        if ((folded >> lane) & 1) {
+#pragma unroll
+          for (int k = 0; k < kHE; ++k)
+            if (k < E) atomicAdd(AT + k * DIPS_WAVE + rep, denc[k]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        live_lanes &= ~folded;
+      }
 #pragma unroll 1
       for (int l = 0; l < H.L; ++l) {
         {
