@@ -451,7 +451,7 @@ def run_hashgrid(args):
     W = max(4, args.warmup - args.warmup % 4)
     eng = HashEngine(model.hashgrid, theta, opt, max_steps=W + K + 8)
     for rec in levels:
-        eng.workspace(rec)
+        eng.workspace(rec, grid)
     L = N.lib()
     stream = N.current_stream()
     inv_b = ctypes.c_float(1.0 / B)
@@ -460,7 +460,7 @@ def run_hashgrid(args):
         per = k // 4
         ents = 0
         for rec in levels:
-            mb = rec.mfma_batches()["batches"]
+            mb = rec.mfma_batches(grid)["batches"]
             for i in range(per):
                 bt = mb[i % nb]
                 ents += bt.t_count

@@ -181,13 +181,21 @@ class DeviceRecords:
         self.row_coeffs = C[self.row_point.to(torch.int64)].contiguous()
         self._finish()
 
-    def mfma_batches(self):
+    def mfma_batches(self, grid=None):
         """Phase-sorted entry arrays (+ entries first inside each batch) and one
-        dips_mfma_batch_t per batch, for the MFMA kernels (built once, cached)."""
+        dips_mfma_batch_t per batch, for the MFMA and hashgrid kernels (built
+        once per ordering, cached).  grid (DeviceGrid): the entries of each
+        (batch, phase) segment are also ordered by their position fl32(p + o h),
+        the position the kernels compute, so that entries at one position (a
+        stencil neighbour of one point that rounds onto the next point) are
+        adjacent; the hashgrid backward then sends one set of table-gradient
+        records for them.  Any order inside a segment is valid: t_csr maps each
+        entry to its CSR slot."""
         import torch
 
-        if getattr(self, "_mfma", None) is not None:
-            return self._mfma
+        cache = "_mfma" if grid is None else "_mfma_pos"
+        if getattr(self, cache, None) is not None:
+            return getattr(self, cache)
         dev = self.row_ptr.device
         ptr = self.row_ptr.to(torch.int64)
         counts = ptr[1:] - ptr[:-1]
@@ -197,7 +205,23 @@ class DeviceRecords:
         phase = (code >> 7) & 1
         batch = ent_row // self.batch_size
         key = batch * 2 + (1 - phase)
-        _, perm = torch.sort(key, stable=True)
+        if grid is None:
+            _, perm = torch.sort(key, stable=True)
+        else:
+            # offsets o in [-2, 2]^3 from bits 0..6 of the code (25 (oz+2) + 5 (oy+2) + (ox+2))
+            c = code & 127
+            o = torch.stack([c % 5 - 2, (c // 5) % 5 - 2, c // 25 - 2], dim=1).to(torch.float32)
+            h = torch.as_tensor(self.h, dtype=torch.float32, device=dev)
+            # o h is exact for |o| <= 2, so the sum rounds once, as the kernels' fused form does
+            X = grid.points()[self.row_point.to(torch.int64)[ent_row]] + o * h
+            del o, c
+            perm = torch.arange(n, device=dev)
+            for col in (0, 1, 2):  # least significant key first: x, y, z, then the segment
+                _, p2 = torch.sort(X[perm, col], stable=True)
+                perm = perm[p2]
+            del X
+            _, p2 = torch.sort(key[perm], stable=True)
+            perm = perm[p2]
         t_row = ent_row[perm].to(torch.int32).contiguous()
         t_code = self.ent_code[:n][perm].contiguous()
         t_w = self.ent_w[:n][perm].contiguous()
@@ -209,10 +233,11 @@ class DeviceRecords:
             e0, e1 = int(ptr_h[b0]), int(ptr_h[b0 + nr])
             batches.append(N.MfmaBatch(int(b0), int(nr), e0, e1, t_row.data_ptr(), t_code.data_ptr(), t_w.data_ptr(),
                                        t_csr.data_ptr(), e0, int(nplus[b]), int(e1 - e0 - nplus[b]), int(e1 - e0)))
-        self._mfma = dict(t_row=t_row, t_code=t_code, t_w=t_w, t_csr=t_csr, batches=batches,
-                          max_nnz=int(max(int(ptr_h[b0 + nr] - ptr_h[b0]) for b0, nr in
-                                          zip(self.batch_begin, self.batch_rows))))
-        return self._mfma
+        mb = dict(t_row=t_row, t_code=t_code, t_w=t_w, t_csr=t_csr, batches=batches,
+                  max_nnz=int(max(int(ptr_h[b0 + nr] - ptr_h[b0]) for b0, nr in
+                                  zip(self.batch_begin, self.batch_rows))))
+        setattr(self, cache, mb)
+        return mb
 
     def tiles(self, grid):
         """Wavefront tiles of every batch (rows packed into tiles of at most 64
@@ -428,7 +453,8 @@ class Engine:
 class HashEngine:
     """Parameters, optimizer state and workspace of the hashgrid network on the
     GPU (dips_hash_*): the two CELU MLPs and the encoder's table, trained over
-    the phase-sorted entries of each batch (DeviceRecords.mfma_batches)."""
+    the phase-sorted entries of each batch, ordered by position inside each
+    phase (DeviceRecords.mfma_batches with the grid)."""
 
     def __init__(self, hashgrid, params, optimizer_dict, max_steps, device="cuda"):
         """hashgrid: (levels, features, table_size, n_min, n_max, bound, encoder)
@@ -453,10 +479,12 @@ class HashEngine:
         self.opt = make_opt(optimizer_dict)
         self._work = None
 
-    def workspace(self, rec):
+    def workspace(self, rec, grid):
+        """Entry arrays of rec ordered by position inside each (batch, phase)
+        segment, and a workspace large enough for its batches."""
         import torch
 
-        mb = rec.mfma_batches()
+        mb = rec.mfma_batches(grid)
         dbl = ctypes.c_int64(0)
         need = int(N.lib().dips_hash_workspace(ctypes.byref(self.hg), int(rec.batch_size), mb["max_nnz"],
                                                ctypes.byref(dbl)))
@@ -471,7 +499,7 @@ class HashEngine:
         """(loss, grad) of batch b, float64 numpy."""
         import torch
 
-        mb = self.workspace(rec)
+        mb = self.workspace(rec, grid)
         g = torch.empty(self.P, dtype=torch.float32, device=self.device)
         loss = torch.empty(1, dtype=torch.float64, device=self.device)
         N.check(N.lib().dips_hash_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb["batches"][b]),
@@ -483,7 +511,7 @@ class HashEngine:
         return float(loss.item()), g.double().cpu().numpy()
 
     def train_level(self, grid, rec, nepochs):
-        mb = self.workspace(rec)
+        mb = self.workspace(rec, grid)
         L = N.lib()
         inv_b = ctypes.c_float(1.0 / rec.batch_size)
         for _ in range(int(nepochs)):

@@ -117,3 +117,55 @@ def test_frequency_tree_and_flat_layout():
     assert lib.dips_hash_table_rows(ctypes.byref(hg)) == 0
     assert lib.dips_hash_param_count(ctypes.byref(_native.HashGrid(9, 2, 1, 1, 1, 1.0, 1))) < 0
     assert lib.dips_hash_param_count(ctypes.byref(_native.HashGrid(10, 2, 1, 1, 1, 1.0, 2))) < 0
+
+
+def test_entries_ordered_by_position():
+    """DeviceRecords.mfma_batches(grid): a permutation that keeps every entry in
+    its (batch, phase) segment (+ entries first) and puts bit-identical
+    positions fl32(p + o h) next to each other; without the grid the order is
+    the phase sort alone.  Host tensors, no GPU."""
+    import torch
+    from jax_dips.records import DeviceGrid, DeviceRecords
+
+    rng = np.random.default_rng(3)
+    n = 6
+    xs = np.linspace(-1, 1, n, dtype=np.float32)
+    grid = DeviceGrid(xs, xs, xs, extra=rng.uniform(-1, 1, (5, 3)), device="cpu")
+    h = np.full(3, xs[1] - xs[0], np.float32)
+    nrows, B = 40, 16
+    row_point = rng.permutation(grid.npts)[:nrows].astype(np.int32)
+    counts = rng.integers(1, 9, nrows)
+    row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    nnz = int(row_ptr[-1])
+    off = rng.integers(-1, 2, (nnz, 3))
+    code = 25 * (off[:, 2] + 2) + 5 * (off[:, 1] + 2) + (off[:, 0] + 2) + 128 * rng.integers(0, 2, nnz)
+    bb = np.arange(0, nrows, B)
+    br = np.minimum(B, nrows - bb)
+    T = lambda a, dt: torch.as_tensor(np.asarray(a), dtype=dt)
+    rec = DeviceRecords.from_device(T(row_point, torch.int32), T(row_ptr, torch.int32),
+                                    T(np.zeros(nrows), torch.float32), T(code, torch.int32),
+                                    T(rng.standard_normal(nnz), torch.float32), nnz, h, B, bb, br, br)
+    plain = rec.mfma_batches()
+    mb = rec.mfma_batches(grid)
+    perm = mb["t_csr"].numpy().astype(np.int64)
+    assert np.array_equal(np.sort(perm), np.arange(nnz))
+    ent_row = np.repeat(np.arange(nrows), counts)
+    assert np.array_equal(mb["t_row"].numpy(), ent_row[perm])
+    assert np.array_equal(mb["t_code"].numpy(), code[perm])
+    # the segments: batch, then + entries before - entries, as in the plain phase sort
+    seg = lambda p: (ent_row[p] // B) * 2 + (1 - (code[p] >> 7) & 1)
+    assert np.array_equal(seg(perm), seg(plain["t_csr"].numpy().astype(np.int64)))
+    assert np.all(np.diff(seg(perm)) >= 0)
+    for a, b in zip(plain["batches"], mb["batches"]):
+        assert (a.t_count, a.n_plus) == (b.t_count, b.n_plus)
+    # positions as the kernels compute them; equal positions of a segment are adjacent
+    pts = grid.points().numpy()
+    X = pts[row_point[ent_row[perm]]] + off[perm].astype(np.float32) * h
+    keys = [(s,) + tuple(x.view(np.uint32)) for s, x in zip(seg(perm), X.astype(np.float32))]
+    seen, last = set(), None
+    for k in keys:
+        if k != last:
+            assert k not in seen, "a position recurs after another one"
+            seen.add(k)
+            last = k
+    assert len(seen) < nnz  # the test data has repeated positions
