@@ -41,8 +41,8 @@ constexpr int kHashBlocks = 256;      // forward / backward workgroups (persiste
 constexpr int kXcds = 8;              // accelerator dies (L2 domains) of the GPU
 constexpr int kHStride = 68;          // staged operand row: 64 entries + 4
 constexpr int kHStage = 2 * 16 * kHStride;
-// the backward's staging area holds d enc (kHE rows of a wavefront) and the duplicate-position slots
-static_assert(kHE * DIPS_WAVE + DIPS_WAVE <= kHStage, "staging area too small");
+// the backward's staging area holds d enc (kHE rows of a wavefront)
+static_assert(kHE * DIPS_WAVE <= kHStage, "staging area too small");
 
 struct HashLevels {
   int L;    // levels (hashgrid) or frequencies (frequency encoding)
@@ -437,32 +437,26 @@ __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_
   }
 }
 
-// Lanes of live at bit-identical positions (a stencil neighbour of one point is
-// often the centre of the next) share every corner row and weight on every
-// level, so their table-gradient values can be summed before the scatter.
-// Each live lane hashes its position to one of 64 slots of `slot` (64 ints of
-// the wavefront's LDS); the slot keeps the lowest lane (LDS atomic min, so the
-// grouping is the same in the counting pass and the backward).  A lane folds
-// into that lane, rep, when rep's position is bit-identical to its own.
-// Returns the mask of folding lanes; rep is the lane itself for every other
-// lane.  Wave-uniform call.
-__device__ __forceinline__ uint64_t hash_dups(int* slot, float x, float y, float z, uint64_t live, int lane,
-                                              int& rep) {
-  slot[lane] = DIPS_WAVE;
-  __builtin_amdgcn_wave_barrier();
-  const uint32_t xb = __float_as_uint(x), yb = __float_as_uint(y), zb = __float_as_uint(z);
-  const int key = (int)(((xb * 0x9E3779B1u) ^ (yb * 0x85EBCA77u) ^ (zb * 0xC2B2AE3Du)) >> 26);
-  const bool in = (live >> lane) & 1;
-  if (in) atomicMin(slot + key, lane);
-  __builtin_amdgcn_wave_barrier();
-  const int r = slot[key];
-  rep = (in && r < DIPS_WAVE) ? r : lane;
-  const uint32_t rx = __float_as_uint(__shfl(x, rep)), ry = __float_as_uint(__shfl(y, rep)),
-                 rz = __float_as_uint(__shfl(z, rep));
-  const bool fold = in && rep != lane && rx == xb && ry == yb && rz == zb;
-  if (!fold) rep = lane;
-  __builtin_amdgcn_wave_barrier();
-  return __ballot(fold);
+// Lanes of live at bit-identical positions (a stencil neighbour of one point
+// that rounds onto the next point; the host orders each batch's entries by
+// position, so such entries are adjacent) share every corner row and weight on
+// every level, so their table-gradient values can be summed before the scatter.
+// A live lane whose position equals the previous lane's (also live) folds into
+// the first lane of that run, rep; rep is the lane itself for every other lane.
+// Returns the mask of folding lanes.  The grouping depends on the positions and
+// the mask only, so the counting pass and the backward see the same records.
+// Wave-uniform call.
+__device__ __forceinline__ uint64_t hash_dups(float x, float y, float z, uint64_t live, int lane, int& rep) {
+  const int prev = lane > 0 ? lane - 1 : 0;
+  const uint32_t px = __float_as_uint(__shfl(x, prev)), py = __float_as_uint(__shfl(y, prev)),
+                 pz = __float_as_uint(__shfl(z, prev));
+  const bool fold = lane > 0 && ((live >> lane) & 1) && ((live >> prev) & 1) && px == __float_as_uint(x) &&
+                    py == __float_as_uint(y) && pz == __float_as_uint(z);
+  const uint64_t folded = __ballot(fold);
+  // the last run start at or below this lane (lane 0 always starts one)
+  const uint64_t upto = ~folded & (lane == DIPS_WAVE - 1 ? ~0ull : ((1ull << (lane + 1)) - 1));
+  rep = DIPS_WAVE - 1 - __builtin_clzll(upto);
+  return folded;
 }
 
 // The 64-entry chunks a wavefront of the persistent forward / backward grid
@@ -523,7 +517,7 @@ __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashL
       // the backward's records: lanes at a position another lane holds send none
       if (live_lanes) {
         int rep;
-        live_lanes &= ~hash_dups(reinterpret_cast<int*>(lenc), x, y, z, live_lanes, lane, rep);
+        live_lanes &= ~hash_dups(x, y, z, live_lanes, lane, rep);
       }
     }
     float p[3];
@@ -748,7 +742,7 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
       // loop need not be unrolled (an unrolled loop overflows the instruction cache)
       __builtin_amdgcn_wave_barrier();
       int rep;
-      const uint64_t folded = hash_dups(reinterpret_cast<int*>(AT + kHE * DIPS_WAVE), x, y, z, live_lanes, lane, rep);
+      const uint64_t folded = hash_dups(x, y, z, live_lanes, lane, rep);
 #pragma unroll
       for (int k = 0; k < kHE; ++k) AT[k * DIPS_WAVE + lane] = denc[k];
       __builtin_amdgcn_wave_barrier();
