@@ -313,7 +313,10 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
  * width 16 and a linear output (blocks/nerfs_flax.py:142-170).
  * Flat parameters: the + MLP, the - MLP (W0[E][16], b0, Wout[16], bout with
  * E = levels * features), then the table (rows x features).  Entries are the
- * phase-sorted arrays of dips_mfma_batch_t.  The table gradient is summed per
+ * phase-sorted arrays of dips_mfma_batch_t; inside each (batch, phase) segment
+ * any order is valid (t_csr names each entry's slot), and an order by position
+ * is faster: entries of one wavefront's 64 at bit-identical positions, adjacent
+ * in that order, send one set of table-gradient records.  The table gradient is summed per
  * bucket of table rows in an order that is not fixed (results vary in the last
  * bits); tables above 2^25 rows use atomic adds.
  * ------------------------------------------------------------------------- */
