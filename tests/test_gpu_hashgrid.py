@@ -65,6 +65,52 @@ def test_loss_grad_matches_oracle(cfg, level, table_scale, B, b):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("cfg,level", [(DEFAULT, 0), (SMALL, 1)])
+def test_entry_order_invariance(cfg, level):
+    """The loss and gradient do not depend on the entry order inside a (batch,
+    phase) segment: the phase sort alone (entries at one position rarely
+    adjacent) and the order by position (such entries adjacent; the kernels
+    send one set of table-gradient records for each run of them) agree with
+    each other and with the oracle.  Zoom level 0 of a float32 linspace grid
+    puts most stencil neighbours bit-exactly on the next point."""
+    import ctypes
+
+    import torch
+
+    from jax_dips import _native as N
+    from jax_dips.records import DeviceGrid, DeviceRecords, HashEngine, pack_dense
+
+    prob = problems.sphere(mu_m=1.0, mu_p=10.0)
+    axes, P, h, bounds, cb, rec = oracle_level(prob, 16, level)
+    model, theta, grid, n_mlp = _model(cfg, 5, 1e4)
+    B, b = 2048, 0
+    eng = HashEngine(model.hashgrid, theta, {}, max_steps=2)
+    dgrid, drec = DeviceGrid(*axes), DeviceRecords(pack_dense(rec.W, rec.c, B), rec.h)
+    eng.workspace(drec, dgrid)
+    out = {}
+    for name, mb in (("phase", drec.mfma_batches()), ("position", drec.mfma_batches(dgrid))):
+        print(f"hashgrid {cfg} level {level}: {name} order", flush=True)
+        g = torch.empty(eng.P, dtype=torch.float32, device="cuda")
+        loss = torch.empty(1, dtype=torch.float64, device="cuda")
+        N.check(N.lib().dips_hash_loss_grad(ctypes.byref(dgrid.c), ctypes.byref(drec.c), ctypes.byref(mb["batches"][b]),
+                                            ctypes.byref(eng.hg), ctypes.c_float(1.0 / B), N.ptr(eng.params), N.ptr(g),
+                                            N.ptr(loss), N.ptr(eng._work), int(eng._work.numel()), N.ptr(eng._dwork),
+                                            int(eng._dwork.numel()), N.current_stream()), "dips_hash_loss_grad")
+        torch.cuda.synchronize()
+        out[name] = (float(loss.item()), g.double().cpu().numpy())
+    assert not np.array_equal(drec.mfma_batches()["t_csr"].cpu().numpy(),
+                              drec.mfma_batches(dgrid)["t_csr"].cpu().numpy())
+    (l0, g0), (l1, g1) = out["phase"], out["position"]
+    rows = np.arange(b * B, (b + 1) * B)
+    ol, og, _ = HG.loss_and_grad(theta.astype(np.float64), grid, [16], rec, rows)
+    print(f"  loss {l0:.9e} / {l1:.9e} oracle {ol:.9e}; grad rel {_rel(g1, g0):.2e}")
+    assert abs(l0 - l1) <= 1e-6 * abs(ol) and abs(l1 - ol) <= 1e-5 * abs(ol)
+    assert _rel(g1[:n_mlp], g0[:n_mlp]) <= 1e-5 and _rel(g1[n_mlp:], g0[n_mlp:]) <= 1e-5
+    assert _rel(g1[:n_mlp], og[:n_mlp]) <= 1e-4 and _rel(g1[n_mlp:], og[n_mlp:]) <= 1e-4
+    assert np.array_equal(g1[n_mlp:] == 0, og[n_mlp:] == 0)
+
+
+@pytest.mark.gpu
 def test_large_table_atomic_path_matches_oracle():
     """A table above 2^25 rows (a dense level of 256^3 rows and a hashed level of
     T = 2^25 rows) takes the atomic-add table gradient instead of the buckets."""
