@@ -10,20 +10,21 @@
 // encoding is level-major (l * 2 + f).  Network (model.py:53-94,
 // blocks/nerfs_flax.py:142-170): per phase W0[E][16], b0, CELU, Wout[16], bout.
 //
-// Step: forward (one lane per phase-sorted entry: encoding + MLP; it also
-// counts the backward's table-gradient records per bucket and workgroup) ->
-// row sums (the MFMA path's kernel) -> two scans of the counts -> backward (one
+// Step: forward (one lane per phase-sorted entry: encoding + MLP, both kept
+// for the backward) -> row sums (the MFMA path's kernel) -> backward (one
 // wavefront per 64 entries: the MLP weight gradient W0 = enc^T delta is a
 // K = 64 contraction on the matrix cores through LDS staging, biases and
 // output weights per lane; the table gradient d enc = W0 delta goes to the 8
-// corners of every level as (row, value) records in buckets of 8192 rows) ->
-// bucket sums (every table row written) -> reduction (fixed order for the
+// corners of every level as (row, value) records in buckets of 8192 rows, one
+// fixed-capacity slot per workgroup and bucket) -> bucket sums (every table
+// row written) -> reduction (fixed order for the
 // MLPs; float64 norm partials) -> Adam over every parameter, the table's
 // gradient read in place.  Tables above 2^25 rows use atomic adds instead of
 // the buckets.
 #include <math.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "dips_common.h"
 #include "dips_hip.h"
@@ -327,9 +328,8 @@ struct HashGrad {
 };
 
 // Table-gradient sinks of hash_scatter.  Atomic: adds into the table gradient.
-// Buckets: appends (row, v0, v1) to the row's bucket at the workgroup's LDS
-// cursor (bucket-major record array, positions counted beforehand).  Count:
-// counts the records the bucket sink will append.
+// Buckets: appends (row, v0, v1) to the workgroup's slot of the row's bucket at
+// the workgroup's LDS cursor.
 //
 // Buckets are block-cyclic over groups of 64 consecutive rows: group G = row >>
 // 6 belongs to bucket G mod nb (nb a power of two), at local rows ((G / nb) <<
@@ -355,31 +355,35 @@ struct AtomicSink {
   }
 };
 
+// Records of one workgroup: bucket b's slot holds records [b * cap, (b + 1) * cap)
+// of `slots`; records past a slot's capacity go to gtab by atomic adds (gtab is
+// zeroed before the backward and the bucket sums add to it).
 struct BucketSink {
-  int* cursor;  // LDS, per bucket
-  uint4* recs;  // records (row, v0 bits, v1 bits, 0): one 16-byte store each
-  int cap;      // records the array holds
+  int* cursor;   // LDS, per bucket: records this workgroup has sent to it
+  uint4* slots;  // records (row, v0 bits, v1 bits, 0): one 16-byte store each
+  int cap;       // records a slot holds
   BucketMap map;
+  float* gtab;
   __device__ __forceinline__ void put(int32_t row, float v0, float v1) const {
-    const int pos = atomicAdd(cursor + map.bucket(row), 1);
-    if (pos < cap) recs[pos] = make_uint4((uint32_t)row, __float_as_uint(v0), __float_as_uint(v1), 0u);
+    const int b = map.bucket(row);
+    const int pos = atomicAdd(cursor + b, 1);
+    if (pos < cap) {
+      slots[(int64_t)b * cap + pos] = make_uint4((uint32_t)row, __float_as_uint(v0), __float_as_uint(v1), 0u);
+    } else {
+      unsafeAtomicAdd(gtab + 2 * (int64_t)row, v0);
+      unsafeAtomicAdd(gtab + 2 * (int64_t)row + 1, v1);
+    }
   }
 };
 
-struct CountSink {
-  int* cnt;  // LDS, per bucket
-  BucketMap map;
-  __device__ __forceinline__ void put(int32_t row, float, float) const { atomicAdd(cnt + map.bucket(row), 1); }
-};
 
 // Sends (v0, v1) of every lane in live for table row `row` to the sink.  Lanes
 // that share a row are summed over the wavefront first: up to kAggIters rows of
 // at least kAggMin lanes each, the other lanes sending their own values.
-// Wave-uniform call; the grouping depends on the rows and the mask only, so
-// that a counting pass (kValues false) sees the same records.
+// Wave-uniform call.
 constexpr int kAggIters = 8;
 constexpr int kAggMin = 4;
-template <bool kValues, typename Sink>
+template <typename Sink>
 __device__ __forceinline__ void hash_scatter(const Sink& sink, int32_t row, float v0, float v1, uint64_t live,
                                              int lane) {
   uint64_t rem = live;
@@ -389,12 +393,9 @@ __device__ __forceinline__ void hash_scatter(const Sink& sink, int32_t row, floa
     const int32_t key = __builtin_amdgcn_readlane(row, leader);
     const uint64_t same = rem & __ballot(row == key);
     if (__builtin_popcountll(same) < kAggMin) break;
-    float s0 = 0.0f, s1 = 0.0f;
-    if constexpr (kValues) {
-      const bool in = (same >> lane) & 1;
-      s0 = dips_wave_sum(in ? v0 : 0.0f);
-      s1 = dips_wave_sum(in ? v1 : 0.0f);
-    }
+    const bool in = (same >> lane) & 1;
+    const float s0 = dips_wave_sum(in ? v0 : 0.0f);
+    const float s1 = dips_wave_sum(in ? v1 : 0.0f);
     if (lane == leader) sink.put(key, s0, s1);
     rem &= ~same;
   }
@@ -404,8 +405,8 @@ __device__ __forceinline__ void hash_scatter(const Sink& sink, int32_t row, floa
 // The 8 corner records of every lane in live on a dense (coarse) level, where a
 // wavefront's entries fall into a few cells: lanes whose 8 corner rows all
 // equal are summed over the wavefront and their leader sends the 8 records.
-// Wave-uniform call; the grouping depends on the rows and the mask only.
-template <bool kValues, typename Sink>
+// Wave-uniform call.
+template <typename Sink>
 __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_t (&rows)[8], const float (&v0)[8],
                                                   const float (&v1)[8], uint64_t live, int lane) {
   uint64_t rem = live;
@@ -425,11 +426,8 @@ __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_
       const bool in = (same >> lane) & 1;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        float s0 = 0.0f, s1 = 0.0f;
-        if constexpr (kValues) {
-          s0 = dips_wave_sum(in ? v0[c] : 0.0f);
-          s1 = dips_wave_sum(in ? v1[c] : 0.0f);
-        }
+        const float s0 = dips_wave_sum(in ? v0[c] : 0.0f);
+        const float s1 = dips_wave_sum(in ? v1[c] : 0.0f);
         if (lane == leader) sink.put(rows[c], s0, s1);
       }
     }
@@ -443,9 +441,7 @@ __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_
 // every level, so their table-gradient values can be summed before the scatter.
 // A live lane whose position equals the previous lane's (also live) folds into
 // the first lane of that run, rep; rep is the lane itself for every other lane.
-// Returns the mask of folding lanes.  The grouping depends on the positions and
-// the mask only, so the counting pass and the backward see the same records.
-// Wave-uniform call.
+// Returns the mask of folding lanes.  Wave-uniform call.
 __device__ __forceinline__ uint64_t hash_dups(float x, float y, float z, uint64_t live, int lane, int& rep) {
   const int prev = lane > 0 ? lane - 1 : 0;
   const uint32_t px = __float_as_uint(__shfl(x, prev)), py = __float_as_uint(__shfl(y, prev)),
@@ -462,8 +458,8 @@ __device__ __forceinline__ uint64_t hash_dups(float x, float y, float z, uint64_
 // The 64-entry chunks a wavefront of the persistent forward / backward grid
 // takes: each XCD (workgroups blockIdx % 8, dispatched round-robin) takes a
 // contiguous eighth of the entries, so that the table rows of a region stay in
-// one L2; both kernels use the same assignment (the forward counts the
-// backward's records per workgroup).
+// one L2; both kernels use the same assignment (the backward reads what the
+// forward stored for the same entries from the same XCD's L2).
 // kSplit: workgroups of kSplit * kHashWaves wavefronts cover the same chunks
 // (wavefront w takes every kSplit-th round of slot w % kHashWaves), so that a
 // kernel with more wavefronts per workgroup keeps the per-workgroup chunk sets.
@@ -483,27 +479,17 @@ constexpr int kFwdSplit = 2;  // the forward runs 16 wavefronts per workgroup (i
 constexpr int kFwdThreads = kFwdSplit * kHashThreads;
 
 // ---- forward: u per entry --------------------------------------------------------
-// Persistent, on the chunks of ChunkRange.  kCount: also counts the
-// table-gradient records of the bucketed backward per (bucket, workgroup) — the
-// same chunks, rows and grouping as hash_bwd_kernel<true> — into cnt_out,
-// bucket-major [bucket][workgroup].
-template <bool kCount>
+// Persistent, on the chunks of ChunkRange.
 __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params,
                                                                 float* __restrict__ u_out,
-                                                                float* __restrict__ encbuf, int nb, BucketMap map,
-                                                                int* __restrict__ cnt_out) {
-  // LDS: the per-bucket counts (kCount), then each wavefront's encodings
-  // (level loop not unrolled: an unrolled one overflows the instruction cache)
-  extern __shared__ int hash_cnt[];
+                                                                float* __restrict__ encbuf) {
+  // LDS: each wavefront's encodings (level loop not unrolled: an unrolled one
+  // overflows the instruction cache)
+  extern __shared__ float hash_lenc[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
-  float* const lenc = reinterpret_cast<float*>(hash_cnt + kMaxBuckets) + wave * (kHE * DIPS_WAVE);
-  if constexpr (kCount) {
-    for (int b = threadIdx.x; b < nb; b += kFwdThreads) hash_cnt[b] = 0;
-    __syncthreads();
-  }
-  const CountSink sink{hash_cnt, map};
+  float* const lenc = hash_lenc + wave * (kHE * DIPS_WAVE);
   const int E = 2 * H.L, PM = hash_mlp_size(E);
   const float* table = params + 2 * (int64_t)PM;
   const ChunkRange<kFwdSplit> cr(A.t_count, wave);
@@ -512,14 +498,6 @@ __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashL
     float x, y, z;
     int row;
     const bool valid = hash_entry(A, t, x, y, z, row);
-    uint64_t live_lanes = __ballot(valid);
-    if constexpr (kCount) {
-      // the backward's records: lanes at a position another lane holds send none
-      if (live_lanes) {
-        int rep;
-        live_lanes &= ~hash_dups(x, y, z, live_lanes, lane, rep);
-      }
-    }
     float p[3];
     hash_pos(H, x, y, z, p);
 #pragma unroll 1
@@ -528,13 +506,11 @@ __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashL
       float frac[3];
       bool live[3];
       hash_cell(H, l, p, lo, hi, frac, live);
-      int32_t rows[8];
       float e0 = 0.0f, e1 = 0.0f;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        rows[c] = (int32_t)hash_row(H, l, lo, hi, c);
         const float w = hash_weight(frac, c);
-        const float2 tv = reinterpret_cast<const float2*>(table)[rows[c]];
+        const float2 tv = reinterpret_cast<const float2*>(table)[hash_row(H, l, lo, hi, c)];
         e0 = fmaf(w, tv.x, e0);
         e1 = fmaf(w, tv.y, e1);
       }
@@ -544,15 +520,6 @@ __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashL
         // the encoding, feature-major, for the backward
         encbuf[(int64_t)(2 * l) * A.t_count + t] = e0;
         encbuf[(int64_t)(2 * l + 1) * A.t_count + t] = e1;
-      }
-      if constexpr (kCount) {
-        if (H.res[l] != 0) {
-          const float zero[8] = {};
-          hash_scatter_cell<false>(sink, rows, zero, zero, live_lanes, lane);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 8; ++c) hash_scatter<false>(sink, rows[c], 0.0f, 0.0f, live_lanes, lane);
-        }
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -577,62 +544,6 @@ __global__ __launch_bounds__(kFwdThreads) void hash_fwd_kernel(HashArgs A, HashL
       for (int j = 0; j < kHW; ++j) encbuf[(int64_t)(E + j) * A.t_count + t] = act[j];
     }
   }
-  if constexpr (kCount) {
-    __syncthreads();
-    for (int b = threadIdx.x; b < nb; b += kFwdThreads) cnt_out[(int64_t)b * kHashBlocks + blockIdx.x] = hash_cnt[b];
-  }
-}
-
-// exclusive scan of each bucket's kHashBlocks counts (one workgroup per bucket)
-__global__ __launch_bounds__(kHashBlocks) void hash_scan_local_kernel(const int* __restrict__ cnt,
-                                                                      int* __restrict__ loff,
-                                                                      int* __restrict__ total) {
-  __shared__ int wsum[kHashBlocks / 64];
-  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t / 64;
-  const int v = cnt[(int64_t)b * kHashBlocks + t];
-  int incl = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int o = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += o;
-  }
-  if (lane == 63) wsum[wave] = incl;
-  __syncthreads();
-  int base = 0;
-  for (int w = 0; w < wave; ++w) base += wsum[w];
-  loff[(int64_t)b * kHashBlocks + t] = base + incl - v;
-  if (t == kHashBlocks - 1) total[b] = base + incl;
-}
-
-// exclusive scan of the bucket totals: bstart[0..nb] (one workgroup)
-__global__ __launch_bounds__(1024) void hash_scan_total_kernel(const int* __restrict__ total, int nb,
-                                                               int* __restrict__ bstart) {
-  __shared__ int part[1024];
-  constexpr int kPer = kMaxBuckets / 1024;
-  const int t = threadIdx.x;
-  int v[kPer], s = 0;
-#pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const int b = t * kPer + i;
-    v[i] = b < nb ? total[b] : 0;
-    s += v[i];
-  }
-  part[t] = s;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int o = t >= d ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += o;
-    __syncthreads();
-  }
-  int run = part[t] - s;
-#pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const int b = t * kPer + i;
-    if (b < nb) bstart[b] = run;
-    run += v[i];
-  }
-  if (t == 1023) bstart[nb] = part[1023];
 }
 
 // Table gradient rows of one bucket (one workgroup per bucket): the bucket's
@@ -640,32 +551,39 @@ __global__ __launch_bounds__(1024) void hash_scan_total_kernel(const int* __rest
 // touches get 0).
 constexpr int kBThreads = 1024;
 __global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint4* __restrict__ recs,
-                                                                const int* __restrict__ bstart, BucketMap map,
-                                                                int64_t rows, float* __restrict__ gtab) {
+                                                                const int* __restrict__ cnt, int cap, int nb,
+                                                                BucketMap map, int64_t rows,
+                                                                float* __restrict__ gtab) {
   __shared__ float acc[2 * kBRows];
   const int b = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x / DIPS_WAVE;
   for (int i = threadIdx.x; i < 2 * kBRows; i += kBThreads) acc[i] = 0.0f;
   __syncthreads();
-  const int e = bstart[b + 1];
-  for (int i = bstart[b] + threadIdx.x; i < e; i += kBThreads) {
-    const uint4 rc = recs[i];
-    const int k = map.local(rc.x);
-    const float2 v = make_float2(__uint_as_float(rc.y), __uint_as_float(rc.z));
-    atomicAdd(acc + 2 * k, v.x);
-    atomicAdd(acc + 2 * k + 1, v.y);
+  // the slots of bucket b, one per backward workgroup, a wavefront each in turn
+  for (int blk = wave; blk < kHashBlocks; blk += kBThreads / DIPS_WAVE) {
+    const int n = cnt[(int64_t)b * kHashBlocks + blk];
+    const uint4* seg = recs + ((int64_t)blk * nb + b) * cap;
+    for (int i = lane; i < n; i += DIPS_WAVE) {
+      const uint4 rc = seg[i];
+      const int k = map.local(rc.x);
+      atomicAdd(acc + 2 * k, __uint_as_float(rc.y));
+      atomicAdd(acc + 2 * k + 1, __uint_as_float(rc.z));
+    }
   }
   __syncthreads();
-  // local row k of group lg is table row ((lg * nb + b) << 6) + (k & 63)
+  // local row k of group lg is table row ((lg * nb + b) << 6) + (k & 63); gtab holds
+  // the records past a slot's capacity
   for (int i = threadIdx.x; i < 2 * kBRows; i += kBThreads) {
     const int k = i >> 1;
     const int64_t g = ((int64_t)(k >> kGShift) << map.shift) + b;
     const int64_t row = (g << kGShift) + (k & ((1 << kGShift) - 1));
-    if (row < rows) gtab[2 * row + (i & 1)] = acc[i];
+    if (row < rows) gtab[2 * row + (i & 1)] += acc[i];
   }
 }
 
-// kBuckets: the table gradient goes to bucket records (cursors from the scans
-// of hash_fwd_kernel<true>'s counts), else by atomic adds into gtab
+// kBuckets: the table gradient goes to bucket records (the workgroup's slot of
+// each bucket; the records per slot to cnt, bucket-major), else by atomic adds
+// into gtab
 template <bool kBuckets>
 __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, HashLevels H,
                                                                 const float* __restrict__ params,
@@ -673,17 +591,14 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
                                                                 const float* __restrict__ r_row,
                                                                 float* __restrict__ gtab,
                                                                 float* __restrict__ partial, int nb,
-                                                                const int* __restrict__ bstart,
-                                                                const int* __restrict__ loff,
-                                                                uint4* __restrict__ recs, int cap,
-                                                                BucketMap map) {
+                                                                int* __restrict__ cnt, uint4* __restrict__ recs,
+                                                                int cap, BucketMap map) {
   extern __shared__ float hash_lds[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
   int* const cursor = reinterpret_cast<int*>(hash_lds + kHashWaves * kHStage);
   if constexpr (kBuckets) {
-    for (int b = threadIdx.x; b < nb; b += kHashThreads)
-      cursor[b] = bstart[b] + loff[(int64_t)b * kHashBlocks + blockIdx.x];
+    for (int b = threadIdx.x; b < nb; b += kHashThreads) cursor[b] = 0;
     __syncthreads();
   }
   float* const AT = hash_lds + wave * kHStage;
@@ -775,20 +690,20 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
           }
           const bool dense = H.res[l] != 0;
           if constexpr (kBuckets) {
-            const BucketSink sink{cursor, recs, cap, map};
+            const BucketSink sink{cursor, recs + (int64_t)blockIdx.x * nb * cap, cap, map, gtab};
             if (dense) {
-              hash_scatter_cell<true>(sink, rows, v0, v1, live_lanes, lane);
+              hash_scatter_cell(sink, rows, v0, v1, live_lanes, lane);
             } else {
 #pragma unroll
-              for (int c = 0; c < 8; ++c) hash_scatter<true>(sink, rows[c], v0[c], v1[c], live_lanes, lane);
+              for (int c = 0; c < 8; ++c) hash_scatter(sink, rows[c], v0[c], v1[c], live_lanes, lane);
             }
           } else {
             const AtomicSink sink{gtab};
             if (dense) {
-              hash_scatter_cell<true>(sink, rows, v0, v1, live_lanes, lane);
+              hash_scatter_cell(sink, rows, v0, v1, live_lanes, lane);
             } else {
 #pragma unroll
-              for (int c = 0; c < 8; ++c) hash_scatter<true>(sink, rows[c], v0[c], v1[c], live_lanes, lane);
+              for (int c = 0; c < 8; ++c) hash_scatter(sink, rows[c], v0[c], v1[c], live_lanes, lane);
             }
           }
         }
@@ -798,6 +713,11 @@ __global__ __launch_bounds__(kHashThreads) void hash_bwd_kernel(HashArgs A, Hash
   // workgroup reduction: per-wavefront rows in LDS, then the wavefronts in order
   const int stride = 2 * PM;
   __syncthreads();
+  if constexpr (kBuckets) {
+    // records in this workgroup's slot of each bucket, bucket-major for the bucket sums
+    for (int b = threadIdx.x; b < nb; b += kHashThreads)
+      cnt[(int64_t)b * kHashBlocks + blockIdx.x] = min(cursor[b], cap);
+  }
   float* red = hash_lds + wave * stride;
   gp.flush(red, lane, E);
   gm.flush(red + PM, lane, E);
@@ -1104,6 +1024,21 @@ static bool hash_bucketed(const HashLevels& H, int64_t nent, int* nb, int* shift
   return *nb <= kMaxBuckets && hash_records(H, nent) < INT_MAX;
 }
 
+// records a (workgroup, bucket) slot holds: the mean per slot if no entries
+// were aggregated (aggregation leaves about half), at least 64; records past it
+// take atomic adds.  DIPS_HASH_SLOT_CAP (read once per process) lowers it, so
+// that tests can run the atomic overflow path.
+static int hash_slot_cap(const HashLevels& H, int64_t nent, int nb) {
+  static const int64_t forced = [] {
+    const char* e = std::getenv("DIPS_HASH_SLOT_CAP");
+    return e ? std::max<int64_t>(1, std::atoll(e)) : (int64_t)0;
+  }();
+  const int64_t slots = (int64_t)kHashBlocks * nb;
+  const int64_t mean = (hash_records(H, nent) + slots - 1) / slots;
+  const int64_t cap = std::max<int64_t>(64, (mean + 15) / 16 * 16);
+  return (int)(forced > 0 ? std::min(cap, forced) : cap);
+}
+
 int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_nnz, int64_t* doubles) {
   HashLevels H;
   if (hash_levels(hg, H) || max_rows < 1 || max_nnz < 0) return -1;
@@ -1114,7 +1049,7 @@ int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_
   int nb;
   n += max_nnz * (hash_width(H) + (H.enc == 0 ? kHW : 0));  // the forward's encodings (and activations)
   if (H.enc == 0 && hash_bucketed(H, max_nnz, &nb))
-    n += 4 * hash_records(H, max_nnz) + 4 + (2 * (int64_t)kHashBlocks + 2) * nb + 2;
+    n += 4 * (int64_t)kHashBlocks * nb * hash_slot_cap(H, max_nnz, nb) + 4 + (int64_t)kHashBlocks * nb;
   return n;
 }
 
@@ -1145,8 +1080,8 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("hash step: too many parameters");
     return DIPS_EINVAL;
   }
-  // float workspace: u | r | MLP partial rows | table gradient | g | encodings | [records (16 B) | counts |
-  // offsets | bucket totals | bucket starts]
+  // float workspace: u | r | MLP partial rows | table gradient | g | encodings | [record slots (16 B records,
+  // kHashBlocks x nb slots of cap) | records per slot]
   float* u = work;
   float* r = u + nent;
   float* partial = r + nrows;
@@ -1155,15 +1090,13 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   int nb = 0, nb_shift = 0;
   const bool buckets = H.enc == 0 && hash_bucketed(H, nent, &nb, &nb_shift);
   const BucketMap bmap{nb_shift};
-  const int64_t nrec = buckets ? hash_records(H, nent) : 0;
+  const int cap = buckets ? hash_slot_cap(H, nent, nb) : 0;
+  const int64_t nrec = buckets ? (int64_t)kHashBlocks * nb * cap : 0;
   float* encbuf = gvec + P;
   float* rbase = encbuf + nent * (hash_width(H) + (H.enc == 0 ? kHW : 0));
   rbase += (4 - (reinterpret_cast<uintptr_t>(rbase) / sizeof(float)) % 4) % 4;  // 16-byte alignment
   uint4* recs = reinterpret_cast<uint4*>(rbase);
   int* cnt = reinterpret_cast<int*>(rbase + 4 * nrec);
-  int* loff = cnt + (int64_t)nb * kHashBlocks;
-  int* total = loff + (int64_t)nb * kHashBlocks;
-  int* bstart = total + nb;
   const int nrb = rowsum_blocks(nrows);
   double* loss_part = dwork;
   double* sq_part = dwork + nrb;
@@ -1189,19 +1122,14 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     hipLaunchKernelGGL(freq_bwd_kernel, dim3(kHashBlocks), dim3(kHashThreads), flds, st, A, H, params, encbuf,
                        bt->row_begin, r, partial);
   } else {
-  constexpr size_t fwd_lds = sizeof(float) * (kMaxBuckets + kFwdSplit * kHashWaves * kHE * DIPS_WAVE);
-  static const bool fwd_set =
-      hipFuncSetAttribute((const void*)hash_fwd_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)fwd_lds) == hipSuccess &&
-      hipFuncSetAttribute((const void*)hash_fwd_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)fwd_lds) == hipSuccess;
+  constexpr size_t fwd_lds = sizeof(float) * kFwdSplit * kHashWaves * kHE * DIPS_WAVE;
+  static const bool fwd_set = hipFuncSetAttribute((const void*)hash_fwd_kernel,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)fwd_lds) == hipSuccess;
   (void)fwd_set;
-  if (buckets)
-    hipLaunchKernelGGL(hash_fwd_kernel<true>, dim3(kHashBlocks), dim3(kFwdThreads), fwd_lds, st, A, H, params,
-                       u - bt->ent_begin, encbuf, nb, bmap, cnt);
-  else if (bt->t_count > 0)
-    hipLaunchKernelGGL(hash_fwd_kernel<false>, dim3(kHashBlocks), dim3(kFwdThreads), fwd_lds, st, A, H, params,
-                       u - bt->ent_begin, encbuf, 0, bmap, nullptr);
+  if (bt->t_count > 0)
+    hipLaunchKernelGGL(hash_fwd_kernel, dim3(kHashBlocks), dim3(kFwdThreads), fwd_lds, st, A, H, params,
+                       u - bt->ent_begin, encbuf);
   rowsum_launch(rec->row_ptr, rec->row_c, bt->row_begin, nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part,
                 st);
   const size_t lds = sizeof(float) * std::max<int64_t>((int64_t)kHashWaves * kHStage + (buckets ? nb : 0),
@@ -1216,17 +1144,17 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("hash step: encoder too wide for the workgroup's LDS");
     return DIPS_EUNSUPPORTED;
   }
+  // gtab is zeroed first: the atomic path adds every record to it, the bucketed
+  // path the records past a slot's capacity (the bucket sums then add theirs)
+  hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
+                     H.rows * kHF);
   if (buckets) {
-    hipLaunchKernelGGL(hash_scan_local_kernel, dim3(nb), dim3(kHashBlocks), 0, st, cnt, loff, total);
-    hipLaunchKernelGGL(hash_scan_total_kernel, dim3(1), dim3(1024), 0, st, total, nb, bstart);
     hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
-                       bt->row_begin, r, gtab, partial, nb, bstart, loff, recs, (int)nrec, bmap);
-    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, recs, bstart, bmap, H.rows, gtab);
+                       bt->row_begin, r, gtab, partial, nb, cnt, recs, cap, bmap);
+    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, recs, cnt, cap, nb, bmap, H.rows, gtab);
   } else {
-    hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
-                       H.rows * kHF);
     hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
-                       bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, nullptr, 0, bmap);
+                       bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, 0, bmap);
   }
   }
   float* const gout = grad_out ? grad_out : gvec;
