@@ -553,8 +553,11 @@ constexpr int kBThreads = 1024;
 __global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint4* __restrict__ recs,
                                                                 const int* __restrict__ cnt, int cap, int nb,
                                                                 BucketMap map, int64_t rows,
-                                                                float* __restrict__ gtab) {
+                                                                float* __restrict__ gtab, float inv_b,
+                                                                float* __restrict__ gout,
+                                                                double* __restrict__ sq_part) {
   __shared__ float acc[2 * kBRows];
+  __shared__ double wsum[kBThreads / DIPS_WAVE];
   const int b = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x / DIPS_WAVE;
   for (int i = threadIdx.x; i < 2 * kBRows; i += kBThreads) acc[i] = 0.0f;
@@ -572,12 +575,30 @@ __global__ __launch_bounds__(kBThreads) void hash_bucket_kernel(const uint4* __r
   }
   __syncthreads();
   // local row k of group lg is table row ((lg * nb + b) << 6) + (k & 63); gtab holds
-  // the records past a slot's capacity
+  // the records past a slot's capacity.  Also the bucket's float64 sum of squares
+  // of the mean gradient inv_b * gtab (fixed order) for the clipping norm, and,
+  // when gout is given, that gradient.
+  double v = 0.0;
   for (int i = threadIdx.x; i < 2 * kBRows; i += kBThreads) {
     const int k = i >> 1;
     const int64_t g = ((int64_t)(k >> kGShift) << map.shift) + b;
     const int64_t row = (g << kGShift) + (k & ((1 << kGShift) - 1));
-    if (row < rows) gtab[2 * row + (i & 1)] += acc[i];
+    if (row < rows) {
+      const int64_t idx = 2 * row + (i & 1);
+      const float tot = gtab[idx] + acc[i];
+      gtab[idx] = tot;
+      const float gg = tot * inv_b;
+      if (gout) gout[idx] = gg;
+      v += (double)gg * (double)gg;
+    }
+  }
+  v = dips_wave_sum_d(v);
+  if (lane == 0) wsum[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBThreads / DIPS_WAVE; ++w) t += wsum[w];
+    sq_part[b] = t;
   }
 }
 
@@ -1044,7 +1065,8 @@ int64_t hash_workspace(const dips_hashgrid_t* hg, int64_t max_rows, int64_t max_
   if (hash_levels(hg, H) || max_rows < 1 || max_nnz < 0) return -1;
   const int64_t PM2 = 2 * (int64_t)hash_mlp_size(hash_width(H));
   const int64_t P = PM2 + H.rows * kHF;
-  if (doubles) *doubles = rowsum_blocks(max_rows) + kRedBlocks + 2;
+  // norm partials: kRedBlocks, or one per bucket and one for the MLPs
+  if (doubles) *doubles = rowsum_blocks(max_rows) + std::max(kRedBlocks, kMaxBuckets + 1) + 2;
   int64_t n = max_nnz + max_rows + (int64_t)kHashBlocks * PM2 + H.rows * kHF + P + 64;
   int nb;
   n += max_nnz * (hash_width(H) + (H.enc == 0 ? kHW : 0));  // the forward's encodings (and activations)
@@ -1100,7 +1122,7 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const int nrb = rowsum_blocks(nrows);
   double* loss_part = dwork;
   double* sq_part = dwork + nrb;
-  const int nsq = (int)std::min<int64_t>((P + 255) / 256, kRedBlocks);
+  const int nsq = buckets ? nb + 1 : (int)std::min<int64_t>((P + 255) / 256, kRedBlocks);
   HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
   if (H.enc == 1) {
@@ -1151,7 +1173,8 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   if (buckets) {
     hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
                        bt->row_begin, r, gtab, partial, nb, cnt, recs, cap, bmap);
-    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, recs, cnt, cap, nb, bmap, H.rows, gtab);
+    hipLaunchKernelGGL(hash_bucket_kernel, dim3(nb), dim3(kBThreads), 0, st, recs, cnt, cap, nb, bmap, H.rows, gtab,
+                       inv_b, grad_out ? grad_out + stride : nullptr, sq_part);
   } else {
     hipLaunchKernelGGL(hash_bwd_kernel<false>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
                        bt->row_begin, r, gtab, partial, 0, nullptr, nullptr, 0, bmap);
@@ -1160,8 +1183,12 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   float* const gout = grad_out ? grad_out : gvec;
   hipLaunchKernelGGL(hash_mlp_reduce_kernel, dim3((unsigned)((stride + 3) / 4)), dim3(256), 0, st, partial,
                      kHashBlocks, (int)stride, inv_b, gout);
-  hipLaunchKernelGGL(hash_reduce_kernel, dim3(nsq), dim3(256), 0, st, (int)stride, gtab, P, inv_b, grad_out ? 1 : 0,
-                     gout, sq_part);
+  if (buckets)  // the bucket sums gave the table's norm partials: the MLPs' alone, after them
+    hipLaunchKernelGGL(hash_reduce_kernel, dim3(1), dim3(256), 0, st, (int)stride, gtab, stride, inv_b, 0, gout,
+                       sq_part + nb);
+  else
+    hipLaunchKernelGGL(hash_reduce_kernel, dim3(nsq), dim3(256), 0, st, (int)stride, gtab, P, inv_b,
+                       grad_out ? 1 : 0, gout, sq_part);
   if (!opt) {
     loss_out_launch(loss_part, nrb, inv_b, loss_out, st);
     return hip_status(hipGetLastError(), "hash step launch");
