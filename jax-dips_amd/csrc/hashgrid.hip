@@ -790,10 +790,18 @@ __global__ __launch_bounds__(256) void hash_reduce_kernel(int stride, const floa
   if (threadIdx.x == 0) sq_part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// p[0, n) = 0: 16-byte stores from the first 16-byte boundary, single floats for
+// the (at most 3) floats before it and after the last whole 16 bytes.  Grid:
+// hash_zero_blocks(n) workgroups of 256.
 __global__ __launch_bounds__(256) void hash_zero_kernel(float* __restrict__ p, int64_t n) {
+  const int64_t head = std::min<int64_t>(n, (int64_t)((16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15) / 4);
+  const int64_t n4 = (n - head) / 4, tail = head + 4 * n4;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = 0.0f;
+  if (i < n4) reinterpret_cast<float4*>(p + head)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i < head) p[i] = 0.0f;
+  if (i < n - tail) p[tail + i] = 0.0f;
 }
+static unsigned hash_zero_blocks(int64_t n) { return (unsigned)((n / 4 + 4 + 255) / 256); }
 
 // ---- evaluation: u and d u / d x ----------------------------------------------
 __global__ __launch_bounds__(256) void hash_eval_kernel(HashLevels H, const float* __restrict__ params,
@@ -1168,8 +1176,7 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   }
   // gtab is zeroed first: the atomic path adds every record to it, the bucketed
   // path the records past a slot's capacity (the bucket sums then add theirs)
-  hipLaunchKernelGGL(hash_zero_kernel, dim3((unsigned)((H.rows * kHF + 255) / 256)), dim3(256), 0, st, gtab,
-                     H.rows * kHF);
+  hipLaunchKernelGGL(hash_zero_kernel, dim3(hash_zero_blocks(H.rows * kHF)), dim3(256), 0, st, gtab, H.rows * kHF);
   if (buckets) {
     hipLaunchKernelGGL(hash_bwd_kernel<true>, dim3(kHashBlocks), dim3(kHashThreads), lds, st, A, H, params, encbuf,
                        bt->row_begin, r, gtab, partial, nb, cnt, recs, cap, bmap);
