@@ -1,13 +1,24 @@
 """Benchmark: NBM residual+grad training points/s on one MI355X.
 
-Workload (default): the 128^3 sphere interface problem on [-1,1]^3 (phi = |x| -
-0.5, mu- = 1, mu+ = 10, k = 0, manufactured u- = exp(-|x|^2), u+ = sin x cos y
-sin z), two-phase tanh MLP 1x1 (Omega-) / 2x10 (Omega+), batch 65536, optimizer
-clip -> Adam -> exponential decay (lr 1e-3, decay 0.9), the four zoom levels
-of a 2000-epoch run.  A step is one optimizer step on one batch of 65536
-training points (residual + loss + parameter gradient + Adam), inputs resident
-in HBM.  Timed steps are split evenly over the four zoom levels and walk the
-batches in order, as the training loop does.
+Workload (default, --workload star): the 128^3 star interface on [-1,1]^3,
+phi = r - (0.5 + 0.1 sin(5 theta) sin(3 varphi)), mu- = 1, mu+ = 100,
+k- = k+ = 1, manufactured u- = exp(-|x|^2), u+ = sin x cos y sin z (f, alpha
+and beta derived), two-phase tanh MLP 3x64 per phase, batch 262144 (8
+batches per epoch), optimizer clip -> Adam -> exponential decay (lr 1e-3,
+decay 0.9), the four zoom levels of a 2000-epoch run.  A step is one optimizer
+step on one batch of training points (residual + loss + parameter gradient +
+Adam), inputs resident in HBM.  Timed steps are split evenly over the four
+zoom levels and walk the batches in order, as the training loop does.
+
+After the timed steps, the drop-in trainer (setup -> init_fn -> solve_fn)
+trains the same configuration for its full epoch schedule (--accuracy-epochs,
+2000) end to end -- records of every zoom level, training, evaluation on the
+eval grid -- and the line reports that run's wall time and the L-inf / L2
+error of the trained network against the manufactured solution
+(tests/test_poisson.py:277 of the reference computes the same norms).
+
+--workload sphere: the 128^3 sphere (phi = |x| - 0.5, mu 1/10, k = 0), MLP
+1x1/2x10, batch 65536.  Other workloads: sweep, pb, hashgrid (see --help).
 
 Prints one JSON line.  --gpus N > 1 exits: this build targets one GPU.
 """
@@ -53,13 +64,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--n", type=int, default=128, help="training grid points per axis")
-    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--n", type=int, default=0, help="training grid points per axis (0: the workload's, 128; pb 256)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="batch size (0: the workload's: star 262144, sphere 65536, pb 2^20)")
     ap.add_argument("--roofline-launches", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "sweep", "pb", "hashgrid"],
-                    help="sphere: 1x1/2x10 MLP, batch 65536 (default); star: 3x64/3x64 MLP, batch 262144; "
+    ap.add_argument("--accuracy-epochs", type=int, default=-1,
+                    help="epochs of the end-to-end drop-in trainer run whose error against the manufactured "
+                         "solution the line reports (-1: the workload's schedule, 2000; 0: skip)")
+    ap.add_argument("--workload", default="star", choices=["sphere", "star", "sweep", "pb", "hashgrid"],
+                    help="star: 3x64/3x64 MLP, batch 262144 (default); sphere: 1x1/2x10 MLP, batch 65536; "
                          "sweep: 64 problems of 32^3..96^3 with 2x16/2x16 MLPs, batched vs one at a time; "
                          "pb: Poisson-Boltzmann box of 2000 atoms at 256^3, 3x64/3x64 MLP, batch 2^20; "
                          "hashgrid: sphere 128^3 with the hash network (hashgrid encoder, 16 levels, T 2^19)")
@@ -102,26 +117,22 @@ def main():
     from jax_dips.level_records import build_level
     from jax_dips.records import DeviceGrid, Engine
 
-    dev = "cuda"
-    n, B = args.n, args.batch
+    n, B = args.n or 128, args.batch
     lo, hi = -1.0, 1.0
     if args.workload == "pb":
         prob = problems.pb_box(n_atoms=2000, half_width=20.0, seed=0)
         lo, hi = -20.0, 20.0
-        if args.n == 128:
-            n = 256
-        if args.batch == 65536:
-            B = 1 << 20
+        n, B = args.n or 256, B or (1 << 20)
         mlp = {"hidden_layers_m": 3, "hidden_dim_m": 64, "activation_m": "jnp.tanh",
                "hidden_layers_p": 3, "hidden_dim_p": 64, "activation_p": "jnp.tanh"}
     elif args.workload == "star":
         prob = problems.star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0)
-        if args.batch == 65536:
-            B = 262144
+        B = B or 262144
         mlp = {"hidden_layers_m": 3, "hidden_dim_m": 64, "activation_m": "jnp.tanh",
                "hidden_layers_p": 3, "hidden_dim_p": 64, "activation_p": "jnp.tanh"}
     else:
         prob = problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0)
+        B = B or 65536
         mlp = {"hidden_layers_m": 1, "hidden_dim_m": 1, "activation_m": "jnp.tanh",
                "hidden_layers_p": 2, "hidden_dim_p": 10, "activation_p": "jnp.tanh"}
     cbs = problems.oracle_callbacks(prob)
@@ -267,7 +278,16 @@ def main():
     avg_s = r0.elapsed_time(r1) / 1e3 / R
     achieved = flops_sum / R / avg_s / 1e12
 
-    traffic = measured_traffic(f"{path} {args.workload}{n}") if args.batch == 65536 else None
+    traffic = measured_traffic(f"{path} {args.workload}{n}") if args.batch == 0 else None
+    # the tile step of the small networks issues its FLOPs on the FP32 VALU (tanh and deltas) with MFMA
+    # only for the weight-gradient blocks; the MFMA step of the wide networks is bound by the matrix cores
+    bound = "mfma" if path == "mfma" else "valu"
+    accuracy = None
+    acc_epochs = 2000 if args.accuracy_epochs < 0 else args.accuracy_epochs
+    if acc_epochs > 0 and "exact_fn" in prob:
+        del levels, eng
+        torch.cuda.empty_cache()
+        accuracy = accuracy_run(prob, n, lo, hi, mlp, B, acc_epochs, opt)
     result = {
         "metric": "NBM residual+grad points/s vs JAX ref on same GPU at equal Linf error",
         "value": value,
@@ -284,20 +304,82 @@ def main():
         "config": {"workload": f"{args.workload} {n}^3 on [{lo:g},{hi:g}]^3, MLP {dims[0]}x{dims[1]}/{dims[2]}x{dims[3]} tanh, batch {B}, "
                                f"4 zoom levels (2000-epoch schedule), {path} kernels",
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": bound, "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": {"mfma": "mfma step: fwd + rowsum + bwd + reduce + apply (5 launches)",
+                     "kernel": {"mfma": "mfma step: mfma_fwd_kernel + mfma_rowsum_kernel + mfma_bwd_kernel + "
+                                        "mfma_reduce_kernel + mfma_apply_kernel (one optimizer step)",
                                 "lane": "loss_grad_kernel<grad,fused> (one launch per optimizer step)",
                                 "tile": "tile_step_kernel<grad> (gradient launch of a step; the optimizer "
-                                        "runs after it in a one-workgroup tile_apply_kernel)"}[path],
+                                        "runs after it in a one-workgroup tile_apply_kernel); FP32 VALU "
+                                        "bound (non-packed VALU ceiling 78.6 TFLOP/s)"}[path],
                      "avg_launch_us": avg_s * 1e6,
                      "flop_per_launch": flops_sum / R},
         "records_build_s": t_records,
         "loss_tail": loss_tail,
     }
+    if accuracy is not None:
+        result["accuracy"] = accuracy
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.workload, n, args.cpu_seconds)
     print(json.dumps(result), flush=True)
+
+
+def accuracy_run(prob, n, lo, hi, mlp, B, epochs, opt):
+    """The workload trained end to end through the drop-in trainer
+    (setup -> init_fn -> solve_fn, initial parameters from the model's own
+    initialiser, seed 42) for ``epochs`` epochs with the zoom schedule, the
+    training grid as eval grid.  Returns the wall time of solve_fn (records of
+    the four zoom levels, training, evaluation), its parts, the training
+    throughput and the error of the trained network against the manufactured
+    solution at the eval nodes: L-inf = max |u - u*|, RMS, L2 = sqrt(sum (u - u*)^2)
+    and relative L2 = L2 / sqrt(sum u*^2) (the norms the reference logs,
+    tests/test_poisson.py:277-281)."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    from jax_dips.domain import mesh
+    from jax_dips.solvers.poisson import trainer
+
+    init_mesh_fn, _ = mesh.construct(3)
+    x, y, z = problems_axes(n, lo, hi)
+    g = init_mesh_fn(x, y, z)
+    work = tempfile.mkdtemp(prefix="dips_bench_")
+    try:
+        init_fn = trainer.setup(prob["initial_value_fn"], prob["dirichlet_bc_fn"], prob["lvl_set_fn"],
+                                prob["mu_m_fn"], prob["mu_p_fn"], prob["k_m_fn"], prob["k_p_fn"], prob["f_m_fn"],
+                                prob["f_p_fn"], prob["alpha_fn"], prob["beta_fn"])
+        state, solve_fn = init_fn(lvl_gstate=g, tr_gstate=g, eval_gstate=g, num_epochs=epochs, batch_size=B,
+                                  checkpoint_dir=os.path.join(work, "ck"), results_dir=work, optimizer_dict=opt,
+                                  model_dict={"model_type": "mlp", "mlp": mlp}, print_rate=epochs)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        new_state, _, loss_epochs = solve_fn(sim_state=state)
+        torch.cuda.synchronize()
+        wall = time.time() - t0
+        tr = solve_fn.trainer
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    R = np.asarray(g.R, np.float64)
+    exact = np.asarray(prob["exact_fn"](R.T), np.float64)
+    err = np.asarray(new_state.solution, np.float64) - exact
+    npts = R.shape[0]
+    print(f"# accuracy run: {epochs} epochs in {wall:.1f} s (records {tr.timing['records_s']:.1f} s, training "
+          f"{tr.timing['train_s']:.1f} s); L-inf {np.abs(err).max():.4e}", file=sys.stderr, flush=True)
+    return {"epochs": epochs, "steps": int(tr.steps), "end_to_end_s": wall,
+            "records_s": tr.timing["records_s"], "train_s": tr.timing["train_s"],
+            "train_points_per_s": npts * epochs / tr.timing["train_s"],
+            "linf": float(np.abs(err).max()), "rms": float(np.sqrt(np.mean(err ** 2))),
+            "l2": float(np.sqrt(np.sum(err ** 2))), "rel_l2": float(np.sqrt(np.sum(err ** 2) / np.sum(exact ** 2))),
+            "loss_first": float(loss_epochs[0]), "loss_last": float(loss_epochs[-1]),
+            "eval_grid": f"{n}^3 training nodes"}
+
+
+def problems_axes(n, lo, hi):
+    from jax_dips import problems
+
+    return problems.grid_axes(n, lo, hi)
 
 
 def run_sweep(args):
@@ -328,7 +410,7 @@ def run_sweep(args):
     for prob, n in specs:
         x, y, z = problems.grid_axes(n)
         N_ = n ** 3
-        B = min(N_, args.batch)
+        B = min(N_, args.batch or 65536)
         nb = -(-N_ // B)
         P = np.stack(np.meshgrid(x, y, z, indexing="ij"), -1).reshape(-1, 3)
         extra = generate_random_points_like(P, nb * B - N_) if nb * B > N_ else None
@@ -428,7 +510,7 @@ def run_hashgrid(args):
     from jax_dips.nn.hash_network import HashNetwork
     from jax_dips.records import DeviceGrid, HashEngine
 
-    n, B = args.n, args.batch
+    n, B = args.n or 128, args.batch or 65536
     prob = problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0)
     cbs = problems.oracle_callbacks(prob)
     x, y, z = problems.grid_axes(n)

@@ -262,7 +262,9 @@ int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t n
 
 /* Network at m points pts (m,3): u (m), optionally grad_x u (m,3) and, with
  * normals (m,3), the normal derivative grad u . n (m).  phase (m) uint8:
- * 1 -> Omega+ network (phi >= 0), 0 -> Omega- network.  Replaces
+ * 1 -> Omega+ network (phi >= 0), 0 -> Omega- network.  Shapes of
+ * dips_net_supported run one lane per point, those of dips_mfma_supported in
+ * MFMA tiles of 16 points (grad u in forward mode).  Replaces
  * evaluate_solution_fn and the gradient post-processing of
  * evaluate_solution_and_gradients (trainer.py:836-844, :914-977). */
 int dips_eval(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase, const float* normals,
@@ -282,7 +284,8 @@ typedef struct {
   const int32_t* t_csr;       /* ... index of the entry in the CSR arrays */
   int64_t t_begin;            /* first phase-sorted entry of this batch */
   int64_t n_plus, n_minus;    /* + entries [t_begin, t_begin+n_plus), then - entries */
-  int64_t t_count;            /* n_plus + n_minus */
+  int64_t t_count;            /* n_plus + n_minus; must equal ent_end - ent_begin (every entry of the batch
+                                 listed once), else the step entry points return DIPS_EINVAL */
 } dips_mfma_batch_t;
 
 /* 1 if (Lm, ceil(Hm/16), Lp, ceil(Hp/16)) has an MFMA kernel. */

@@ -53,8 +53,8 @@ int eval_launch(const dips_net_t* net, const float* params, const float* pts, co
   X(2, 16, 2, 16, 1)
 #undef XT
 #undef X
-  set_error("dips_eval: network shape has no compiled kernel");
-  return DIPS_EUNSUPPORTED;
+  // wide networks: MFMA tiles of 16 points
+  return mfma_eval_launch(net, params, pts, phase, nrm, m, u, grad, gradn, st);
 }
 
 }  // namespace dips

@@ -1091,8 +1091,9 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   int s = hash_levels(hg, H);
   if (s) return s;
   if (!bt || bt->t_count < 0 || bt->n_plus + bt->n_minus != bt->t_count ||
-      bt->t_count > bt->ent_end - bt->ent_begin) {
-    set_error("hash step: invalid batch");
+      bt->t_count != bt->ent_end - bt->ent_begin) {
+    // every entry of the batch's CSR range must be listed once: the row sums read all of their values
+    set_error("hash step: invalid batch (t_count must equal ent_end - ent_begin and n_plus + n_minus)");
     return DIPS_EINVAL;
   }
   const int PM = hash_mlp_size(hash_width(H));

@@ -611,6 +611,139 @@ __global__ void count_inc_kernel(int32_t* count) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *count = *count + 1;
 }
 
+// ---- evaluation at arbitrary points: u and grad_x u (forward mode) ----------------
+// Value and the three input tangents of one 16-position tile, propagated
+// together: a_0 = tanh(W_0 [x,1]), t_0^k = (1 - a_0^2) W_0[k,:]; hidden layers
+// a_l = tanh(W_l a_{l-1} + b_l), t_l^k = (1 - a_l^2) (W_l t_{l-1}^k), sharing each
+// weight operand between the four MFMA chains.  Returns u and du/dx_k of the
+// tile position lane%16 (identical in the 4 lanes of a position).
+template <int L, int NB>
+__device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, float xin, float& u, float (&du)[3]) {
+  using PK = Packed<L, NB, false>;
+  const int lane = threadIdx.x & 63;
+  float a[4 * NB], t[3][4 * NB];
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mfma4(W[PK::l0() + ob * 64 + lane], xin, c);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float v = dips_tanh(c[r]);
+      const float g = 1.0f - v * v;
+      a[ob * 4 + r] = v;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t[k][ob * 4 + r] = g * W[PK::l0() + ob * 64 + 16 * k + 4 * (lane / 16) + r];
+    }
+  }
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    f4 c[NB], ct[3][NB];
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) c[ob][r] = W[PK::bias(l) + 16 * ob + 4 * (lane / 16) + r];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) ct[k][ob] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) {
+        const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::fwd(l) + ((ob * NB + fb) * 64 + lane) * 4]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          c[ob] = mfma4(w4[r], a[fb * 4 + r], c[ob]);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) ct[k][ob] = mfma4(w4[r], t[k][fb * 4 + r], ct[k][ob]);
+        }
+      }
+    }
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = dips_tanh(c[ob][r]);
+        const float g = 1.0f - v * v;
+        a[ob * 4 + r] = v;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t[k][ob * 4 + r] = g * ct[k][ob][r];
+      }
+    }
+  }
+  float s = 0.0f, d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
+#pragma unroll
+  for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float wo = W[PK::wout() + 16 * fb + 4 * (lane / 16) + r];
+      s = fmaf(a[fb * 4 + r], wo, s);
+      d0 = fmaf(t[0][fb * 4 + r], wo, d0);
+      d1 = fmaf(t[1][fb * 4 + r], wo, d1);
+      d2 = fmaf(t[2][fb * 4 + r], wo, d2);
+    }
+  }
+  s += __shfl_xor(s, 16, 64);
+  s += __shfl_xor(s, 32, 64);
+  d0 += __shfl_xor(d0, 16, 64);
+  d0 += __shfl_xor(d0, 32, 64);
+  d1 += __shfl_xor(d1, 16, 64);
+  d1 += __shfl_xor(d1, 32, 64);
+  d2 += __shfl_xor(d2, 16, 64);
+  d2 += __shfl_xor(d2, 32, 64);
+  u = s + W[PK::bout()];
+  du[0] = d0;
+  du[1] = d1;
+  du[2] = d2;
+}
+
+// u, grad u and grad u . n at m points; both networks packed once per
+// workgroup (+ net first); a wavefront takes 16 consecutive points and runs
+// each network only if one of its points selects it.
+template <int Lm, int NBm, int Lp, int NBp>
+__global__ __launch_bounds__(kMT) void mfma_eval_kernel(const float* __restrict__ params, int Hm, int Hp,
+                                                        const float* __restrict__ pts,
+                                                        const uint8_t* __restrict__ phase,
+                                                        const float* __restrict__ nrm, int64_t m,
+                                                        float* __restrict__ u, float* __restrict__ grad,
+                                                        float* __restrict__ gradn) {
+  extern __shared__ float lds[];
+  constexpr int kOffM = Packed<Lp, NBp, false>::size();
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
+  pack_weights<Lp, NBp, false>(params, FlatNet{Hp, Lp, 0}, lds);
+  pack_weights<Lm, NBm, false>(params, FlatNet{Hm, Lm, PPp}, lds + kOffM);
+  __syncthreads();
+  const int64_t ntiles = (m + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
+    const int64_t p = tile * 16 + (lane & 15);
+    const bool valid = p < m;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    bool plus = false;
+    if (valid) {
+      x = pts[3 * p];
+      y = pts[3 * p + 1];
+      z = pts[3 * p + 2];
+      plus = phase[p] != 0;
+    }
+    const int k = lane / 16;
+    const float xin = (k == 0) ? x : (k == 1) ? y : (k == 2) ? z : 1.0f;
+    float up = 0.0f, um = 0.0f, gp[3] = {0.f, 0.f, 0.f}, gm[3] = {0.f, 0.f, 0.f};
+    if (__any(valid && plus)) tile_value_grad<Lp, NBp>(lds, xin, up, gp);
+    if (__any(valid && !plus)) tile_value_grad<Lm, NBm>(lds + kOffM, xin, um, gm);
+    if (valid && lane < 16) {
+      const float v = plus ? up : um;
+      const float g0 = plus ? gp[0] : gm[0], g1 = plus ? gp[1] : gm[1], g2 = plus ? gp[2] : gm[2];
+      u[p] = v;
+      if (grad) {
+        grad[3 * p] = g0;
+        grad[3 * p + 1] = g1;
+        grad[3 * p + 2] = g2;
+      }
+      if (gradn && nrm) gradn[p] = g0 * nrm[3 * p] + g1 * nrm[3 * p + 1] + g2 * nrm[3 * p + 2];
+    }
+  }
+}
+
 // loss = inv_b * sum of the row-sum partials (fixed order), gradient copy-out
 __global__ __launch_bounds__(256) void mfma_loss_out_kernel(const double* __restrict__ loss_part, int nloss,
                                                             float inv_b, double* __restrict__ loss_out) {
@@ -678,6 +811,33 @@ int64_t mfma_workspace_floats(const dips_net_t* n, int64_t max_rows, int64_t max
   return max_nnz + max_rows + (int64_t)kMfmaBwdBlocks * stride + P + 64;
 }
 
+// dips_eval for the network shapes of the MFMA path
+int mfma_eval_launch(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase,
+                     const float* nrm, int64_t m, float* u, float* grad, float* gradn, hipStream_t st) {
+  if (!mfma_supported(net)) {
+    set_error("dips_eval: network shape has no compiled kernel");
+    return DIPS_EUNSUPPORTED;
+  }
+  if (m <= 0) return DIPS_OK;
+  const int Hm = net->Hm, Hp = net->Hp;
+  const int a = net->Lm, b = nb_of(Hm), c = net->Lp, d = nb_of(Hp);
+  const int64_t ntiles = (m + 15) / 16;
+  const int nblk = (int)std::min<int64_t>((ntiles + 3) / 4, 1024);
+#define X(A, B, C, D)                                                                                            \
+  if (a == A && b == B && c == C && d == D) {                                                                   \
+    const size_t lds = sizeof(float) * (Packed<C, D, false>::size() + Packed<A, B, false>::size());             \
+    hipFuncSetAttribute((const void*)mfma_eval_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                        (int)lds);                                                                             \
+    hipLaunchKernelGGL((mfma_eval_kernel<A, B, C, D>), dim3(nblk), dim3(kMT), lds, st, params, Hm, Hp, pts, phase, \
+                       nrm, m, u, grad, gradn);                                                                 \
+    return hip_status(hipGetLastError(), "mfma_eval_kernel launch");                                            \
+  }
+  DIPS_MFMA_SHAPES(X)
+#undef X
+  set_error("dips_eval: network shape has no compiled kernel");
+  return DIPS_EUNSUPPORTED;
+}
+
 // opt == nullptr: stop after the reduction (gradient in grad_out, loss in loss_out)
 int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, const dips_net_t* net,
               const dips_opt_t* opt, float inv_b, float* params, float* m, float* v, int32_t* count, float* loss_hist,
@@ -693,8 +853,9 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const int stride = std::max(PPp, PPm);
   const int64_t nrows = bt->nrows, np_ = bt->n_plus, nm_ = bt->n_minus;
   const int64_t nent = bt->ent_end - bt->ent_begin;
-  if (np_ + nm_ != bt->t_count || bt->t_count < 0) {
-    set_error("dips_mfma_train_step: phase counts do not add up");
+  if (np_ + nm_ != bt->t_count || bt->t_count < 0 || bt->t_count != nent) {
+    // every entry of the batch's CSR range must be listed once: the row sums read all of their values
+    set_error("dips_mfma_train_step: invalid batch (t_count must equal ent_end - ent_begin and n_plus + n_minus)");
     return DIPS_EINVAL;
   }
   // workspace layout (floats): u over the batch's CSR entries | r rows | gradient partials | g

@@ -142,6 +142,203 @@ def star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k
     return make_problem(phi, gphi, mu_m, mu_p, k_m, k_p, "star")
 
 
+def _problem(phi_raw, grad_phi, mu_m, mu_p, grad_mu_m, grad_mu_p, um, up, grad_um, grad_up, f_m, f_p, name,
+             normalize_beta=False, k_m=0.0, k_p=0.0):
+    """Callbacks of a problem with spatially varying mu: alpha = u+ - u-,
+    beta = -(mu+ grad u+ - mu- grad u-) . grad phi (the reference's sign; the
+    level-set gradient as is, or normalised), g = u+ (the box lies in
+    Omega+)."""
+    phi = perturb(phi_raw)
+
+    def alpha(r):
+        return up(r) - um(r)
+
+    def beta(r):
+        g = grad_phi(r)
+        if normalize_beta:
+            nn = xnp.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
+            g = (g[0] / nn, g[1] / nn, g[2] / nn)
+        a, b = grad_up(r), grad_um(r)
+        mp, mm = mu_p(r), mu_m(r)
+        v = sum((mp * a[i] - mm * b[i]) * g[i] for i in range(3))
+        return xnp.nan_to_num(-v)
+
+    def exact(r):
+        return xnp.where(phi(r) >= 0, up(r), um(r))
+
+    return dict(name=name, initial_value_fn=_const(0.0), dirichlet_bc_fn=up, lvl_set_fn=phi, mu_m_fn=mu_m,
+                mu_p_fn=mu_p, k_m_fn=_const(k_m), k_p_fn=_const(k_p), f_m_fn=f_m, f_p_fn=f_p, alpha_fn=alpha,
+                beta_fn=beta, exact_fn=exact, exact_m_fn=um, exact_p_fn=up, grad_mu_m_fn=grad_mu_m,
+                grad_mu_p_fn=grad_mu_p, grad_exact_m_fn=grad_um, grad_exact_p_fn=grad_up, phi_raw_fn=phi_raw,
+                grad_phi_fn=grad_phi)
+
+
+def ref_sphere():
+    """The reference's variable-coefficient sphere (tests/confs/experiment_configs.py:21-186):
+    phi = |x| - 0.5, u- = exp(z), u+ = sin y cos x, mu- = y^2 log(x + 2) + 4,
+    mu+ = exp(-z), k = 0; f = -div(mu grad u) on each side (derived here:
+    f- = -exp(z) mu-, f+ = 2 exp(-z) cos x sin y), alpha = u+ - u-, beta with
+    the unnormalised grad phi (a unit vector for this phi), g = u+.
+    mu- > mu+ wherever |x| <= 1 (the mu- > mu+ ghost-value branch)."""
+
+    def phi(r):
+        return xnp.sqrt(r[0] ** 2 + r[1] ** 2 + r[2] ** 2) - 0.5
+
+    def gphi(r):
+        rr = xnp.sqrt(r[0] ** 2 + r[1] ** 2 + r[2] ** 2)
+        return (r[0] / rr, r[1] / rr, r[2] / rr)
+
+    def um(r):
+        return xnp.exp(r[2])
+
+    def up(r):
+        return xnp.sin(r[1]) * xnp.cos(r[0])
+
+    def gum(r):
+        z = xnp.zeros_like(r[0])
+        return (z, z, xnp.exp(r[2]))
+
+    def gup(r):
+        return (-xnp.sin(r[1]) * xnp.sin(r[0]), xnp.cos(r[1]) * xnp.cos(r[0]), xnp.zeros_like(r[0]))
+
+    def mu_m(r):
+        return r[1] * r[1] * xnp.log(r[0] + 2.0) + 4.0
+
+    def mu_p(r):
+        return xnp.exp(-r[2])
+
+    def gmu_m(r):
+        return (r[1] * r[1] / (r[0] + 2.0), 2.0 * r[1] * xnp.log(r[0] + 2.0), xnp.zeros_like(r[0]))
+
+    def gmu_p(r):
+        z = xnp.zeros_like(r[0])
+        return (z, z, -xnp.exp(-r[2]))
+
+    def f_m(r):
+        # -d/dz (mu- exp(z)); mu- does not depend on z
+        return -mu_m(r) * xnp.exp(r[2])
+
+    def f_p(r):
+        # -(mu+ lap u+ + grad mu+ . grad u+) with lap u+ = -2 u+ and grad mu+ . grad u+ = 0
+        return 2.0 * xnp.exp(-r[2]) * xnp.cos(r[0]) * xnp.sin(r[1])
+
+    return _problem(phi, gphi, mu_m, mu_p, gmu_m, gmu_p, um, up, gum, gup, f_m, f_p, "ref_sphere")
+
+
+# the reference star's level set: three angular modes (n, beta, theta) about the z axis
+_STAR_R0 = 0.483
+_STAR_MODES = ((3.0, 0.1, 0.5), (4.0, -0.1, 1.8), (7.0, 0.15, 0.0))
+
+
+def ref_star(normalize_beta=False):
+    """The reference's variable-coefficient star (tests/confs/experiment_configs.py:196-412):
+    phi = |x| - r0 (1 + ((x^2 + y^2)/|x|^2)^2 sum_i b_i cos(n_i (atan2(y, x) - t_i)))
+    (r0 = 0.483, modes (3, 0.1, 0.5), (4, -0.1, 1.8), (7, 0.15, 0)), not finite
+    at the origin -> 0 before the perturbation; u- = sin 2x cos 2y exp(z),
+    u+ = T5((y - x)/3) log(x + y + 3) cos z with T5 the Chebyshev polynomial
+    16t^5 - 20t^3 + 5t, mu- = 10 (1 + 0.2 cos(2 pi (x + y)) sin(2 pi (x - y)) cos z),
+    mu+ = 1, k = 0; f = -div(mu grad u) derived here; beta with the
+    unnormalised grad phi as the reference writes it (normalize_beta=True
+    divides by |grad phi|, which the manufactured solution needs to satisfy the
+    flux jump), not finite values -> 0; g = u+.  mu- > mu+ everywhere."""
+    r0 = _STAR_R0
+
+    def _core(ps):
+        c = 0.0
+        dc = 0.0
+        for n_i, b_i, t_i in _STAR_MODES:
+            c = c + b_i * xnp.cos(n_i * (ps - t_i))
+            dc = dc - b_i * n_i * xnp.sin(n_i * (ps - t_i))
+        return c, dc
+
+    def phi(r):
+        x, y, z = r[0], r[1], r[2]
+        rho2 = x ** 2 + y ** 2 + z ** 2
+        c, _ = _core(xnp.arctan2(y, x))
+        v = xnp.sqrt(rho2) - r0 * (1.0 + ((x ** 2 + y ** 2) / rho2) ** 2 * c)
+        return xnp.nan_to_num(v)
+
+    def gphi(r):
+        x, y, z = r[0], r[1], r[2]
+        q = x ** 2 + y ** 2
+        rho2 = q + z ** 2
+        rho = xnp.sqrt(rho2)
+        s = q / rho2
+        c, dc = _core(xnp.arctan2(y, x))
+        # grad(s^2 c) = 2 s c grad s + s^2 c' grad psi with
+        # grad s = (2x z^2, 2y z^2, -2 q z) / rho^4 and s^2 grad psi = q (-y, x, 0) / rho^4
+        sa = 2.0 * s * c / rho2 ** 2
+        pb = q * dc / rho2 ** 2
+        gx = x / rho - r0 * (sa * 2.0 * x * z ** 2 - pb * y)
+        gy = y / rho - r0 * (sa * 2.0 * y * z ** 2 + pb * x)
+        gz = z / rho - r0 * (sa * (-2.0 * q * z))
+        return (gx, gy, gz)
+
+    def um(r):
+        return xnp.sin(2.0 * r[0]) * xnp.cos(2.0 * r[1]) * xnp.exp(r[2])
+
+    def gum(r):
+        e = xnp.exp(r[2])
+        return (2.0 * xnp.cos(2.0 * r[0]) * xnp.cos(2.0 * r[1]) * e,
+                -2.0 * xnp.sin(2.0 * r[0]) * xnp.sin(2.0 * r[1]) * e,
+                xnp.sin(2.0 * r[0]) * xnp.cos(2.0 * r[1]) * e)
+
+    def _t(r):
+        t = (r[1] - r[0]) / 3.0
+        T = 16.0 * t ** 5 - 20.0 * t ** 3 + 5.0 * t
+        dT = 80.0 * t ** 4 - 60.0 * t ** 2 + 5.0
+        ddT = 320.0 * t ** 3 - 120.0 * t
+        return T, dT, ddT
+
+    def up(r):
+        T, _, _ = _t(r)
+        return T * xnp.log(r[0] + r[1] + 3.0) * xnp.cos(r[2])
+
+    def gup(r):
+        T, dT, _ = _t(r)
+        w = r[0] + r[1] + 3.0
+        L, C = xnp.log(w), xnp.cos(r[2])
+        return (-dT / 3.0 * L * C + T * C / w, dT / 3.0 * L * C + T * C / w, -T * L * xnp.sin(r[2]))
+
+    def _mu_parts(r):
+        A = xnp.cos(2.0 * xnp.pi * (r[0] + r[1]))
+        B = xnp.sin(2.0 * xnp.pi * (r[0] - r[1]))
+        C = xnp.cos(r[2])
+        return A, B, C
+
+    def mu_m(r):
+        A, B, C = _mu_parts(r)
+        return 10.0 * (1.0 + 0.2 * A * B * C)
+
+    def gmu_m(r):
+        A, B, C = _mu_parts(r)
+        dA = -2.0 * xnp.pi * xnp.sin(2.0 * xnp.pi * (r[0] + r[1]))  # d/dx = d/dy
+        dB = 2.0 * xnp.pi * xnp.cos(2.0 * xnp.pi * (r[0] - r[1]))  # d/dx = -d/dy
+        return (2.0 * (dA * B + A * dB) * C, 2.0 * (dA * B - A * dB) * C, -2.0 * A * B * xnp.sin(r[2]))
+
+    def mu_p(r):
+        return xnp.zeros_like(r[0]) + 1.0
+
+    def gmu_p(r):
+        z = xnp.zeros_like(r[0])
+        return (z, z, z)
+
+    def f_m(r):
+        # lap u- = -7 u-
+        g, gm = gum(r), gmu_m(r)
+        return 7.0 * mu_m(r) * um(r) - (gm[0] * g[0] + gm[1] * g[1] + gm[2] * g[2])
+
+    def f_p(r):
+        # lap u+ = (2/9) T'' L C - 2 T C / w^2 - T L C (the mixed terms of d_xx and d_yy cancel)
+        T, _, ddT = _t(r)
+        w = r[0] + r[1] + 3.0
+        L, C = xnp.log(w), xnp.cos(r[2])
+        return -((2.0 / 9.0) * ddT * L * C - 2.0 * T * C / w ** 2 - T * L * C)
+
+    return _problem(phi, gphi, mu_m, mu_p, gmu_m, gmu_p, um, up, gum, gup, f_m, f_p, "ref_star",
+                    normalize_beta=normalize_beta)
+
+
 class AtomField:
     """Union of spheres and a sum of Gaussian charges, evaluated through a cell
     list so that a query reads only the atoms of its 27 neighbouring cells.

@@ -213,19 +213,21 @@ class Trainer:
         bounds = (lv.xmin(), lv.xmax(), lv.ymin(), lv.ymax(), lv.zmin(), lv.zmax())
         E = self.num_epochs
         steps_total = (E - self.epoch_start) * nb
+        # the kernels write the loss of optimizer step c at loss_hist[c], c the global step count, which a
+        # restart resumes from the checkpoint's count
+        count0 = int(self.opt_restore["count"]) if self.opt_restore is not None else 0
+        cap = max(1, count0 + steps_total)
         if getattr(self.model, "encoder", "identity") in ("hashgrid", "frequency"):
-            engine = HashEngine(self.model.hashgrid, self.flatten(self.params), self.optimizer_dict,
-                                max_steps=max(1, steps_total))
+            engine = HashEngine(self.model.hashgrid, self.flatten(self.params), self.optimizer_dict, max_steps=cap)
         else:
             engine = Engine(self.model.dims, self.flatten(self.params), self.optimizer_dict, max_rows=B,
-                            max_steps=max(1, steps_total),
+                            max_steps=cap,
                             precond=self.precond.engine_config() if self.precond is not None else None,
                             act=getattr(self.model, "act", 0))
         if self.opt_restore is not None:
             engine.m.copy_(torch.as_tensor(self.flatten(self.opt_restore["mu"]), device="cuda"))
             engine.v.copy_(torch.as_tensor(self.flatten(self.opt_restore["nu"]), device="cuda"))
-            engine.count.fill_(int(self.opt_restore["count"]))
-        count0 = int(engine.count.item())
+            engine.count.fill_(count0)
         prev = signal.signal(signal.SIGINT, signalHandler)
         stop_training = False
         t0 = time.time()
@@ -253,7 +255,7 @@ class Trainer:
                 self.timing["train_s"] += time.time() - tt0
                 hist = engine.loss_hist.cpu().numpy()
                 for e in range(epoch, done):
-                    k = (e - self.epoch_start) * nb
+                    k = count0 + (e - self.epoch_start) * nb
                     self.loss_epochs[e - self.epoch_start] = hist[k:k + nb].mean()
                     if e % self.print_rate == 0:
                         logger.info(f"epoch # {e} loss is {self.loss_epochs[e - self.epoch_start]:.6e}")

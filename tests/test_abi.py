@@ -42,22 +42,20 @@ STRUCTS = {"Grid": "dips_grid_t", "Records": "dips_records_t", "Net": "dips_net_
            "BatchProblem": "dips_batch_problem_t", "HashGrid": "dips_hashgrid_t"}
 
 
-def test_ctypes_structs_match_header(tmp_path):
-    """Size and every field offset of the ctypes mirrors equal the C header's
-    (a C program built with gcc prints offsetof/sizeof)."""
+def check_layout(classes, tmp_path):
+    """Size and every field offset of ctypes structures ({C struct name: class})
+    equal the C header's (a C program built with gcc prints offsetof/sizeof)."""
     import shutil
     import subprocess
 
     import pytest
-    from jax_dips import _native
 
     gcc = shutil.which("gcc")
     if gcc is None:
         pytest.skip("gcc not available")
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dips_hip.h"', "int main(void) {"]
     expect = []
-    for py, c in STRUCTS.items():
-        cls = getattr(_native, py)
+    for c, cls in classes.items():
         lines.append(f'  printf("%zu\\n", sizeof({c}));')
         expect.append(ctypes.sizeof(cls))
         for name, _ in cls._fields_:
@@ -70,6 +68,36 @@ def test_ctypes_structs_match_header(tmp_path):
     subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert got == expect
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """The package's ctypes mirrors (jax_dips._native) have the C layout."""
+    from jax_dips import _native
+
+    check_layout({c: getattr(_native, py) for py, c in STRUCTS.items()}, tmp_path)
+
+
+def test_integration_stub_matches_header(tmp_path):
+    """The ctypes stub that INTEGRATION.md shows a maintainer (its python block
+    with the Structure classes) declares every field of each C struct at the
+    header's offsets and sizes."""
+    import ast
+
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```python\n(.*?)```", txt, flags=re.S)
+    stub = [b for b in blocks if "ctypes.Structure" in b]
+    assert len(stub) == 1
+    tree = ast.parse(stub[0])
+    # the imports, type aliases and Structure classes (not the CDLL load or the calls)
+    keep = [n for n in tree.body if isinstance(n, (ast.Import, ast.ClassDef))
+            or (isinstance(n, ast.Assign) and not isinstance(n.value, ast.Call)
+                and all(isinstance(t, (ast.Name, ast.Tuple)) for t in n.targets))]
+    ns = {}
+    exec(compile(ast.Module(body=keep, type_ignores=[]), "INTEGRATION.md", "exec"), ns)
+    names = {"Grid": "dips_grid_t", "Records": "dips_records_t", "Net": "dips_net_t", "Opt": "dips_opt_t",
+             "Tiles": "dips_tiles_t", "MfmaBatch": "dips_mfma_batch_t"}
+    assert set(names) <= set(ns)
+    check_layout({c: ns[py] for py, c in names.items()}, tmp_path)
 
 
 def test_precond_shapes_and_counts():

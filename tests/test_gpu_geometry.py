@@ -16,16 +16,27 @@ from helpers import grid_points, oracle_level
 from jax_dips import problems
 
 CASES = [("sphere", 16, 0), ("sphere", 16, 1), ("sphere", 16, 2), ("sphere", 16, 3),
-         ("star", 18, 0), ("star", 18, 2), ("sphere_mu_m_big", 16, 0), ("sphere_k", 14, 1)]
+         ("star", 18, 0), ("star", 18, 2), ("sphere_mu_m_big", 16, 0), ("sphere_k", 14, 1),
+         ("c3_star", 16, 0), ("c3_star", 16, 1), ("c3_star", 16, 2), ("c3_star", 16, 3), ("c3_star", 22, 1),
+         ("ref_sphere", 16, 0), ("ref_sphere", 16, 1), ("ref_sphere", 16, 2), ("ref_sphere", 16, 3),
+         ("ref_star", 16, 0), ("ref_star", 16, 1), ("ref_star", 16, 2), ("ref_star", 16, 3)]
 
 
 def make(name):
     if name == "sphere":
         return problems.sphere()
     if name == "star":
-        # centre off the lattice lines: the normal of phi is singular on the
-        # star's polar axis, where host and device libm results legitimately differ
         return problems.star(r0=0.55, amp=0.15, centre=(0.0123, -0.0171, 0.0093))
+    if name == "c3_star":
+        # the 128^3 star workload's interface and coefficients, centred at the origin: at zoom levels >= 1
+        # stencil points and at every level cell corners lie on the x = 0 and y = 0 planes and on the polar axis
+        return problems.star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0)
+    if name == "ref_sphere":
+        # spatially varying mu- = y^2 log(x + 2) + 4 > mu+ = exp(-z) (tests/confs/experiment_configs.py:21-186)
+        return problems.ref_sphere()
+    if name == "ref_star":
+        # mu- = 10 (1 + 0.2 cos(2 pi (x + y)) sin(2 pi (x - y)) cos z) > mu+ = 1 (experiment_configs.py:196-412)
+        return problems.ref_star()
     if name == "sphere_mu_m_big":
         return problems.sphere(radius=0.45, mu_m=20.0, mu_p=2.0)
     if name == "sphere_k":
@@ -79,6 +90,36 @@ def test_loss_on_device_records_matches_oracle():
     for b0, nr in zip(drec.batch_begin, drec.batch_rows):
         loss, grad = eng.loss_grad(grid, drec, b0, nr, 1.0 / 1024)
         ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, orec, np.arange(b0, b0 + nr))
+        assert abs(loss - ol) <= 1e-5 * abs(ol)
+        assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,level,dims", [("ref_sphere", 0, (1, 1, 2, 10)), ("ref_sphere", 2, (1, 1, 2, 10)),
+                                             ("ref_star", 1, (1, 1, 2, 10)), ("ref_star", 3, (3, 64, 3, 64)),
+                                             ("ref_sphere", 1, (3, 64, 3, 64)), ("c3_star", 1, (3, 64, 3, 64))])
+def test_variable_mu_loss_grad_on_device_records(name, level, dims):
+    """Loss and mean gradient on records the device builds for the spatially
+    varying coefficients (mu at face centres, at the stencil point s and at
+    r_G all differ) equal the oracle's on its own records, through the tile
+    kernel (1x1/2x10) and the MFMA kernels (3x64)."""
+    from jax_dips.records import Engine
+    from oracle import mlp as M
+    from oracle import trainer as T
+
+    prob = make(name)
+    axes, P, h, bounds, cb, orec = oracle_level(prob, 16, level)
+    B = 2048
+    grid, drec = device_records(prob, 16, level, B)
+    theta = M.init_params(dims, seed=13)
+    eng = Engine(dims, theta, {}, max_rows=B, max_steps=2)
+    for b in range(len(drec.batch_begin)):
+        if eng.tile_path:
+            loss, grad = eng.tile_loss_grad(grid, drec, b, 1.0 / B)
+        else:
+            loss, grad = eng.mfma_loss_grad(grid, drec, b, 1.0 / B)
+        ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, orec, np.arange(b * B, (b + 1) * B))
+        print(name, level, dims, b, f"loss {loss:.9e} {ol:.9e} grad rel {np.linalg.norm(grad - og) / np.linalg.norm(og):.2e}")
         assert abs(loss - ol) <= 1e-5 * abs(ol)
         assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)
 

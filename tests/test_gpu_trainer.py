@@ -21,7 +21,13 @@ MODEL = {"model_type": "mlp", "mlp": {"hidden_layers_m": 1, "hidden_dim_m": 1, "
 OPT = {"optimizer_name": "custom", "learning_rate": 1e-2, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
 
 
-def run_trainer(prob, n, epochs, batch, theta0, tmp_path):
+def mlp_dict(dims):
+    Lm, Hm, Lp, Hp = dims
+    return {"model_type": "mlp", "mlp": {"hidden_layers_m": Lm, "hidden_dim_m": Hm, "activation_m": "jnp.tanh",
+                                         "hidden_layers_p": Lp, "hidden_dim_p": Hp, "activation_p": "jnp.tanh"}}
+
+
+def run_trainer(prob, n, epochs, batch, theta0, tmp_path, model=MODEL, **kw):
     from jax_dips.domain import mesh
     from jax_dips.solvers.poisson import trainer
 
@@ -33,7 +39,7 @@ def run_trainer(prob, n, epochs, batch, theta0, tmp_path):
                             prob["alpha_fn"], prob["beta_fn"])
     state, solve_fn = init_fn(lvl_gstate=g, tr_gstate=g, eval_gstate=g, num_epochs=epochs, batch_size=batch,
                               checkpoint_dir=str(tmp_path / "ck"), results_dir=str(tmp_path), optimizer_dict=OPT,
-                              model_dict=MODEL, init_params=theta0)
+                              model_dict=model, init_params=theta0, **kw)
     new_state, epoch_store, loss_epochs = solve_fn(sim_state=state)
     return g, new_state, epoch_store, loss_epochs, solve_fn.trainer
 
@@ -60,23 +66,40 @@ def test_trainer_matches_oracle_loop(tmp_path):
 
 
 @pytest.mark.gpu
-def test_eval_gradients_match_finite_differences():
+@pytest.mark.parametrize("dims", [(1, 1, 2, 10), (3, 64, 3, 64), (3, 32, 3, 32), (1, 1, 3, 64)])
+def test_eval_gradients_match_finite_differences(dims):
+    """u against the float64 oracle network; grad u against central finite
+    differences of it (h = 1e-3); d_n u = grad u . n.  The first shape runs the
+    per-lane kernel, the others the MFMA tiles (16 points per wavefront, both
+    networks where a tile mixes phases; 517 points leave a partial tile)."""
     from jax_dips.evaluation import evaluate_points
     from jax_dips.nn.mlp.MLP import DoubleMLP
 
-    model = DoubleMLP(**MODEL)
+    model = DoubleMLP(**mlp_dict(dims))
     theta = M.init_params(model.dims, seed=9)
     rng = np.random.default_rng(0)
-    X = rng.uniform(-1, 1, (512, 3)).astype(np.float32)
-    phase = rng.uniform(size=512) > 0.5
-    u, gr = evaluate_points(model, theta, X, phase, want_grad=True)
-    assert np.max(np.abs(u - M.forward(theta, model.dims, X, phase))) <= 1e-6
+    m = 517
+    X = rng.uniform(-1, 1, (m, 3)).astype(np.float32)
+    phase = rng.uniform(size=m) > 0.5
+    phase[:64] = True  # whole tiles of one phase as well as mixed ones
+    phase[64:128] = False
+    nrm = rng.normal(size=(m, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    u, gr, gn = evaluate_points(model, theta, X, phase, normals=nrm)
+    u_or = M.forward(theta, model.dims, X, phase)
+    scale = max(1.0, float(np.max(np.abs(u_or))))
+    print(dims, "u err", np.max(np.abs(u - u_or)))
+    assert np.max(np.abs(u - u_or)) <= 2e-6 * scale
     eps = 1e-3
+    fd = np.zeros((m, 3))
     for a in range(3):
         e = np.zeros(3)
         e[a] = eps
-        fd = (M.forward(theta, model.dims, X + e, phase) - M.forward(theta, model.dims, X - e, phase)) / (2 * eps)
-        assert np.max(np.abs(gr[:, a] - fd)) <= 1e-4
+        fd[:, a] = (M.forward(theta, model.dims, X + e, phase) - M.forward(theta, model.dims, X - e, phase)) / (2 * eps)
+    gscale = max(1.0, float(np.max(np.abs(fd))))
+    print(dims, "grad err", np.max(np.abs(gr - fd)))
+    assert np.max(np.abs(gr - fd)) <= 1e-4 * gscale
+    assert np.max(np.abs(gn - (gr * nrm).sum(axis=1))) <= 1e-6 * gscale
 
 
 @pytest.mark.gpu
@@ -96,3 +119,72 @@ def test_trainer_padded_batches(tmp_path):
     assert np.all(np.abs(le - ole) <= 1e-4 * np.abs(ole))
     dev_theta = tr.model.flatten(tr.params)
     assert np.linalg.norm(dev_theta - th) <= 1e-4 * np.linalg.norm(th)
+
+
+@pytest.mark.gpu
+def test_trainer_wide_network_matches_oracle_loop(tmp_path):
+    """The drop-in trainer with 3x64 networks per phase (MFMA kernels for the
+    training steps and the evaluation): per-epoch losses and final parameters
+    follow the oracle loop, and solve_fn returns u, grad u and d_n u of the
+    trained network on the eval grid."""
+    prob = problems.star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0)
+    dims = (3, 64, 3, 64)
+    theta0 = M.init_params(dims, seed=7)
+    g, st, es, le, tr = run_trainer(prob, 16, 4, 2048, theta0, tmp_path, model=mlp_dict(dims))
+    axes, P, h, bounds = grid_points(16)
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    th, ole, steps, _ = T.train(P, h, bounds, cb, dims, theta0, 4, 2048, OPT)
+    print("device", le, "oracle", ole, "rel", np.abs(le - ole) / np.abs(ole))
+    assert np.all(np.abs(le - ole) <= 1e-5 * np.abs(ole))
+    dev_theta = tr.model.flatten(tr.params)
+    assert np.linalg.norm(dev_theta - th) <= 1e-4 * np.linalg.norm(th)
+    phase = cb("phi", P) >= 0
+    u_or = M.forward(dev_theta, dims, P, phase)
+    assert np.max(np.abs(st.solution - u_or)) <= 2e-6 * max(1.0, np.max(np.abs(u_or)))
+    assert st.grad_solution.shape == (P.shape[0], 3) and np.all(np.isfinite(st.grad_normal_solution))
+
+
+@pytest.mark.gpu
+def test_restart_after_sigint_matches_uninterrupted_run(tmp_path, monkeypatch):
+    """SIGINT sent from the host after epoch 3 of 8 stops training after that
+    epoch (the flag of trainer.py:68-78); the checkpoint written at the stop,
+    checkpoint_3, resumes with restart=True and the same num_epochs
+    (trainer.py:210-220, :327-351).  The resumed run's epoch_store is 3..7, its
+    loss_epochs equal the uninterrupted run's epochs 3..7 and its final
+    parameters equal the uninterrupted run's, bit for bit (the same kernels
+    with fixed-order reductions, the optimizer state and step count restored
+    from the checkpoint)."""
+    import os
+    import signal
+
+    from jax_dips import records
+
+    prob = problems.sphere()
+    dims = (1, 1, 2, 10)
+    theta0 = M.init_params(dims, seed=42)
+    _, _, es_full, le_full, tr_full = run_trainer(prob, 16, 8, 1024, theta0, tmp_path / "full")
+    calls = {"n": 0}
+    orig = records.Engine.train_level
+
+    def interrupt_after_third(self, grid, rec, nepochs):
+        orig(self, grid, rec, nepochs)
+        calls["n"] += 1
+        if calls["n"] == 3:
+            os.kill(os.getpid(), signal.SIGINT)
+
+    monkeypatch.setattr(records.Engine, "train_level", interrupt_after_third)
+    part = tmp_path / "part"
+    _, _, es1, le1, tr1 = run_trainer(prob, 16, 8, 1024, theta0, part)
+    monkeypatch.setattr(records.Engine, "train_level", orig)
+    assert calls["n"] == 3
+    assert sorted(os.listdir(part / "ck")) == ["checkpoint_3"]
+    assert tr1.steps == 3 * 4
+    assert np.array_equal(le1[:3], le_full[:3])
+    _, st2, es2, le2, tr2 = run_trainer(prob, 16, 8, 1024, None, tmp_path / "resume", restart=True,
+                                        restart_checkpoint_dir=str(part / "ck"))
+    print("full", le_full, "resumed", le2)
+    assert np.array_equal(es2, np.arange(3, 8))
+    assert tr2.steps == 5 * 4
+    assert np.array_equal(le2, le_full[3:])
+    assert np.array_equal(tr2.model.flatten(tr2.params), tr_full.model.flatten(tr_full.params))
+    assert sorted(os.listdir(tmp_path / "resume" / "ck")) == ["checkpoint_8"]

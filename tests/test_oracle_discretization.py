@@ -162,3 +162,96 @@ def test_beta_not_finite_at_origin_zeroes_crossed_rhs():
     lhs_part = s1["c"] + np.nan_to_num(s1["rhs"] / s1["diag"])
     assert np.allclose(s2["c"][cut], lhs_part[cut])
     assert np.allclose(s2["c"][~cut], s1["c"][~cut])
+
+
+def _linear(c0, g):
+    g = np.asarray(g, np.float64)
+    return lambda r: c0 + g[0] * r[0] + g[1] * r[1] + g[2] * r[2]
+
+
+@pytest.mark.parametrize("mbig,pos", [(False, True), (False, False), (True, True), (True, False)])
+def test_unequal_mu_ghost_values_closed_form(mbig, pos):
+    """The four ghost-value branches of get_u_mp_by_regression_at_point_fn
+    (discretization.py:453-514) for mu- != mu+, pinned by closed forms.
+
+    Planar phi = n.(x - x0), two arbitrary linear functions u-, u+ (the network
+    values U_q are u+ where phi >= 0, u- elsewhere), linear alpha, beta, mu-,
+    mu+.  Least squares reproduces a linear field exactly, so with
+    D(U - u_c) = grad u on the matching side and c = n . D 1 (the fitted
+    gradient of the constant 1, computed here with numpy.linalg.lstsq on the
+    masked stencil rows, not with the oracle's pinv), Delta = u+(s) - u-(s),
+    J- = alpha + delta beta/mu- and J+ = alpha + delta beta/mu+ at
+    r_G = s - delta n:
+      mu- <= mu+, delta > 0:  u- = u+(s) - k dn u+ - J-,       k = (mu+ - mu-)/mu- delta
+      mu- <= mu+, delta <= 0: u+ = u-(s) + (z Delta - k dn u+ + J-)/(1 + z),  z = -k c+
+      mu- > mu+,  delta > 0:  u- = u+(s) - (k dn u- + z Delta)/(1 - z) - (1 - 2z)/(1 - z) J+,
+                              k = (mu+ - mu-)/mu+ delta, z = -k c-
+      mu- > mu+,  delta <= 0: u+ = u-(s) - k dn u- + J+
+    and the other value is u_c.  The first-order term has the opposite sign to
+    a Taylor expansion from r_G (u-(s) = u+(s) + k dn u+ - J- when the jump
+    conditions hold), which the oracle reproduces as the reference writes it."""
+    rng = np.random.default_rng(11 + 2 * mbig + pos)
+    ntest = 0
+    for trial in range(60):
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        x0 = rng.uniform(-0.3, 0.3, 3)
+        hs = np.float32(rng.uniform(0.03, 0.12))
+        h = np.array([hs, hs, hs], np.float32)
+        delta_want = rng.uniform(0.05, 0.45) * float(hs) * (1 if pos else -1)
+        tang = rng.normal(size=3)
+        tang -= (tang @ nrm) * nrm
+        s = (x0 + delta_want * nrm + 0.2 * tang).astype(np.float32)[None]
+        gm, gp = rng.normal(size=3), rng.normal(size=3)
+        cm0, cp0 = rng.normal(), rng.normal()
+        # mu linear and positive near s, ordered as the branch needs at s
+        lo, hi = rng.uniform(1.0, 2.0), rng.uniform(3.0, 8.0)
+        mu_m0, mu_p0 = (hi, lo) if mbig else (lo, hi)
+        gmu_m, gmu_p = rng.normal(size=3) * 0.3, rng.normal(size=3) * 0.3
+        sv = s[0].astype(np.float64)
+        fns = dict(phi=lambda r: nrm[0] * (r[0] - x0[0]) + nrm[1] * (r[1] - x0[1]) + nrm[2] * (r[2] - x0[2]),
+                   mu_m=_linear(mu_m0 - gmu_m @ sv, gmu_m), mu_p=_linear(mu_p0 - gmu_p @ sv, gmu_p),
+                   alpha=_linear(rng.normal(), rng.normal(size=3)), beta=_linear(rng.normal(), rng.normal(size=3)))
+        cb = D.Callbacks(fns, np.float64)
+        a_m, a_p, b_m, b_p, aux = D.u_mp_forms(s, h, cb, np.float64)
+        if aux["cut"].size != 1:
+            continue
+        pos_q = (s[0][None, :] + (D.CUBE * h[None, :]).astype(np.float32)).astype(np.float32).astype(np.float64)
+        phi_q = cb("phi", pos_q)
+        um_f, up_f = _linear(cm0, gm), _linear(cp0, gp)
+        U = np.where(phi_q >= 0, up_f(pos_q.T), um_f(pos_q.T))
+        u_m, u_p = a_m[0] @ U + b_m[0], a_p[0] @ U + b_p[0]
+        delta = float(cb("phi", s)[0])
+        assert (delta > 0) == pos
+        X = (D.CUBE * h[None, :]).astype(np.float32).astype(np.float64)
+        side = phi_q > 0 if not mbig else phi_q < 0
+        if np.linalg.matrix_rank(X[side]) < 3:
+            continue
+        c1 = float(nrm @ np.linalg.lstsq(X[side], np.ones(int(side.sum())), rcond=None)[0])
+        rG = (sv - delta * nrm)[:, None]
+        al, be = fns["alpha"](rG)[0], fns["beta"](rG)[0]
+        Jm = al + delta * be / fns["mu_m"](rG)[0]
+        Jp = al + delta * be / fns["mu_p"](rG)[0]
+        mm, mp = fns["mu_m"](sv[:, None])[0], fns["mu_p"](sv[:, None])[0]
+        assert (mm > mp) == mbig
+        ups, ums = up_f(sv[:, None])[0], um_f(sv[:, None])[0]
+        Dl = ups - ums
+        if not mbig:
+            k = (mp - mm) / mm * delta
+            z = -k * c1
+            if pos:
+                want_m, want_p = ups - k * (nrm @ gp) - Jm, ups
+            else:
+                want_m, want_p = ums, ums + (z * Dl - k * (nrm @ gp) + Jm) / (1 + z)
+        else:
+            k = (mp - mm) / mp * delta
+            z = -k * c1
+            if pos:
+                want_m, want_p = ups - (k * (nrm @ gm) + z * Dl) / (1 - z) - (1 - 2 * z) / (1 - z) * Jp, ups
+            else:
+                want_m, want_p = ums, ums - k * (nrm @ gm) + Jp
+        scale = 1.0 + abs(want_m) + abs(want_p)
+        assert abs(u_m - want_m) <= 2e-5 * scale, (trial, u_m, want_m)
+        assert abs(u_p - want_p) <= 2e-5 * scale, (trial, u_p, want_p)
+        ntest += 1
+    assert ntest >= 20
