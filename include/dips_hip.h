@@ -300,6 +300,18 @@ int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
                          const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
                          float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* work,
                          int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream);
+/* dips_mfma_train_step without a workspace argument and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The workspace is
+ * a fixed device array of the library (6 Mi floats + 8 Ki doubles, 24.1 MiB,
+ * one copy per device, held until the process ends): a batch needing more
+ * (dips_mfma_workspace floats, rows/256 + P/256 + 1 doubles; e.g. 2^20-row
+ * batches) returns DIPS_EINVAL.  The device of `stream` must be current.
+ * Calls on one device share the array: calls that overlap on one device race.
+ * Same results as dips_mfma_train_step. */
+int dips_mfma_train_step_noalloc(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                                 const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
+                                 float* v, int32_t* count, float* loss_hist, int32_t loss_cap, void* stream);
 /* Loss (device double *loss) and mean gradient (device grad[P]) of the batch
  * through the MFMA kernels, without an optimizer update. */
 int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,

@@ -388,6 +388,27 @@ int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
                    dwork, dwork_doubles, nullptr, nullptr, (hipStream_t)stream);
 }
 
+int dips_mfma_train_step_noalloc(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
+                                 const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
+                                 float* v, int32_t* count, float* loss_hist, int32_t loss_cap, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if ((st = check_records(grid, rec))) return st;
+  if (!batch || !opt || !params || !m || !v || !count || !loss_hist || batch->nrows < 1 || batch->row_begin < 0 ||
+      batch->row_begin + batch->nrows > rec->nrows || batch->ent_begin < 0 || batch->ent_end > rec->nnz ||
+      batch->ent_begin > batch->ent_end || batch->t_begin < 0 || batch->t_begin + batch->t_count > rec->nnz ||
+      (batch->t_count > 0 && (!batch->t_row || !batch->t_code || !batch->t_w || !batch->t_csr))) {
+    set_error("dips_mfma_train_step_noalloc: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  if (!mfma_supported(net)) {
+    set_error("dips_mfma_train_step_noalloc: network shape has no MFMA kernel");
+    return DIPS_EUNSUPPORTED;
+  }
+  return mfma_step_noalloc(grid, rec, batch, net, opt, inv_batch, params, m, v, count, loss_hist, loss_cap,
+                           (hipStream_t)stream);
+}
+
 int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
                         const dips_net_t* net, float inv_batch, const float* params, float* grad, double* loss,
                         float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {

@@ -21,6 +21,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "dips_common.h"
@@ -937,6 +939,62 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                      loss_part, nrb, inv_b, *opt, params, m, v, count, loss_hist, loss_cap);
   hipLaunchKernelGGL(count_inc_kernel, dim3(1), dim3(64), 0, st, count);
   return hip_status(hipGetLastError(), "mfma reduce/apply launch");
+}
+
+// ---- the same step without a caller-given workspace ------------------------------
+// The float workspace of mfma_step (u per entry, r per row, per-workgroup
+// gradient partials, g) and its float64 partials live in one file-scope device
+// array, kNoAllocFloats floats followed by kNoAllocDoubles doubles; each device
+// holds its own copy until the process ends.  Every value a step reads from it
+// is written earlier in the same step, so nothing is cleared between calls;
+// calls that overlap on one device share the array and race.
+constexpr int64_t kNoAllocFloats = int64_t(6) << 20;  // 24 MiB
+constexpr int64_t kNoAllocDoubles = int64_t(1) << 13;  // 64 KiB
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocFloats + 2 * kNoAllocDoubles];
+
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+// device address of the current device's copy of scratch, looked up once per device
+static int scratch_of_current_device(float** out) {
+  int dev = -1;
+  int s = hip_status(hipGetDevice(&dev), "dips_mfma_train_step_noalloc: hipGetDevice");
+  if (s) return s;
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error("dips_mfma_train_step_noalloc: device index out of range");
+    return DIPS_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    s = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                   "dips_mfma_train_step_noalloc: hipGetSymbolAddress(scratch)");
+    if (s) return s;
+    g_scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return DIPS_OK;
+}
+
+int mfma_step_noalloc(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt,
+                      const dips_net_t* net, const dips_opt_t* opt, float inv_b, float* params, float* m, float* v,
+                      int32_t* count, float* loss_hist, int loss_cap, hipStream_t st) {
+  const int64_t nrows = bt->nrows, nent = bt->ent_end - bt->ent_begin;
+  const int64_t floats = mfma_workspace_floats(net, nrows, nent);
+  const int64_t P = net_params(net->Lm, net->Hm) + net_params(net->Lp, net->Hp);
+  const int64_t doubles = (nrows + kMT - 1) / kMT + (P + 255) / 256 + 1;
+  if (floats > kNoAllocFloats || doubles > kNoAllocDoubles) {
+    set_error("dips_mfma_train_step_noalloc: the batch needs " + std::to_string(floats) + " floats and " +
+              std::to_string(doubles) + " doubles of scratch, beyond the fixed " + std::to_string(kNoAllocFloats) +
+              " and " + std::to_string(kNoAllocDoubles) + " (use dips_mfma_train_step with a workspace)");
+    return DIPS_EINVAL;
+  }
+  float* base = nullptr;
+  const int s = scratch_of_current_device(&base);
+  if (s) return s;
+  return mfma_step(g, rec, bt, net, opt, inv_b, params, m, v, count, loss_hist, loss_cap, base, kNoAllocFloats,
+                   reinterpret_cast<double*>(base + kNoAllocFloats), kNoAllocDoubles, nullptr, nullptr, st);
 }
 
 }  // namespace dips

@@ -131,6 +131,9 @@ SIGNATURES = {
     "dips_mfma_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
                                             ctypes.POINTER(Net), ctypes.POINTER(Opt), ctypes.c_float, _P, _P, _P, _P,
                                             _P, _I32, _P, _I64, _P, _I64, _P]),
+    "dips_mfma_train_step_noalloc": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records),
+                                                    ctypes.POINTER(MfmaBatch), ctypes.POINTER(Net), ctypes.POINTER(Opt),
+                                                    ctypes.c_float, _P, _P, _P, _P, _P, _I32, _P]),
     "dips_mfma_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
                                            ctypes.POINTER(Net), ctypes.c_float, _P, _P, _P, _P, _I64, _P, _I64, _P]),
     "dips_hash_table_rows": (_I64, [ctypes.POINTER(HashGrid)]),
