@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--epochs-per-level", type=int, default=3, help="sweep: timed epochs per zoom level")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy, 5 times (for scaling HBM counters)")
+    ap.add_argument("--mfma-groups", default="on", choices=["on", "off"],
+                    help="MFMA path: evaluate the network once per group of record entries at one position (on), "
+                         "or once per entry (off)")
     ap.add_argument("--path", default="auto", choices=["auto", "tile", "lane", "mfma"],
                     help="training-step kernels: packed-tile fused step (auto for the small networks), "
                          "lane-per-row fused step, or MFMA layer tiles (auto for the wide networks)")
@@ -169,9 +172,11 @@ def main():
     else:
         path = args.path
     use_mfma = path == "mfma"
+    # MFMA path: entries at one position evaluated once (groups of the position-ordered batches)
+    mgrid = grid if args.mfma_groups == "on" else None
     if use_mfma:
         for rec in levels:
-            eng.mfma_workspace(rec)
+            eng.mfma_workspace(rec, mgrid)
     if path == "tile":
         for rec in levels:
             rec.tiles(grid)
@@ -182,7 +187,7 @@ def main():
         for li, rec in enumerate(levels):
             idx = np.arange(per) % nb
             if use_mfma:
-                mb = rec.mfma_batches()["batches"]
+                mb = rec.mfma_batches(mgrid)["batches"]
                 for b in idx:
                     N.check(L.dips_mfma_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb[b]),
                                                    ctypes.byref(eng.net), ctypes.byref(eng.opt),
@@ -227,29 +232,38 @@ def main():
     value = K * B / (ms / 1e3)
     loss_tail = eng.loss_hist[: int(eng.count.item())][-4:].cpu().numpy().tolist()
 
-    # roofline of the dominant kernel: the fused residual/loss/gradient launch
-    per_launch_flops = []
+    # roofline of the dominant kernel: the fused residual/loss/gradient launch.  FLOPs per launch: 3 F_fwd
+    # per network evaluation of the launch -- per record entry, or per group of entries at one position
+    # where the MFMA kernels evaluate a group once; the per-entry count is reported beside it
+    fp_p, fp_m = mlp_fwd_flops(dims[2], dims[3]), mlp_fwd_flops(dims[0], dims[1])
+    per_launch_flops, per_launch_entry_flops = [], []
     for rec in levels:
         codes = rec.ent_code[: rec.nnz].cpu().numpy()
         ptr = rec.row_ptr.cpu().numpy()
         ph = (codes >> 7) & 1
-        fl = np.where(ph == 1, mlp_fwd_flops(dims[2], dims[3]), mlp_fwd_flops(dims[0], dims[1])) * 3.0
+        fl = np.where(ph == 1, fp_p, fp_m) * 3.0
         cum = np.concatenate([[0.0], np.cumsum(fl)])
-        for b0, nr in zip(rec.batch_begin, rec.batch_rows):
-            per_launch_flops.append(cum[ptr[b0 + nr]] - cum[ptr[b0]])
+        for bi, (b0, nr) in enumerate(zip(rec.batch_begin, rec.batch_rows)):
+            per_launch_entry_flops.append(cum[ptr[b0 + nr]] - cum[ptr[b0]])
+            if use_mfma and mgrid is not None:
+                bt = rec.mfma_batches(mgrid)["batches"][bi]
+                per_launch_flops.append(3.0 * (bt.g_plus * fp_p + bt.g_minus * fp_m))
+            else:
+                per_launch_flops.append(per_launch_entry_flops[-1])
     R = max(4, args.roofline_launches - args.roofline_launches % 4)
     nblk_rl = ctypes.c_int32(0)
     torch.cuda.synchronize()
     r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    flops_sum = 0.0
+    flops_sum = entry_flops_sum = 0.0
     r0.record()
     for i in range(R):
         li = (4 * i) // R
         rec = levels[li]
         b = i % nb
         flops_sum += per_launch_flops[li * nb + b]
+        entry_flops_sum += per_launch_entry_flops[li * nb + b]
         if use_mfma:
-            mb = rec.mfma_batches()["batches"]
+            mb = rec.mfma_batches(mgrid)["batches"]
             N.check(L.dips_mfma_train_step(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb[b]),
                                            ctypes.byref(eng.net), ctypes.byref(eng.opt), ctypes.c_float(1.0 / B),
                                            N.ptr(eng.params), N.ptr(eng.m), N.ptr(eng.v), N.ptr(eng.count),
@@ -313,7 +327,9 @@ def main():
                                         "runs after it in a one-workgroup tile_apply_kernel); FP32 VALU "
                                         "bound (non-packed VALU ceiling 78.6 TFLOP/s)"}[path],
                      "avg_launch_us": avg_s * 1e6,
-                     "flop_per_launch": flops_sum / R},
+                     "flop_per_launch": flops_sum / R,
+                     "entry_flop_per_launch": entry_flops_sum / R,
+                     "entry_rate_tflops": entry_flops_sum / R / avg_s / 1e12},
         "records_build_s": t_records,
         "loss_tail": loss_tail,
     }

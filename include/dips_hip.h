@@ -286,6 +286,14 @@ typedef struct {
   int64_t n_plus, n_minus;    /* + entries [t_begin, t_begin+n_plus), then - entries */
   int64_t t_count;            /* n_plus + n_minus; must equal ent_end - ent_begin (every entry of the batch
                                  listed once), else the step entry points return DIPS_EINVAL */
+  /* Optional groups of entries at one position (bit-identical fl32(p + o h), hence one phase): the MFMA
+   * kernels evaluate a group's network once, write its u to every member and sum the members' seeds
+   * r(row) w.  Group g = phase-sorted entries [t_group[g], t_group[g+1]); this batch's groups are
+   * [g_begin, g_begin + g_plus) (+ entries) then the next g_minus (- entries), and must cover its
+   * entries exactly.  t_group = NULL: every entry is evaluated on its own (the hashgrid entry points
+   * ignore the groups). */
+  const int32_t* t_group;
+  int64_t g_begin, g_plus, g_minus;
 } dips_mfma_batch_t;
 
 /* 1 if (Lm, ceil(Hm/16), Lp, ceil(Hp/16)) has an MFMA kernel. */

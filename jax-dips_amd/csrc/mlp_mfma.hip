@@ -24,6 +24,8 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "dips_common.h"
 #include "dips_hip.h"
@@ -34,7 +36,7 @@ namespace dips {
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMT = 256;  // threads per workgroup (4 wavefronts)
-constexpr int kTP = 17;   // padded row length of a transposed 16-position tile
+constexpr int kTP = 20;   // padded row length of a transposed 16-position tile (16-byte aligned rows)
 constexpr int kMfmaBwdBlocks = 256;   // workgroups of the backward kernel (both phases), one per CU
 constexpr int kMfmaFwdBlocks = 1024;  // workgroups of the forward kernel (both phases), four per CU
 
@@ -115,7 +117,7 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
     f4 c = {0.f, 0.f, 0.f, 0.f};
     c = mfma4(W[PK::l0() + ob * 64 + lane], xin, c);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh(c[r]);
+    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh_fast(c[r]);
   }
 #pragma unroll
   for (int l = 1; l < L; ++l) {
@@ -137,7 +139,7 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
 #pragma unroll
     for (int ob = 0; ob < NB; ++ob) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) act[l][ob * 4 + r] = dips_tanh(c[ob][r]);
+      for (int r = 0; r < 4; ++r) act[l][ob * 4 + r] = dips_tanh_fast(c[ob][r]);
     }
   }
 }
@@ -167,7 +169,20 @@ struct MfmaTileArgs {
   const uint32_t* t_code;    // phase-sorted entries: code (bit 7 = phase)
   const float* t_w;          // phase-sorted entries: weight
   const int32_t* t_csr;      // phase-sorted entries: CSR entry index
+  const int32_t* t_group;    // items = groups of entries at one position: group g is the phase-sorted entries
+                             // [t_group[g], t_group[g+1]); null: every entry is an item of its own
 };
+
+// phase-sorted entries [b, e) of item `item`
+__device__ __forceinline__ void item_members(const MfmaTileArgs& T, int64_t item, int64_t& b, int64_t& e) {
+  if (T.t_group) {
+    b = T.t_group[item];
+    e = T.t_group[item + 1];
+  } else {
+    b = item;
+    e = item + 1;
+  }
+}
 
 // position of phase-sorted entry e; valid = false for padding lanes
 __device__ __forceinline__ bool entry_pos(const MfmaTileArgs& T, int64_t e, int64_t e_end, float& x, float& y, float& z,
@@ -188,11 +203,16 @@ __device__ __forceinline__ bool entry_pos(const MfmaTileArgs& T, int64_t e, int6
 }
 
 // ---- kernel 1: u for every entry of the batch ----------------------------------
-// Persistent: workgroups [0, nblk_p) take the + tiles (+ entries [e0, e0+np)),
+// Persistent: workgroups [0, nblk_p) take the + tiles (+ items [e0, e0+np)),
 // the others the - tiles; each packs its network's weights into LDS once and
-// its wavefronts stride over the tiles.
+// its wavefronts stride over the tiles.  An item (an entry, or a group of
+// entries at one position) is evaluated once and its u written to the CSR
+// slot of every member entry.
+constexpr int kFT = 512;  // threads per forward workgroup: 8 wavefronts share one packed weight copy
+constexpr int kFW = kFT / 64;
+
 template <int Lm, int NBm, int Lp, int NBp>
-__global__ __launch_bounds__(kMT) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
+__global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_out,
                                                        int nblk_p) {
   extern __shared__ float lds[];
@@ -209,11 +229,13 @@ __global__ __launch_bounds__(kMT) void mfma_fwd_kernel(MfmaTileArgs T, const flo
   __syncthreads();
   const int64_t eb = plus ? e0 : e0 + np_;
   const int64_t ee = plus ? e0 + np_ : e0 + np_ + nm_;
-  for (int64_t tile = (int64_t)bid * 4 + wave; tile < ntiles; tile += (int64_t)nblk * 4) {
-    const int64_t ebeg = eb + tile * 16;
+  for (int64_t tile = (int64_t)bid * kFW + wave; tile < ntiles; tile += (int64_t)nblk * kFW) {
+    const int64_t item = eb + tile * 16 + (lane & 15);
+    int64_t mb = 0, me = 0;
+    if (item < ee) item_members(T, item, mb, me);
     float x, y, z;
     int row;
-    const bool valid = entry_pos(T, ebeg + (lane & 15), ee, x, y, z, row);
+    const bool valid = entry_pos(T, mb, me, x, y, z, row);
     const int k = lane / 16;
     const float xin = (k == 0) ? x : (k == 1) ? y : (k == 2) ? z : 1.0f;
     float u;
@@ -226,7 +248,8 @@ __global__ __launch_bounds__(kMT) void mfma_fwd_kernel(MfmaTileArgs T, const flo
       tile_forward<Lm, NBm, false>(lds, xin, act);
       u = tile_output<Lm, NBm, false>(lds, act);
     }
-    if (valid && lane < 16) u_out[T.t_csr[ebeg + lane]] = u;
+    if (valid && lane < 16)
+      for (int64_t e = mb; e < me; ++e) u_out[T.t_csr[e]] = u;
   }
 }
 
@@ -300,6 +323,8 @@ __device__ __forceinline__ float row16_sum(float v) {
 }
 
 // D += A^T B over the kBW staged tiles of the workgroup: A rows arow.., B rows brow..
+// The 4 contraction steps of lane group kq take positions 4 kq .. 4 kq + 3 of a
+// tile, one 16-byte LDS read per operand (rows are kTP = 20 floats, 16-byte aligned).
 __device__ __forceinline__ f4 coop_block(const float* __restrict__ stage, int stride_w, int arow, int brow, bool a_inputs,
                                          f4 acc) {
   const int lane = threadIdx.x & 63;
@@ -308,13 +333,14 @@ __device__ __forceinline__ f4 coop_block(const float* __restrict__ stage, int st
   for (int s = 0; s < kBW; ++s) {
     const float* A = stage + s * stride_w;
     const float* D = A + stride_w / 2;
+    const f4 b4 = *reinterpret_cast<const f4*>(&D[(brow + m) * kTP + 4 * kq]);
+    f4 a4;
+    if (a_inputs)
+      a4 = m < 4 ? *reinterpret_cast<const f4*>(&A[m * kTP + 4 * kq]) : f4{0.f, 0.f, 0.f, 0.f};
+    else
+      a4 = *reinterpret_cast<const f4*>(&A[(arow + m) * kTP + 4 * kq]);
 #pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const int pos = 4 * st + kq;
-      const float a = a_inputs ? (m < 4 ? A[m * kTP + pos] : 0.0f) : A[(arow + m) * kTP + pos];
-      const float b = D[(brow + m) * kTP + pos];
-      acc = mfma4(a, b, acc);
-    }
+    for (int st = 0; st < 4; ++st) acc = mfma4(a4[st], b4[st], acc);
   }
   return acc;
 }
@@ -458,12 +484,13 @@ __global__ __launch_bounds__(kBT) void mfma_bwd_kernel(MfmaTileArgs T, const flo
                                                        int64_t e0, int64_t np_, int64_t nm_, int64_t row_begin,
                                                        const float* __restrict__ r_row, float* __restrict__ gpart,
                                                        int nblk_p, int stride) {
-  extern __shared__ float lds[];
+  extern __shared__ __align__(16) float lds[];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool plus = (int)blockIdx.x < nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
-  constexpr int PKsz = (Packed<Lp, NBp>::size() > Packed<Lm, NBm>::size()) ? Packed<Lp, NBp>::size()
-                                                                            : Packed<Lm, NBm>::size();
+  // the operand stage starts on a 16-byte boundary after the packed weights
+  constexpr int PKsz = ((Packed<Lp, NBp>::size() > Packed<Lm, NBm>::size() ? Packed<Lp, NBp>::size()
+                                                                           : Packed<Lm, NBm>::size()) + 3) & ~3;
   constexpr int NBx = NBp > NBm ? NBp : NBm;
   constexpr int kStrideW = 2 * 16 * NBx * kTP;  // floats of one wavefront's operand pair
   float* stage = lds + PKsz;
@@ -498,11 +525,18 @@ __global__ __launch_bounds__(kBT) void mfma_bwd_kernel(MfmaTileArgs T, const flo
     const int64_t rounds = (ntiles + per_round - 1) / per_round;
     for (int64_t rd = 0; rd < rounds; ++rd) {
       const int64_t tile = (rd * nblk + bid) * kBW + wave;
-      const int64_t ebeg = eb + tile * 16;
+      const int64_t item = eb + tile * 16 + (lane & 15);
+      int64_t mb = 0, me = 0;
+      if (item < ee) item_members(T, item, mb, me);
       float x, y, z;
       int row;
-      const bool valid = entry_pos(T, ebeg + (lane & 15), ee, x, y, z, row);
-      const float seed = valid ? r_row[row - row_begin] * T.t_w[ebeg + (lane & 15)] : 0.0f;
+      const bool valid = entry_pos(T, mb, me, x, y, z, row);
+      // seed of the item: sum over its member entries of r(row) * w (entries in position order)
+      float seed = 0.0f;
+      if (valid) {
+        seed = r_row[row - row_begin] * T.t_w[mb];
+        for (int64_t e = mb + 1; e < me; ++e) seed = fmaf(r_row[T.t_row[e] - row_begin], T.t_w[e], seed);
+      }
       const int kq = lane / 16;
       const float xin = (kq == 0) ? x : (kq == 1) ? y : (kq == 2) ? z : 1.0f;
       float act[L][4 * NB];
@@ -630,7 +664,7 @@ __device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, flo
     c = mfma4(W[PK::l0() + ob * 64 + lane], xin, c);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float v = dips_tanh(c[r]);
+      const float v = dips_tanh_fast(c[r]);
       const float g = 1.0f - v * v;
       a[ob * 4 + r] = v;
 #pragma unroll
@@ -664,7 +698,7 @@ __device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, flo
     for (int ob = 0; ob < NB; ++ob) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float v = dips_tanh(c[ob][r]);
+        const float v = dips_tanh_fast(c[ob][r]);
         const float g = 1.0f - v * v;
         a[ob * 4 + r] = v;
 #pragma unroll
@@ -840,6 +874,27 @@ int mfma_eval_launch(const dips_net_t* net, const float* params, const float* pt
   return DIPS_EUNSUPPORTED;
 }
 
+// Workgroups of a persistent kernel resident at once on the current device
+// (compute units x occupancy), looked up once per kernel and device; fallback
+// if the runtime cannot say.  A persistent grid of exactly this size runs in
+// one wave of workgroups, so no workgroup starts after others have finished.
+static int resident_blocks(const void* fn, int threads, size_t lds, int fallback) {
+  static std::mutex mu;
+  static std::vector<std::pair<std::pair<const void*, int>, int>> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fallback;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const auto& e : cache)
+    if (e.first.first == fn && e.first.second == dev) return e.second;
+  int per_cu = 0, cus = 0;
+  int n = fallback;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 && cus > 0)
+    n = per_cu * cus;
+  cache.push_back({{fn, dev}, n});
+  return n;
+}
+
 // opt == nullptr: stop after the reduction (gradient in grad_out, loss in loss_out)
 int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, const dips_net_t* net,
               const dips_opt_t* opt, float inv_b, float* params, float* m, float* v, int32_t* count, float* loss_hist,
@@ -877,10 +932,19 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("dips_mfma_train_step: float64 workspace too small");
     return DIPS_EINVAL;
   }
+  // items of the forward and backward tiles: the batch's groups of entries at one position when it lists
+  // them (each evaluated once), else its entries
+  const bool grouped = bt->t_group != nullptr;
+  if (grouped && (bt->g_begin < 0 || bt->g_plus < 0 || bt->g_minus < 0 || bt->g_plus + bt->g_minus > bt->t_count)) {
+    set_error("dips_mfma_train_step: invalid entry groups");
+    return DIPS_EINVAL;
+  }
+  const int64_t i0 = grouped ? bt->g_begin : bt->t_begin;
+  const int64_t ip = grouped ? bt->g_plus : np_, im = grouped ? bt->g_minus : nm_;
   MfmaTileArgs T{point_set(g), rec->hx, rec->hy, rec->hz, rec->row_point, bt->t_row, bt->t_code, bt->t_w,
-                 bt->t_csr};
+                 bt->t_csr, bt->t_group};
   const int a = Lm, b = nb_of(Hm), c = Lp, d = nb_of(Hp);
-  const int64_t ntp = (np_ + 15) / 16, ntm = (nm_ + 15) / 16;
+  const int64_t ntp = (ip + 15) / 16, ntm = (im + 15) / 16;
   // workgroups split by phase in proportion to the tiles (each at least 1 if it has tiles)
   auto split = [&](int total, int& p, int& m) {
     p = (ntp > 0) ? (int)std::max<int64_t>(1, (int64_t)((double)total * ntp / std::max<int64_t>(1, ntp + ntm))) : 0;
@@ -889,12 +953,12 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   };
   int nbp, nbm, nfp, nfm;
   split(kMfmaBwdBlocks, nbp, nbm);
-  split(kMfmaFwdBlocks, nfp, nfm);
   bool launched = false;
 #define X(A, B, C, D)                                                                                              \
   if (!launched && a == A && b == B && c == C && d == D) {                                                      \
     launched = true;                                                                                               \
-    constexpr int PK = Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size() : Packed<A, B>::size(); \
+    constexpr int PK = ((Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size()                       \
+                                                                     : Packed<A, B>::size()) + 3) & ~3;           \
     constexpr int PKF = Packed<C, D, false>::size() > Packed<A, B, false>::size() ? Packed<C, D, false>::size()   \
                                                                                   : Packed<A, B, false>::size();  \
     constexpr int NBX = D > B ? D : B;                                                                             \
@@ -910,14 +974,15 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                         (int)lds_f);                                                                               \
     hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                         (int)bwd_lds);                                                                             \
+    split(resident_blocks((const void*)mfma_fwd_kernel<A, B, C, D>, kFT, lds_f, kMfmaFwdBlocks), nfp, nfm);       \
     if (nfp + nfm > 0)                                                                                             \
-      hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kMT), lds_f, st, T, params, Hm, Hp,   \
-                         bt->t_begin, np_, nm_, u - bt->ent_begin, nfp);                                           \
+      hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kFT), lds_f, st, T, params, Hm, Hp,   \
+                         i0, ip, im, u - bt->ent_begin, nfp);                                                      \
     hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, rec->row_ptr, rec->row_c, bt->row_begin,   \
                        nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part);                              \
     if (nbp + nbm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kBT), bwd_lds, st, T,                 \
-                         params, Hm, Hp, bt->t_begin, np_, nm_, bt->row_begin, r, gpart, nbp, stride);             \
+                         params, Hm, Hp, i0, ip, im, bt->row_begin, r, gpart, nbp, stride);                        \
   }
   DIPS_MFMA_SHAPES(X)
 #undef X

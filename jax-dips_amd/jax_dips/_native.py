@@ -46,7 +46,9 @@ class MfmaBatch(ctypes.Structure):
     _fields_ = [("row_begin", ctypes.c_int64), ("nrows", ctypes.c_int64), ("ent_begin", ctypes.c_int64),
                 ("ent_end", ctypes.c_int64), ("t_row", ctypes.c_void_p), ("t_code", ctypes.c_void_p),
                 ("t_w", ctypes.c_void_p), ("t_csr", ctypes.c_void_p), ("t_begin", ctypes.c_int64),
-                ("n_plus", ctypes.c_int64), ("n_minus", ctypes.c_int64), ("t_count", ctypes.c_int64)]
+                ("n_plus", ctypes.c_int64), ("n_minus", ctypes.c_int64), ("t_count", ctypes.c_int64),
+                ("t_group", ctypes.c_void_p), ("g_begin", ctypes.c_int64), ("g_plus", ctypes.c_int64),
+                ("g_minus", ctypes.c_int64)]
 
 
 class Tiles(ctypes.Structure):

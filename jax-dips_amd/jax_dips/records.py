@@ -189,7 +189,9 @@ class DeviceRecords:
         the position the kernels compute, so that entries at one position (a
         stencil neighbour of one point that rounds onto the next point) are
         adjacent; the hashgrid backward then sends one set of table-gradient
-        records for them.  Any order inside a segment is valid: t_csr maps each
+        records for them, and the batches carry these runs as groups
+        (t_group, g_begin/g_plus/g_minus) whose network the MFMA kernels
+        evaluate once.  Any order inside a segment is valid: t_csr maps each
         entry to its CSR slot."""
         import torch
 
@@ -219,23 +221,45 @@ class DeviceRecords:
             for col in (0, 1, 2):  # least significant key first: x, y, z, then the segment
                 _, p2 = torch.sort(X[perm, col], stable=True)
                 perm = perm[p2]
-            del X
             _, p2 = torch.sort(key[perm], stable=True)
             perm = perm[p2]
+            # groups: runs of bit-identical positions inside a (batch, phase) segment of this order
+            Xs = X[perm]
+            del X
+            ks = key[perm]
+            new = torch.ones(n, dtype=torch.bool, device=dev)
+            if n > 1:
+                new[1:] = (ks[1:] != ks[:-1]) | (Xs[1:] != Xs[:-1]).any(dim=1)
+            del Xs, ks
+            starts = torch.nonzero(new).flatten()
+            t_group = torch.cat([starts, torch.tensor([n], device=dev, dtype=torch.int64)]).to(torch.int32).contiguous()
         t_row = ent_row[perm].to(torch.int32).contiguous()
         t_code = self.ent_code[:n][perm].contiguous()
         t_w = self.ent_w[:n][perm].contiguous()
         t_csr = perm.to(torch.int32).contiguous()
         nplus = torch.zeros(len(self.batch_begin), dtype=torch.int64, device=dev).index_add_(0, batch, phase).cpu().numpy()
         ptr_h = ptr.cpu().numpy()
+        if grid is not None:
+            # each batch's groups: + groups from its first entry, - groups from its first - entry
+            bnd = np.array([[int(ptr_h[b0]), int(ptr_h[b0]) + int(nplus[b]), int(ptr_h[b0 + nr])]
+                            for b, (b0, nr) in enumerate(zip(self.batch_begin, self.batch_rows))], np.int64)
+            gidx = torch.searchsorted(starts, torch.as_tensor(bnd.ravel(), device=dev)).cpu().numpy().reshape(-1, 3)
         batches = []
         for b, (b0, nr) in enumerate(zip(self.batch_begin, self.batch_rows)):
             e0, e1 = int(ptr_h[b0]), int(ptr_h[b0 + nr])
-            batches.append(N.MfmaBatch(int(b0), int(nr), e0, e1, t_row.data_ptr(), t_code.data_ptr(), t_w.data_ptr(),
-                                       t_csr.data_ptr(), e0, int(nplus[b]), int(e1 - e0 - nplus[b]), int(e1 - e0)))
+            bt = N.MfmaBatch(int(b0), int(nr), e0, e1, t_row.data_ptr(), t_code.data_ptr(), t_w.data_ptr(),
+                             t_csr.data_ptr(), e0, int(nplus[b]), int(e1 - e0 - nplus[b]), int(e1 - e0))
+            if grid is not None:
+                bt.t_group = t_group.data_ptr()
+                bt.g_begin, bt.g_plus, bt.g_minus = (int(gidx[b, 0]), int(gidx[b, 1] - gidx[b, 0]),
+                                                     int(gidx[b, 2] - gidx[b, 1]))
+            batches.append(bt)
         mb = dict(t_row=t_row, t_code=t_code, t_w=t_w, t_csr=t_csr, batches=batches,
                   max_nnz=int(max(int(ptr_h[b0 + nr] - ptr_h[b0]) for b0, nr in
                                   zip(self.batch_begin, self.batch_rows))))
+        if grid is not None:
+            mb["t_group"] = t_group
+            mb["groups"] = int(starts.numel())
         setattr(self, cache, mb)
         return mb
 
@@ -396,10 +420,13 @@ class Engine:
             N.current_stream())
         N.check(st, "dips_tile_train_level")
 
-    def mfma_workspace(self, rec):
+    def mfma_workspace(self, rec, grid=None):
+        """Phase-sorted batches of rec for the MFMA kernels (with the grid: in
+        position order, with groups of entries at one position) and a
+        workspace large enough for them."""
         import torch
 
-        mb = rec.mfma_batches()
+        mb = rec.mfma_batches(grid)
         need = int(N.lib().dips_mfma_workspace(ctypes.byref(self.net), int(rec.batch_size), mb["max_nnz"]))
         if getattr(self, "_mwork", None) is None or self._mwork.numel() < need:
             self._mwork = torch.empty(need, dtype=torch.float32, device=self.device)
@@ -407,11 +434,12 @@ class Engine:
                                        device=self.device)
         return mb
 
-    def mfma_loss_grad(self, grid, rec, b, inv_batch):
-        """(loss, grad) of batch b through the MFMA kernels, float64 numpy."""
+    def mfma_loss_grad(self, grid, rec, b, inv_batch, grouped=True):
+        """(loss, grad) of batch b through the MFMA kernels, float64 numpy;
+        grouped: entries at one position evaluated once (else each entry)."""
         import torch
 
-        mb = self.mfma_workspace(rec)
+        mb = self.mfma_workspace(rec, grid if grouped else None)
         g = torch.empty(self.P, dtype=torch.float32, device=self.device)
         loss = torch.empty(1, dtype=torch.float64, device=self.device)
         N.check(N.lib().dips_mfma_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb["batches"][b]),
@@ -423,7 +451,7 @@ class Engine:
         return float(loss.item()), g.double().cpu().numpy()
 
     def train_level_mfma(self, grid, rec, nepochs):
-        mb = self.mfma_workspace(rec)
+        mb = self.mfma_workspace(rec, grid)
         L = N.lib()
         inv_b = ctypes.c_float(1.0 / rec.batch_size)
         for _ in range(int(nepochs)):

@@ -22,9 +22,13 @@ def device(axes, rec, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dims,level", [((2, 16, 2, 16), 0), ((2, 16, 2, 16), 2), ((3, 64, 3, 64), 1),
-                                        ((1, 1, 2, 10), 0), ((3, 32, 3, 32), 3)])
-def test_mfma_loss_grad(dims, level):
+@pytest.mark.parametrize("dims,level,grouped", [((2, 16, 2, 16), 0, True), ((2, 16, 2, 16), 2, True),
+                                                ((3, 64, 3, 64), 1, True), ((3, 64, 3, 64), 0, False),
+                                                ((3, 64, 3, 64), 0, True), ((1, 1, 2, 10), 0, True),
+                                                ((3, 32, 3, 32), 3, True), ((3, 32, 3, 32), 1, False)])
+def test_mfma_loss_grad(dims, level, grouped):
+    """grouped: record entries at one position evaluated once (their u written
+    to every member, their seeds summed); else every entry on its own."""
     from jax_dips.records import Engine
 
     prob = problems.star(r0=0.55, amp=0.15, centre=(0.0123, -0.0171, 0.0093))
@@ -34,8 +38,11 @@ def test_mfma_loss_grad(dims, level):
     theta = M.init_params(dims, seed=31)
     grid, drec = device(axes, rec, B)
     eng = Engine(dims, theta, {}, max_rows=B, max_steps=4)
+    if grouped and level <= 1:
+        mb = drec.mfma_batches(grid)
+        assert mb["groups"] < drec.nnz  # at zoom levels 0 and 1 stencil neighbours share positions
     for b in range(n // B):
-        loss, grad = eng.mfma_loss_grad(grid, drec, b, 1.0 / B)
+        loss, grad = eng.mfma_loss_grad(grid, drec, b, 1.0 / B, grouped=grouped)
         ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, rec, np.arange(b * B, (b + 1) * B))
         print(dims, level, b, "loss", loss, ol, "grad rel", np.linalg.norm(grad - og) / np.linalg.norm(og))
         assert abs(loss - ol) <= 1e-5 * abs(ol)

@@ -158,6 +158,24 @@ def test_entries_ordered_by_position():
     assert np.all(np.diff(seg(perm)) >= 0)
     for a, b in zip(plain["batches"], mb["batches"]):
         assert (a.t_count, a.n_plus) == (b.t_count, b.n_plus)
+    # groups: maximal runs of one (segment, position) in this order; each batch's + groups then - groups
+    # cover exactly its + entries then its - entries; without the grid there are no groups
+    P = grid.points().numpy()
+    pos = (P[row_point[ent_row[perm]]] + off[perm].astype(np.float32) * h).astype(np.float32)
+    tg = mb["t_group"].numpy().astype(np.int64)
+    assert tg[0] == 0 and tg[-1] == nnz and np.all(np.diff(tg) > 0)
+    s = seg(perm)
+    for g in range(len(tg) - 1):
+        a, b = tg[g], tg[g + 1]
+        assert np.all(s[a:b] == s[a]) and np.all(pos[a:b] == pos[a])
+        if b < nnz:
+            assert s[b] != s[a] or np.any(pos[b] != pos[a])
+    assert mb["groups"] == len(tg) - 1 < nnz  # some positions coincide on this grid
+    for bt in mb["batches"]:
+        assert tg[bt.g_begin] == bt.t_begin
+        assert tg[bt.g_begin + bt.g_plus] == bt.t_begin + bt.n_plus
+        assert tg[bt.g_begin + bt.g_plus + bt.g_minus] == bt.t_begin + bt.t_count
+    assert all(bt.t_group is None for bt in plain["batches"])
     # positions as the kernels compute them; equal positions of a segment are adjacent
     pts = grid.points().numpy()
     X = pts[row_point[ent_row[perm]]] + off[perm].astype(np.float32) * h

@@ -39,7 +39,7 @@ def test_noalloc_step_equals_workspace_step():
     stream = N.current_stream()
     sizes = []
     for rec, bi in ((big, 0), (small, 1)):
-        mb = a.mfma_workspace(rec)
+        mb = a.mfma_workspace(rec, grid)
         bt = mb["batches"][bi]
         sizes.append((bt.nrows, bt.ent_end - bt.ent_begin))
         inv_b = ctypes.c_float(1.0 / rec.batch_size)
