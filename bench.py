@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--epochs-per-level", type=int, default=3, help="sweep: timed epochs per zoom level")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy, 5 times (for scaling HBM counters)")
+    ap.add_argument("--graph", default="on", choices=["on", "off"],
+                    help="replay each zoom level's epoch from a captured HIP graph (on) or launch every step")
     ap.add_argument("--mfma-groups", default="on", choices=["on", "off"],
                     help="MFMA path: evaluate the network once per group of record entries at one position (on), "
                          "or once per entry (off)")
@@ -181,11 +183,24 @@ def main():
         for rec in levels:
             rec.tiles(grid)
 
+    # each level's epoch captured once in a HIP graph (tile and MFMA paths); the timed steps replay it
+    eng.mfma_grouped = mgrid is not None
+    graphs = {}
+    if args.graph == "on" and path in ("tile", "mfma"):
+        for li, rec in enumerate(levels):
+            graphs[li] = eng.capture_epoch(grid, rec)
+
     def run_steps(k, stream):
-        # k steps split over the 4 zoom levels, batches in order (cyclic)
+        # k steps split over the 4 zoom levels, batches in order (cyclic): whole epochs from the level's
+        # graph, the remaining steps launched directly
         per = k // 4
         for li, rec in enumerate(levels):
-            idx = np.arange(per) % nb
+            steps = per
+            if li in graphs:
+                for _ in range(per // nb):
+                    graphs[li].replay()
+                steps = per % nb
+            idx = np.arange(steps) % nb
             if use_mfma:
                 mb = rec.mfma_batches(mgrid)["batches"]
                 for b in idx:
@@ -200,7 +215,7 @@ def main():
             if path == "tile":
                 tl = rec.tiles(grid)
                 btb = tl["batch_tile_begin"]
-                full, part = divmod(per, nb)
+                full, part = divmod(steps, nb)
                 for ne, nbat in ((full, nb), (1 if part else 0, part)):
                     if ne == 0:
                         continue
@@ -213,7 +228,7 @@ def main():
                             "dips_tile_train_level")
                 continue
             st = L.dips_train_level(ctypes.byref(grid.c), ctypes.byref(rec.c), N.i64_array(rec.batch_begin[idx]),
-                                    N.i64_array(rec.batch_rows[idx]), N.i64_array(rec.batch_nbulk[idx]), per,
+                                    N.i64_array(rec.batch_rows[idx]), N.i64_array(rec.batch_nbulk[idx]), steps,
                                     rec.batch_size, 1,
                                     ctypes.byref(eng.net), ctypes.byref(eng.opt), N.ptr(eng.params), N.ptr(eng.m),
                                     N.ptr(eng.v), N.ptr(eng.count), N.ptr(eng.loss_hist), int(eng.loss_hist.numel()),

@@ -970,10 +970,16 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
       set_error("dips_mfma_train_step: network too large for the LDS of one workgroup");                            \
       return DIPS_EUNSUPPORTED;                                                                                    \
     }                                                                                                              \
-    hipFuncSetAttribute((const void*)mfma_fwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
-                        (int)lds_f);                                                                               \
-    hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
-                        (int)bwd_lds);                                                                             \
+    /* LDS limits set once per instantiation (for its widest networks), so that a step is launches only */      \
+    static const bool attrs_set = [] {                                                                             \
+      const size_t red_max = sizeof(float) * std::max(net_params(C, 16 * D), net_params(A, 16 * B));             \
+      hipFuncSetAttribute((const void*)mfma_fwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                          (int)(sizeof(float) * PKF));                                                             \
+      hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                          (int)std::max(sizeof(float) * (PK + kBW * 2 * 16 * NBX * kTP), red_max));               \
+      return true;                                                                                                 \
+    }();                                                                                                           \
+    (void)attrs_set;                                                                                               \
     split(resident_blocks((const void*)mfma_fwd_kernel<A, B, C, D>, kFT, lds_f, kMfmaFwdBlocks), nfp, nfm);       \
     if (nfp + nfm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kFT), lds_f, st, T, params, Hm, Hp,   \

@@ -372,6 +372,8 @@ class Engine:
         # tile steps as two launches (step kernel, then a one-workgroup reduction + optimizer kernel),
         # which is faster than the single launch whose last workgroup applies the optimizer
         self.tile_single_launch = False
+        # MFMA path: one network evaluation per group of entries at one position (position-ordered batches)
+        self.mfma_grouped = True
         self.opt = make_opt(optimizer_dict)
 
     def loss_grad(self, grid, rec, row_begin, nrows, inv_batch, want_grad=True, nbulk=None):
@@ -451,7 +453,7 @@ class Engine:
         return float(loss.item()), g.double().cpu().numpy()
 
     def train_level_mfma(self, grid, rec, nepochs):
-        mb = self.mfma_workspace(rec, grid)
+        mb = self.mfma_workspace(rec, grid if self.mfma_grouped else None)
         L = N.lib()
         inv_b = ctypes.c_float(1.0 / rec.batch_size)
         for _ in range(int(nepochs)):
@@ -476,6 +478,33 @@ class Engine:
             int(self.loss_hist.numel()), N.ptr(self.partial), self.partial_cap,
             N.ptr(self.sync) if self.fused else None, N.current_stream())
         N.check(st, "dips_train_level")
+
+    def prepare(self, grid, rec):
+        """Build what train_level needs for rec (tiles, MFMA batches and
+        workspace) ahead of time, outside any graph capture."""
+        if self.tile_path and (self.use_tiles or self.precond is not None or self.act != 0):
+            rec.tiles(grid)
+        elif not self.lane_path:
+            self.mfma_workspace(rec, grid if self.mfma_grouped else None)
+
+    def capture_epoch(self, grid, rec):
+        """One epoch over rec's batches (the launches of train_level(grid, rec, 1))
+        captured in a HIP graph; each replay() trains one more epoch (step
+        counter, schedule and loss history advance on the device)."""
+        return capture_epoch(self, grid, rec)
+
+
+def capture_epoch(engine, grid, rec):
+    """HIP graph of engine.train_level(grid, rec, 1): every launch of an epoch
+    goes to the capture stream; nothing runs until replay()."""
+    import torch
+
+    engine.prepare(grid, rec)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        engine.train_level(grid, rec, 1)
+    return g
 
 
 class HashEngine:

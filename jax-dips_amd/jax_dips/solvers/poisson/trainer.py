@@ -49,7 +49,7 @@ from ...data.data_management import generate_random_points_like
 from ...level_records import build_level, sample
 from ...nn.configure import get_model
 from ...nn.preconditioner import Preconditioner
-from ...records import DeviceGrid, Engine, HashEngine
+from ...records import DeviceGrid, Engine, HashEngine, capture_epoch
 from ..simulation_states import PoissonSimState, PoissonSimStateFn
 from ..._jaxmd_modules import dataclasses
 
@@ -81,6 +81,9 @@ def _callbacks(sim_state_fn):
 
 class Trainer:
     """Trains the two-phase network of one interface problem on the GPU."""
+
+    # replay each zoom level's epoch from a captured HIP graph (False: launch every step from the host)
+    use_graphs = True
 
     def __init__(self, lvl_gstate, tr_gstate, eval_gstate, sim_state, sim_state_fn, algorithm=0,
                  mgrad_over_pgrad_scalefactor=1, lvl_set_fn=None, num_epochs=1000, multi_gpu=False,
@@ -246,11 +249,16 @@ class Trainer:
                 torch.cuda.synchronize()
                 self.timing["records_s"] += time.time() - tr0
                 tt0 = time.time()
+                # the level's epoch (all its batches) is captured once in a HIP graph and replayed per epoch
+                graph = None
+                if self.use_graphs and isinstance(engine, Engine) and last > epoch:
+                    graph = capture_epoch(engine, grid, rec)
                 for e in range(epoch, last + 1):
                     if stop_training:
                         break
-                    engine.train_level(grid, rec, 1)
+                    self._epoch(engine, grid, rec, graph)
                     done = e + 1
+                del graph
                 torch.cuda.synchronize()
                 self.timing["train_s"] += time.time() - tt0
                 hist = engine.loss_hist.cpu().numpy()
@@ -271,6 +279,14 @@ class Trainer:
         self.engine = engine
         u, gu, gnu = self.evaluate_solution_and_gradients(self.params, self.eval_gstate)
         return u, gu, gnu, self.epoch_store, self.loss_epochs
+
+    @staticmethod
+    def _epoch(engine, grid, rec, graph):
+        """One epoch over the level's batches: a replay of the captured graph, or the launches themselves."""
+        if graph is not None:
+            graph.replay()
+        else:
+            engine.train_level(grid, rec, 1)
 
     def _plot(self):
         try:

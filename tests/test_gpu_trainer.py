@@ -157,25 +157,25 @@ def test_restart_after_sigint_matches_uninterrupted_run(tmp_path, monkeypatch):
     import os
     import signal
 
-    from jax_dips import records
+    from jax_dips.solvers.poisson import trainer
 
     prob = problems.sphere()
     dims = (1, 1, 2, 10)
     theta0 = M.init_params(dims, seed=42)
     _, _, es_full, le_full, tr_full = run_trainer(prob, 16, 8, 1024, theta0, tmp_path / "full")
     calls = {"n": 0}
-    orig = records.Engine.train_level
+    orig = trainer.Trainer._epoch
 
-    def interrupt_after_third(self, grid, rec, nepochs):
-        orig(self, grid, rec, nepochs)
+    def interrupt_after_third(engine, grid, rec, graph):
+        orig(engine, grid, rec, graph)
         calls["n"] += 1
         if calls["n"] == 3:
             os.kill(os.getpid(), signal.SIGINT)
 
-    monkeypatch.setattr(records.Engine, "train_level", interrupt_after_third)
+    monkeypatch.setattr(trainer.Trainer, "_epoch", staticmethod(interrupt_after_third))
     part = tmp_path / "part"
     _, _, es1, le1, tr1 = run_trainer(prob, 16, 8, 1024, theta0, part)
-    monkeypatch.setattr(records.Engine, "train_level", orig)
+    monkeypatch.setattr(trainer.Trainer, "_epoch", staticmethod(orig))
     assert calls["n"] == 3
     assert sorted(os.listdir(part / "ck")) == ["checkpoint_3"]
     assert tr1.steps == 3 * 4
