@@ -35,8 +35,8 @@ __device__ __forceinline__ float dips_tanh(float x) {
 // tanh as 1 - 2/(1 + exp(2x)) with the hardware exp2 and reciprocal: 3 VALU
 // instructions and 2 transcendentals (dips_tanh: 14 and 2).  Absolute error
 // below 4e-7 for every x with both hardware ops 1 ulp off (relative accuracy
-// degrades only where |tanh x| is small); saturates to +-1, propagates NaN.  Used
-// where tanh is the VALU bottleneck next to the matrix cores (the MFMA tiles).
+// degrades only where |tanh x| is small); saturates to +-1, propagates NaN.  The
+// hidden activation of every network kernel (tanh is their VALU bottleneck).
 __device__ __forceinline__ float dips_tanh_fast(float x) {
   const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // exp(2x)
   return fmaf(-2.0f, __builtin_amdgcn_rcpf(1.0f + e), 1.0f);
@@ -55,7 +55,7 @@ __device__ __forceinline__ float act_f(float s) {
   if constexpr (Act == kActCelu)
     return s > 0.0f ? s : expm1f(s);
   else
-    return dips_tanh(s);
+    return dips_tanh_fast(s);
 }
 
 template <int Act>
