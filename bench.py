@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--epochs-per-level", type=int, default=3, help="sweep: timed epochs per zoom level")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy, 5 times (for scaling HBM counters)")
+    ap.add_argument("--batch-cfg", default="auto",
+                    help="sweep: shape of the batched steps, 'auto' (autotuned) or 'blk_cap,grid_blocks'")
     ap.add_argument("--graph", default="on", choices=["on", "off"],
                     help="replay each zoom level's epoch from a captured HIP graph (on) or launch every step")
     ap.add_argument("--mfma-groups", default="on", choices=["on", "off"],
@@ -465,14 +467,19 @@ def run_sweep(args):
             r.tiles(g)
         torch.cuda.synchronize()
         t_rec += time.time() - t0
-        if lvl == 0:  # warm up both paths (module load, LDS attributes)
+        if lvl == 0:  # warm up both paths (module load, LDS attributes), then choose the batched shape
             beng.train_level(grids, recs, 1)
             for e, g, r in zip(sengs, grids, recs):
                 e.train_level(g, r, 1)
             torch.cuda.synchronize()
+            ta = time.time()
+            cfg = beng.autotune(grids, recs) if args.batch_cfg == "auto" else tuple(
+                int(v) for v in args.batch_cfg.split(","))
+            torch.cuda.synchronize()
+            t_tune = time.time() - ta
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         ev[0].record()
-        beng.train_level(grids, recs, E)
+        beng.train_level(grids, recs, E, cfg=cfg)
         ev[1].record()
         ev[2].record()
         for e, g, r in zip(sengs, grids, recs):
@@ -508,9 +515,12 @@ def run_sweep(args):
                    "global_batch": int(sum(B for (_, _, _, B, _, _) in info)), "parallelism": "single GPU"},
         "sequential_value": points / t_seq,
         "speedup_vs_sequential": t_seq / t_batch,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "batch_cfg": {"blk_cap": cfg[0], "grid_blocks": cfg[1], "autotune_s": t_tune,
+                      "candidates_ms_per_step": getattr(beng, "tuning", {}).get("candidates_ms")},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "tile_batch_step_kernel + tile_batch_apply_kernel (whole batched step)"},
+                     "kernel": "tile_batch_step_kernel (persistent, work queue) + tile_batch_apply_kernel + counter "
+                               "update, per global step (whole batched step; FP32 VALU bound)"},
         "records_build_s": t_rec,
         "loss_tail": [float(e.loss_hist[int(e.count.item()) - 1]) for e in beng.engines[:4]],
     }

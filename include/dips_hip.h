@@ -256,9 +256,45 @@ int64_t dips_batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, c
 
 /* nepochs epochs of every problem (probs: host array).  The problem table is
  * copied into work (device, work_bytes >= dips_batch_workspace) on the stream,
- * which is then synchronized once before the steps are enqueued. */
+ * which is then synchronized once before the steps are enqueued.  Same as
+ * dips_batch_setup with the default configuration, then dips_batch_steps for
+ * every global step. */
 int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
                      const dips_opt_t* opt, void* work, int64_t work_bytes, void* stream);
+
+/* Shape of the batched steps (chosen by an autotuner, e.g. jax_dips.records.BatchEngine.autotune).
+ * Each global step is one persistent launch of grid_blocks workgroups that take work items (problem p,
+ * block k) from a device queue; problem p's step has min(blk_cap, its tile blocks) blocks, each writing
+ * one partial row reduced in block order by the problem's optimizer workgroup.  blk_cap = 256 (or 0)
+ * reproduces the single-problem step bit for bit; fewer blocks make larger work items. */
+typedef struct {
+  int32_t blk_cap;      /* 1..256; 0: 256 */
+  int32_t grid_blocks;  /* > 0; 0: as many as are resident at once on the device */
+} dips_batch_cfg_t;
+
+/* Host record of a set-up batched run, for dips_batch_steps. */
+typedef struct {
+  int32_t nprob;        /* problems */
+  int32_t nitems;       /* work items per global step */
+  int32_t blk_cap;
+  int32_t grid_blocks;
+  int32_t max_steps;    /* global steps of the nepochs epochs (max over problems of nepochs * nbatches) */
+  int32_t ctrl_units;   /* offset of the device step counter and queue head in work, in 256-byte units */
+} dips_batch_plan_t;
+
+/* dips_batch_workspace for a configuration (cfg may be NULL: the default). */
+int64_t dips_batch_workspace_cfg(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net,
+                                 const dips_batch_cfg_t* cfg);
+/* Copies the problem table into work, sets the device global step to 0 and
+ * synchronizes the stream; fills *plan.  No steps run. */
+int dips_batch_setup(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                     const dips_opt_t* opt, const dips_batch_cfg_t* cfg, void* work, int64_t work_bytes,
+                     dips_batch_plan_t* plan, void* stream);
+/* Enqueues nsteps global steps from the device step counter on (3 launches each: the persistent step,
+ * the optimizer workgroups, the counter update); launches only, so a caller may capture them in a HIP
+ * graph (e.g. one epoch, plan->max_steps / nepochs steps) and replay.  Steps past a problem's last
+ * leave it unchanged. */
+int dips_batch_steps(const dips_net_t* net, const dips_batch_plan_t* plan, int32_t nsteps, void* work, void* stream);
 
 /* Network at m points pts (m,3): u (m), optionally grad_x u (m,3) and, with
  * normals (m,3), the normal derivative grad u . n (m).  phase (m) uint8:

@@ -255,7 +255,33 @@ int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, co
 
 int64_t dips_batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net) {
   if (check_net(net)) return -1;
-  return batch_workspace(probs, nprob, net);
+  return batch_workspace(probs, nprob, net, nullptr);
+}
+
+int64_t dips_batch_workspace_cfg(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net,
+                                 const dips_batch_cfg_t* cfg) {
+  if (check_net(net)) return -1;
+  return batch_workspace(probs, nprob, net, cfg);
+}
+
+int dips_batch_setup(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                     const dips_opt_t* opt, const dips_batch_cfg_t* cfg, void* work, int64_t work_bytes,
+                     dips_batch_plan_t* plan, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  if (!probs || nprob < 1) {
+    set_error("dips_batch_setup: no problems");
+    return DIPS_EINVAL;
+  }
+  for (int p = 0; p < nprob; ++p)
+    if ((st = check_records(&probs[p].grid, &probs[p].rec))) return st;
+  return batch_setup(probs, nprob, nepochs, net, opt, cfg, work, work_bytes, plan, (hipStream_t)stream);
+}
+
+int dips_batch_steps(const dips_net_t* net, const dips_batch_plan_t* plan, int32_t nsteps, void* work, void* stream) {
+  int st = check_net(net);
+  if (st) return st;
+  return batch_steps(net, plan, nsteps, work, (hipStream_t)stream);
 }
 
 int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,

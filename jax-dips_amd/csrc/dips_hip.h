@@ -48,9 +48,16 @@ void apply_launch(const float* g, int64_t P, const float* g2, int64_t split, flo
                   float* m, float* v, int32_t* count, float* loss_hist, int loss_cap, hipStream_t st);
 constexpr int kApplyBlocks = 2048;  // grid cap of the optimizer kernel (it strides over the parameters)
 void loss_out_launch(const double* loss_part, int nloss, float inv_b, double* loss_out, hipStream_t st);
-int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net);
+int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net,
+                        const dips_batch_cfg_t* cfg);
 int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
                 const dips_opt_t* opt, void* work, int64_t work_bytes, hipStream_t stream);
+int batch_setup(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                const dips_opt_t* opt, const dips_batch_cfg_t* cfg, void* work, int64_t work_bytes,
+                dips_batch_plan_t* plan, hipStream_t stream);
+int batch_steps(const dips_net_t* net, const dips_batch_plan_t* plan, int32_t nsteps, void* work, hipStream_t stream);
+// workgroups of a kernel resident at once on the current device (cached per kernel and device)
+int resident_blocks(const void* fn, int threads, size_t lds, int fallback);
 int tile_supported(const dips_net_t* net);
 int tile_slots_launch(const dips_grid_t* g, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
                       float* sx, float* sy, float* sz, float* sw, float* sc, uint32_t* sinfo, int32_t* srow,

@@ -878,7 +878,7 @@ int mfma_eval_launch(const dips_net_t* net, const float* params, const float* pt
 // (compute units x occupancy), looked up once per kernel and device; fallback
 // if the runtime cannot say.  A persistent grid of exactly this size runs in
 // one wave of workgroups, so no workgroup starts after others have finished.
-static int resident_blocks(const void* fn, int threads, size_t lds, int fallback) {
+int resident_blocks(const void* fn, int threads, size_t lds, int fallback) {
   static std::mutex mu;
   static std::vector<std::pair<std::pair<const void*, int>, int>> cache;
   int dev = 0;

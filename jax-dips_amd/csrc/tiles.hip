@@ -1009,72 +1009,107 @@ int tile_train_step(const dips_grid_t* g, const dips_records_t* rec, const dips_
 }
 
 // ---------------------------------------------------------------------------
-// Batched training of many problems.  Problem p owns the workgroups
-// [blk_begin, blk_begin + blk_count) of every step launch (blk_count = the
-// largest block count over its batches); at global step s it trains batch
-// s mod nbatches while s < steps.  Its partial rows and parameters are its own,
-// so one problem's step is exactly the single-problem two-launch step.
+// Batched training of many problems.  A global step is one persistent launch
+// whose workgroups pull items from a device work queue: item (p, k) is block k
+// of the nblk_p workgroup blocks that problem p's step uses on its own
+// (nblk_p = min(cap, blocks for the tiles of its batch s mod nbatches_p)), and
+// writes exactly that block's partial row, whichever workgroup takes it.  Then
+// one reduction + optimizer workgroup per problem, in block order.  With
+// cap = kMaxTileBlocks every problem follows the trajectory the
+// single-problem two-launch step gives it, bit for bit.  The global step and
+// the queue head live in device memory (BatchCtrl), advanced by a one-thread
+// kernel after each step, so an epoch of launches can be captured in a graph
+// and replayed.
 struct BatchDesc {
   TileStepArgs a;                   // tile_begin / ntiles are set per step
   const int64_t* batch_tile_begin;  // device copy, nbatches + 1
   int32_t nbatches;
   int32_t steps;
-  int32_t blk_begin;
-  int32_t blk_count;
+  int32_t blk_begin;                // first item of the problem
+  int32_t blk_count;                // items of the problem: its most blocks over the batches
   float* params;
   float* partial;                   // blk_count * stride floats
   FusedOpt fo;
 };
 
+struct BatchCtrl {
+  uint32_t queue;  // next item of the current global step
+  int32_t step;    // current global step
+};
+
 // the step's tile range and block count of problem d at global step s
-__device__ __forceinline__ bool batch_step_range(const BatchDesc& d, int s, int waves, int64_t& tb, int& nt,
+__device__ __forceinline__ bool batch_step_range(const BatchDesc& d, int s, int waves, int cap, int64_t& tb, int& nt,
                                                  int& nblk) {
   if (s >= d.steps) return false;
   const int b = s % d.nbatches;
   tb = gld(d.batch_tile_begin, b);
   nt = (int)(gld(d.batch_tile_begin, b + 1) - tb - 1);
-  nblk = tile_blocks(nt, waves);
+  nblk = min(tile_blocks(nt, waves), cap);
   return true;
 }
 
 template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2, int Act>
 __global__ __launch_bounds__((TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads)) void tile_batch_step_kernel(
-    const BatchDesc* __restrict__ descs, int nprob, int step) {
+    const BatchDesc* __restrict__ descs, int nprob, int nitems, int cap, BatchCtrl* __restrict__ ctrl) {
   constexpr int kTileWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
-  // owner problem of this workgroup: the last p with blk_begin <= blockIdx.x
-  const int bx = (int)blockIdx.x;
-  int lo = 0, hi = nprob - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].blk_begin <= bx)
-      lo = mid;
-    else
-      hi = mid - 1;
+  const int s = ctrl->step;
+  __shared__ int item_s;
+  for (;;) {
+    __syncthreads();  // every thread has read the previous item
+    if (threadIdx.x == 0) item_s = (int)atomicAdd(&ctrl->queue, 1u);
+    __syncthreads();
+    const int item = item_s;
+    if (item >= nitems) return;
+    // owner problem: the last p with blk_begin <= item
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].blk_begin <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const BatchDesc& d = descs[lo];
+    int64_t tb;
+    int nt, nblk;
+    if (!batch_step_range(d, s, kTileWaves, cap, tb, nt, nblk)) continue;
+    const int k = item - d.blk_begin;
+    if (k >= nblk) continue;
+    TileStepArgs a = d.a;
+    a.tile_begin = tb;
+    a.ntiles = nt;
+    tile_step_body<Lm, Hm, Lp, Hp, true, 1, PC1, PC2, Act>(a, d.params, d.partial, k, nblk);
   }
-  const BatchDesc& d = descs[lo];
-  int64_t tb;
-  int nt, nblk;
-  if (!batch_step_range(d, step, kTileWaves, tb, nt, nblk)) return;
-  const int k = bx - d.blk_begin;
-  if (k >= nblk) return;
-  TileStepArgs a = d.a;
-  a.tile_begin = tb;
-  a.ntiles = nt;
-  tile_step_body<Lm, Hm, Lp, Hp, true, 1, PC1, PC2, Act>(a, d.params, d.partial, k, nblk);
 }
 
 template <int kStride, int kWaves>
-__global__ __launch_bounds__(1024) void tile_batch_apply_kernel(const BatchDesc* __restrict__ descs, int step) {
+__global__ __launch_bounds__(1024) void tile_batch_apply_kernel(const BatchDesc* __restrict__ descs, int cap,
+                                                                const BatchCtrl* __restrict__ ctrl) {
   const BatchDesc& d = descs[blockIdx.x];
   int64_t tb;
   int nt, nblk;
-  if (!batch_step_range(d, step, kWaves, tb, nt, nblk)) return;
+  if (!batch_step_range(d, ctrl->step, kWaves, cap, tb, nt, nblk)) return;
   tile_reduce_apply<kStride, 16>(d.partial, nblk, d.fo, d.params);
+}
+
+// after a step: empty queue, next global step
+__global__ void batch_ctrl_next_kernel(BatchCtrl* ctrl) {
+  if (threadIdx.x == 0) {
+    ctrl->queue = 0u;
+    ctrl->step = ctrl->step + 1;
+  }
+}
+
+__global__ void batch_ctrl_init_kernel(BatchCtrl* ctrl, int step) {
+  if (threadIdx.x == 0) {
+    ctrl->queue = 0u;
+    ctrl->step = step;
+  }
 }
 
 namespace {
 struct BatchLayout {
-  int64_t desc_off, table_off, partial_off, bytes;
+  int64_t desc_off, table_off, partial_off, ctrl_off, bytes;
   int32_t nblk_total;
   std::vector<int32_t> blk_count;
 };
@@ -1090,7 +1125,12 @@ int tile_waves_of(const dips_net_t* net) {
   return 0;
 }
 
-int batch_layout(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net, BatchLayout& L) {
+int batch_cap(const dips_batch_cfg_t* cfg) {
+  const int c = cfg ? cfg->blk_cap : 0;
+  return c <= 0 ? kMaxTileBlocks : std::min(c, kMaxTileBlocks);
+}
+
+int batch_layout(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net, int cap, BatchLayout& L) {
   const int waves = tile_waves_of(net);
   if (!waves) {
     set_error("batch train: network shape has no compiled tile kernel");
@@ -1116,7 +1156,7 @@ int batch_layout(const dips_batch_problem_t* probs, int32_t nprob, const dips_ne
         set_error("batch train: problem " + std::to_string(p) + ": batch without tiles or outside the tile table");
         return DIPS_EINVAL;
       }
-      cnt = std::max(cnt, tile_blocks(nt, waves));
+      cnt = std::max(cnt, std::min(tile_blocks(nt, waves), cap));
     }
     L.blk_count[p] = cnt;
     blocks += cnt;
@@ -1131,42 +1171,58 @@ int batch_layout(const dips_batch_problem_t* probs, int32_t nprob, const dips_ne
   L.desc_off = 0;
   L.table_off = align_up((int64_t)sizeof(BatchDesc) * nprob);
   L.partial_off = align_up(L.table_off + (int64_t)sizeof(int64_t) * tables);
-  L.bytes = align_up(L.partial_off + (int64_t)sizeof(float) * partials);
+  L.ctrl_off = align_up(L.partial_off + (int64_t)sizeof(float) * partials);
+  L.bytes = align_up(L.ctrl_off + (int64_t)sizeof(BatchCtrl));
   return DIPS_OK;
+}
+
+// workgroups of the persistent step launch: as many as are resident at once, or the configured count
+template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2, int Act>
+int batch_grid_shape(const dips_batch_cfg_t* cfg) {
+  if (cfg && cfg->grid_blocks > 0) return cfg->grid_blocks;
+  return resident_blocks((const void*)tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2, Act>,
+                         TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads, TileCfg<Lm, Hm, Lp, Hp, PC1>::kLdsBytes,
+                         kMaxTileBlocks);
 }
 }  // namespace
 
-int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net) {
+int64_t batch_workspace(const dips_batch_problem_t* probs, int32_t nprob, const dips_net_t* net,
+                        const dips_batch_cfg_t* cfg) {
   BatchLayout L;
-  if (batch_layout(probs, nprob, net, L)) return -1;
+  if (batch_layout(probs, nprob, net, batch_cap(cfg), L)) return -1;
   return L.bytes;
 }
 
 template <int Lm, int Hm, int Lp, int Hp, int PC1, int PC2, int Act>
-static void batch_launch_shape(const BatchDesc* descs, int nprob, int nblk_total, int step, hipStream_t st) {
+static void batch_launch_shape(const BatchDesc* descs, BatchCtrl* ctrl, const dips_batch_plan_t& plan, hipStream_t st) {
   constexpr int kWaves = TileCfg<Lm, Hm, Lp, Hp, PC1>::kWaves;
   const size_t lds = TileCfg<Lm, Hm, Lp, Hp, PC1>::kLdsBytes;
   static const bool lds_set =
       hipFuncSetAttribute((const void*)tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2, Act>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
   (void)lds_set;
-  hipLaunchKernelGGL((tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2, Act>), dim3(nblk_total),
-                     dim3(TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads), lds, st, descs, nprob, step);
+  hipLaunchKernelGGL((tile_batch_step_kernel<Lm, Hm, Lp, Hp, PC1, PC2, Act>), dim3(plan.grid_blocks),
+                     dim3(TileCfg<Lm, Hm, Lp, Hp, PC1>::kThreads), lds, st, descs, plan.nprob, plan.nitems,
+                     plan.blk_cap, ctrl);
   constexpr int kStride = tile_stride<Lm, Hm, Lp, Hp, PC1, PC2>();
-  hipLaunchKernelGGL((tile_batch_apply_kernel<kStride, kWaves>), dim3(nprob), dim3(1024), 0, st, descs, step);
+  hipLaunchKernelGGL((tile_batch_apply_kernel<kStride, kWaves>), dim3(plan.nprob), dim3(1024), 0, st, descs,
+                     plan.blk_cap, ctrl);
+  hipLaunchKernelGGL(batch_ctrl_next_kernel, dim3(1), dim3(64), 0, st, ctrl);
 }
 
-int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
-                const dips_opt_t* opt, void* work, int64_t work_bytes, hipStream_t stream) {
+int batch_setup(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                const dips_opt_t* opt, const dips_batch_cfg_t* cfg, void* work, int64_t work_bytes,
+                dips_batch_plan_t* plan, hipStream_t stream) {
   BatchLayout L;
-  int st = batch_layout(probs, nprob, net, L);
+  const int cap = batch_cap(cfg);
+  int st = batch_layout(probs, nprob, net, cap, L);
   if (st) return st;
   if (!work || work_bytes < L.bytes) {
     set_error("batch train: workspace too small (need " + std::to_string(L.bytes) + " bytes)");
     return DIPS_EINVAL;
   }
-  if (nepochs < 0 || !opt) {
-    set_error("batch train: invalid epochs or optimizer");
+  if (nepochs < 0 || !opt || !plan || (cfg && cfg->grid_blocks < 0)) {
+    set_error("batch train: invalid epochs, optimizer, plan or grid size");
     return DIPS_EINVAL;
   }
   const bool pc = net->pc_w1 > 0;
@@ -1213,30 +1269,64 @@ int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepoch
     poff += (int64_t)L.blk_count[p] * stride;
     d.fo = FusedOpt{*opt, 1.0f / (float)q.batch_size, q.m, q.v, q.count, q.loss_hist, q.loss_cap, nullptr};
   }
+  int grid = 0;
+#define X(A, B, C, D, E, F, G) \
+  if (!grid && tile_shape_is(net, A, B, C, D, E, F, G)) grid = batch_grid_shape<A, B, C, D, E, F, G>(cfg);
+  DIPS_TILE_SHAPES(X)
+#undef X
+  if (grid < 1) {
+    set_error("batch train: network shape has no compiled tile kernel");
+    return DIPS_EUNSUPPORTED;
+  }
   // the tables are read by the kernels: copy, then wait so the host vectors may go
   hipError_t e = hipMemcpyAsync(base + L.desc_off, descs.data(), sizeof(BatchDesc) * nprob, hipMemcpyHostToDevice,
                                 stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(base + L.table_off, tables.data(), sizeof(int64_t) * tables.size(), hipMemcpyHostToDevice,
                        stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(batch_ctrl_init_kernel, dim3(1), dim3(64), 0, stream,
+                       reinterpret_cast<BatchCtrl*>(base + L.ctrl_off), 0);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) return hip_status(e, "batch train: table copy");
-  const BatchDesc* dd = reinterpret_cast<const BatchDesc*>(base + L.desc_off);
-  for (int64_t s = 0; s < maxsteps; ++s) {
-#define X(A, B, C, D, E, F, G)                                                         \
-  if (tile_shape_is(net, A, B, C, D, E, F, G)) {                                      \
-    batch_launch_shape<A, B, C, D, E, F, G>(dd, nprob, L.nblk_total, (int)s, stream); \
+  *plan = dips_batch_plan_t{nprob, L.nblk_total, cap, grid, (int32_t)maxsteps, (int32_t)(L.ctrl_off / kAlign)};
+  return DIPS_OK;
+}
+
+int batch_steps(const dips_net_t* net, const dips_batch_plan_t* plan, int32_t nsteps, void* work, hipStream_t stream) {
+  if (!net || !plan || !work || nsteps < 0 || plan->nprob < 1 || plan->nitems < 1 || plan->grid_blocks < 1 ||
+      plan->blk_cap < 1 || plan->ctrl_units < 1) {
+    set_error("batch steps: invalid plan");
+    return DIPS_EINVAL;
+  }
+  char* base = static_cast<char*>(work);
+  const BatchDesc* dd = reinterpret_cast<const BatchDesc*>(base);
+  BatchCtrl* ctrl = reinterpret_cast<BatchCtrl*>(base + (int64_t)plan->ctrl_units * kAlign);
+  for (int32_t s = 0; s < nsteps; ++s) {
+#define X(A, B, C, D, E, F, G)                                         \
+  if (tile_shape_is(net, A, B, C, D, E, F, G)) {                      \
+    batch_launch_shape<A, B, C, D, E, F, G>(dd, ctrl, *plan, stream); \
   } else
     DIPS_TILE_SHAPES(X)
 #undef X
     {
-      set_error("batch train: network shape has no compiled tile kernel");
+      set_error("batch steps: network shape has no compiled tile kernel");
       return DIPS_EUNSUPPORTED;
     }
-    e = hipGetLastError();
-    if (e != hipSuccess) return hip_status(e, "batch train: launch");
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_status(e, "batch steps: launch");
   }
   return DIPS_OK;
+}
+
+int batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
+                const dips_opt_t* opt, void* work, int64_t work_bytes, hipStream_t stream) {
+  dips_batch_plan_t plan;
+  int st = batch_setup(probs, nprob, nepochs, net, opt, nullptr, work, work_bytes, &plan, stream);
+  if (st) return st;
+  return batch_steps(net, &plan, plan.max_steps, work, stream);
 }
 
 // Slot layout of every tile (once per zoom level): one wavefront per tile

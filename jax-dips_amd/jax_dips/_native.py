@@ -64,6 +64,15 @@ class BatchProblem(ctypes.Structure):
                 ("loss_hist", ctypes.c_void_p), ("loss_cap", ctypes.c_int32)]
 
 
+class BatchCfg(ctypes.Structure):
+    _fields_ = [("blk_cap", ctypes.c_int32), ("grid_blocks", ctypes.c_int32)]
+
+
+class BatchPlan(ctypes.Structure):
+    _fields_ = [("nprob", ctypes.c_int32), ("nitems", ctypes.c_int32), ("blk_cap", ctypes.c_int32),
+                ("grid_blocks", ctypes.c_int32), ("max_steps", ctypes.c_int32), ("ctrl_units", ctypes.c_int32)]
+
+
 class HashGrid(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("features", ctypes.c_int32), ("table_size", ctypes.c_int64),
                 ("n_min", ctypes.c_int32), ("n_max", ctypes.c_int32), ("bound", ctypes.c_float),
@@ -128,6 +137,12 @@ SIGNATURES = {
     "dips_batch_workspace": (_I64, [ctypes.POINTER(BatchProblem), _I32, ctypes.POINTER(Net)]),
     "dips_batch_train": (ctypes.c_int, [ctypes.POINTER(BatchProblem), _I32, _I32, ctypes.POINTER(Net),
                                         ctypes.POINTER(Opt), _P, _I64, _P]),
+    "dips_batch_workspace_cfg": (_I64, [ctypes.POINTER(BatchProblem), _I32, ctypes.POINTER(Net),
+                                        ctypes.POINTER(BatchCfg)]),
+    "dips_batch_setup": (ctypes.c_int, [ctypes.POINTER(BatchProblem), _I32, _I32, ctypes.POINTER(Net),
+                                        ctypes.POINTER(Opt), ctypes.POINTER(BatchCfg), _P, _I64,
+                                        ctypes.POINTER(BatchPlan), _P]),
+    "dips_batch_steps": (ctypes.c_int, [ctypes.POINTER(Net), ctypes.POINTER(BatchPlan), _I32, _P, _P]),
     "dips_mfma_supported": (ctypes.c_int, [ctypes.POINTER(Net)]),
     "dips_mfma_workspace": (_I64, [ctypes.POINTER(Net), _I64, _I64]),
     "dips_mfma_train_step": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),

@@ -583,10 +583,11 @@ class HashEngine:
 
 class BatchEngine:
     """Parameters and optimizer state of many independent problems with one
-    network shape, trained together by dips_batch_train (one step launch over
-    the current batch of every problem, then one optimizer workgroup per
-    problem).  Each problem follows the trajectory Engine.train_level gives it
-    alone."""
+    network shape, trained together (dips_batch_setup / dips_batch_steps: per
+    global step one persistent launch whose workgroups take (problem, block)
+    items from a device queue, then one optimizer workgroup per problem).  With
+    the default shape each problem follows the trajectory Engine.train_level
+    gives it alone; autotune() picks a faster shape for a set of problems."""
 
     def __init__(self, dims, params_list, optimizer_dict, max_steps, device="cuda", precond=None, act=0):
         """max_steps: one optimizer-step capacity (loss history length) per problem."""
@@ -622,18 +623,122 @@ class BatchEngine:
             q.loss_cap = int(eng.loss_hist.numel())
         return probs, keep
 
-    def train_level(self, grids, recs, nepochs):
+    def setup(self, grids, recs, nepochs, cfg=None):
+        """dips_batch_setup for nepochs epochs of these records (copies the
+        problem table, synchronizes); cfg = (blk_cap, grid_blocks) or None for
+        the default, which reproduces the single-problem steps bit for bit.
+        Returns the plan for steps()."""
         import torch
 
         L = N.lib()
         probs, keep = self.problems(grids, recs)
         n = len(self.engines)
-        need = int(L.dips_batch_workspace(probs, n, ctypes.byref(self.net)))
+        c = N.BatchCfg(*(cfg if cfg is not None else (0, 0)))
+        need = int(L.dips_batch_workspace_cfg(probs, n, ctypes.byref(self.net), ctypes.byref(c)))
         if need < 0:
-            N.check(-1, "dips_batch_workspace")
+            N.check(-1, "dips_batch_workspace_cfg")
         if self._work is None or self._work.numel() < need:
             self._work = torch.empty(need, dtype=torch.uint8, device=self.device)
-        N.check(L.dips_batch_train(probs, n, int(nepochs), ctypes.byref(self.net), ctypes.byref(self.opt),
-                                   N.ptr(self._work), int(self._work.numel()), N.current_stream()),
-                "dips_batch_train")
+        plan = N.BatchPlan()
+        N.check(L.dips_batch_setup(probs, n, int(nepochs), ctypes.byref(self.net), ctypes.byref(self.opt),
+                                   ctypes.byref(c), N.ptr(self._work), int(self._work.numel()), ctypes.byref(plan),
+                                   N.current_stream()), "dips_batch_setup")
         del keep
+        return plan
+
+    def steps(self, plan, nsteps):
+        """Enqueue nsteps global steps (launches only)."""
+        N.check(N.lib().dips_batch_steps(ctypes.byref(self.net), ctypes.byref(plan), int(nsteps), N.ptr(self._work),
+                                         N.current_stream()), "dips_batch_steps")
+
+    def train_level(self, grids, recs, nepochs, cfg=None, graph=True):
+        """nepochs epochs of every problem at one zoom level: one setup, then
+        the global steps of an epoch (the most batches over the problems)
+        captured once in a HIP graph and replayed per epoch (graph=False:
+        launched directly)."""
+        import torch
+
+        plan = self.setup(grids, recs, nepochs, cfg)
+        if nepochs <= 0:
+            return plan
+        per_epoch = plan.max_steps // int(nepochs)
+        if graph and nepochs > 1:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.steps(plan, per_epoch)
+            for _ in range(int(nepochs)):
+                g.replay()
+            del g
+        else:
+            self.steps(plan, plan.max_steps)
+        return plan
+
+    def _snapshot(self):
+        return [[t.clone() for t in (e.params, e.m, e.v, e.count, e.loss_hist)] for e in self.engines]
+
+    def _restore(self, snap):
+        for e, s in zip(self.engines, snap):
+            for t, v in zip((e.params, e.m, e.v, e.count, e.loss_hist), s):
+                t.copy_(v)
+
+    def autotune(self, grids, recs, caps=(256, 128, 64, 32, 16), steps=6, cache_path=None):
+        """Choose the batched step's shape (dips_batch_cfg_t: blocks per
+        problem and step, persistent grid size) by timing `steps` global steps
+        of these records for every candidate, from the same state (restored
+        after each).  Results are cached in memory and in a JSON file keyed by
+        (GPU architecture, network shape, the problems' point counts and batch
+        sizes); returns the chosen (blk_cap, grid_blocks)."""
+        import json
+        import os
+
+        import torch
+
+        arch = torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName
+        sizes = sorted((int(g.npts), int(r.batch_size)) for g, r in zip(grids, recs))
+        key = json.dumps([arch, [int(self.net.Lm), int(self.net.Hm), int(self.net.Lp), int(self.net.Hp),
+                                 int(self.net.pc_w1), int(self.net.pc_w2), int(self.net.act)], sizes])
+        path = cache_path or os.environ.get("DIPS_AUTOTUNE_CACHE",
+                                            os.path.join(os.path.expanduser("~"), ".cache", "jax_dips_amd",
+                                                         "batch_autotune.json"))
+        if key in _BATCH_TUNED:
+            return _BATCH_TUNED[key]
+        try:
+            with open(path) as f:
+                disk = json.load(f)
+        except (OSError, ValueError):
+            disk = {}
+        if key in disk:
+            _BATCH_TUNED[key] = tuple(disk[key]["cfg"])
+            return _BATCH_TUNED[key]
+        snap = self._snapshot()
+        # persistent grids of one and two workgroups per resident slot
+        resident = int(self.setup(grids, recs, 1, (0, 0)).grid_blocks)
+        timings = {}
+        for cap in caps:
+            for grid in (resident, 2 * resident):
+                plan = self.setup(grids, recs, 1, (cap, grid))
+                self.steps(plan, 1)  # warm-up
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record()
+                self.steps(plan, steps)
+                ev[1].record()
+                torch.cuda.synchronize()
+                timings[(cap, grid)] = ev[0].elapsed_time(ev[1]) / steps
+                self._restore(snap)
+        torch.cuda.synchronize()
+        best = min(timings, key=timings.get)
+        self.tuning = {"candidates_ms": {f"{c}/{g}": t for (c, g), t in timings.items()}, "cfg": list(best),
+                       "resident_blocks": resident}
+        _BATCH_TUNED[key] = best
+        disk[key] = {"cfg": list(best), "ms_per_step": timings[best]}
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            with open(path, "w") as f:
+                json.dump(disk, f)
+        except OSError:
+            pass
+        return best
+
+
+# in-process cache of BatchEngine.autotune: key -> (blk_cap, grid_blocks)
+_BATCH_TUNED = {}

@@ -39,7 +39,8 @@ def test_host_only_queries():
 
 STRUCTS = {"Grid": "dips_grid_t", "Records": "dips_records_t", "Net": "dips_net_t", "Opt": "dips_opt_t",
            "Tiles": "dips_tiles_t", "MfmaBatch": "dips_mfma_batch_t", "Assemble": "dips_assemble_t",
-           "BatchProblem": "dips_batch_problem_t", "HashGrid": "dips_hashgrid_t"}
+           "BatchProblem": "dips_batch_problem_t", "HashGrid": "dips_hashgrid_t", "BatchCfg": "dips_batch_cfg_t",
+           "BatchPlan": "dips_batch_plan_t"}
 
 
 def check_layout(classes, tmp_path):

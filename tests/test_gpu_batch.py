@@ -139,3 +139,75 @@ def test_solve_batch_matches_trainer_runs(tmp_path):
         assert np.allclose(res[i]["loss_epochs"], le, rtol=1e-6, atol=0)
         assert np.array_equal(tr.model.flatten(res[i]["params"]), tr.model.flatten(tr.params))
         assert np.max(np.abs(res[i]["solution"] - st.solution)) <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg", [(32, 0), (8, 64)])
+def test_batch_shape_equals_each_problem_alone(cfg):
+    """With a configured shape (fewer blocks per problem and step, a given
+    persistent grid) every problem of the batched run equals its run alone
+    through the same entry point and shape, bit for bit; the epochs replayed
+    from a captured graph equal the launched ones."""
+    from jax_dips.records import BatchEngine
+    import torch
+
+    dims = (2, 16, 2, 16)
+    levels = [_level(p, n, 0, b) for p, n, b in CASES]
+    grids = [g for g, _ in levels]
+    recs = [r for _, r in levels]
+    params = [M.init_params(dims, seed=60 + i) for i in range(len(CASES))]
+    epochs = 3
+    together = BatchEngine(dims, params, CFG, [64] * len(CASES))
+    together.train_level(grids, recs, epochs, cfg=cfg, graph=True)
+    launched = BatchEngine(dims, params, CFG, [64] * len(CASES))
+    launched.train_level(grids, recs, epochs, cfg=cfg, graph=False)
+    torch.cuda.synchronize()
+    for i in range(len(CASES)):
+        alone = BatchEngine(dims, [params[i]], CFG, [64])
+        alone.train_level([grids[i]], [recs[i]], epochs, cfg=cfg, graph=False)
+        torch.cuda.synchronize()
+        a, b, c = together.engines[i], alone.engines[0], launched.engines[i]
+        n = epochs * len(recs[i].batch_begin)
+        assert int(a.count.item()) == int(b.count.item()) == int(c.count.item()) == n
+        for x, y, z in ((a.params, b.params, c.params), (a.m, b.m, c.m), (a.v, b.v, c.v),
+                        (a.loss_hist, b.loss_hist, c.loss_hist)):
+            assert torch.equal(x, y) and torch.equal(x, z), i
+        print(f"problem {i}: {n} steps, last loss {float(a.loss_hist[n - 1]):.6e}")
+
+
+@pytest.mark.gpu
+def test_batch_autotune_picks_a_measured_shape(tmp_path):
+    """The autotuner times every candidate shape from the same state (the
+    problems' parameters and step counters are restored after each), caches
+    the choice in memory and in its JSON file, and training with the chosen
+    shape matches the oracle loop like any other shape."""
+    import json
+
+    from jax_dips import records
+    from jax_dips.records import BatchEngine
+    import torch
+
+    dims = (2, 16, 2, 16)
+    levels = [_level(p, n, 1, b) for p, n, b in CASES]
+    grids = [g for g, _ in levels]
+    recs = [r for _, r in levels]
+    params = [M.init_params(dims, seed=70 + i) for i in range(len(CASES))]
+    be = BatchEngine(dims, params, CFG, [64] * len(CASES))
+    records._BATCH_TUNED.clear()
+    path = str(tmp_path / "tune.json")
+    cfg = be.autotune(grids, recs, caps=(256, 16), steps=2, cache_path=path)
+    assert cfg in {(256, be.tuning["resident_blocks"]), (256, 2 * be.tuning["resident_blocks"]),
+                   (16, be.tuning["resident_blocks"]), (16, 2 * be.tuning["resident_blocks"])}
+    assert len(be.tuning["candidates_ms"]) == 4
+    for e, p in zip(be.engines, params):
+        assert int(e.count.item()) == 0 and np.array_equal(e.params.cpu().numpy(), p)
+    disk = json.load(open(path))
+    assert [tuple(v["cfg"]) for v in disk.values()] == [cfg]
+    records._BATCH_TUNED.clear()
+    assert be.autotune(grids, recs, caps=(256, 16), cache_path=path) == cfg  # from the file
+    be.train_level(grids, recs, 2, cfg=cfg)
+    torch.cuda.synchronize()
+    for i, (g, r) in enumerate(levels):
+        e = be.engines[i]
+        assert int(e.count.item()) == 2 * len(r.batch_begin)
+        assert np.all(np.isfinite(e.params.cpu().numpy()))
