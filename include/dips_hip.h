@@ -433,6 +433,23 @@ int dips_bicgstab(const dips_grid_t* grid, const dips_records_t* rec, float* x, 
                   int32_t max_iter, double rtol, int32_t* iters, double* relres, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Molecular level set and charge density, the user callbacks of the
+ * Poisson-Boltzmann problems (replaces the per-query scan over every atom of
+ * examples/biomolecules/geometry.py:10-39 by a cell list).  atoms (natoms,5)
+ * float64: x, y, z, radius, charge; the atoms of cell c = (ix*nc + iy)*nc + iz
+ * are cell_atoms[cell_start[c] .. cell_start[c+1]), cells of size `cell` from
+ * `lo` on every axis (a query's cell is clamped to the grid).  At the m
+ * positions pos (m,3) float64: phi = min(cap, min over the atoms of the 27
+ * neighbouring cells of |x - c_i| - r_i) (NULL to skip) and charge = sum over
+ * the same atoms of q_i exp(-|x - c_i|^2 / (2 sigma^2)) / (2 pi sigma^2)^1.5
+ * (NULL to skip).  With cell >= cap + max radius, phi is the union-of-spheres
+ * distance wherever it is below cap.
+ * ------------------------------------------------------------------------- */
+int dips_atoms_eval(const double* atoms, int32_t natoms, const int32_t* cell_start, const int32_t* cell_atoms,
+                    int32_t nc, double lo, double cell, double cap, double sigma, const double* pos, int64_t m,
+                    double* phi, double* charge, void* stream);
+
+/* ---------------------------------------------------------------------------
  * FP64 cut-cell geometry, regression rows and record assembly.
  * (geometry/geometric_integrations_per_point.py:32-1009;
  *  solvers/poisson/discretization.py:164-519)

@@ -297,6 +297,13 @@ int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t n
   return batch_train(probs, nprob, nepochs, net, opt, work, work_bytes, (hipStream_t)stream);
 }
 
+int dips_atoms_eval(const double* atoms, int32_t natoms, const int32_t* cell_start, const int32_t* cell_atoms,
+                    int32_t nc, double lo, double cell, double cap, double sigma, const double* pos, int64_t m,
+                    double* phi, double* charge, void* stream) {
+  return atoms_launch(atoms, natoms, cell_start, cell_atoms, nc, lo, cell, cap, sigma, pos, m, phi, charge,
+                      (hipStream_t)stream);
+}
+
 int dips_geom_regions(const double* phic, int64_t n7, int8_t* region, void* stream) {
   if (n7 < 0 || (n7 > 0 && (!phic || !region))) {
     set_error("dips_geom_regions: invalid arguments");

@@ -56,6 +56,9 @@ int batch_setup(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepoch
                 const dips_opt_t* opt, const dips_batch_cfg_t* cfg, void* work, int64_t work_bytes,
                 dips_batch_plan_t* plan, hipStream_t stream);
 int batch_steps(const dips_net_t* net, const dips_batch_plan_t* plan, int32_t nsteps, void* work, hipStream_t stream);
+int atoms_launch(const double* atoms, int32_t natoms, const int32_t* cell_start, const int32_t* cell_atoms, int32_t nc,
+                 double lo, double cell, double cap, double sigma, const double* pos, int64_t m, double* phi,
+                 double* charge, hipStream_t st);
 // workgroups of a kernel resident at once on the current device (cached per kernel and device)
 int resident_blocks(const void* fn, int threads, size_t lds, int fallback);
 int tile_supported(const dips_net_t* net);

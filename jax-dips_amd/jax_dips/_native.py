@@ -166,6 +166,8 @@ SIGNATURES = {
     "dips_grid_apply": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _P, ctypes.c_int, _P]),
     "dips_bicgstab": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), _P, _P, _P, _I32, ctypes.c_double,
                                      ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double), _P]),
+    "dips_atoms_eval": (ctypes.c_int, [_P, _I32, _P, _P, _I32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, _P, _I64, _P, _P, _P]),
     "dips_geom_regions": (ctypes.c_int, [_P, _I64, _P, _P]),
     "dips_geom_cells": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _I64,
                                        _P, _P, _P, _P, _P, _P]),

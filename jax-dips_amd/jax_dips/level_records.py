@@ -66,6 +66,59 @@ def _offsets(vals, device):
     return torch.tensor(vals, dtype=torch.float32, device=device)
 
 
+def stencil_corner_phi(grid, S, ht, half, phi):
+    """phi at the 7 stencil points s_e = fl32(p + e h) of every training point,
+    (N,7), and at the 8 corners fl32(s_e + c h/2) of each stencil cell,
+    (N,7,8), float64.
+
+    For the grid nodes every coordinate of these positions depends on its own
+    axis only (node index, stencil sign, corner sign), so the positions are
+    formed from per-axis tables of the distinct float32 values, and phi is
+    evaluated once per distinct position: neighbouring points share cells and
+    corners (at the training spacing a node's 63 samples have about 5
+    distinct positions, at h/8 about 45).  The values are the same floats the
+    direct construction gives (same operations, same rounding).  Extra points
+    (batch padding) are sampled directly."""
+    import torch
+
+    device = S.device
+    nx, ny, nz = grid.shape
+    ng = grid.ngrid
+    sgn = torch.tensor([-1.0, 0.0, 1.0], dtype=torch.float32, device=device)
+    tabs = []
+    for a, axv in enumerate((grid.xs, grid.ys, grid.zs)):
+        V = axv[:, None] + sgn[None, :] * ht[a]  # stencil coordinate: node value + e h
+        W = V[:, :, None] + sgn[None, None, :] * half[a]  # corner coordinate; sign 0 gives V itself
+        U, inv = torch.unique(W.reshape(-1), sorted=True, return_inverse=True)
+        tabs.append((U, inv.reshape(len(axv), 3, 3)))
+    (Ux, Ix), (Uy, Iy), (Uz, Iz) = tabs
+    My, Mz = Uy.numel(), Uz.numel()
+    # keys of the 63 samples of every node: per stencil point e, the point (sign 0) then its 8 corners
+    combos = [(e, c) for e in STENCIL for c in ((0, 0, 0),) + CORNER_SIGNS]
+    keys = torch.empty((ng, len(combos)), dtype=torch.int64, device=device)
+    for q, (e, c) in enumerate(combos):
+        kx = Ix[:, e[0] + 1, c[0] + 1].to(torch.int64)
+        ky = Iy[:, e[1] + 1, c[1] + 1].to(torch.int64)
+        kz = Iz[:, e[2] + 1, c[2] + 1].to(torch.int64)
+        keys[:, q] = ((kx[:, None, None] * My + ky[None, :, None]) * Mz + kz[None, None, :]).reshape(-1)
+    uniq, inv = torch.unique(keys.reshape(-1), sorted=True, return_inverse=True)
+    del keys
+    pos = torch.stack([Ux[uniq // (My * Mz)], Uy[(uniq // Mz) % My], Uz[uniq % Mz]], dim=1)
+    del uniq
+    vals = sample(phi, pos)[inv].reshape(ng, 7, 9)
+    del inv, pos
+    phis = torch.empty((grid.npts, 7), dtype=torch.float64, device=device)
+    phic = torch.empty((grid.npts, 7, 8), dtype=torch.float64, device=device)
+    phis[:ng] = vals[:, :, 0]
+    phic[:ng] = vals[:, :, 1:]
+    del vals
+    if grid.npts > ng:
+        Se = S[ng:]
+        phis[ng:] = sample(phi, Se)
+        phic[ng:] = sample(phi, Se[:, :, None, :] + _offsets(CORNER_SIGNS, device)[None, None] * half)
+    return phis.contiguous(), phic.contiguous()
+
+
 def build_level(grid, h, bounds, cbs, batch_size, device="cuda", coeffs=False):
     """DeviceRecords of one zoom level.
 
@@ -89,10 +142,7 @@ def build_level(grid, h, bounds, cbs, batch_size, device="cuda", coeffs=False):
     # stencil points and the corners of their cells (float32, reference order of operations)
     S = P[:, None, :] + _offsets(STENCIL, device)[None] * ht  # (N,7,3)
     half = ht * 0.5
-    C = S[:, :, None, :] + _offsets(CORNER_SIGNS, device)[None, None] * half  # (N,7,8,3)
-    phic = sample(cbs["phi"], C).contiguous()
-    del C
-    phis = sample(cbs["phi"], S).contiguous()
+    phis, phic = stencil_corner_phi(grid, S, ht, half, cbs["phi"])
     region = torch.empty((npts, 7), dtype=torch.int8, device=device)
     N.check(L.dips_geom_regions(N.ptr(phic), npts * 7, N.ptr(region), st), "dips_geom_regions")
 

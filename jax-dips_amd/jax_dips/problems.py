@@ -378,7 +378,31 @@ class AtomField:
         self.cx = np.concatenate([self.c, [[1e6, 1e6, 1e6]]])
         self.rx = np.concatenate([self.r, [0.0]])
         self.qx = np.concatenate([self.q, [0.0]])
+        # the same cell list as compressed rows, for the device kernel (dips_atoms_eval)
+        self.cell_start = np.zeros(self.nc ** 3 + 1, np.int32)
+        np.cumsum(counts, out=self.cell_start[1:])
+        self.cell_atoms = np.argsort(flat, kind="stable").astype(np.int32)
+        self.atoms5 = np.ascontiguousarray(np.concatenate([self.c, self.r[:, None], self.q[:, None]], axis=1))
         self._dev = {}
+
+    def _eval_device(self, r, what):
+        """phi or charge of GPU positions r (3, m) through dips_atoms_eval."""
+        import torch
+
+        from . import _native as N
+
+        key = ("kernel", str(r.device))
+        if key not in self._dev:
+            self._dev[key] = tuple(torch.as_tensor(a, device=r.device) for a in
+                                   (self.atoms5, self.cell_start, self.cell_atoms))
+        atoms, cs, ca = self._dev[key]
+        pos = r.t().to(torch.float64).contiguous()
+        m = int(pos.shape[0])
+        out = torch.empty(m, dtype=torch.float64, device=r.device)
+        N.check(N.lib().dips_atoms_eval(N.ptr(atoms), len(self.c), N.ptr(cs), N.ptr(ca), self.nc, self.lo, self.cell,
+                                        self.cap, self.sigma, N.ptr(pos), m, N.ptr(out) if what == "phi" else None,
+                                        N.ptr(out) if what != "phi" else None, N.current_stream()), "dips_atoms_eval")
+        return out
 
     def _arrays(self, like):
         if not xnp._is_torch(like):
@@ -392,6 +416,8 @@ class AtomField:
         return (torch,) + self._dev[key]
 
     def _eval(self, r, what):
+        if xnp._is_torch(r) and r.is_cuda:
+            return self._eval_device(r, what)
         lib, table, cx, rx, qx = self._arrays(r)
         m = r.shape[1]
         out = lib.empty(m, dtype=lib.float64) if lib is np else lib.empty(m, dtype=lib.float64, device=r.device)

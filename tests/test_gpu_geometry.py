@@ -207,3 +207,27 @@ def test_pb_box_records_match_oracle(level):
     print(f"loss gpu {loss:.9e} oracle {ol:.9e} grad rel {np.linalg.norm(grad - og) / np.linalg.norm(og):.2e}")
     assert abs(loss - ol) <= 1e-5 * abs(ol)
     assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)
+
+
+@pytest.mark.gpu
+def test_atom_field_kernel_matches_host():
+    """dips_atoms_eval (cell list on the device, float64) gives the host cell
+    list's phi bit for bit (same subtraction order, no contraction) and its
+    charge to 1e-13 relative (exp and summation order differ), and both equal
+    the all-atom brute force where phi is below cap; also through the padded
+    query sizes of the record build (m not a multiple of the workgroup)."""
+    import torch
+
+    prob = problems.pb_box(n_atoms=400, half_width=10.0, seed=6)
+    f = prob["field"]
+    rng = np.random.default_rng(2)
+    for m in (1, 255, 5003):
+        x = rng.uniform(-10.5, 10.5, (3, m))
+        xd = torch.as_tensor(x, device="cuda")
+        phi = f.phi(xd).cpu().numpy()
+        chg = f.charge(xd).cpu().numpy()
+        assert np.array_equal(phi, f.phi(x)), m
+        hc = f.charge(x)
+        assert np.max(np.abs(chg - hc)) <= 1e-13 * max(np.max(np.abs(hc)), 1e-300), m
+        d2 = ((x.T[:, None, :] - f.c[None]) ** 2).sum(axis=2)
+        assert np.array_equal(phi, np.minimum((np.sqrt(d2) - f.r[None]).min(axis=1), f.cap)), m

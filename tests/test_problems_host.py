@@ -53,3 +53,33 @@ def test_reference_problems_are_consistent(name):
     box = rng.uniform(-1, 1, (3, 5000))
     assert np.all(prob["mu_m_fn"](box) > prob["mu_p_fn"](box))
     assert np.all(np.isfinite(prob["lvl_set_fn"](np.zeros((3, 1)))))
+
+
+def brute_atoms(c, rad, q, x, sigma=0.5):
+    """phi = min over every atom of |x - c_i| - r_i and the full Gaussian charge
+    sum: the reference's brute-force scan (examples/biomolecules/geometry.py:10-39)."""
+    d2 = ((x.T[:, None, :] - c[None]) ** 2).sum(axis=2)
+    phi = (np.sqrt(d2) - rad[None]).min(axis=1)
+    chg = (q[None] * np.exp(-d2 / (2 * sigma ** 2))).sum(axis=1) / (2 * np.pi * sigma ** 2) ** 1.5
+    return phi, chg
+
+
+def test_atom_field_cell_list_matches_brute_force():
+    """The 27-cell query equals the all-atom minimum wherever it is below cap
+    (and is cap elsewhere), and the charge equals the all-atom sum to within the
+    Gaussian tails cut beyond one cell; the compressed rows hold the same atoms
+    per cell as the padded table."""
+    prob = problems.pb_box(n_atoms=300, half_width=8.0, seed=4)
+    f = prob["field"]
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-8.5, 8.5, (3, 4000))
+    phi, chg = f.phi(x), f.charge(x)
+    bphi, bchg = brute_atoms(f.c, f.r, f.q, x)
+    assert np.array_equal(phi, np.minimum(bphi, f.cap))
+    assert np.max(np.abs(chg - bchg)) <= 1e-12 * np.max(np.abs(bchg))
+    pad = len(f.c)
+    for cell in range(f.nc ** 3):
+        rows = f.cell_atoms[f.cell_start[cell]:f.cell_start[cell + 1]]
+        assert sorted(rows.tolist()) == sorted(a for a in f.table[cell].tolist() if a != pad)
+    assert f.cell_start[-1] == pad and np.array_equal(np.sort(f.cell_atoms), np.arange(pad))
+    assert np.array_equal(f.atoms5[:, :3], f.c) and np.array_equal(f.atoms5[:, 3], f.r)
