@@ -180,7 +180,7 @@ def main():
     mgrid = grid if args.mfma_groups == "on" else None
     if use_mfma:
         for rec in levels:
-            eng.mfma_workspace(rec, mgrid)
+            eng.mfma_workspace(rec, grid, grouped=mgrid is not None)
     if path == "tile":
         for rec in levels:
             rec.tiles(grid)

@@ -330,6 +330,14 @@ typedef struct {
    * ignore the groups). */
   const int32_t* t_group;
   int64_t g_begin, g_plus, g_minus;
+  /* Items of the level (the groups when t_group is set, else the phase-sorted entries), built once per
+   * level by dips_mfma_items and required by the MFMA step entry points: i_pos holds the item positions
+   * SoA (x at [item], y at [i_count + item], z at [2 i_count + item]); e_item[e] is the item of CSR entry
+   * e.  The forward tiles read a position with one load and write u per item, the row sums read u
+   * through e_item, and the backward tiles read a per-item seed computed after the row sums. */
+  const float* i_pos;
+  int64_t i_count;
+  const int32_t* e_item;
 } dips_mfma_batch_t;
 
 /* 1 if (Lm, ceil(Hm/16), Lp, ceil(Hp/16)) has an MFMA kernel. */
@@ -356,6 +364,14 @@ int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
 int dips_mfma_train_step_noalloc(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
                                  const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
                                  float* v, int32_t* count, float* loss_hist, int32_t loss_cap, void* stream);
+/* Item positions and entry -> item map of one batch (see dips_mfma_batch_t), for every batch of a level
+ * once per level: fills i_pos[item], i_pos[i_count + item], i_pos[2 i_count + item] for the batch's items
+ * (batch->i_count gives the stride) and e_item[e] for its CSR entries.  The position of an item is that
+ * of its first member, fl32(p + o h) with p its row's point (row_point must name valid points).  No
+ * counterpart in the reference: part of the record build (trainer.py:892-912 evaluates the network at
+ * these positions every step). */
+int dips_mfma_items(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch, float* i_pos,
+                    int32_t* e_item, void* stream);
 /* Loss (device double *loss) and mean gradient (device grad[P]) of the batch
  * through the MFMA kernels, without an optimizer update. */
 int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,

@@ -462,6 +462,19 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
                    nullptr, 0, work, work_floats, dwork, dwork_doubles, grad, loss, (hipStream_t)stream);
 }
 
+int dips_mfma_items(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch, float* i_pos,
+                    int32_t* e_item, void* stream) {
+  int st = check_records(grid, rec);
+  if (st) return st;
+  if (!batch || batch->ent_begin < 0 || batch->ent_end > rec->nnz || batch->ent_begin > batch->ent_end ||
+      batch->t_begin < 0 || batch->t_begin + batch->t_count > rec->nnz || !batch->t_row || !batch->t_code ||
+      !batch->t_csr) {
+    set_error("dips_mfma_items: invalid arguments");
+    return DIPS_EINVAL;
+  }
+  return mfma_items_launch(grid, rec, batch, i_pos, e_item, (hipStream_t)stream);
+}
+
 int64_t dips_hash_table_rows(const dips_hashgrid_t* hg) { return hash_table_rows(hg); }
 
 int64_t dips_hash_param_count(const dips_hashgrid_t* hg) { return hash_param_count(hg); }

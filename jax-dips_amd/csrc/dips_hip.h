@@ -108,6 +108,8 @@ int mfma_supported(const dips_net_t* n);
 int64_t mfma_workspace_floats(const dips_net_t* n, int64_t max_rows, int64_t max_nnz);
 int mfma_eval_launch(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase,
                      const float* nrm, int64_t m, float* u, float* grad, float* gradn, hipStream_t st);
+int mfma_items_launch(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, float* i_pos,
+                      int32_t* e_item, hipStream_t st);
 int mfma_step_noalloc(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt,
                       const dips_net_t* net, const dips_opt_t* opt, float inv_b, float* params, float* m, float* v,
                       int32_t* count, float* loss_hist, int loss_cap, hipStream_t st);

@@ -171,7 +171,16 @@ struct MfmaTileArgs {
   const int32_t* t_csr;      // phase-sorted entries: CSR entry index
   const int32_t* t_group;    // items = groups of entries at one position: group g is the phase-sorted entries
                              // [t_group[g], t_group[g+1]); null: every entry is an item of its own
+  const float* i_pos;        // item positions of the level, SoA: x at [item], y at [i_count + item], z at
+                             // [2 i_count + item] (built once per level by mfma_items_kernel)
+  int64_t i_count;
 };
+
+// input of lane group k (x, y, z or the constant 1 of the bias row) for item `item`; 0 / 1 past the end
+__device__ __forceinline__ float item_input(const MfmaTileArgs& T, int64_t item, int64_t item_end, int k) {
+  if (k == 3) return 1.0f;
+  return item < item_end ? T.i_pos[k * T.i_count + item] : 0.0f;
+}
 
 // phase-sorted entries [b, e) of item `item`
 __device__ __forceinline__ void item_members(const MfmaTileArgs& T, int64_t item, int64_t& b, int64_t& e) {
@@ -202,18 +211,37 @@ __device__ __forceinline__ bool entry_pos(const MfmaTileArgs& T, int64_t e, int6
   return true;
 }
 
-// ---- kernel 1: u for every entry of the batch ----------------------------------
+// ---- per level: item positions and the entry -> item map ------------------------
+// One thread per item of a batch: its position (that of its first member entry;
+// all members share it) into i_pos, and its index into e_item at the CSR slot of
+// every member, through which the row sums read u per item.
+__global__ __launch_bounds__(256) void mfma_items_kernel(MfmaTileArgs T, int64_t i0, int64_t n, int32_t* __restrict__ e_item,
+                                                         float* __restrict__ i_pos) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t item = i0 + t;
+  int64_t mb, me;
+  item_members(T, item, mb, me);
+  float x, y, z;
+  int row;
+  entry_pos(T, mb, me, x, y, z, row);  // a row naming no valid point gives the position (0, 0, 0)
+  i_pos[item] = x;
+  i_pos[T.i_count + item] = y;
+  i_pos[2 * T.i_count + item] = z;
+  for (int64_t e = mb; e < me; ++e) e_item[T.t_csr[e]] = (int32_t)item;
+}
+
+// ---- kernel 1: u for every item of the batch -----------------------------------
 // Persistent: workgroups [0, nblk_p) take the + tiles (+ items [e0, e0+np)),
 // the others the - tiles; each packs its network's weights into LDS once and
 // its wavefronts stride over the tiles.  An item (an entry, or a group of
-// entries at one position) is evaluated once and its u written to the CSR
-// slot of every member entry.
+// entries at one position) is evaluated once and its u written to u_item[item - e0].
 constexpr int kFT = 512;  // threads per forward workgroup: 8 wavefronts share one packed weight copy
 constexpr int kFW = kFT / 64;
 
 template <int Lm, int NBm, int Lp, int NBp>
 __global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
-                                                       int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_out,
+                                                       int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_item,
                                                        int nblk_p) {
   extern __shared__ float lds[];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
@@ -229,15 +257,14 @@ __global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const 
   __syncthreads();
   const int64_t eb = plus ? e0 : e0 + np_;
   const int64_t ee = plus ? e0 + np_ : e0 + np_ + nm_;
-  for (int64_t tile = (int64_t)bid * kFW + wave; tile < ntiles; tile += (int64_t)nblk * kFW) {
+  const int k = lane / 16;
+  int64_t tile = (int64_t)bid * kFW + wave;
+  float xin_next = item_input(T, eb + tile * 16 + (lane & 15), ee, k);
+  for (; tile < ntiles; tile += (int64_t)nblk * kFW) {
     const int64_t item = eb + tile * 16 + (lane & 15);
-    int64_t mb = 0, me = 0;
-    if (item < ee) item_members(T, item, mb, me);
-    float x, y, z;
-    int row;
-    const bool valid = entry_pos(T, mb, me, x, y, z, row);
-    const int k = lane / 16;
-    const float xin = (k == 0) ? x : (k == 1) ? y : (k == 2) ? z : 1.0f;
+    // this tile's input was loaded one tile ahead; load the next one's now
+    const float xin = xin_next;
+    xin_next = item_input(T, item + (int64_t)nblk * kFW * 16, ee, k);
     float u;
     if (plus) {
       float act[Lp][4 * NBp];
@@ -248,17 +275,17 @@ __global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const 
       tile_forward<Lm, NBm, false>(lds, xin, act);
       u = tile_output<Lm, NBm, false>(lds, act);
     }
-    if (valid && lane < 16)
-      for (int64_t e = mb; e < me; ++e) u_out[T.t_csr[e]] = u;
+    if (item < ee && lane < 16) u_item[item - e0] = u;
   }
 }
 
 // ---- kernel 2: r per row, loss partials ---------------------------------------
+// u of CSR entry e: u[e] (per-entry values, e_item null), else u[e_item[e]] (per-item values).
 __global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restrict__ row_ptr,
                                                           const float* __restrict__ row_c, int64_t row_begin,
                                                           int64_t nrows, const float* __restrict__ ent_w, int64_t nnz,
-                                                          const float* __restrict__ u, float* __restrict__ r_out,
-                                                          double* __restrict__ loss_part) {
+                                                          const float* __restrict__ u, const int32_t* __restrict__ e_item,
+                                                          float* __restrict__ r_out, double* __restrict__ loss_part) {
   const int64_t i = (int64_t)blockIdx.x * kMT + threadIdx.x;
   double l = 0.0;
   if (i < nrows) {
@@ -266,7 +293,10 @@ __global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restr
     const int beg = max(row_ptr[row], 0);
     const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
     float r = row_c[row];
-    for (int e = beg; e < end; ++e) r = fmaf(ent_w[e], u[e], r);
+    if (e_item)
+      for (int e = beg; e < end; ++e) r = fmaf(ent_w[e], u[e_item[e]], r);
+    else
+      for (int e = beg; e < end; ++e) r = fmaf(ent_w[e], u[e], r);
     r_out[i] = r;
     l = 0.5 * (double)r * (double)r;
   }
@@ -275,6 +305,21 @@ __global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restr
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = l;
   __syncthreads();
   if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ---- kernel 2b: seed per item ----------------------------------------------------
+// seed of an item = sum over its member entries (in order) of r(row) w: the
+// derivative of the batch's half sum of squares with respect to the item's u.
+// Written to seed[item - i0] (the buffer that held u, consumed by the row sums).
+__global__ __launch_bounds__(256) void mfma_seed_kernel(MfmaTileArgs T, int64_t i0, int64_t n, int64_t row_begin,
+                                                        const float* __restrict__ r_row, float* __restrict__ seed) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t mb, me;
+  item_members(T, i0 + t, mb, me);
+  float s = r_row[T.t_row[mb] - row_begin] * T.t_w[mb];
+  for (int64_t e = mb + 1; e < me; ++e) s = fmaf(r_row[T.t_row[e] - row_begin], T.t_w[e], s);
+  seed[t] = s;
 }
 
 // ---- kernel 3: parameter-gradient partials ---------------------------------------
@@ -481,8 +526,8 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
 
 template <int Lm, int NBm, int Lp, int NBp>
 __global__ __launch_bounds__(kBT) void mfma_bwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
-                                                       int64_t e0, int64_t np_, int64_t nm_, int64_t row_begin,
-                                                       const float* __restrict__ r_row, float* __restrict__ gpart,
+                                                       int64_t e0, int64_t np_, int64_t nm_,
+                                                       const float* __restrict__ seed_item, float* __restrict__ gpart,
                                                        int nblk_p, int stride) {
   extern __shared__ __align__(16) float lds[];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
@@ -523,22 +568,20 @@ __global__ __launch_bounds__(kBT) void mfma_bwd_kernel(MfmaTileArgs T, const flo
     // tiles past the end carry no entries (zero seeds, zero inputs)
     const int64_t per_round = (int64_t)nblk * kBW;
     const int64_t rounds = (ntiles + per_round - 1) / per_round;
+    const int kq = lane / 16;
+    // input and seed of a round's item are loaded one round ahead, so that their
+    // latency overlaps the previous round's matrix work
+    auto item_of = [&](int64_t rd) { return eb + ((rd * nblk + bid) * kBW + wave) * 16 + (lane & 15); };
+    auto seed_of = [&](int64_t item) { return item < ee ? seed_item[item - e0] : 0.0f; };
+    float xin_next = item_input(T, item_of(0), ee, kq);
+    float seed_next = seed_of(item_of(0));
     for (int64_t rd = 0; rd < rounds; ++rd) {
-      const int64_t tile = (rd * nblk + bid) * kBW + wave;
-      const int64_t item = eb + tile * 16 + (lane & 15);
-      int64_t mb = 0, me = 0;
-      if (item < ee) item_members(T, item, mb, me);
-      float x, y, z;
-      int row;
-      const bool valid = entry_pos(T, mb, me, x, y, z, row);
-      // seed of the item: sum over its member entries of r(row) * w (entries in position order)
-      float seed = 0.0f;
-      if (valid) {
-        seed = r_row[row - row_begin] * T.t_w[mb];
-        for (int64_t e = mb + 1; e < me; ++e) seed = fmaf(r_row[T.t_row[e] - row_begin], T.t_w[e], seed);
+      const float xin = xin_next, seed = seed_next;
+      if (rd + 1 < rounds) {
+        const int64_t nx = item_of(rd + 1);
+        xin_next = item_input(T, nx, ee, kq);
+        seed_next = seed_of(nx);
       }
-      const int kq = lane / 16;
-      const float xin = (kq == 0) ? x : (kq == 1) ? y : (kq == 2) ? z : 1.0f;
       float act[L][4 * NB];
       tile_forward<L, NB>(lds, xin, act);
       coop_backward<L, NB>(lds, seed, xin, act, stage, kStrideW, G);
@@ -800,7 +843,7 @@ void rowsum_launch(const int32_t* row_ptr, const float* row_c, int64_t row_begin
                    int64_t nnz, const float* u, float* r, double* loss_part, hipStream_t st) {
   const int nrb = (int)((nrows + kMT - 1) / kMT);
   hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, row_ptr, row_c, row_begin, nrows, ent_w, nnz, u,
-                     r, loss_part);
+                     (const int32_t*)nullptr, r, loss_part);
 }
 
 int rowsum_blocks(int64_t nrows) { return (int)((nrows + kMT - 1) / kMT); }
@@ -915,8 +958,9 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     set_error("dips_mfma_train_step: invalid batch (t_count must equal ent_end - ent_begin and n_plus + n_minus)");
     return DIPS_EINVAL;
   }
-  // workspace layout (floats): u over the batch's CSR entries | r rows | gradient partials | g
-  float* u = work;  // indexed by CSR entry - ent_begin (offset by the caller through t_csr)
+  // workspace layout (floats): u per item, then seed per item (at most one item per entry) | r rows |
+  // gradient partials | g
+  float* u = work;  // indexed by item - i0
   float* r = u + nent;
   float* gpart = r + nrows;
   float* gvec = gpart + (int64_t)kMfmaBwdBlocks * stride;
@@ -941,8 +985,13 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   }
   const int64_t i0 = grouped ? bt->g_begin : bt->t_begin;
   const int64_t ip = grouped ? bt->g_plus : np_, im = grouped ? bt->g_minus : nm_;
+  if (!bt->i_pos || !bt->e_item || bt->i_count < i0 + ip + im) {
+    set_error("dips_mfma_train_step: the batch needs its level's item positions and entry -> item map "
+              "(i_pos, i_count, e_item; dips_mfma_items)");
+    return DIPS_EINVAL;
+  }
   MfmaTileArgs T{point_set(g), rec->hx, rec->hy, rec->hz, rec->row_point, bt->t_row, bt->t_code, bt->t_w,
-                 bt->t_csr, bt->t_group};
+                 bt->t_csr, bt->t_group, bt->i_pos, bt->i_count};
   const int a = Lm, b = nb_of(Hm), c = Lp, d = nb_of(Hp);
   const int64_t ntp = (ip + 15) / 16, ntm = (im + 15) / 16;
   // workgroups split by phase in proportion to the tiles (each at least 1 if it has tiles)
@@ -983,12 +1032,15 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     split(resident_blocks((const void*)mfma_fwd_kernel<A, B, C, D>, kFT, lds_f, kMfmaFwdBlocks), nfp, nfm);       \
     if (nfp + nfm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kFT), lds_f, st, T, params, Hm, Hp,   \
-                         i0, ip, im, u - bt->ent_begin, nfp);                                                      \
+                         i0, ip, im, u, nfp);                                                                      \
     hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, rec->row_ptr, rec->row_c, bt->row_begin,   \
-                       nrows, rec->ent_w, rec->nnz, u - bt->ent_begin, r, loss_part);                              \
+                       nrows, rec->ent_w, rec->nnz, u - i0, bt->e_item, r, loss_part);                             \
+    if (ip + im > 0)                                                                                               \
+      hipLaunchKernelGGL(mfma_seed_kernel, dim3((unsigned)((ip + im + 255) / 256)), dim3(256), 0, st, T, i0,       \
+                         ip + im, bt->row_begin, r, u);                                                            \
     if (nbp + nbm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kBT), bwd_lds, st, T,                 \
-                         params, Hm, Hp, i0, ip, im, bt->row_begin, r, gpart, nbp, stride);                        \
+                         params, Hm, Hp, i0, ip, im, u, gpart, nbp, stride);                                       \
   }
   DIPS_MFMA_SHAPES(X)
 #undef X
@@ -1010,6 +1062,24 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                      loss_part, nrb, inv_b, *opt, params, m, v, count, loss_hist, loss_cap);
   hipLaunchKernelGGL(count_inc_kernel, dim3(1), dim3(64), 0, st, count);
   return hip_status(hipGetLastError(), "mfma reduce/apply launch");
+}
+
+// item positions (SoA, stride bt->i_count) and entry -> item map of one batch, once per level
+int mfma_items_launch(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_batch_t* bt, float* i_pos,
+                      int32_t* e_item, hipStream_t st) {
+  const bool grouped = bt->t_group != nullptr;
+  const int64_t i0 = grouped ? bt->g_begin : bt->t_begin;
+  const int64_t n = grouped ? bt->g_plus + bt->g_minus : bt->n_plus + bt->n_minus;
+  if (!i_pos || !e_item || i0 < 0 || n < 0 || bt->i_count < i0 + n || bt->t_count != bt->ent_end - bt->ent_begin ||
+      (grouped && (bt->g_plus < 0 || bt->g_minus < 0 || n > bt->t_count))) {
+    set_error("dips_mfma_items: invalid batch or outputs (i_count must cover the batch's items)");
+    return DIPS_EINVAL;
+  }
+  if (n == 0) return DIPS_OK;
+  MfmaTileArgs T{point_set(g), rec->hx, rec->hy, rec->hz, rec->row_point, bt->t_row, bt->t_code, bt->t_w,
+                 bt->t_csr, bt->t_group, i_pos, bt->i_count};
+  hipLaunchKernelGGL(mfma_items_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, i0, n, e_item, i_pos);
+  return hip_status(hipGetLastError(), "mfma_items_kernel launch");
 }
 
 // ---- the same step without a caller-given workspace ------------------------------

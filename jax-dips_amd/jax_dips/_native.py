@@ -48,7 +48,8 @@ class MfmaBatch(ctypes.Structure):
                 ("t_w", ctypes.c_void_p), ("t_csr", ctypes.c_void_p), ("t_begin", ctypes.c_int64),
                 ("n_plus", ctypes.c_int64), ("n_minus", ctypes.c_int64), ("t_count", ctypes.c_int64),
                 ("t_group", ctypes.c_void_p), ("g_begin", ctypes.c_int64), ("g_plus", ctypes.c_int64),
-                ("g_minus", ctypes.c_int64)]
+                ("g_minus", ctypes.c_int64), ("i_pos", ctypes.c_void_p), ("i_count", ctypes.c_int64),
+                ("e_item", ctypes.c_void_p)]
 
 
 class Tiles(ctypes.Structure):
@@ -153,6 +154,8 @@ SIGNATURES = {
                                                     ctypes.c_float, _P, _P, _P, _P, _P, _I32, _P]),
     "dips_mfma_loss_grad": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch),
                                            ctypes.POINTER(Net), ctypes.c_float, _P, _P, _P, _P, _I64, _P, _I64, _P]),
+    "dips_mfma_items": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Records), ctypes.POINTER(MfmaBatch), _P, _P,
+                                       _P]),
     "dips_hash_table_rows": (_I64, [ctypes.POINTER(HashGrid)]),
     "dips_hash_param_count": (_I64, [ctypes.POINTER(HashGrid)]),
     "dips_hash_workspace": (_I64, [ctypes.POINTER(HashGrid), _I64, _I64, ctypes.POINTER(_I64)]),

@@ -263,6 +263,28 @@ class DeviceRecords:
         setattr(self, cache, mb)
         return mb
 
+    def mfma_items(self, grid, mb):
+        """Item positions and the entry -> item map of the batches in mb (one
+        ordering of mfma_batches), which the MFMA step kernels read: built on
+        the device once per ordering (dips_mfma_items per batch) and attached
+        to every batch (i_pos, i_count, e_item).  Items are the groups when mb
+        has them, else the entries."""
+        import torch
+
+        if "i_pos" in mb:
+            return mb
+        dev = self.row_ptr.device
+        ni = mb["groups"] if "t_group" in mb else self.nnz
+        i_pos = torch.zeros(3 * max(ni, 1), dtype=torch.float32, device=dev)
+        e_item = torch.zeros(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        for bt in mb["batches"]:
+            bt.i_count = ni
+            N.check(N.lib().dips_mfma_items(ctypes.byref(grid.c), ctypes.byref(self.c), ctypes.byref(bt), N.ptr(i_pos),
+                                            N.ptr(e_item), N.current_stream()), "dips_mfma_items")
+            bt.i_pos, bt.e_item = i_pos.data_ptr(), e_item.data_ptr()
+        mb["i_pos"], mb["e_item"] = i_pos, e_item
+        return mb
+
     def tiles(self, grid):
         """Wavefront tiles of every batch (rows packed into tiles of at most 64
         slots, dips_pack_tiles) and their slot layout (dips_tile_slots), built
@@ -422,13 +444,13 @@ class Engine:
             N.current_stream())
         N.check(st, "dips_tile_train_level")
 
-    def mfma_workspace(self, rec, grid=None):
-        """Phase-sorted batches of rec for the MFMA kernels (with the grid: in
-        position order, with groups of entries at one position) and a
-        workspace large enough for them."""
+    def mfma_workspace(self, rec, grid, grouped=True):
+        """Phase-sorted batches of rec for the MFMA kernels (grouped: in
+        position order, with groups of entries at one position), their items,
+        and a workspace large enough for them."""
         import torch
 
-        mb = rec.mfma_batches(grid)
+        mb = rec.mfma_items(grid, rec.mfma_batches(grid if grouped else None))
         need = int(N.lib().dips_mfma_workspace(ctypes.byref(self.net), int(rec.batch_size), mb["max_nnz"]))
         if getattr(self, "_mwork", None) is None or self._mwork.numel() < need:
             self._mwork = torch.empty(need, dtype=torch.float32, device=self.device)
@@ -441,7 +463,7 @@ class Engine:
         grouped: entries at one position evaluated once (else each entry)."""
         import torch
 
-        mb = self.mfma_workspace(rec, grid if grouped else None)
+        mb = self.mfma_workspace(rec, grid, grouped)
         g = torch.empty(self.P, dtype=torch.float32, device=self.device)
         loss = torch.empty(1, dtype=torch.float64, device=self.device)
         N.check(N.lib().dips_mfma_loss_grad(ctypes.byref(grid.c), ctypes.byref(rec.c), ctypes.byref(mb["batches"][b]),
@@ -453,7 +475,7 @@ class Engine:
         return float(loss.item()), g.double().cpu().numpy()
 
     def train_level_mfma(self, grid, rec, nepochs):
-        mb = self.mfma_workspace(rec, grid if self.mfma_grouped else None)
+        mb = self.mfma_workspace(rec, grid, self.mfma_grouped)
         L = N.lib()
         inv_b = ctypes.c_float(1.0 / rec.batch_size)
         for _ in range(int(nepochs)):
@@ -485,7 +507,7 @@ class Engine:
         if self.tile_path and (self.use_tiles or self.precond is not None or self.act != 0):
             rec.tiles(grid)
         elif not self.lane_path:
-            self.mfma_workspace(rec, grid if self.mfma_grouped else None)
+            self.mfma_workspace(rec, grid, self.mfma_grouped)
 
     def capture_epoch(self, grid, rec):
         """One epoch over rec's batches (the launches of train_level(grid, rec, 1))
