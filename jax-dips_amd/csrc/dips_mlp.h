@@ -37,10 +37,15 @@ __device__ __forceinline__ float dips_tanh(float x) {
 // below 4e-7 for every x with both hardware ops 1 ulp off (relative accuracy
 // degrades only where |tanh x| is small); saturates to +-1, propagates NaN.  The
 // hidden activation of every network kernel (tanh is their VALU bottleneck).
-__device__ __forceinline__ float dips_tanh_fast(float x) {
-  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // exp(2x)
-  return fmaf(-2.0f, __builtin_amdgcn_rcpf(1.0f + e), 1.0f);
+constexpr float kTanhExp2Scale = 2.8853900817779268f;  // 2 log2(e): exp(2x) = exp2(kTanhExp2Scale x)
+
+// tanh(s / kTanhExp2Scale): the same form for an argument already scaled, as the
+// MFMA kernels have it when the scale is folded into the packed weights.
+__device__ __forceinline__ float dips_tanh_exp2(float s) {
+  return fmaf(-2.0f, __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(s)), 1.0f);
 }
+
+__device__ __forceinline__ float dips_tanh_fast(float x) { return dips_tanh_exp2(x * kTanhExp2Scale); }
 
 // Hidden activations: 0 = tanh (DoubleMLP, nn/mlp/MLP.py:64-69), 1 = CELU with
 // alpha 1 (the coordinate MLPs of the hash network, nn/hash_encoding/blocks/

@@ -68,6 +68,10 @@ struct FlatNet {
   __device__ int bout() const { return wout() + Hh; }
 };
 
+// The forward copies (layer 0, the hidden layers' fwd matrices and all hidden
+// biases) are stored times kTanhExp2Scale, so that the accumulator of a hidden
+// layer is already the argument of dips_tanh_exp2; the transposed (bwd) copies
+// and the output layer are stored as they are.
 template <int L, int NB, bool kBwd = true>
 __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
   using PK = Packed<L, NB, kBwd>;
@@ -77,7 +81,7 @@ __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float
     if (idx < NB * 64) {
       const int ob = idx / 64, lane = idx % 64;
       const int out = 16 * ob + lane % 16, k = lane / 16;
-      if (out < Hh) v = (k < 3) ? params[fn.w(0) + k * Hh + out] : params[fn.b(0) + out];
+      if (out < Hh) v = kTanhExp2Scale * ((k < 3) ? params[fn.w(0) + k * Hh + out] : params[fn.b(0) + out]);
     } else if (idx >= PK::wout()) {
       const int h = idx - PK::wout();
       if (h < PK::H) {
@@ -97,10 +101,10 @@ __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float
         // fwd: W[16fb + 4(lane/16) + r][16xb + lane%16] ; bwd: W[16xb + lane%16][16fb + 4(lane/16) + r]
         const int a = 16 * fb + 4 * (lane / 16) + r, b = 16 * xb + lane % 16;
         const int in = bw ? b : a, out = bw ? a : b;
-        if (in < Hh && out < Hh) v = params[fn.w(l) + in * Hh + out];
+        if (in < Hh && out < Hh) v = (bw ? 1.0f : kTanhExp2Scale) * params[fn.w(l) + in * Hh + out];
       } else {
         const int h = q - (kBwd ? 2 : 1) * PK::kMat;
-        if (h < Hh) v = params[fn.b(l) + h];
+        if (h < Hh) v = kTanhExp2Scale * params[fn.b(l) + h];
       }
     }
     lds[idx] = v;
@@ -117,7 +121,7 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
     f4 c = {0.f, 0.f, 0.f, 0.f};
     c = mfma4(W[PK::l0() + ob * 64 + lane], xin, c);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh_fast(c[r]);
+    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh_exp2(c[r]);
   }
 #pragma unroll
   for (int l = 1; l < L; ++l) {
@@ -139,7 +143,7 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
 #pragma unroll
     for (int ob = 0; ob < NB; ++ob) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) act[l][ob * 4 + r] = dips_tanh_fast(c[ob][r]);
+      for (int r = 0; r < 4; ++r) act[l][ob * 4 + r] = dips_tanh_exp2(c[ob][r]);
     }
   }
 }
@@ -293,10 +297,23 @@ __global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restr
     const int beg = max(row_ptr[row], 0);
     const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
     float r = row_c[row];
-    if (e_item)
-      for (int e = beg; e < end; ++e) r = fmaf(ent_w[e], u[e_item[e]], r);
-    else
-      for (int e = beg; e < end; ++e) r = fmaf(ent_w[e], u[e], r);
+    // entries in chunks of 8: every load of a chunk is issued before the first use (the gathers through
+    // e_item depend on one load, not on the previous entry), the sum keeps the entry order
+    for (int e0 = beg; e0 < end; e0 += 8) {
+      float w[8], uv[8];
+      int it[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool in = e0 + j < end;
+        w[j] = in ? ent_w[e0 + j] : 0.0f;
+        it[j] = in ? (e_item ? e_item[e0 + j] : e0 + j) : -1;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) uv[j] = it[j] >= 0 ? u[it[j]] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (e0 + j < end) r = fmaf(w[j], uv[j], r);
+    }
     r_out[i] = r;
     l = 0.5 * (double)r * (double)r;
   }
@@ -317,8 +334,27 @@ __global__ __launch_bounds__(256) void mfma_seed_kernel(MfmaTileArgs T, int64_t 
   if (t >= n) return;
   int64_t mb, me;
   item_members(T, i0 + t, mb, me);
-  float s = r_row[T.t_row[mb] - row_begin] * T.t_w[mb];
-  for (int64_t e = mb + 1; e < me; ++e) s = fmaf(r_row[T.t_row[e] - row_begin], T.t_w[e], s);
+  // members in chunks of 4, loads of a chunk issued before the first use; the sum keeps the member order
+  float s = 0.0f;
+  for (int64_t c = mb; c < me; c += 4) {
+    float w[4], rv[4];
+    int64_t rr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool in = c + j < me;
+      w[j] = in ? T.t_w[c + j] : 0.0f;
+      rr[j] = in ? (int64_t)T.t_row[c + j] - row_begin : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) rv[j] = rr[j] >= 0 ? r_row[rr[j]] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (c + j == mb)
+        s = rv[j] * w[j];
+      else if (c + j < me)
+        s = fmaf(rv[j], w[j], s);
+    }
+  }
   seed[t] = s;
 }
 
@@ -694,8 +730,10 @@ __global__ void count_inc_kernel(int32_t* count) {
 // Value and the three input tangents of one 16-position tile, propagated
 // together: a_0 = tanh(W_0 [x,1]), t_0^k = (1 - a_0^2) W_0[k,:]; hidden layers
 // a_l = tanh(W_l a_{l-1} + b_l), t_l^k = (1 - a_l^2) (W_l t_{l-1}^k), sharing each
-// weight operand between the four MFMA chains.  Returns u and du/dx_k of the
-// tile position lane%16 (identical in the 4 lanes of a position).
+// weight operand between the four MFMA chains.  The packed forward weights carry
+// the factor kTanhExp2Scale, which the tangents divide out layer by layer.
+// Returns u and du/dx_k of the tile position lane%16 (identical in the 4 lanes
+// of a position).
 template <int L, int NB>
 __device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, float xin, float& u, float (&du)[3]) {
   using PK = Packed<L, NB, false>;
@@ -707,8 +745,8 @@ __device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, flo
     c = mfma4(W[PK::l0() + ob * 64 + lane], xin, c);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float v = dips_tanh_fast(c[r]);
-      const float g = 1.0f - v * v;
+      const float v = dips_tanh_exp2(c[r]);
+      const float g = (1.0f - v * v) * (1.0f / kTanhExp2Scale);
       a[ob * 4 + r] = v;
 #pragma unroll
       for (int k = 0; k < 3; ++k) t[k][ob * 4 + r] = g * W[PK::l0() + ob * 64 + 16 * k + 4 * (lane / 16) + r];
@@ -741,8 +779,8 @@ __device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, flo
     for (int ob = 0; ob < NB; ++ob) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float v = dips_tanh_fast(c[ob][r]);
-        const float g = 1.0f - v * v;
+        const float v = dips_tanh_exp2(c[ob][r]);
+        const float g = (1.0f - v * v) * (1.0f / kTanhExp2Scale);
         a[ob * 4 + r] = v;
 #pragma unroll
         for (int k = 0; k < 3; ++k) t[k][ob * 4 + r] = g * ct[k][ob][r];
