@@ -355,7 +355,7 @@ int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
 /* dips_mfma_train_step without a workspace argument and without allocating:
  * no counterpart in the reference; for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops).  The workspace is
- * a fixed device array of the library (6 Mi floats + 8 Ki doubles, 24.1 MiB,
+ * a fixed device array of the library (8 Mi floats + 8 Ki doubles, 32.1 MiB,
  * one copy per device, held until the process ends): a batch needing more
  * (dips_mfma_workspace floats, rows/256 + P/256 + 1 doubles; e.g. 2^20-row
  * batches) returns DIPS_EINVAL.  The device of `stream` must be current.

@@ -37,26 +37,31 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMT = 256;  // threads per workgroup (4 wavefronts)
 constexpr int kTP = 20;   // padded row length of a transposed 16-position tile (16-byte aligned rows)
-constexpr int kMfmaBwdBlocks = 256;   // workgroups of the backward kernel (both phases), one per CU
+constexpr int kMfmaBwdBlocks = 512;   // workgroups of the backward kernel (both phases), two per CU
 constexpr int kMfmaFwdBlocks = 1024;  // workgroups of the forward kernel (both phases), four per CU
 
 __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
 // ---- packed weights of one phase net in LDS ---------------------------------
-// [L0: NB*64] [for hidden layer 1..L-1: fwd NB*NB*64*4, (bwd NB*NB*64*4,) bias NB*16] [wout NB*16] [bout 1]
-// The transposed (bwd) copies are packed only for the backward kernel.
-template <int L, int NB, bool kBwd = true>
+// [L0: NB*64] [for hidden layer 1..L-1: NB*NB blocks of kBlk, bias NB*16] [wout NB*16] [bout 1]
+// Block (ob, fb) of hidden layer l holds W_l[16fb + 4(lane/16) + r][16ob + lane%16]
+// as lane's 4-float unit at unit(lane) = lane + lane/16: the forward pass reads
+// one 16-byte unit per lane (the A operands of 4 MFMA steps), and the backward
+// pass reads the transpose from the same copy, 4 floats per lane 4 apart; the
+// unit left free after every 16 spreads both patterns over all LDS banks.
+template <int L, int NB>
 struct Packed {
   static constexpr int H = 16 * NB;
-  static constexpr int kMat = NB * NB * 256;
-  static constexpr int kPer = (kBwd ? 2 : 1) * kMat + H;
+  static constexpr int kBlk = 4 * 68;  // floats of one 16x16 block: 64 units of 4 plus 4 free units
+  static constexpr int kMat = NB * NB * kBlk;
+  static constexpr int kPer = kMat + H;
   static constexpr int l0() { return 0; }
   static constexpr int fwd(int l) { return NB * 64 + (l - 1) * kPer; }
-  static constexpr int bwd(int l) { return fwd(l) + kMat; }
-  static constexpr int bias(int l) { return fwd(l) + (kBwd ? 2 : 1) * kMat; }
+  static constexpr int bias(int l) { return fwd(l) + kMat; }
   static constexpr int wout() { return NB * 64 + (L - 1) * kPer; }
   static constexpr int bout() { return wout() + H; }
   static constexpr int size() { return bout() + 1; }
+  static __device__ __forceinline__ int unit(int lane) { return lane + (lane >> 4); }
 };
 
 // flat-parameter offsets of one phase net of true width Hh
@@ -68,13 +73,13 @@ struct FlatNet {
   __device__ int bout() const { return wout() + Hh; }
 };
 
-// The forward copies (layer 0, the hidden layers' fwd matrices and all hidden
-// biases) are stored times kTanhExp2Scale, so that the accumulator of a hidden
-// layer is already the argument of dips_tanh_exp2; the transposed (bwd) copies
-// and the output layer are stored as they are.
-template <int L, int NB, bool kBwd = true>
+// Layer 0, the hidden-layer matrices and the hidden biases are stored times
+// kTanhExp2Scale, so that the accumulator of a hidden layer is already the
+// argument of dips_tanh_exp2 (the backward pass divides the factor out of its
+// gradients); the output layer is stored as it is.
+template <int L, int NB>
 __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
-  using PK = Packed<L, NB, kBwd>;
+  using PK = Packed<L, NB>;
   const int Hh = fn.Hh;
   for (int idx = threadIdx.x; idx < PK::size(); idx += blockDim.x) {
     float v = 0.0f;
@@ -91,19 +96,19 @@ __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float
       }
     } else {
       const int rel = idx - NB * 64;
-      const int per = PK::kPer;
-      const int l = rel / per + 1;
-      const int q = rel % per;
-      if (q < (kBwd ? 2 : 1) * PK::kMat) {
-        const bool bw = q >= PK::kMat;
-        const int t = bw ? q - PK::kMat : q;
-        const int r = t % 4, lane = (t / 4) % 64, fb = (t / 256) % NB, xb = t / (256 * NB);
-        // fwd: W[16fb + 4(lane/16) + r][16xb + lane%16] ; bwd: W[16xb + lane%16][16fb + 4(lane/16) + r]
-        const int a = 16 * fb + 4 * (lane / 16) + r, b = 16 * xb + lane % 16;
-        const int in = bw ? b : a, out = bw ? a : b;
-        if (in < Hh && out < Hh) v = (bw ? 1.0f : kTanhExp2Scale) * params[fn.w(l) + in * Hh + out];
+      const int l = rel / PK::kPer + 1;
+      const int q = rel % PK::kPer;
+      if (q < PK::kMat) {
+        const int blk = q / PK::kBlk, t = q % PK::kBlk;
+        const int up = t / 4, r = t % 4;  // unit position up = 17 a + b holds lane 16 a + b (b < 16)
+        if (up % 17 < 16) {
+          const int lane = 16 * (up / 17) + up % 17;
+          const int ob = blk / NB, fb = blk % NB;
+          const int in = 16 * fb + 4 * (lane / 16) + r, out = 16 * ob + lane % 16;
+          if (in < Hh && out < Hh) v = kTanhExp2Scale * params[fn.w(l) + in * Hh + out];
+        }
       } else {
-        const int h = q - (kBwd ? 2 : 1) * PK::kMat;
+        const int h = q - PK::kMat;
         if (h < Hh) v = kTanhExp2Scale * params[fn.b(l) + h];
       }
     }
@@ -112,9 +117,9 @@ __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float
 }
 
 // forward of one tile: act[l][fb*4+r] feature-major; returns nothing (u by caller)
-template <int L, int NB, bool kBwd = true>
+template <int L, int NB>
 __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
-  using PK = Packed<L, NB, kBwd>;
+  using PK = Packed<L, NB>;
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int ob = 0; ob < NB; ++ob) {
@@ -135,7 +140,7 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
     for (int fb = 0; fb < NB; ++fb) {
 #pragma unroll
       for (int ob = 0; ob < NB; ++ob) {
-        const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::fwd(l) + ((ob * NB + fb) * 64 + lane) * 4]);
+        const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::fwd(l) + (ob * NB + fb) * PK::kBlk + 4 * PK::unit(lane)]);
 #pragma unroll
         for (int r = 0; r < 4; ++r) c[ob] = mfma4(w4[r], act[l - 1][fb * 4 + r], c[ob]);
       }
@@ -149,9 +154,9 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
 }
 
 // output u of the tile position l%16 (identical in the 4 lanes of that position)
-template <int L, int NB, bool kBwd = true>
+template <int L, int NB>
 __device__ __forceinline__ float tile_output(const float* __restrict__ W, const float (&act)[L][4 * NB]) {
-  using PK = Packed<L, NB, kBwd>;
+  using PK = Packed<L, NB>;
   const int lane = threadIdx.x & 63;
   float s = 0.0f;
 #pragma unroll
@@ -247,7 +252,7 @@ template <int Lm, int NBm, int Lp, int NBp>
 __global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_item,
                                                        int nblk_p) {
-  extern __shared__ float lds[];
+  extern __shared__ __align__(16) float lds[];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool plus = (int)blockIdx.x < nblk_p;
   const int64_t ntiles = plus ? (np_ + 15) / 16 : (nm_ + 15) / 16;
@@ -255,9 +260,9 @@ __global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const 
   const int bid = plus ? (int)blockIdx.x : (int)blockIdx.x - nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
   if (plus)
-    pack_weights<Lp, NBp, false>(params, FlatNet{Hp, Lp, 0}, lds);
+    pack_weights<Lp, NBp>(params, FlatNet{Hp, Lp, 0}, lds);
   else
-    pack_weights<Lm, NBm, false>(params, FlatNet{Hm, Lm, PPp}, lds);
+    pack_weights<Lm, NBm>(params, FlatNet{Hm, Lm, PPp}, lds);
   __syncthreads();
   const int64_t eb = plus ? e0 : e0 + np_;
   const int64_t ee = plus ? e0 + np_ : e0 + np_ + nm_;
@@ -272,12 +277,12 @@ __global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const 
     float u;
     if (plus) {
       float act[Lp][4 * NBp];
-      tile_forward<Lp, NBp, false>(lds, xin, act);
-      u = tile_output<Lp, NBp, false>(lds, act);
+      tile_forward<Lp, NBp>(lds, xin, act);
+      u = tile_output<Lp, NBp>(lds, act);
     } else {
       float act[Lm][4 * NBm];
-      tile_forward<Lm, NBm, false>(lds, xin, act);
-      u = tile_output<Lm, NBm, false>(lds, act);
+      tile_forward<Lm, NBm>(lds, xin, act);
+      u = tile_output<Lm, NBm>(lds, act);
     }
     if (item < ee && lane < 16) u_item[item - e0] = u;
   }
@@ -364,13 +369,16 @@ __global__ __launch_bounds__(256) void mfma_seed_kernel(MfmaTileArgs T, int64_t 
 // back-propagates its deltas (registers); per layer, each wavefront stages its
 // tile's operands (activation and delta, feature-major rows of 16 positions)
 // in LDS, and after a barrier each wavefront contracts only the weight-gradient
-// blocks it owns over all kBW tiles (K = 16 kBW positions).  Owning 2 of the
-// 16 blocks of a 64-wide layer instead of all 16 keeps a lane's registers below
-// 256, so two workgroups' worth of wavefronts (8 per CU) run per SIMD pair and
-// the matrix cores see twice the wavefronts of the one-tile-per-wavefront form.
+// blocks it owns over all kBW tiles (K = 16 kBW positions).  Owning 4 of the
+// 16 blocks of a 64-wide layer keeps a lane's registers within 256, and the
+// packed weights (one copy, read transposed for the deltas) plus the stage fit
+// twice into a CU's LDS: two workgroups per CU, whose barriers are independent,
+// so one workgroup's wavefronts issue while the other's wait.
 // Bias, output-weight and output-bias sums stay per lane and are reduced over
-// the 16 positions with row-local DPP sums at the end.
-constexpr int kBW = 8;              // wavefronts per backward workgroup
+// the 16 positions with row-local DPP sums at the end.  The packed weights carry
+// the factor kTanhExp2Scale, so each layer's delta comes out that factor larger
+// than the one after it; the flush divides it out.
+constexpr int kBW = 4;              // wavefronts per backward workgroup
 constexpr int kBT = 64 * kBW;       // its threads
 
 template <int L, int NB>
@@ -444,6 +452,12 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
     for (int l = 1; l < L; ++l) G.db[l][q] = row16_sum(G.db[l][q]);
   }
   G.dbo = row16_sum(G.dbo);
+  // the delta of hidden layer l carries kTanhExp2Scale^(L-1-l) (weights packed times the factor): inv[l]
+  // undoes it for layer l's weight and bias gradients (layer 0: its weights and bias, rows 0..3 of dW0)
+  float inv[L];
+  inv[L - 1] = 1.0f;
+#pragma unroll
+  for (int l = L - 2; l >= 0; --l) inv[l] = inv[l + 1] * (1.0f / kTanhExp2Scale);
   // wavefronts add in order (fixed order: deterministic); weight blocks have one owner each
   for (int w = 0; w < kBW; ++w) {
     if (wave == w) {
@@ -457,7 +471,7 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int in = 16 * ib + 4 * (lane / 16) + r, o = 16 * ob + (lane & 15);
-              if (in < Hh && o < Hh) red[fn.w(l) - base + in * Hh + o] = G.dW[l - 1][j][r];
+              if (in < Hh && o < Hh) red[fn.w(l) - base + in * Hh + o] = G.dW[l - 1][j][r] * inv[l];
             }
           }
         }
@@ -466,8 +480,8 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = 4 * (lane / 16) + r, o = 16 * wave + (lane & 15);
-          if (row < 3 && o < Hh) red[fn.w(0) - base + row * Hh + o] = G.dW0[r];
-          if (row == 3 && o < Hh) red[fn.b(0) - base + o] = G.dW0[r];
+          if (row < 3 && o < Hh) red[fn.w(0) - base + row * Hh + o] = G.dW0[r] * inv[0];
+          if (row == 3 && o < Hh) red[fn.b(0) - base + o] = G.dW0[r] * inv[0];
         }
       }
       if ((lane & 15) == 15) {
@@ -479,7 +493,7 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
             if (h < Hh) {
               red[fn.wout() - base + h] += G.dwo[fb * 4 + r];
 #pragma unroll
-              for (int l = 1; l < L; ++l) red[fn.b(l) - base + h] += G.db[l][fb * 4 + r];
+              for (int l = 1; l < L; ++l) red[fn.b(l) - base + h] += G.db[l][fb * 4 + r] * inv[l];
             }
           }
         }
@@ -532,7 +546,11 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
         const int q = wave * CA::kW + j;
         if (q < NB * NB) G.dW[l - 1][j] = coop_block(stage, stride_w, 16 * (q / NB), 16 * (q % NB), false, G.dW[l - 1][j]);
       }
-      // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2), from this tile's registers
+      // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2), from this tile's registers: the A operand of step r over
+      // output block fb is W_l[16 ib + m][16 fb + 4 kq + r] (m = lane%16, kq = lane/16), the float m%4 of
+      // unit 16 (m/4) + 4 kq + r of block (fb, ib) -- 4 floats 4 apart from this lane's offset
+      const int m = lane & 15, kq = lane >> 4;
+      const int tofs = 4 * (17 * (m >> 2) + 4 * kq) + (m & 3);
       f4 c[NB];
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib) c[ib] = f4{0.f, 0.f, 0.f, 0.f};
@@ -540,9 +558,9 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
       for (int fb = 0; fb < NB; ++fb) {
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib) {
-          const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::bwd(l) + ((ib * NB + fb) * 64 + lane) * 4]);
+          const float* wt = W + PK::fwd(l) + (fb * NB + ib) * PK::kBlk + tofs;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) c[ib] = mfma4(w4[r], d[fb * 4 + r], c[ib]);
+          for (int r = 0; r < 4; ++r) c[ib] = mfma4(wt[4 * r], d[fb * 4 + r], c[ib]);
         }
       }
 #pragma unroll
@@ -561,7 +579,7 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
 }
 
 template <int Lm, int NBm, int Lp, int NBp>
-__global__ __launch_bounds__(kBT) void mfma_bwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
+__global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_,
                                                        const float* __restrict__ seed_item, float* __restrict__ gpart,
                                                        int nblk_p, int stride) {
@@ -736,7 +754,7 @@ __global__ void count_inc_kernel(int32_t* count) {
 // of a position).
 template <int L, int NB>
 __device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, float xin, float& u, float (&du)[3]) {
-  using PK = Packed<L, NB, false>;
+  using PK = Packed<L, NB>;
   const int lane = threadIdx.x & 63;
   float a[4 * NB], t[3][4 * NB];
 #pragma unroll
@@ -766,7 +784,7 @@ __device__ __forceinline__ void tile_value_grad(const float* __restrict__ W, flo
     for (int fb = 0; fb < NB; ++fb) {
 #pragma unroll
       for (int ob = 0; ob < NB; ++ob) {
-        const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::fwd(l) + ((ob * NB + fb) * 64 + lane) * 4]);
+        const f4 w4 = *reinterpret_cast<const f4*>(&W[PK::fwd(l) + (ob * NB + fb) * PK::kBlk + 4 * PK::unit(lane)]);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           c[ob] = mfma4(w4[r], a[fb * 4 + r], c[ob]);
@@ -823,12 +841,12 @@ __global__ __launch_bounds__(kMT) void mfma_eval_kernel(const float* __restrict_
                                                         const float* __restrict__ nrm, int64_t m,
                                                         float* __restrict__ u, float* __restrict__ grad,
                                                         float* __restrict__ gradn) {
-  extern __shared__ float lds[];
-  constexpr int kOffM = Packed<Lp, NBp, false>::size();
+  extern __shared__ __align__(16) float lds[];
+  constexpr int kOffM = (Packed<Lp, NBp>::size() + 3) & ~3;  // the - net starts 16-byte aligned
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
-  pack_weights<Lp, NBp, false>(params, FlatNet{Hp, Lp, 0}, lds);
-  pack_weights<Lm, NBm, false>(params, FlatNet{Hm, Lm, PPp}, lds + kOffM);
+  pack_weights<Lp, NBp>(params, FlatNet{Hp, Lp, 0}, lds);
+  pack_weights<Lm, NBm>(params, FlatNet{Hm, Lm, PPp}, lds + kOffM);
   __syncthreads();
   const int64_t ntiles = (m + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
@@ -942,7 +960,7 @@ int mfma_eval_launch(const dips_net_t* net, const float* params, const float* pt
   const int nblk = (int)std::min<int64_t>((ntiles + 3) / 4, 1024);
 #define X(A, B, C, D)                                                                                            \
   if (a == A && b == B && c == C && d == D) {                                                                   \
-    const size_t lds = sizeof(float) * (Packed<C, D, false>::size() + Packed<A, B, false>::size());             \
+    const size_t lds = sizeof(float) * (((Packed<C, D>::size() + 3) & ~3) + Packed<A, B>::size());             \
     hipFuncSetAttribute((const void*)mfma_eval_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
                         (int)lds);                                                                             \
     hipLaunchKernelGGL((mfma_eval_kernel<A, B, C, D>), dim3(nblk), dim3(kMT), lds, st, params, Hm, Hp, pts, phase, \
@@ -1046,8 +1064,7 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     launched = true;                                                                                               \
     constexpr int PK = ((Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size()                       \
                                                                      : Packed<A, B>::size()) + 3) & ~3;           \
-    constexpr int PKF = Packed<C, D, false>::size() > Packed<A, B, false>::size() ? Packed<C, D, false>::size()   \
-                                                                                  : Packed<A, B, false>::size();  \
+    constexpr int PKF = Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size() : Packed<A, B>::size(); \
     constexpr int NBX = D > B ? D : B;                                                                             \
     const size_t lds_f = sizeof(float) * PKF;                                                                      \
     const size_t lds_b = sizeof(float) * (PK + kBW * 2 * 16 * NBX * kTP);                                         \
@@ -1127,7 +1144,7 @@ int mfma_items_launch(const dips_grid_t* g, const dips_records_t* rec, const dip
 // holds its own copy until the process ends.  Every value a step reads from it
 // is written earlier in the same step, so nothing is cleared between calls;
 // calls that overlap on one device share the array and race.
-constexpr int64_t kNoAllocFloats = int64_t(6) << 20;  // 24 MiB
+constexpr int64_t kNoAllocFloats = int64_t(8) << 20;  // 32 MiB
 constexpr int64_t kNoAllocDoubles = int64_t(1) << 13;  // 64 KiB
 constexpr int kNoAllocMaxDevices = 64;
 static __device__ __align__(256) float scratch[kNoAllocFloats + 2 * kNoAllocDoubles];
