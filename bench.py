@@ -337,8 +337,9 @@ def main():
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": bound, "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": {"mfma": "mfma step: mfma_fwd_kernel + mfma_rowsum_kernel + mfma_bwd_kernel + "
-                                        "mfma_reduce_kernel + mfma_apply_kernel (one optimizer step)",
+                     "kernel": {"mfma": "mfma step: mfma_fwd_kernel + mfma_rowsum_kernel + mfma_seed_kernel + "
+                                        "mfma_bwd_kernel + mfma_reduce_kernel + mfma_apply_kernel (one optimizer "
+                                        "step; FP32 MFMA with the tanh on the VALU, which does not overlap it)",
                                 "lane": "loss_grad_kernel<grad,fused> (one launch per optimizer step)",
                                 "tile": "tile_step_kernel<grad> (gradient launch of a step; the optimizer "
                                         "runs after it in a one-workgroup tile_apply_kernel); FP32 VALU "

@@ -288,14 +288,28 @@ __global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const 
   }
 }
 
+// ---- XCD-contiguous chunks -------------------------------------------------------
+// Workgroups go to the 8 XCDs round-robin by index; a grid of 8 * ceil(n / 8)
+// workgroups maps workgroup b to chunk (b % 8) ceil(n / 8) + b / 8, so that each
+// XCD (and its L2) takes one contiguous eighth of the n chunks.  Chunks >= n exit.
+constexpr int kXcds = 8;
+inline unsigned xcd_grid(int64_t n) { return (unsigned)(kXcds * ((n + kXcds - 1) / kXcds)); }
+__device__ __forceinline__ int64_t xcd_chunk(int64_t n) {
+  const int64_t per = (n + kXcds - 1) / kXcds;
+  return (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 // ---- kernel 2: r per row, loss partials ---------------------------------------
 // u of CSR entry e: u[e] (per-entry values, e_item null), else u[e_item[e]] (per-item values).
+// Launched on xcd_grid(rowsum_blocks(nrows)) workgroups; loss_part[chunk] per chunk of kMT rows.
 __global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restrict__ row_ptr,
                                                           const float* __restrict__ row_c, int64_t row_begin,
                                                           int64_t nrows, const float* __restrict__ ent_w, int64_t nnz,
                                                           const float* __restrict__ u, const int32_t* __restrict__ e_item,
                                                           float* __restrict__ r_out, double* __restrict__ loss_part) {
-  const int64_t i = (int64_t)blockIdx.x * kMT + threadIdx.x;
+  const int64_t chunk = xcd_chunk((nrows + kMT - 1) / kMT);
+  if (chunk >= (nrows + kMT - 1) / kMT) return;  // the whole workgroup
+  const int64_t i = chunk * kMT + threadIdx.x;
   double l = 0.0;
   if (i < nrows) {
     const int64_t row = row_begin + i;
@@ -326,16 +340,17 @@ __global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restr
   __shared__ double red[kMT / 64];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = l;
   __syncthreads();
-  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) loss_part[chunk] = red[0] + red[1] + red[2] + red[3];
 }
 
 // ---- kernel 2b: seed per item ----------------------------------------------------
 // seed of an item = sum over its member entries (in order) of r(row) w: the
 // derivative of the batch's half sum of squares with respect to the item's u.
 // Written to seed[item - i0] (the buffer that held u, consumed by the row sums).
+// Launched on xcd_grid(ceil(n / 256)) workgroups of 256 threads.
 __global__ __launch_bounds__(256) void mfma_seed_kernel(MfmaTileArgs T, int64_t i0, int64_t n, int64_t row_begin,
                                                         const float* __restrict__ r_row, float* __restrict__ seed) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_chunk((n + 255) / 256) * 256 + threadIdx.x;
   if (t >= n) return;
   int64_t mb, me;
   item_members(T, i0 + t, mb, me);
@@ -652,27 +667,47 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
 
 // ---- kernel 4/5: reduction of gradient partials and optimizer -------------------
 // g[p] = inv_b * sum over the workgroups of its phase (fixed order); sumsq partial per workgroup
+// 64 parameters per workgroup, each summed over its phase's partial rows by 4
+// wavefronts (a quarter of the rows each, in order), then the quarters in order;
+// sq_part[workgroup] = sum of g^2 over its 64 parameters.  With count_io, the
+// first thread copies the step counter to count_snap (read by the optimizer)
+// and advances it, in place of a separate launch.
+constexpr int kRedParams = 64;
 __global__ __launch_bounds__(256) void mfma_reduce_kernel(const float* __restrict__ gpart, int stride, int nblk_p,
                                                           int nblk_m, int PPp, int PPm, float inv_b,
-                                                          float* __restrict__ g, double* __restrict__ sq_part) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  double v = 0.0;
+                                                          float* __restrict__ g, double* __restrict__ sq_part,
+                                                          int32_t* __restrict__ count_io,
+                                                          int32_t* __restrict__ count_snap) {
+  const int j = threadIdx.x & 63, k = threadIdx.x >> 6;
+  const int p = blockIdx.x * kRedParams + j;
+  double acc = 0.0;
   if (p < PPp + PPm) {
     const bool plus = p < PPp;
-    const int b0 = plus ? 0 : nblk_p, b1 = plus ? nblk_p : nblk_p + nblk_m;
+    const int b0 = plus ? 0 : nblk_p, nb = plus ? nblk_p : nblk_m;
     const int q = plus ? p : p - PPp;
-    double acc = 0.0;
-#pragma unroll 16
-    for (int b = b0; b < b1; ++b) acc += (double)gpart[(int64_t)b * stride + q];  // 16 loads in flight
-    const float gg = (float)(acc * (double)inv_b);
-    g[p] = gg;
-    v = (double)gg * (double)gg;
+    const int per = (nb + 3) / 4;
+    const int lo = b0 + k * per, hi = b0 + min(nb, (k + 1) * per);
+#pragma unroll 8
+    for (int b = lo; b < hi; ++b) acc += (double)gpart[(int64_t)b * stride + q];
   }
-  v = dips_wave_sum_d(v);
-  __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+  __shared__ double part[4][kRedParams];
+  part[k][j] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) sq_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (k == 0) {
+    double v = 0.0;
+    if (p < PPp + PPm) {
+      const float gg = (float)((((part[0][j] + part[1][j]) + part[2][j]) + part[3][j]) * (double)inv_b);
+      g[p] = gg;
+      v = (double)gg * (double)gg;
+    }
+    v = dips_wave_sum_d(v);
+    if (j == 0) sq_part[blockIdx.x] = v;
+  }
+  if (count_io && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int32_t c = *count_io;
+    *count_snap = c;
+    *count_io = c + 1;
+  }
 }
 
 // Parameters [0, split) take their gradient from g, [split, P) from
@@ -898,8 +933,8 @@ __global__ __launch_bounds__(256) void mfma_loss_out_kernel(const double* __rest
 void rowsum_launch(const int32_t* row_ptr, const float* row_c, int64_t row_begin, int64_t nrows, const float* ent_w,
                    int64_t nnz, const float* u, float* r, double* loss_part, hipStream_t st) {
   const int nrb = (int)((nrows + kMT - 1) / kMT);
-  hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, row_ptr, row_c, row_begin, nrows, ent_w, nnz, u,
-                     (const int32_t*)nullptr, r, loss_part);
+  hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(xcd_grid(nrb)), dim3(kMT), 0, st, row_ptr, row_c, row_begin, nrows, ent_w,
+                     nnz, u, (const int32_t*)nullptr, r, loss_part);
 }
 
 int rowsum_blocks(int64_t nrows) { return (int)((nrows + kMT - 1) / kMT); }
@@ -1015,17 +1050,18 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     return DIPS_EINVAL;
   }
   // workspace layout (floats): u per item, then seed per item (at most one item per entry) | r rows |
-  // gradient partials | g
+  // gradient partials | g | the step counter's value for the optimizer (one int32)
   float* u = work;  // indexed by item - i0
   float* r = u + nent;
   float* gpart = r + nrows;
   float* gvec = gpart + (int64_t)kMfmaBwdBlocks * stride;
-  if ((int64_t)(gvec + P - work) > work_floats) {
+  int32_t* count_snap = reinterpret_cast<int32_t*>(gvec + P);
+  if ((int64_t)(gvec + P + 1 - work) > work_floats) {
     set_error("dips_mfma_train_step: workspace too small");
     return DIPS_EINVAL;
   }
   const int nrb = (int)((nrows + kMT - 1) / kMT);
-  const int nred = (P + 255) / 256;
+  const int nred = (P + kRedParams - 1) / kRedParams;
   double* loss_part = dwork;
   double* sq_part = dwork + nrb;
   if (nrb + nred + 1 > dwork_doubles) {
@@ -1088,10 +1124,10 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     if (nfp + nfm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kFT), lds_f, st, T, params, Hm, Hp,   \
                          i0, ip, im, u, nfp);                                                                      \
-    hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(nrb), dim3(kMT), 0, st, rec->row_ptr, rec->row_c, bt->row_begin,   \
-                       nrows, rec->ent_w, rec->nnz, u - i0, bt->e_item, r, loss_part);                             \
+    hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(xcd_grid(nrb)), dim3(kMT), 0, st, rec->row_ptr, rec->row_c,       \
+                       bt->row_begin, nrows, rec->ent_w, rec->nnz, u - i0, bt->e_item, r, loss_part);              \
     if (ip + im > 0)                                                                                               \
-      hipLaunchKernelGGL(mfma_seed_kernel, dim3((unsigned)((ip + im + 255) / 256)), dim3(256), 0, st, T, i0,       \
+      hipLaunchKernelGGL(mfma_seed_kernel, dim3(xcd_grid((ip + im + 255) / 256)), dim3(256), 0, st, T, i0,          \
                          ip + im, bt->row_begin, r, u);                                                            \
     if (nbp + nbm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kBT), bwd_lds, st, T,                 \
@@ -1107,15 +1143,16 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   if (status) return status;
   // the reduction reads the + net partials from workgroups [0, nbp) and the - net from [nbp, nbp+nbm);
   // parameters of the + net come first in the flat vector
+  // (with the optimizer) the reduction also hands the step counter over to count_snap and advances it
   hipLaunchKernelGGL(mfma_reduce_kernel, dim3(nred), dim3(256), 0, st, gpart, stride, nbp, nbm, PPp, PPm, inv_b,
-                     grad_out ? grad_out : gvec, sq_part);
+                     grad_out ? grad_out : gvec, sq_part, opt ? count : nullptr, count_snap);
   if (!opt) {
     hipLaunchKernelGGL(mfma_loss_out_kernel, dim3(1), dim3(256), 0, st, loss_part, nrb, inv_b, loss_out);
     return hip_status(hipGetLastError(), "mfma reduce launch");
   }
-  hipLaunchKernelGGL(mfma_apply_kernel, dim3(nred), dim3(256), 0, st, gvec, P, nullptr, P, 1.0f, sq_part, nred,
-                     loss_part, nrb, inv_b, *opt, params, m, v, count, loss_hist, loss_cap);
-  hipLaunchKernelGGL(count_inc_kernel, dim3(1), dim3(64), 0, st, count);
+  hipLaunchKernelGGL(mfma_apply_kernel, dim3((unsigned)std::min(nred, kApplyBlocks)), dim3(256), 0, st, gvec, P,
+                     nullptr, P, 1.0f, sq_part, nred, loss_part, nrb, inv_b, *opt, params, m, v, count_snap, loss_hist,
+                     loss_cap);
   return hip_status(hipGetLastError(), "mfma reduce/apply launch");
 }
 
@@ -1179,7 +1216,7 @@ int mfma_step_noalloc(const dips_grid_t* g, const dips_records_t* rec, const dip
   const int64_t nrows = bt->nrows, nent = bt->ent_end - bt->ent_begin;
   const int64_t floats = mfma_workspace_floats(net, nrows, nent);
   const int64_t P = net_params(net->Lm, net->Hm) + net_params(net->Lp, net->Hp);
-  const int64_t doubles = (nrows + kMT - 1) / kMT + (P + 255) / 256 + 1;
+  const int64_t doubles = (nrows + kMT - 1) / kMT + (P + kRedParams - 1) / kRedParams + 1;
   if (floats > kNoAllocFloats || doubles > kNoAllocDoubles) {
     set_error("dips_mfma_train_step_noalloc: the batch needs " + std::to_string(floats) + " floats and " +
               std::to_string(doubles) + " doubles of scratch, beyond the fixed " + std::to_string(kNoAllocFloats) +

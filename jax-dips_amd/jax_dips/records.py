@@ -218,7 +218,9 @@ class DeviceRecords:
             X = grid.points()[self.row_point.to(torch.int64)[ent_row]] + o * h
             del o, c
             perm = torch.arange(n, device=dev)
-            for col in (0, 1, 2):  # least significant key first: x, y, z, then the segment
+            # least significant key first: z, y, x, then the segment -- inside a segment the order is x-major,
+            # z fastest, as the rows (grid points) are, so the row sums' gathers of u per item stay local
+            for col in (2, 1, 0):
                 _, p2 = torch.sort(X[perm, col], stable=True)
                 perm = perm[p2]
             _, p2 = torch.sort(key[perm], stable=True)
@@ -454,7 +456,7 @@ class Engine:
         need = int(N.lib().dips_mfma_workspace(ctypes.byref(self.net), int(rec.batch_size), mb["max_nnz"]))
         if getattr(self, "_mwork", None) is None or self._mwork.numel() < need:
             self._mwork = torch.empty(need, dtype=torch.float32, device=self.device)
-            self._mdwork = torch.empty(int(rec.batch_size) // 256 + self.P // 256 + 64, dtype=torch.float64,
+            self._mdwork = torch.empty(int(rec.batch_size) // 256 + self.P // 64 + 64, dtype=torch.float64,
                                        device=self.device)
         return mb
 

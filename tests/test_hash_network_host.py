@@ -171,6 +171,10 @@ def test_entries_ordered_by_position():
         if b < nnz:
             assert s[b] != s[a] or np.any(pos[b] != pos[a])
     assert mb["groups"] == len(tg) - 1 < nnz  # some positions coincide on this grid
+    # inside a segment the positions run x-major, z fastest (the order of the grid points)
+    for a in range(nnz - 1):
+        if s[a + 1] == s[a]:
+            assert tuple(pos[a][[0, 1, 2]]) <= tuple(pos[a + 1][[0, 1, 2]])
     for bt in mb["batches"]:
         assert tg[bt.g_begin] == bt.t_begin
         assert tg[bt.g_begin + bt.g_plus] == bt.t_begin + bt.n_plus
