@@ -396,7 +396,8 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
  * bits); tables above 2^25 rows use atomic adds.
  * ------------------------------------------------------------------------- */
 typedef struct {
-  int32_t levels;      /* L, 2..16; with encoder 1 the number of frequencies, 10 */
+  int32_t levels;      /* L, 2..16; with encoder 1 the number of frequencies, 10; with encoder 2 the highest
+                          order, 1..8 */
   int32_t features;    /* F, must be 2 (encoder 0) */
   int64_t table_size;  /* T, rows per hashed level (encoder 0) */
   int32_t n_min;       /* coarsest resolution (encoder 0) */
@@ -404,7 +405,11 @@ typedef struct {
   float bound;         /* positions are mapped from [-bound, bound] to [0, 1] */
   int32_t encoder;     /* 0 = hashgrid; 1 = frequency encoding (FrequencyEncoder, encoders_flax.py:269-307):
                           per axis d and frequency l < levels, sin and cos of 2^l pi pos_d, feature
-                          (d levels + l) 2 + {0: sin, 1: cos}; no table */
+                          (d levels + l) 2 + {0: sin, 1: cos}; no table; 2 = spherical harmonics
+                          (SphericalHarmonicsEncoder, encoders_flax.py:311-538) of the raw position, orders
+                          l < levels, feature l^2 + l + m = (-1)^|m| c_l^|m| d^|m|P_l/dz^|m|(z) times
+                          Re (x+iy)^|m| (m > 0), Im (x+iy)^|m| (m < 0) or 1 (m = 0), c the real-harmonic
+                          normalisation (times sqrt 2 for m != 0); levels^2 features; no table */
 } dips_hashgrid_t;
 
 /* Table rows of the encoder (sum over levels of min(res^3, table_size)); < 0 if invalid. */

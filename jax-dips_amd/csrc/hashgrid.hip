@@ -45,15 +45,18 @@ constexpr int kHStage = 2 * 16 * kHStride;
 // the backward's staging area holds d enc (kHE rows of a wavefront)
 static_assert(kHE * DIPS_WAVE <= kHStage, "staging area too small");
 
+constexpr int kSHMax = 8;  // highest order of the spherical-harmonics encoder (SphericalHarmonicsEncoder, L <= 8)
+
 struct HashLevels {
-  int L;    // levels (hashgrid) or frequencies (frequency encoding)
-  int enc;  // 0 = hashgrid, 1 = frequency encoding
+  int L;    // levels (hashgrid), frequencies (frequency encoding) or highest order (spherical harmonics)
+  int enc;  // 0 = hashgrid, 1 = frequency encoding, 2 = spherical harmonics
   uint32_t T;
   int64_t rows;
   float bound, two_bound;
   float scale[kHL];
   uint32_t res[kHL];     // resolution of a dense level, 0 for a hashed level
   uint32_t offset[kHL];  // first table row of the level (tables of < 2^31 parameters)
+  float shc[kSHMax * kSHMax];  // spherical harmonics: (-1)^|m| c_l^|m| of feature l^2 + l + m
 };
 
 __host__ __device__ inline int hash_mlp_size(int E) { return E * kHW + kHW + kHW + 1; }
@@ -63,6 +66,30 @@ constexpr int kFE = 6 * kFL;     // its encoding width
 constexpr int kFEPad = 64;       // ... padded to whole 16-row MFMA blocks
 
 static int hash_levels(const dips_hashgrid_t* hg, HashLevels& H) {
+  if (hg && hg->encoder == 2) {
+    if (hg->levels < 1 || hg->levels > kSHMax) {
+      set_error("spherical harmonics encoding: invalid description (levels = highest order, 1..8)");
+      return DIPS_EINVAL;
+    }
+    H = HashLevels{};
+    H.L = hg->levels;
+    H.enc = 2;
+    H.T = 1;
+    H.rows = 0;
+    H.bound = hg->bound;
+    H.two_bound = 2.0f * hg->bound;
+    // c_l^k = sqrt((2l+1)/(4 pi) (l-k)!/(l+k)!), times sqrt 2 for k > 0, with the sign (-1)^k
+    for (int l = 0; l < H.L; ++l)
+      for (int m = -l; m <= l; ++m) {
+        const int k = m < 0 ? -m : m;
+        double ratio = 1.0;  // (l-k)! / (l+k)!
+        for (int j = l - k + 1; j <= l + k; ++j) ratio /= (double)j;
+        double c = sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * ratio) * (k > 0 ? sqrt(2.0) : 1.0);
+        if (k & 1) c = -c;
+        H.shc[l * l + l + m] = (float)c;
+      }
+    return DIPS_OK;
+  }
   if (hg && hg->encoder == 1) {
     if (hg->levels != kFL || !(hg->bound > 0.0f)) {
       set_error("frequency encoding: invalid description (levels = 10 frequencies, bound > 0)");
@@ -883,6 +910,61 @@ __device__ __forceinline__ void freq_encode(const HashLevels& H, const float (&p
     }
 }
 
+// ---- spherical-harmonics encoding -------------------------------------------------
+// SphericalHarmonicsEncoder (encoders_flax.py:311-538) of highest order L <= 8 on
+// the raw position (model.py:84; the bound is not used): feature l^2 + l + m is
+// shc[l^2 + l + m] Q_l^k(z) T_m(x, y), k = |m|, with Q_l^k = d^k P_l / dz^k from
+// the three-term recurrence (l - k) Q_l^k = (2l - 1) z Q_{l-1}^k - (l + k - 1)
+// Q_{l-2}^k (Q_k^k = (2k - 1)!!, Q_{k+1}^k = (2k + 1) z Q_k^k) and T_m = Re (x +
+// iy)^k (m > 0), Im (x + iy)^k (m < 0), 1 (m = 0).  Features past L^2 are zero.
+__device__ __forceinline__ void sh_encode(const HashLevels& H, float x, float y, float z, float (&enc)[kFEPad]) {
+#pragma unroll
+  for (int f = 0; f < kFEPad; ++f) enc[f] = 0.0f;
+  float ck = 1.0f, sk = 0.0f, dfact = 1.0f;  // Re / Im (x + iy)^k, (2k - 1)!!
+#pragma unroll
+  for (int k = 0; k < kSHMax; ++k) {
+    if (k > 0) {
+      const float c1 = x * ck - y * sk;
+      sk = x * sk + y * ck;
+      ck = c1;
+      dfact *= (float)(2 * k - 1);
+    }
+    float q2 = 0.0f, q1 = dfact;  // Q_{l-1}^k, Q_l^k
+#pragma unroll
+    for (int l = k; l < kSHMax; ++l) {
+      if (l > k) {
+        const float q = (l == k + 1) ? (float)(2 * k + 1) * z * q1
+                                     : ((float)(2 * l - 1) * z * q1 - (float)(l + k - 1) * q2) / (float)(l - k);
+        q2 = q1;
+        q1 = q;
+      }
+      if (l < H.L) {
+        const int f = l * l + l;
+        if (k == 0) {
+          enc[f] = H.shc[f] * q1;
+        } else {
+          enc[f + k] = H.shc[f + k] * q1 * ck;
+          enc[f - k] = H.shc[f - k] * q1 * sk;
+        }
+      }
+    }
+  }
+}
+
+// the dense encoders' features of a position (frequency: of pos = (x + bound) / 2 bound; spherical harmonics:
+// of x itself)
+__device__ __forceinline__ void dense_encode(const HashLevels& H, float x, float y, float z, float (&enc)[kFEPad]) {
+  if (H.enc == 2) {
+    sh_encode(H, x, y, z, enc);
+  } else {
+    float p[3];
+    hash_pos(H, x, y, z, p);
+    freq_encode(H, p, enc);
+  }
+}
+
+__device__ __forceinline__ int dense_width(const HashLevels& H) { return H.enc == 2 ? H.L * H.L : 6 * H.L; }
+
 // forward: u per entry (one lane per phase-sorted entry) and the encoding for the backward
 __global__ __launch_bounds__(256) void freq_fwd_kernel(HashArgs A, HashLevels H, const float* __restrict__ params,
                                                        float* __restrict__ u_out, float* __restrict__ encbuf) {
@@ -890,11 +972,9 @@ __global__ __launch_bounds__(256) void freq_fwd_kernel(HashArgs A, HashLevels H,
   float x, y, z;
   int row;
   const bool valid = hash_entry(A, t, x, y, z, row);
-  const int E = 6 * H.L, PM = hash_mlp_size(E);
-  float p[3];
-  hash_pos(H, x, y, z, p);
+  const int E = dense_width(H), PM = hash_mlp_size(E);
   float enc[kFEPad];
-  freq_encode(H, p, enc);
+  dense_encode(H, x, y, z, enc);
   const bool plus = t < A.n_plus;
   float act[kHW];
   const float* Praw = params;
@@ -925,7 +1005,7 @@ __global__ __launch_bounds__(kHashThreads) void freq_bwd_kernel(HashArgs A, Hash
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / DIPS_WAVE);
   float* const AT = hash_lds + wave * kHStage;
   float* const BT = AT + 16 * kHStride;
-  const int E = 6 * H.L, PM = hash_mlp_size(E);
+  const int E = dense_width(H), PM = hash_mlp_size(E);
   HashGrad<kFEPad / 16> gp, gm;
   gp.zero();
   gm.zero();
@@ -983,7 +1063,9 @@ __global__ __launch_bounds__(kHashThreads) void freq_bwd_kernel(HashArgs A, Hash
   }
 }
 
-// evaluation: u and d u / d x (d enc / d x through sin' = cos, cos' = -sin)
+// evaluation: u and d u / d x.  Frequency: d enc / d x through sin' = cos,
+// cos' = -sin.  Spherical harmonics: d/dz of Q_l^k is Q_l^{k+1}; d/dx, d/dy of
+// (x + iy)^k are k (x + iy)^{k-1} and i k (x + iy)^{k-1}.
 __global__ __launch_bounds__(256) void freq_eval_kernel(HashLevels H, const float* __restrict__ params,
                                                         const float* __restrict__ pts,
                                                         const uint8_t* __restrict__ phase, int64_t m,
@@ -991,39 +1073,85 @@ __global__ __launch_bounds__(256) void freq_eval_kernel(HashLevels H, const floa
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= m) return;
   const float x = pts[3 * t], y = pts[3 * t + 1], z = pts[3 * t + 2];
-  const int E = 6 * H.L, PM = hash_mlp_size(E);
-  float p[3];
-  hash_pos(H, x, y, z, p);
+  const int E = dense_width(H), PM = hash_mlp_size(E);
   float enc[kFEPad], act[kHW];
-  freq_encode(H, p, enc);
+  dense_encode(H, x, y, z, enc);
   const float* W = params + (phase[t] ? 0 : PM);
   u[t] = hash_mlp(W, E, enc, act);
   if (!grad) return;
   float dj[kHW];
 #pragma unroll
   for (int j = 0; j < kHW; ++j) dj[j] = W[E * kHW + kHW + j] * act_grad<kActCelu>(act[j]);
+  auto du_of = [&](int f) {  // d u / d enc_f
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kHW; ++j) s = fmaf(W[f * kHW + j], dj[j], s);
+    return s;
+  };
   float g[3] = {0.f, 0.f, 0.f};
+  if (H.enc == 2) {
+    // Q[l][k] = d^k P_l / dz^k (zero for k > l), T from the complex powers
+    float Q[kSHMax][kSHMax + 1];
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
+    for (int l = 0; l < kSHMax; ++l)
 #pragma unroll
-    for (int l = 0; l < kFL; ++l) {
-      const int ks = (d * kFL + l) * 2;
-      float du_s = 0.0f, du_c = 0.0f;
+      for (int k = 0; k <= kSHMax; ++k) Q[l][k] = 0.0f;
+    float dfact = 1.0f;
 #pragma unroll
-      for (int j = 0; j < kHW; ++j) {
-        du_s = fmaf(W[ks * kHW + j], dj[j], du_s);
-        du_c = fmaf(W[(ks + 1) * kHW + j], dj[j], du_c);
-      }
-      const float da = ldexpf(3.14159265358979323846f, l) / H.two_bound;  // d A / d x_d
-      g[d] = fmaf(du_s * enc[ks + 1] - du_c * enc[ks], da, g[d]);
+    for (int k = 0; k < kSHMax; ++k) {
+      if (k > 0) dfact *= (float)(2 * k - 1);
+      Q[k][k] = dfact;
+#pragma unroll
+      for (int l = k + 1; l < kSHMax; ++l)
+        Q[l][k] = (l == k + 1) ? (float)(2 * k + 1) * z * Q[k][k]
+                               : ((float)(2 * l - 1) * z * Q[l - 1][k] - (float)(l + k - 1) * Q[l - 2][k]) / (float)(l - k);
     }
+    float C[kSHMax], S[kSHMax];
+    C[0] = 1.0f;
+    S[0] = 0.0f;
+#pragma unroll
+    for (int k = 1; k < kSHMax; ++k) {
+      C[k] = x * C[k - 1] - y * S[k - 1];
+      S[k] = x * S[k - 1] + y * C[k - 1];
+    }
+#pragma unroll
+    for (int l = 0; l < kSHMax; ++l) {
+      if (l >= H.L) break;
+#pragma unroll
+      for (int k = 0; k <= l; ++k) {
+        const int f = l * l + l;
+        const float q = Q[l][k], dq = Q[l][k + 1];
+        if (k == 0) {
+          const float c = du_of(f) * H.shc[f];
+          g[2] = fmaf(c, dq, g[2]);
+        } else {
+          const float kf = (float)k;
+          const float cp = du_of(f + k) * H.shc[f + k], cm = du_of(f - k) * H.shc[f - k];
+          // m = +k: q C_k;  m = -k: q S_k
+          g[0] = fmaf(q * kf, cp * C[k - 1] + cm * S[k - 1], g[0]);
+          g[1] = fmaf(q * kf, cm * C[k - 1] - cp * S[k - 1], g[1]);
+          g[2] = fmaf(dq, cp * C[k] + cm * S[k], g[2]);
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int l = 0; l < kFL; ++l) {
+        const int ks = (d * kFL + l) * 2;
+        const float du_s = du_of(ks), du_c = du_of(ks + 1);
+        const float da = ldexpf(3.14159265358979323846f, l) / H.two_bound;  // d A / d x_d
+        g[d] = fmaf(du_s * enc[ks + 1] - du_c * enc[ks], da, g[d]);
+      }
+  }
   grad[3 * t] = g[0];
   grad[3 * t + 1] = g[1];
   grad[3 * t + 2] = g[2];
 }
 
 // ---- host side ------------------------------------------------------------------
-static int hash_width(const HashLevels& H) { return H.enc == 1 ? 6 * H.L : 2 * H.L; }
+static int hash_width(const HashLevels& H) { return H.enc == 1 ? 6 * H.L : H.enc == 2 ? H.L * H.L : 2 * H.L; }
 
 int64_t hash_table_rows(const dips_hashgrid_t* hg) {
   HashLevels H;
@@ -1134,8 +1262,8 @@ int hash_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
   const int nsq = buckets ? nb + 1 : (int)std::min<int64_t>((P + 255) / 256, kRedBlocks);
   HashArgs A{point_set(g), rec->hx,  rec->hy,    rec->hz,      rec->row_point, bt->t_row, bt->t_code,
              bt->t_w,      bt->t_csr, bt->t_begin, bt->t_count, bt->n_plus};
-  if (H.enc == 1) {
-    // frequency encoding: no table, so no records and no table gradient
+  if (H.enc != 0) {
+    // frequency and spherical-harmonics encodings: no table, so no records and no table gradient
     if (bt->t_count > 0)
       hipLaunchKernelGGL(freq_fwd_kernel, dim3((unsigned)((bt->t_count + 255) / 256)), dim3(256), 0, st, A, H,
                          params, u - bt->ent_begin, encbuf);
@@ -1216,7 +1344,7 @@ int hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, 
     return DIPS_EINVAL;
   }
   if (m <= 0) return DIPS_OK;
-  if (H.enc == 1) {
+  if (H.enc != 0) {
     hipLaunchKernelGGL(freq_eval_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, H, params, pts, phase,
                        m, u, grad);
     return hip_status(hipGetLastError(), "freq_eval_kernel launch");

@@ -14,7 +14,7 @@ def evaluate_points(model, flat_params, X, phase_p, normals=None, want_grad=Fals
     """u (m,) [and grad (m,3), grad.n (m,)] of the two-phase network at X (m,3)."""
     import torch
 
-    if getattr(model, "encoder", "identity") in ("hashgrid", "frequency"):
+    if getattr(model, "encoder", "identity") in ("hashgrid", "frequency", "spherical_harmonics"):
         return _evaluate_hash(model, flat_params, X, phase_p, normals, want_grad, device)
     net = N.Net(*model.dims, 0, 0, 0.0, int(getattr(model, "act", 0)))
     X = torch.as_tensor(np.asarray(X, np.float32), device=device).contiguous()

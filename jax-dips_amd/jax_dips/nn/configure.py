@@ -1,5 +1,5 @@
 """Model factory (jax_dips/nn/configure.py:28-50): "mlp" (DoubleMLP, tanh) and
-"multiresolution_hash_network" with the identity, hashgrid or frequency
+"multiresolution_hash_network" with the identity, hashgrid, frequency or spherical-harmonics
 encoder (CELU coordinate MLPs, nn/hash_network.py).
 
 ``get_model(model_dict, model_type)`` returns ``(forward, framework)``;

@@ -29,16 +29,23 @@ Parameter tree (flax): {"params": {"sol_p": {"Dense_i": {"kernel", "bias"}},
   blocks/encoders_flax.py:269-307): sin and cos of 2^l pi pos per axis and
   l < 10, 60 features; the same device kernels (dips_hashgrid_t.encoder = 1),
   no table, one hidden layer of 16.
+* ``pos_enc="spherical_harmonics"`` (model.py:135-136,
+  SphericalHarmonicsEncoder(L=highest_sh_order), blocks/encoders_flax.py:
+  311-538): the real spherical harmonics of orders below highest_sh_order
+  (1..8, highest_sh_order^2 features) on the raw position; the same device
+  kernels (dips_hashgrid_t.encoder = 2), no table, one hidden layer of 16.
 
-Not implemented (NotImplementedError): the spherical harmonics encoder, skip
-connections, output activations other than linear, other hidden widths, and
-the total variation term (it does not enter the solution, model.py:82-94).
+Not implemented (NotImplementedError): skip connections, output activations
+other than linear, other hidden widths, and the total variation term (it does
+not enter the solution, model.py:82-94).
 """
 import numpy as np
 
 SUPPORTED_WIDTHS = ([16], [16, 16])
 HASHGRID_WIDTHS = ([16],)
 FREQUENCIES = 10
+# encoders that train through dips_hash_* (the identity encoder runs in the tile kernels)
+DENSE_ENCODERS = ("hashgrid", "frequency", "spherical_harmonics")
 TABLE_KEY = "latent codes stored on grid vertices"
 
 
@@ -48,15 +55,15 @@ class HashNetwork:
     def __init__(self, bound=1.0, pos_enc="identity", tv_scale=0.0, pos_levels=16, layer_widths=(64,),
                  sol_skip_in_layers=(), sol_act="linear", highest_sh_order=8, N_min=16, N_max=2048, F=2,
                  T=2 ** 19, **kwargs):
-        if pos_enc not in ("identity", "hashgrid", "frequency"):
-            raise NotImplementedError(f"pos_enc={pos_enc!r}: only the identity, hashgrid and frequency encoders have "
-                                      "device kernels")
+        if pos_enc not in ("identity",) + DENSE_ENCODERS:
+            raise NotImplementedError(f"pos_enc={pos_enc!r}: only the identity, hashgrid, frequency and "
+                                      "spherical_harmonics encoders have device kernels")
         if list(sol_skip_in_layers):
             raise NotImplementedError("skip connections have no device implementation")
         if sol_act != "linear":
             raise NotImplementedError(f"sol_act={sol_act!r}: only 'linear' has a device implementation")
         widths = [int(w) for w in layer_widths]
-        allowed = HASHGRID_WIDTHS if pos_enc in ("hashgrid", "frequency") else SUPPORTED_WIDTHS
+        allowed = HASHGRID_WIDTHS if pos_enc in DENSE_ENCODERS else SUPPORTED_WIDTHS
         if widths not in allowed:
             raise NotImplementedError(f"layer_widths={widths}: compiled for {allowed} with pos_enc={pos_enc!r}")
         self.bound = float(bound)
@@ -79,6 +86,13 @@ class HashNetwork:
             # FrequencyEncoder(L=10) (model.py:122-123): sin and cos of 2^l pi pos per axis
             self.hashgrid = (FREQUENCIES, 2, 1, 1, 1, self.bound, 1)
             self.in_dim = 6 * FREQUENCIES
+        elif pos_enc == "spherical_harmonics":
+            # SphericalHarmonicsEncoder(L=highest_sh_order) (model.py:135-136): orders below highest_sh_order
+            order = int(highest_sh_order)
+            if not 1 <= order <= 8:
+                raise NotImplementedError(f"highest_sh_order={highest_sh_order}: the reference implements 1..8")
+            self.hashgrid = (order, 2, 1, 1, 1, self.bound, 2)
+            self.in_dim = order * order
 
     def _table_rows(self):
         """Rows of the table: per level min(res^3, T) (encoders_flax.py:98-112)."""
@@ -94,7 +108,7 @@ class HashNetwork:
 
     @property
     def dims(self):
-        if self.encoder in ("hashgrid", "frequency"):
+        if self.encoder in DENSE_ENCODERS:
             raise AttributeError("the hashgrid network has no (Lm, Hm, Lp, Hp) shape; it trains through dips_hash_*")
         return (self.L, self.H, self.L, self.H)
 

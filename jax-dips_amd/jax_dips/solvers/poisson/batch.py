@@ -68,7 +68,7 @@ def solve_batch(problems, num_epochs=1000, batch_size=65536, model_dict=DEFAULT_
         raise ValueError("num_epochs must be at least 4 (four zoom levels of E // 4 epochs)")
     forward, _ = get_model(model_dict, model_type=model_dict.get("model_type", "mlp"))
     model = forward.model
-    if getattr(model, "encoder", "identity") in ("hashgrid", "frequency"):
+    if getattr(model, "encoder", "identity") in ("hashgrid", "frequency", "spherical_harmonics"):
         raise NotImplementedError("batched training of the hashgrid / frequency encoders has no device "
                                   "implementation; train each problem with trainer.setup")
     pc = model_dict.get("preconditioner") or {}

@@ -29,11 +29,11 @@ under params["preconditioner"] and its initial draw comes from numpy.
 model_type "multiresolution_hash_network" runs with the identity position
 encoder (CELU coordinate MLPs in the tile kernel), the hashgrid encoder
 (dips_hash_train_step per batch: the MLPs and the encoder's table train
-together) or the frequency encoder (the same entry points, no table),
-nn/hash_network.py.  Unsupported reference options raise
+together) or the frequency and spherical-harmonics encoders (the same entry
+points, no table), nn/hash_network.py.  Unsupported reference options raise
 NotImplementedError: algorithm 1, lbfgs, nonlinear operators, activations
-other than tanh in "mlp", the preconditioner with the hashgrid or frequency
-encoders, and the spherical-harmonics encoder.
+other than tanh in "mlp", and the preconditioner with the hashgrid, frequency
+or spherical-harmonics encoders.
 """
 import logging
 import os
@@ -116,7 +116,7 @@ class Trainer:
                                                        np.float32(tr_gstate.dz))
         self.forward, self.framework = get_model(model_dict, model_type=model_dict.get("model_type", "mlp"))
         self.model = self.forward.model
-        if self.precond is not None and getattr(self.model, "encoder", "identity") in ("hashgrid", "frequency"):
+        if self.precond is not None and getattr(self.model, "encoder", "identity") in ("hashgrid", "frequency", "spherical_harmonics"):
             raise NotImplementedError("the preconditioner with the hashgrid / frequency encoders has no device "
                                       "implementation")
         self.cbs = _callbacks(sim_state_fn)
@@ -220,7 +220,7 @@ class Trainer:
         # restart resumes from the checkpoint's count
         count0 = int(self.opt_restore["count"]) if self.opt_restore is not None else 0
         cap = max(1, count0 + steps_total)
-        if getattr(self.model, "encoder", "identity") in ("hashgrid", "frequency"):
+        if getattr(self.model, "encoder", "identity") in ("hashgrid", "frequency", "spherical_harmonics"):
             engine = HashEngine(self.model.hashgrid, self.flatten(self.params), self.optimizer_dict, max_steps=cap)
         else:
             engine = Engine(self.model.dims, self.flatten(self.params), self.optimizer_dict, max_rows=B,

@@ -31,14 +31,18 @@ fraction puts the whole weight of that axis on the lower corner (parity
 unpinned for such positions: the reference's own tests hold no vectors there).
 
 ``Frequency`` restates the frequency encoder of the same model type
-(encoders_flax.py:269-307); forward / vjp take either encoder.
+(encoders_flax.py:269-307), ``SphericalHarmonics`` its spherical-harmonics
+encoder (encoders_flax.py:311-538); forward / vjp take any of the encoders.
 
 Pinned by tests/test_oracle_hashgrid.py: dense-level indices and a linear table
 reproduce a linear function exactly (trilinear interpolation), a constant
 table gives a constant encoding (weights sum to 1), and the VJP against central
 finite differences for table rows and MLP weights; the frequency encoding's
 closed forms (pos = 0 gives sin 0 = 0 and cos 0 = 1 on every frequency,
-pos = 1/2 gives sin(2^l pi / 2)) and its VJP against finite differences.
+pos = 1/2 gives sin(2^l pi / 2)) and its VJP against finite differences; the
+spherical harmonics against scipy's complex spherical harmonics on the unit
+sphere (ordering, signs, normalisation) and against numpy's Legendre
+polynomials off the sphere (the tabulated form, no r^2 terms).
 """
 import math
 
@@ -128,6 +132,71 @@ class Frequency:
         A = (c[None, None, :] * pos[:, :, None]).astype(np.float32)  # (n, 3, L)
         enc = np.stack([np.sin(A.astype(dt)), np.cos(A.astype(dt))], axis=-1)
         return enc.reshape(X.shape[0], 6 * self.L), None
+
+
+class SphericalHarmonics:
+    """SphericalHarmonicsEncoder (encoders_flax.py:311-538) of highest order L
+    (1..8, L^2 features), applied to the raw position x (model.py:84 passes x
+    itself; the bound is not used).  Feature l^2 + l + m (|m| <= l < L) is
+
+        (-1)^k c_l^k Q_l^k(z) T_m(x, y),   k = |m|,
+        c_l^k = sqrt((2l+1)/(4 pi) (l-k)!/(l+k)!) * (sqrt 2 if k > 0),
+        Q_l^k = d^k P_l / dz^k (P_l the Legendre polynomial),
+        T_m = Re (x + i y)^k for m > 0, Im (x + i y)^k for m < 0, 1 for m = 0,
+
+    the real spherical harmonics in the form the reference tabulates them (the
+    unit-vector forms, with no r^2 terms): on the unit sphere they equal
+    sqrt 2 Re / Im of the Condon-Shortley Y_l^k.  Q_l^k from the three-term
+    recurrence (l - k) Q_l^k = (2l - 1) z Q_{l-1}^k - (l + k - 1) Q_{l-2}^k with
+    Q_k^k = (2k - 1)!!, Q_{k+1}^k = (2k + 1) z Q_k^k; (x + iy)^k by complex
+    multiplication.  No table (rows = 0)."""
+
+    def __init__(self, L=8, bound=1.0, exact_input=False):
+        if not 1 <= int(L) <= 8:
+            raise ValueError("spherical harmonics of order 1..8")
+        self.L, self.bound = int(L), float(bound)
+        self.rows, self.F = 0, 2
+        self.exact_input = bool(exact_input)  # positions not rounded to float32 first (finite differences)
+
+    @property
+    def n_features(self):
+        return self.L * self.L
+
+    @staticmethod
+    def coeff(l, k):
+        c = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - k) / math.factorial(l + k))
+        return (-1) ** k * c * (math.sqrt(2.0) if k > 0 else 1.0)
+
+    def legendre_derivs(self, z):
+        """Q[l][k] = d^k P_l / dz^k at z, l, k < L (arrays of z's shape)."""
+        L = self.L
+        Q = [[np.zeros_like(z) for _ in range(L)] for _ in range(L)]
+        for k in range(L):
+            dfact = 1.0
+            for j in range(1, 2 * k, 2):
+                dfact *= j
+            Q[k][k] = np.full_like(z, dfact)
+            if k + 1 < L:
+                Q[k + 1][k] = (2 * k + 1) * z * Q[k][k]
+            for l in range(k + 2, L):
+                Q[l][k] = ((2 * l - 1) * z * Q[l - 1][k] - (l + k - 1) * Q[l - 2][k]) / (l - k)
+        return Q
+
+    def encode(self, table, X, dt=np.float64):
+        X = np.asarray(X, np.float64 if self.exact_input else np.float32).astype(dt)
+        x, y, z = X[:, 0], X[:, 1], X[:, 2]
+        Q = self.legendre_derivs(z)
+        C, S = [np.ones_like(x)], [np.zeros_like(x)]
+        for k in range(1, self.L):
+            C.append(x * C[-1] - y * S[-1])
+            S.append(x * S[-1] + y * C[-2])
+        enc = np.zeros((X.shape[0], self.L * self.L), dt)
+        for l in range(self.L):
+            for m in range(-l, l + 1):
+                k = abs(m)
+                t = C[k] if m > 0 else (S[k] if m < 0 else 1.0)
+                enc[:, l * l + l + m] = self.coeff(l, k) * Q[l][k] * t
+        return enc, None
 
 
 def mlp_size(E, widths):

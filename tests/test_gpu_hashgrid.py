@@ -345,3 +345,97 @@ def test_trainer_frequency_matches_oracle(tmp_path):
     phase = cb("phi", P) >= 0
     uo = HG.forward(dev.astype(np.float64), freq, [16], P, phase)
     assert np.max(np.abs(st.solution - uo)) <= 1e-5 * max(1.0, np.max(np.abs(uo)))
+
+
+def _sh_model(seed, order):
+    from jax_dips.nn.hash_network import HashNetwork
+
+    model = HashNetwork(bound=1.0, pos_enc="spherical_harmonics", layer_widths=[16], highest_sh_order=order)
+    return model, model.flatten(model.init(seed)), HG.SphericalHarmonics(order, 1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,level,B,b", [(2, 0, 4096, 0), (8, 0, 4096, 0), (8, 2, 1024, 3), (5, 1, 2048, 1)])
+def test_spherical_harmonics_loss_grad_matches_oracle(order, level, B, b):
+    """The spherical-harmonics encoder (order 2: the reference's LPBE setting,
+    lpbe.yaml:60; order 8: its maximum, 64 features): loss and gradient of a
+    batch against the oracle."""
+    from jax_dips.records import DeviceGrid, DeviceRecords, HashEngine, pack_dense
+
+    prob = problems.sphere(mu_m=1.0, mu_p=10.0)
+    axes, P, h, bounds, cb, rec = oracle_level(prob, 16, level)
+    model, theta, sh = _sh_model(5, order)
+    assert theta.size == 2 * HG.mlp_size(order * order, [16])
+    print(f"spherical harmonics order {order} level {level} batch {b} of {B}", flush=True)
+    eng = HashEngine(model.hashgrid, theta, {}, max_steps=2)
+    loss, grad = eng.loss_grad(DeviceGrid(*axes), DeviceRecords(pack_dense(rec.W, rec.c, B), rec.h), b, 1.0 / B)
+    ol, og, _ = HG.loss_and_grad(theta.astype(np.float64), sh, [16], rec, np.arange(b * B, (b + 1) * B))
+    r = _rel(grad, og)
+    print(f"  loss gpu {loss:.9e} oracle {ol:.9e}; grad rel {r:.2e}")
+    assert abs(loss - ol) <= 1e-5 * abs(ol)
+    assert r <= 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [3, 8])
+def test_spherical_harmonics_eval_matches_oracle(order):
+    """u against the oracle network; grad u against central finite differences
+    of the oracle network on float64 positions (h = 1e-6)."""
+    from jax_dips.evaluation import evaluate_points
+
+    model, theta, sh = _sh_model(11, order)
+    rng = np.random.default_rng(4)
+    m = 2000
+    X = rng.uniform(-1.0, 1.0, (m, 3)).astype(np.float32)
+    phase = rng.uniform(size=m) > 0.5
+    u, g = evaluate_points(model, theta, X, phase, want_grad=True)
+    th = theta.astype(np.float64)
+    uo = HG.forward(th, sh, [16], X, phase)
+    assert np.max(np.abs(u - uo)) <= 1e-5 * max(1.0, np.max(np.abs(uo)))
+    ex = HG.SphericalHarmonics(order, 1.0, exact_input=True)
+    X64 = X.astype(np.float64)
+    eps = 1e-6
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = eps
+        fd = (HG.forward(th, ex, [16], X64 + e, phase) - HG.forward(th, ex, [16], X64 - e, phase)) / (2 * eps)
+        r = _rel(g[:, a], fd)
+        print(f"order {order} axis {a}: grad rel {r:.2e}")
+        assert r <= 1e-3
+
+
+@pytest.mark.gpu
+def test_trainer_spherical_harmonics_matches_oracle(tmp_path):
+    """The drop-in trainer with the reference's LPBE encoder settings
+    (pos_enc spherical_harmonics, highest_sh_order 2, layer_widths [16]):
+    per-epoch losses and parameters against the oracle loop."""
+    from jax_dips.domain import mesh
+    from jax_dips.solvers.poisson import trainer
+
+    prob = problems.sphere()
+    cfg = {"bound": 1.0, "pos_enc": "spherical_harmonics", "layer_widths": [16], "sol_act": "linear",
+           "tv_scale": 0.0, "highest_sh_order": 2}
+    model_dict = {"model_type": "multiresolution_hash_network", "multiresolution_hash_network": cfg}
+    model, theta0, sh = _sh_model(42, 2)
+    x, y, z = problems.grid_axes(16)
+    g = mesh.construct(3)[0](x, y, z)
+    init_fn = trainer.setup(prob["initial_value_fn"], prob["dirichlet_bc_fn"], prob["lvl_set_fn"], prob["mu_m_fn"],
+                            prob["mu_p_fn"], prob["k_m_fn"], prob["k_p_fn"], prob["f_m_fn"], prob["f_p_fn"],
+                            prob["alpha_fn"], prob["beta_fn"])
+    state, solve_fn = init_fn(lvl_gstate=g, tr_gstate=g, eval_gstate=g, num_epochs=8, batch_size=2048,
+                              checkpoint_dir=str(tmp_path / "ck"), results_dir=str(tmp_path), optimizer_dict=OPT,
+                              model_dict=model_dict, init_params=model.unflatten(theta0))
+    st, es, le = solve_fn(sim_state=state)
+    tr = solve_fn.trainer
+    axes, P, h, bounds = grid_points(16)
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    th, ole, steps, _ = T.train(P, h, bounds, cb, None, theta0, 8, 2048, OPT,
+                                loss_fn=lambda t, rec, rows: HG.loss_and_grad(t.astype(np.float64), sh, [16], rec,
+                                                                              rows))
+    print("device", le, "oracle", ole)
+    assert np.all(np.abs(le - ole) <= 1e-3 * np.abs(ole))
+    dev = tr.model.flatten(tr.params)
+    assert _rel(dev, th) <= 1e-3
+    phase = cb("phi", P) >= 0
+    uo = HG.forward(dev.astype(np.float64), sh, [16], P, phase)
+    assert np.max(np.abs(st.solution - uo)) <= 1e-5 * max(1.0, np.max(np.abs(uo)))

@@ -42,7 +42,8 @@ def test_model_tree_and_flat_layout():
     assert lib.dips_param_count(ctypes.byref(_native.Net(*model.dims, 0, 0, 0.0, 2))) < 0
 
 
-@pytest.mark.parametrize("change", [{"pos_enc": "spherical_harmonics"}, {"pos_enc": "frequency", "layer_widths": [32]},
+@pytest.mark.parametrize("change", [{"pos_enc": "spherical_harmonics", "layer_widths": [64]},
+                                    {"pos_enc": "frequency", "layer_widths": [32]},
                                     {"sol_act": "sigmoid"}, {"layer_widths": [32]}, {"sol_skip_in_layers": [1]},
                                     {"pos_enc": "hashgrid", "layer_widths": [16, 16]},
                                     {"pos_enc": "hashgrid", "F": 4}, {"pos_enc": "hashgrid", "pos_levels": 17}])
@@ -191,3 +192,33 @@ def test_entries_ordered_by_position():
             seen.add(k)
             last = k
     assert len(seen) < nnz  # the test data has repeated positions
+
+
+@pytest.mark.parametrize("order", [1, 2, 8])
+def test_spherical_harmonics_model_and_abi(order):
+    """pos_enc = "spherical_harmonics" (model.py:135-136): highest_sh_order^2
+    features, no table; the library's parameter count agrees; orders outside
+    1..8 are rejected by the model and the library."""
+    from jax_dips import _native
+    from jax_dips.nn.configure import get_model
+    from oracle.hashgrid import mlp_size
+
+    cfg = dict(LPBE, pos_enc="spherical_harmonics", highest_sh_order=order)
+    forward, _ = get_model({"model_type": "multiresolution_hash_network", "multiresolution_hash_network": cfg},
+                           "multiresolution_hash_network")
+    model = forward.model
+    assert model.hashgrid[-1] == 2 and model.hashgrid[0] == order and model.in_dim == order * order
+    tree = forward.init(3)
+    assert set(tree["params"]) == {"sol_p", "sol_m"}
+    assert tree["params"]["sol_m"]["Dense_0"]["kernel"].shape == (order * order, 16)
+    flat = model.flatten(tree)
+    assert flat.size == model.n_params == 2 * mlp_size(order * order, [16])
+    lib = _native.lib()
+    hg = _native.HashGrid(*model.hashgrid)
+    assert lib.dips_hash_param_count(ctypes.byref(hg)) == flat.size
+    assert lib.dips_hash_table_rows(ctypes.byref(hg)) == 0
+    for bad in (0, 9):
+        assert lib.dips_hash_param_count(ctypes.byref(_native.HashGrid(bad, 2, 1, 1, 1, 1.0, 2))) < 0
+        with pytest.raises(NotImplementedError):
+            get_model({"model_type": "multiresolution_hash_network",
+                       "multiresolution_hash_network": dict(cfg, highest_sh_order=bad)}, "multiresolution_hash_network")

@@ -123,3 +123,77 @@ def test_frequency_encoding_closed_forms_and_vjp():
         e[i] = h
         fd = (seed @ HG.forward(theta + e, f, widths, X, phase) - seed @ HG.forward(theta - e, f, widths, X, phase)) / (2 * h)
         assert abs(fd - gr[i]) <= 1e-5 * max(1.0, abs(gr[i])), (i, fd, gr[i])
+
+
+def _sh_scipy(L, X):
+    """Real spherical harmonics of unit vectors X from scipy's complex ones
+    (Condon-Shortley phase): sqrt 2 Re Y_l^k for m = k > 0, sqrt 2 Im Y_l^k for
+    m = -k < 0, Y_l^0 for m = 0, feature l^2 + l + m."""
+    from scipy.special import sph_harm_y
+
+    polar = np.arccos(np.clip(X[:, 2], -1, 1))
+    azim = np.arctan2(X[:, 1], X[:, 0])
+    out = np.zeros((X.shape[0], L * L))
+    for l in range(L):
+        for m in range(-l, l + 1):
+            y = sph_harm_y(l, abs(m), polar, azim)
+            out[:, l * l + l + m] = (np.sqrt(2) * (y.real if m > 0 else y.imag)) if m else y.real
+    return out
+
+
+def test_spherical_harmonics_on_the_sphere_and_off_it():
+    """On unit vectors the encoding equals scipy's real spherical harmonics
+    (ordering, signs, normalisation; the reference's first terms are
+    1/(2 sqrt pi), -sqrt3 y/(2 sqrt pi), sqrt3 z/(2 sqrt pi), -sqrt3 x/(2 sqrt pi));
+    off the sphere it is c Q(z) T(x, y) with Q the derivatives of numpy's
+    Legendre polynomials and T from numpy's complex powers -- the unit-vector
+    form the reference evaluates on raw positions, without r^2 terms (the solid
+    harmonic 3 z^2 - r^2 differs from 3 z^2 - 1 there)."""
+    from numpy.polynomial import legendre
+
+    rng = np.random.default_rng(9)
+    U = rng.normal(size=(300, 3))
+    U /= np.linalg.norm(U, axis=1, keepdims=True)
+    sh = HG.SphericalHarmonics(8)
+    assert sh.n_features == 64
+    enc, _ = sh.encode(None, U.astype(np.float32))
+    Uf = U.astype(np.float32).astype(np.float64)
+    Uf /= np.linalg.norm(Uf, axis=1, keepdims=True)
+    assert np.max(np.abs(enc - _sh_scipy(8, Uf))) <= 1e-5
+    s = 1.0 / (2.0 * np.sqrt(np.pi))
+    assert np.allclose(enc[:, 0], s)
+    assert np.allclose(enc[:, 1:4], np.sqrt(3) * s * np.stack([-Uf[:, 1], Uf[:, 2], -Uf[:, 0]], 1), atol=1e-6)
+    X = rng.uniform(-1.6, 1.6, (200, 3)).astype(np.float32)
+    enc, _ = sh.encode(None, X)
+    Xd = X.astype(np.float64)
+    w = (Xd[:, 0] + 1j * Xd[:, 1])
+    for l in range(8):
+        for m in range(-l, l + 1):
+            k = abs(m)
+            q = legendre.Legendre.basis(l).deriv(k)(Xd[:, 2]) if k <= l else 0.0
+            t = (w ** k).real if m > 0 else ((w ** k).imag if m < 0 else 1.0)
+            want = HG.SphericalHarmonics.coeff(l, k) * q * t
+            assert np.allclose(enc[:, l * l + l + m], want, rtol=1e-9, atol=1e-9), (l, m)
+    # lower orders are the leading features of the higher ones
+    e2, _ = HG.SphericalHarmonics(2).encode(None, X)
+    assert np.array_equal(e2, enc[:, :4])
+    with pytest.raises(ValueError):
+        HG.SphericalHarmonics(9)
+
+
+def test_spherical_harmonics_network_vjp():
+    sh = HG.SphericalHarmonics(3)
+    widths = [16]
+    theta = HG.init_params(sh, widths, seed=4).astype(np.float64)
+    assert theta.size == 2 * HG.mlp_size(9, widths)
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-0.9, 0.9, (40, 3)).astype(np.float32)
+    phase = rng.uniform(size=40) > 0.5
+    seed = rng.normal(size=40)
+    gr = HG.vjp(theta, sh, widths, X, phase, seed)
+    h = 1e-5
+    for i in rng.choice(theta.size, 24, replace=False):
+        e = np.zeros_like(theta)
+        e[i] = h
+        fd = (seed @ HG.forward(theta + e, sh, widths, X, phase) - seed @ HG.forward(theta - e, sh, widths, X, phase)) / (2 * h)
+        assert abs(fd - gr[i]) <= 1e-5 * max(1.0, abs(gr[i])), (i, fd, gr[i])
