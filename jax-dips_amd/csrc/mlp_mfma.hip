@@ -433,7 +433,7 @@ __device__ __forceinline__ f4 coop_block(const float* __restrict__ stage, int st
                                          f4 acc) {
   const int lane = threadIdx.x & 63;
   const int m = lane & 15, kq = lane / 16;
-#pragma unroll 2
+#pragma unroll
   for (int s = 0; s < kBW; ++s) {
     const float* A = stage + s * stride_w;
     const float* D = A + stride_w / 2;
