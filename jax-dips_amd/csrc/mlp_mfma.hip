@@ -249,7 +249,7 @@ constexpr int kFT = 512;  // threads per forward workgroup: 8 wavefronts share o
 constexpr int kFW = kFT / 64;
 
 template <int Lm, int NBm, int Lp, int NBp>
-__global__ __launch_bounds__(kFT, 6) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
+__global__ __launch_bounds__(kFT, 8) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_item,
                                                        int nblk_p) {
   extern __shared__ __align__(16) float lds[];
@@ -554,16 +554,12 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
       to_lds<NB>(bufA, act[l - 1]);
     else
       bufA[(lane / 16) * kTP + (lane & 15)] = xin;
-    __syncthreads();
     if (l > 0) {
-#pragma unroll
-      for (int j = 0; j < CA::kW; ++j) {
-        const int q = wave * CA::kW + j;
-        if (q < NB * NB) G.dW[l - 1][j] = coop_block(stage, stride_w, 16 * (q / NB), 16 * (q % NB), false, G.dW[l - 1][j]);
-      }
-      // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2), from this tile's registers: the A operand of step r over
-      // output block fb is W_l[16 ib + m][16 fb + 4 kq + r] (m = lane%16, kq = lane/16), the float m%4 of
-      // unit 16 (m/4) + 4 kq + r of block (fb, ib) -- 4 floats 4 apart from this lane's offset
+      // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2), from this tile's registers (the contraction below reads the
+      // staged copy of d_l), before the barrier: a wavefront that arrives early does this matrix work while
+      // it waits.  The A operand of step r over output block fb is W_l[16 ib + m][16 fb + 4 kq + r]
+      // (m = lane%16, kq = lane/16), the float m%4 of unit 16 (m/4) + 4 kq + r of block (fb, ib) -- 4
+      // floats 4 apart from this lane's offset
       const int m = lane & 15, kq = lane >> 4;
       const int tofs = 4 * (17 * (m >> 2) + 4 * kq) + (m & 3);
       f4 c[NB];
@@ -585,6 +581,14 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
           const float a = act[l - 1][ib * 4 + r];
           d[ib * 4 + r] = c[ib][r] * (1.0f - a * a);
         }
+      }
+    }
+    __syncthreads();
+    if (l > 0) {
+#pragma unroll
+      for (int j = 0; j < CA::kW; ++j) {
+        const int q = wave * CA::kW + j;
+        if (q < NB * NB) G.dW[l - 1][j] = coop_block(stage, stride_w, 16 * (q / NB), 16 * (q % NB), false, G.dW[l - 1][j]);
       }
     } else if (wave < NB) {
       G.dW0 = coop_block(stage, stride_w, 0, 16 * wave, true, G.dW0);
