@@ -584,7 +584,7 @@ def run_hashgrid(args):
         per = k // 4
         ents = 0
         for rec in levels:
-            mb = rec.mfma_batches(grid)["batches"]
+            mb = rec.mfma_batches(grid, table_order=True)["batches"]
             for i in range(per):
                 bt = mb[i % nb]
                 ents += bt.t_count

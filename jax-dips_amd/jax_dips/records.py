@@ -181,7 +181,7 @@ class DeviceRecords:
         self.row_coeffs = C[self.row_point.to(torch.int64)].contiguous()
         self._finish()
 
-    def mfma_batches(self, grid=None):
+    def mfma_batches(self, grid=None, table_order=False):
         """Phase-sorted entry arrays (+ entries first inside each batch) and one
         dips_mfma_batch_t per batch, for the MFMA and hashgrid kernels (built
         once per ordering, cached).  grid (DeviceGrid): the entries of each
@@ -192,10 +192,15 @@ class DeviceRecords:
         records for them, and the batches carry these runs as groups
         (t_group, g_begin/g_plus/g_minus) whose network the MFMA kernels
         evaluate once.  Any order inside a segment is valid: t_csr maps each
-        entry to its CSR slot."""
+        entry to its CSR slot.  The position order is lexicographic, x-major
+        with z fastest (the order of the grid points, hence of the rows, which
+        keeps the MFMA row sums' gathers of u per item local), or with
+        table_order z-major with x fastest (the order of the hashgrid's dense
+        table rows, x + y res + z res^2, for the hashgrid kernels' table
+        gathers and record aggregation)."""
         import torch
 
-        cache = "_mfma" if grid is None else "_mfma_pos"
+        cache = "_mfma" if grid is None else ("_mfma_tab" if table_order else "_mfma_pos")
         if getattr(self, cache, None) is not None:
             return getattr(self, cache)
         dev = self.row_ptr.device
@@ -218,9 +223,9 @@ class DeviceRecords:
             X = grid.points()[self.row_point.to(torch.int64)[ent_row]] + o * h
             del o, c
             perm = torch.arange(n, device=dev)
-            # least significant key first: z, y, x, then the segment -- inside a segment the order is x-major,
-            # z fastest, as the rows (grid points) are, so the row sums' gathers of u per item stay local
-            for col in (2, 1, 0):
+            # least significant key first, then the segment: z, y, x (x-major, z fastest, as the rows are) or,
+            # in table order, x, y, z (z-major, x fastest, as the hashgrid's dense table rows are)
+            for col in ((0, 1, 2) if table_order else (2, 1, 0)):
                 _, p2 = torch.sort(X[perm, col], stable=True)
                 perm = perm[p2]
             _, p2 = torch.sort(key[perm], stable=True)
@@ -565,7 +570,7 @@ class HashEngine:
         segment, and a workspace large enough for its batches."""
         import torch
 
-        mb = rec.mfma_batches(grid)
+        mb = rec.mfma_batches(grid, table_order=True)
         dbl = ctypes.c_int64(0)
         need = int(N.lib().dips_hash_workspace(ctypes.byref(self.hg), int(rec.batch_size), mb["max_nnz"],
                                                ctypes.byref(dbl)))

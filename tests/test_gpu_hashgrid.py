@@ -88,7 +88,8 @@ def test_entry_order_invariance(cfg, level):
     dgrid, drec = DeviceGrid(*axes), DeviceRecords(pack_dense(rec.W, rec.c, B), rec.h)
     eng.workspace(drec, dgrid)
     out = {}
-    for name, mb in (("phase", drec.mfma_batches()), ("position", drec.mfma_batches(dgrid))):
+    for name, mb in (("phase", drec.mfma_batches()), ("position", drec.mfma_batches(dgrid, table_order=True)),
+                     ("grid position", drec.mfma_batches(dgrid))):
         print(f"hashgrid {cfg} level {level}: {name} order", flush=True)
         g = torch.empty(eng.P, dtype=torch.float32, device="cuda")
         loss = torch.empty(1, dtype=torch.float64, device="cuda")
@@ -99,7 +100,7 @@ def test_entry_order_invariance(cfg, level):
         torch.cuda.synchronize()
         out[name] = (float(loss.item()), g.double().cpu().numpy())
     assert not np.array_equal(drec.mfma_batches()["t_csr"].cpu().numpy(),
-                              drec.mfma_batches(dgrid)["t_csr"].cpu().numpy())
+                              drec.mfma_batches(dgrid, table_order=True)["t_csr"].cpu().numpy())
     (l0, g0), (l1, g1) = out["phase"], out["position"]
     rows = np.arange(b * B, (b + 1) * B)
     ol, og, _ = HG.loss_and_grad(theta.astype(np.float64), grid, [16], rec, rows)

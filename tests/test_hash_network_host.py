@@ -172,10 +172,20 @@ def test_entries_ordered_by_position():
         if b < nnz:
             assert s[b] != s[a] or np.any(pos[b] != pos[a])
     assert mb["groups"] == len(tg) - 1 < nnz  # some positions coincide on this grid
-    # inside a segment the positions run x-major, z fastest (the order of the grid points)
+    # inside a segment the positions run x-major, z fastest (the order of the grid points); in table order
+    # z-major, x fastest (the hashgrid's dense table rows); both keep the same groups
     for a in range(nnz - 1):
         if s[a + 1] == s[a]:
             assert tuple(pos[a][[0, 1, 2]]) <= tuple(pos[a + 1][[0, 1, 2]])
+    tb = rec.mfma_batches(grid, table_order=True)
+    pt = tb["t_csr"].numpy().astype(np.int64)
+    post = (P[row_point[ent_row[pt]]] + off[pt].astype(np.float32) * h).astype(np.float32)
+    st = seg(pt)
+    assert np.array_equal(st, s)
+    for a in range(nnz - 1):
+        if st[a + 1] == st[a]:
+            assert tuple(post[a][[2, 1, 0]]) <= tuple(post[a + 1][[2, 1, 0]])
+    assert tb["groups"] == mb["groups"]
     for bt in mb["batches"]:
         assert tg[bt.g_begin] == bt.t_begin
         assert tg[bt.g_begin + bt.g_plus] == bt.t_begin + bt.n_plus
