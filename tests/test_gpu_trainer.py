@@ -188,3 +188,43 @@ def test_restart_after_sigint_matches_uninterrupted_run(tmp_path, monkeypatch):
     assert np.array_equal(le2, le_full[3:])
     assert np.array_equal(tr2.model.flatten(tr2.params), tr_full.model.flatten(tr_full.params))
     assert sorted(os.listdir(tmp_path / "resume" / "ck")) == ["checkpoint_8"]
+
+
+@pytest.mark.gpu
+def test_wide_network_eval_at_256_cubed():
+    """The 3x64 evaluation (MFMA tiles, forward-mode grad u) on every node of a
+    256^3 grid (16.8 M points, the reference's eval grid for its 128^3 runs):
+    u at 8,192 sampled points against the float64 oracle network, grad u at 256
+    of them against central differences of it."""
+    import time
+
+    import torch
+    from jax_dips.evaluation import evaluate_points
+    from jax_dips.nn.mlp.MLP import DoubleMLP
+
+    model = DoubleMLP(**mlp_dict((3, 64, 3, 64)))
+    theta = M.init_params(model.dims, seed=21)
+    x = problems.grid_axes(256)[0]
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+    del X, Y, Z
+    phase = np.linalg.norm(P, axis=1) >= 0.5
+    torch.cuda.synchronize()
+    t0 = time.time()
+    u, g = evaluate_points(model, theta, P, phase, want_grad=True)
+    torch.cuda.synchronize()
+    print(f"256^3 eval of u and grad u: {time.time() - t0:.2f} s (host copies included)")
+    assert u.shape == (P.shape[0],) and g.shape == (P.shape[0], 3)
+    rng = np.random.default_rng(1)
+    idx = rng.choice(P.shape[0], 8192, replace=False)
+    th = theta.astype(np.float64)
+    uo = M.forward(th, model.dims, P[idx].astype(np.float64), phase[idx])
+    assert np.max(np.abs(u[idx] - uo)) <= 1e-5 * max(1.0, np.max(np.abs(uo)))
+    sub = idx[:256]
+    h = 1e-3
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = h
+        Q = P[sub].astype(np.float64)
+        fd = (M.forward(th, model.dims, Q + e, phase[sub]) - M.forward(th, model.dims, Q - e, phase[sub])) / (2 * h)
+        assert np.max(np.abs(g[sub, a] - fd)) <= 1e-3 * max(1.0, np.max(np.abs(fd))), a
