@@ -314,8 +314,9 @@ def main():
     # only for the weight-gradient blocks; the MFMA step of the wide networks is bound by the matrix cores
     bound = "mfma" if path == "mfma" else "valu"
     accuracy = None
-    acc_epochs = 2000 if args.accuracy_epochs < 0 else args.accuracy_epochs
-    if acc_epochs > 0 and "exact_fn" in prob:
+    # the workload's schedule (2000 epochs); PB: 200 epochs (32 optimizer steps per epoch at 5 ms each)
+    acc_epochs = (200 if args.workload == "pb" else 2000) if args.accuracy_epochs < 0 else args.accuracy_epochs
+    if acc_epochs > 0:
         del levels, eng
         torch.cuda.empty_cache()
         accuracy = accuracy_run(prob, n, lo, hi, mlp, B, acc_epochs, opt)
@@ -396,9 +397,20 @@ def accuracy_run(prob, n, lo, hi, mlp, B, epochs, opt):
     finally:
         shutil.rmtree(work, ignore_errors=True)
     R = np.asarray(g.R, np.float64)
+    npts = R.shape[0]
+    if "exact_fn" not in prob:
+        # no analytic solution (the PB box), and the unpreconditioned BiCGSTAB judge does not converge on its
+        # operator (cut cells with mu 2 / 80 give rows far from diagonal dominance and negative eigenvalues):
+        # the end-to-end run is reported without an error figure
+        print(f"# accuracy run: {epochs} epochs in {wall:.1f} s (records {tr.timing['records_s']:.1f} s, training "
+              f"{tr.timing['train_s']:.1f} s); no reference solution", file=sys.stderr, flush=True)
+        return {"epochs": epochs, "steps": int(tr.steps), "end_to_end_s": wall,
+                "records_s": tr.timing["records_s"], "train_s": tr.timing["train_s"],
+                "train_points_per_s": npts * epochs / tr.timing["train_s"],
+                "loss_first": float(loss_epochs[0]), "loss_last": float(loss_epochs[-1]),
+                "reference": "none: no analytic solution, and the grid solve does not converge on this operator"}
     exact = np.asarray(prob["exact_fn"](R.T), np.float64)
     err = np.asarray(new_state.solution, np.float64) - exact
-    npts = R.shape[0]
     # the trained network on the (2n)^3 eval grid (the reference's tests evaluate 128^3 runs at 256^3,
     # confs/poisson.yaml:13-15): u at nodes between the training nodes too
     x2, y2, z2 = problems_axes(2 * n, lo, hi)
