@@ -340,7 +340,10 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": {"mfma": "mfma step: mfma_fwd_kernel + mfma_rowsum_kernel + mfma_seed_kernel + "
                                         "mfma_bwd_kernel + mfma_reduce_kernel + mfma_apply_kernel (one optimizer "
-                                        "step; FP32 MFMA with the tanh on the VALU, which does not overlap it)",
+                                        "step; forward hidden layers as exact 3-way bf16 splits on the bf16 "
+                                        "matrix cores, 6 products, fp32 accumulate; backward on the FP32 matrix "
+                                        "cores from the forward's stored activations; peak = the FP32 matrix "
+                                        "peak)",
                                 "lane": "loss_grad_kernel<grad,fused> (one launch per optimizer step)",
                                 "tile": "tile_step_kernel<grad> (gradient launch of a step; the optimizer "
                                         "runs after it in a one-workgroup tile_apply_kernel); FP32 VALU "

@@ -116,18 +116,26 @@ __device__ void pack_weights(const float* __restrict__ params, FlatNet fn, float
   }
 }
 
+// layer 0 of one tile (K = 4: x, y, z and the bias row) from its packed weights at W[0, NB*64)
+template <int L, int NB>
+__device__ __forceinline__ void tile_layer0(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mfma4(W[ob * 64 + lane], xin, c);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh_exp2(c[r]);
+  }
+}
+
 // forward of one tile: act[l][fb*4+r] feature-major; returns nothing (u by caller)
 template <int L, int NB>
 __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
   using PK = Packed<L, NB>;
   const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int ob = 0; ob < NB; ++ob) {
-    f4 c = {0.f, 0.f, 0.f, 0.f};
-    c = mfma4(W[PK::l0() + ob * 64 + lane], xin, c);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh_exp2(c[r]);
-  }
+  static_assert(PK::l0() == 0, "layer 0 leads the packed weights");
+  tile_layer0<L, NB>(W, xin, act);
 #pragma unroll
   for (int l = 1; l < L; ++l) {
     f4 c[NB];
@@ -153,10 +161,172 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float 
   }
 }
 
-// output u of the tile position l%16 (identical in the 4 lanes of that position)
+// ---- hidden layers on the bf16 matrix cores, to float32 accuracy --------------
+// Each float32 operand x is split exactly into three bf16 parts by truncation:
+// x0 = x with the low 16 bits cleared, x1 likewise of x - x0, x2 = x - x0 - x1
+// (x has 24 significant bits, each part holds 8: x0 + x1 + x2 == x bit for bit
+// for normal x, |x1| < 2^-7 |x|, |x2| < 2^-15 |x|).  A product a w is then the
+// sum of the six part products with part indices summing to at most 2, each an
+// exact bf16 x bf16 product accumulated in float32 by v_mfma_f32_16x16x32_bf16;
+// the dropped terms a1 w2 + a2 w1 + a2 w2 are below 2^-21 |a w|.  One such MFMA
+// (8,192 multiply-adds) takes ~16 cycles against 256 for the same work as eight
+// v_mfma_f32_16x16x4f32 (the float32 MFMA does not overlap with VALU work on a
+// SIMD; the bf16 one overlaps with the tanh's transcendentals).
+//
+// Operand layout of a K = 32 chunk c of a hidden layer (input blocks fb = 2c, 2c+1):
+// slot t of lane group g = lane/16 is input feature 16 (2c + t/4) + 4 g + t%4,
+// which is where the activation of the previous layer already sits (register
+// (fb, r) of the feature-major layout), so the B operand is the 8 activations
+// of blocks 2c and 2c+1, split, and layers still chain without data movement.
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4 mfma_bf(u4 a, u4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float bf_trunc(float x) { return __uint_as_float(__float_as_uint(x) & 0xffff0000u); }
+
+// bf16 bits of part p of x (the three parts sum to x exactly)
+__device__ __forceinline__ uint32_t bf_part(float x, int p) {
+  const float x0 = bf_trunc(x);
+  const float r1 = x - x0;
+  const float x1 = bf_trunc(r1);
+  const float v = p == 0 ? x0 : (p == 1 ? x1 : r1 - x1);
+  return __float_as_uint(v) >> 16;
+}
+
+// the three parts of (x, y) as packed bf16 pairs (x in the low half)
+__device__ __forceinline__ void bf_split_pair(float x, float y, uint32_t& h, uint32_t& m, uint32_t& l) {
+  const uint32_t xb = __float_as_uint(x), yb = __float_as_uint(y);
+  h = __builtin_amdgcn_perm(yb, xb, 0x07060302u);
+  const float x1 = x - __uint_as_float(xb & 0xffff0000u);
+  const float y1 = y - __uint_as_float(yb & 0xffff0000u);
+  const uint32_t x1b = __float_as_uint(x1), y1b = __float_as_uint(y1);
+  m = __builtin_amdgcn_perm(y1b, x1b, 0x07060302u);
+  const float x2 = x1 - __uint_as_float(x1b & 0xffff0000u);
+  const float y2 = y1 - __uint_as_float(y1b & 0xffff0000u);
+  l = __builtin_amdgcn_perm(__float_as_uint(y2), __float_as_uint(x2), 0x07060302u);
+}
+
+// Packed weights of one phase net for the split-bf16 forward pass:
+// [L0: NB*64 floats][hidden biases: (L-1) x H][wout: H][bout: 1] (padded to 16 bytes), then
+// per hidden layer l, output block ob, chunk c and part p one 64-lane set of 16-byte units
+// (lane = 16 g + i holds W_l[16 (2c + t/4) + 4 g + t%4][16 ob + i], t = 0..7, part p).
+// Scaled like Packed: every hidden-layer weight and bias times kTanhExp2Scale.
 template <int L, int NB>
+struct PackedBf {
+  static constexpr int H = 16 * NB;
+  static constexpr int NC = (NB + 1) / 2;
+  static constexpr int l0() { return 0; }
+  static constexpr int bias(int l) { return NB * 64 + (l - 1) * H; }
+  static constexpr int wout() { return NB * 64 + (L - 1) * H; }
+  static constexpr int bout() { return wout() + H; }
+  static constexpr int nf() { return (bout() + 1 + 3) & ~3; }
+  static constexpr int units() { return (L - 1) * NB * NC * 3 * 64; }
+  static constexpr int size() { return nf() + 4 * units(); }  // floats
+  static constexpr int set(int l, int ob, int c, int p) { return (((l - 1) * NB + ob) * NC + c) * 3 + p; }
+};
+
+template <int L, int NB>
+__device__ void pack_weights_bf(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
+  using PK = PackedBf<L, NB>;
+  const int Hh = fn.Hh;
+  for (int idx = threadIdx.x; idx < PK::nf(); idx += blockDim.x) {
+    float v = 0.0f;
+    if (idx < NB * 64) {
+      const int ob = idx / 64, lane = idx % 64;
+      const int out = 16 * ob + lane % 16, k = lane / 16;
+      if (out < Hh) v = kTanhExp2Scale * ((k < 3) ? params[fn.w(0) + k * Hh + out] : params[fn.b(0) + out]);
+    } else if (idx < PK::wout()) {
+      const int l = (idx - NB * 64) / PK::H + 1, h = (idx - NB * 64) % PK::H;
+      if (h < Hh) v = kTanhExp2Scale * params[fn.b(l) + h];
+    } else if (idx < PK::bout()) {
+      const int h = idx - PK::wout();
+      if (h < Hh) v = params[fn.wout() + h];
+    } else if (idx == PK::bout()) {
+      v = params[fn.bout()];
+    }
+    lds[idx] = v;
+  }
+  uint32_t* wb = reinterpret_cast<uint32_t*>(lds + PK::nf());
+  for (int d = threadIdx.x; d < 4 * PK::units(); d += blockDim.x) {
+    const int u = d % 4, lane = (d / 4) % 64;
+    int rest = d / 256;
+    const int p = rest % 3;
+    rest /= 3;
+    const int c = rest % PK::NC;
+    rest /= PK::NC;
+    const int ob = rest % NB, l = rest / NB + 1;
+    const int g = lane / 16, out = 16 * ob + lane % 16;
+    uint32_t word = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = 2 * u + h, fb = 2 * c + t / 4;
+      const int in = 16 * fb + 4 * g + t % 4;
+      float w = 0.0f;
+      if (fb < NB && in < Hh && out < Hh) w = kTanhExp2Scale * params[fn.w(l) + in * Hh + out];
+      word |= bf_part(w, p) << (16 * h);
+    }
+    wb[d] = word;
+  }
+}
+
+// forward of one tile with the hidden layers on the bf16 matrix cores (layer 0, K = 4, in float32)
+template <int L, int NB>
+__device__ __forceinline__ void tile_forward_bf(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
+  using PK = PackedBf<L, NB>;
+  const int lane = threadIdx.x & 63;
+  const u4* __restrict__ wb = reinterpret_cast<const u4*>(W + PK::nf());
+  static_assert(PK::l0() == 0, "layer 0 leads the packed weights");
+  tile_layer0<L, NB>(W, xin, act);
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    f4 c[NB];
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) c[ob][r] = W[PK::bias(l) + 16 * ob + 4 * (lane / 16) + r];
+    }
+#pragma unroll
+    for (int cc = 0; cc < PK::NC; ++cc) {
+      u4 ah, am, al;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int fb = 2 * cc + u / 2, r = (2 * u) % 4;
+        const float x = fb < NB ? act[l - 1][fb * 4 + r] : 0.0f;
+        const float y = fb < NB ? act[l - 1][fb * 4 + r + 1] : 0.0f;
+        uint32_t h, m, lo;
+        bf_split_pair(x, y, h, m, lo);
+        ah[u] = h;
+        am[u] = m;
+        al[u] = lo;
+      }
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) {
+        const u4 w0 = wb[PK::set(l, ob, cc, 0) * 64 + lane];
+        const u4 w1 = wb[PK::set(l, ob, cc, 1) * 64 + lane];
+        const u4 w2 = wb[PK::set(l, ob, cc, 2) * 64 + lane];
+        // smallest terms first
+        c[ob] = mfma_bf(w2, ah, c[ob]);
+        c[ob] = mfma_bf(w1, am, c[ob]);
+        c[ob] = mfma_bf(w0, al, c[ob]);
+        c[ob] = mfma_bf(w1, ah, c[ob]);
+        c[ob] = mfma_bf(w0, am, c[ob]);
+        c[ob] = mfma_bf(w0, ah, c[ob]);
+      }
+    }
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) act[l][ob * 4 + r] = dips_tanh_exp2(c[ob][r]);
+    }
+  }
+}
+
+// output u of the tile position l%16 (identical in the 4 lanes of that position)
+template <int L, int NB, class PK = Packed<L, NB>>
 __device__ __forceinline__ float tile_output(const float* __restrict__ W, const float (&act)[L][4 * NB]) {
-  using PK = Packed<L, NB>;
   const int lane = threadIdx.x & 63;
   float s = 0.0f;
 #pragma unroll
@@ -248,10 +418,69 @@ __global__ __launch_bounds__(256) void mfma_items_kernel(MfmaTileArgs T, int64_t
 constexpr int kFT = 512;  // threads per forward workgroup: 8 wavefronts share one packed weight copy
 constexpr int kFW = kFT / 64;
 
+// the forward kernel's hidden layers: split bf16 from two 16-wide blocks on (where the matrix work
+// outweighs the splits), float32 MFMA below
+template <int L, int NB>
+struct FwdPack {
+  static constexpr bool kBf = NB >= 2;
+  static constexpr int size() { return kBf ? PackedBf<L, NB>::size() : Packed<L, NB>::size(); }
+};
+template <int L, int NB>
+__device__ __forceinline__ void fwd_pack(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
+  if constexpr (FwdPack<L, NB>::kBf)
+    pack_weights_bf<L, NB>(params, fn, lds);
+  else
+    pack_weights<L, NB>(params, fn, lds);
+}
+// the forward pass of one tile (split bf16 or float32 by FwdPack) into act
+template <int L, int NB>
+__device__ __forceinline__ void fwd_tile_act(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
+  if constexpr (FwdPack<L, NB>::kBf)
+    tile_forward_bf<L, NB>(W, xin, act);
+  else
+    tile_forward<L, NB>(W, xin, act);
+}
+
+// Stored activations of a tile: hidden layers 1..L-1 (layer 0, four MFMA steps, is recomputed more cheaply
+// than it is stored), f4 ((l - 1) * NB + fb) of lane `lane` at act_tile[((l - 1) * NB + fb) * 64 + lane]
+// (act_tile = the tile's 64 * act_f4 f4s): one 1 KB contiguous store or load per wavefront and f4.
+template <int L, int NB>
+__device__ __forceinline__ void store_act(f4* __restrict__ dst, const float (&act)[L][4 * NB]) {
+#pragma unroll
+  for (int l = 1; l < L; ++l)
+#pragma unroll
+    for (int fb = 0; fb < NB; ++fb)
+      __builtin_nontemporal_store(f4{act[l][fb * 4], act[l][fb * 4 + 1], act[l][fb * 4 + 2], act[l][fb * 4 + 3]},
+                                  dst + ((l - 1) * NB + fb) * 64);
+}
+template <int L, int NB>
+__device__ __forceinline__ void load_act(const f4* __restrict__ src, float (&act)[L][4 * NB]) {
+#pragma unroll
+  for (int l = 1; l < L; ++l)
+#pragma unroll
+    for (int fb = 0; fb < NB; ++fb) {
+      const f4 v = __builtin_nontemporal_load(src + ((l - 1) * NB + fb) * 64);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) act[l][fb * 4 + r] = v[r];
+    }
+}
+
+// u of one tile; with act_tile (this lane's f4 0 of the tile's stored activations) also stores them
+template <int L, int NB>
+__device__ __forceinline__ float fwd_tile_u(const float* __restrict__ W, float xin, f4* __restrict__ act_tile) {
+  float act[L][4 * NB];
+  fwd_tile_act<L, NB>(W, xin, act);
+  if (act_tile) store_act<L, NB>(act_tile, act);
+  if constexpr (FwdPack<L, NB>::kBf)
+    return tile_output<L, NB, PackedBf<L, NB>>(W, act);
+  else
+    return tile_output<L, NB>(W, act);
+}
+
 template <int Lm, int NBm, int Lp, int NBp>
-__global__ __launch_bounds__(kFT, 8) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
+__global__ __launch_bounds__(kFT, (FwdPack<Lp, NBp>::kBf || FwdPack<Lm, NBm>::kBf) ? 6 : 8) void mfma_fwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_, float* __restrict__ u_item,
-                                                       int nblk_p) {
+                                                       int nblk_p, f4* __restrict__ act_out, int act_f4) {
   extern __shared__ __align__(16) float lds[];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool plus = (int)blockIdx.x < nblk_p;
@@ -260,9 +489,9 @@ __global__ __launch_bounds__(kFT, 8) void mfma_fwd_kernel(MfmaTileArgs T, const 
   const int bid = plus ? (int)blockIdx.x : (int)blockIdx.x - nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
   if (plus)
-    pack_weights<Lp, NBp>(params, FlatNet{Hp, Lp, 0}, lds);
+    fwd_pack<Lp, NBp>(params, FlatNet{Hp, Lp, 0}, lds);
   else
-    pack_weights<Lm, NBm>(params, FlatNet{Hm, Lm, PPp}, lds);
+    fwd_pack<Lm, NBm>(params, FlatNet{Hm, Lm, PPp}, lds);
   __syncthreads();
   const int64_t eb = plus ? e0 : e0 + np_;
   const int64_t ee = plus ? e0 + np_ : e0 + np_ + nm_;
@@ -274,16 +503,10 @@ __global__ __launch_bounds__(kFT, 8) void mfma_fwd_kernel(MfmaTileArgs T, const 
     // this tile's input was loaded one tile ahead; load the next one's now
     const float xin = xin_next;
     xin_next = item_input(T, item + (int64_t)nblk * kFW * 16, ee, k);
-    float u;
-    if (plus) {
-      float act[Lp][4 * NBp];
-      tile_forward<Lp, NBp>(lds, xin, act);
-      u = tile_output<Lp, NBp>(lds, act);
-    } else {
-      float act[Lm][4 * NBm];
-      tile_forward<Lm, NBm>(lds, xin, act);
-      u = tile_output<Lm, NBm>(lds, act);
-    }
+    // stored activations (if asked for) of tile `tile` of the + net at global tile `tile`, of the - net after
+    // the (np + 15) / 16 tiles of the + net
+    f4* at = act_out ? act_out + ((plus ? tile : (np_ + 15) / 16 + tile) * act_f4) * 64 + lane : nullptr;
+    const float u = plus ? fwd_tile_u<Lp, NBp>(lds, xin, at) : fwd_tile_u<Lm, NBm>(lds, xin, at);
     if (item < ee && lane < 16) u_item[item - e0] = u;
   }
 }
@@ -597,21 +820,39 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
   }
 }
 
-template <int Lm, int NBm, int Lp, int NBp>
+// LDS of the backward kernel (floats): the float32 packed weights (PKsz), with kStored false the forward
+// pass's split-bf16 copy where it differs from them (FPsz), then the operand stage
+template <int Lm, int NBm, int Lp, int NBp, bool kStored>
+struct BwdLds {
+  static constexpr int PKsz = ((Packed<Lp, NBp>::size() > Packed<Lm, NBm>::size() ? Packed<Lp, NBp>::size()
+                                                                                  : Packed<Lm, NBm>::size()) + 3) & ~3;
+  static constexpr int FPp = FwdPack<Lp, NBp>::kBf ? PackedBf<Lp, NBp>::size() : 0;
+  static constexpr int FPm = FwdPack<Lm, NBm>::kBf ? PackedBf<Lm, NBm>::size() : 0;
+  static constexpr int FPsz = kStored ? 0 : (((FPp > FPm ? FPp : FPm) + 3) & ~3);
+  static constexpr int NBx = NBp > NBm ? NBp : NBm;
+  static constexpr int kStrideW = 2 * 16 * NBx * kTP;  // floats of one wavefront's operand pair
+  static constexpr int floats() { return PKsz + FPsz + kBW * kStrideW; }
+};
+
+// kStored: the activations of every tile come from the forward kernel (act_in, act_f4 f4s per lane and
+// tile, + tiles first); else each wavefront recomputes them with the forward kernel's arithmetic (the
+// same split-bf16 or float32 pass, so both variants give the same bits)
+template <int Lm, int NBm, int Lp, int NBp, bool kStored>
 __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const float* __restrict__ params, int Hm, int Hp,
                                                        int64_t e0, int64_t np_, int64_t nm_,
                                                        const float* __restrict__ seed_item, float* __restrict__ gpart,
-                                                       int nblk_p, int stride) {
+                                                       int nblk_p, int stride, const f4* __restrict__ act_in,
+                                                       int act_f4) {
   extern __shared__ __align__(16) float lds[];
+  using BL = BwdLds<Lm, NBm, Lp, NBp, kStored>;
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool plus = (int)blockIdx.x < nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
-  // the operand stage starts on a 16-byte boundary after the packed weights
-  constexpr int PKsz = ((Packed<Lp, NBp>::size() > Packed<Lm, NBm>::size() ? Packed<Lp, NBp>::size()
-                                                                           : Packed<Lm, NBm>::size()) + 3) & ~3;
-  constexpr int NBx = NBp > NBm ? NBp : NBm;
-  constexpr int kStrideW = 2 * 16 * NBx * kTP;  // floats of one wavefront's operand pair
-  float* stage = lds + PKsz;
+  // the forward copy and the operand stage start on 16-byte boundaries after the packed weights
+  constexpr int kStrideW = BL::kStrideW;
+  float* fwdW = lds + BL::PKsz;
+  float* stage = lds + BL::PKsz + BL::FPsz;
+  const int64_t ntp = (np_ + 15) / 16;
   const int64_t ntiles = plus ? (np_ + 15) / 16 : (nm_ + 15) / 16;
   const int nblk = plus ? nblk_p : (int)gridDim.x - nblk_p;
   const int bid = plus ? (int)blockIdx.x : (int)blockIdx.x - nblk_p;
@@ -623,6 +864,7 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
     constexpr int NB = decltype(tagNB)::value;
     using CA = CoopAcc<L, NB>;
     pack_weights<L, NB>(params, fn, lds);
+    if constexpr (!kStored && FwdPack<L, NB>::kBf) pack_weights_bf<L, NB>(params, fn, fwdW);
     __syncthreads();
     CA G;
 #pragma unroll
@@ -656,7 +898,22 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
         seed_next = seed_of(nx);
       }
       float act[L][4 * NB];
-      tile_forward<L, NB>(lds, xin, act);
+      if constexpr (kStored) {
+        const int64_t tile = (rd * nblk + bid) * kBW + wave;
+        if (tile < ntiles) {
+          load_act<L, NB>(act_in + ((plus ? tile : ntp + tile) * act_f4) * 64 + lane, act);
+        } else {
+#pragma unroll
+          for (int l = 1; l < L; ++l)
+#pragma unroll
+            for (int q = 0; q < 4 * NB; ++q) act[l][q] = 0.0f;
+        }
+        tile_layer0<L, NB>(lds, xin, act);
+      } else if constexpr (FwdPack<L, NB>::kBf) {
+        tile_forward_bf<L, NB>(fwdW, xin, act);
+      } else {
+        tile_forward<L, NB>(lds, xin, act);
+      }
       coop_backward<L, NB>(lds, seed, xin, act, stage, kStrideW, G);
     }
     __syncthreads();  // all waves done with the packed weights: reuse LDS for the reduction
@@ -978,11 +1235,24 @@ int mfma_supported(const dips_net_t* n) {
 
 static int net_params(int L, int H) { return 4 * H + (L - 1) * (H * H + H) + H + 1; }
 
-int64_t mfma_workspace_floats(const dips_net_t* n, int64_t max_rows, int64_t max_nnz) {
-  // u per entry, r per row, gradient partials (bwd workgroups x max net size), g
+// f4s per lane of one tile's stored activations (hidden layers 1..L-1 of the wider phase net: (L - 1) x NB)
+static int act_f4_of(const dips_net_t* n) {
+  return std::max((n->Lm - 1) * nb_of(n->Hm), (n->Lp - 1) * nb_of(n->Hp));
+}
+
+// the workspace without the stored activations: u per entry, r per row, gradient partials (bwd
+// workgroups x max net size), g, the step counter's snapshot
+static int64_t mfma_base_floats(const dips_net_t* n, int64_t max_rows, int64_t max_nnz) {
   const int P = net_params(n->Lm, n->Hm) + net_params(n->Lp, n->Hp);
   const int stride = std::max(net_params(n->Lm, n->Hm), net_params(n->Lp, n->Hp));
   return max_nnz + max_rows + (int64_t)kMfmaBwdBlocks * stride + P + 64;
+}
+
+// ... plus, 256-byte aligned, the hidden activations of every forward tile (at most max_nnz / 16 + 2 tiles:
+// an item is at least one entry, and each phase rounds up to whole tiles), which the backward kernel reads
+// instead of recomputing them
+int64_t mfma_workspace_floats(const dips_net_t* n, int64_t max_rows, int64_t max_nnz) {
+  return mfma_base_floats(n, max_rows, max_nnz) + 64 + (max_nnz / 16 + 2) * (int64_t)act_f4_of(n) * 256;
 }
 
 // dips_eval for the network shapes of the MFMA path
@@ -1090,6 +1360,12 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
                  bt->t_csr, bt->t_group, bt->i_pos, bt->i_count};
   const int a = Lm, b = nb_of(Hm), c = Lp, d = nb_of(Hp);
   const int64_t ntp = (ip + 15) / 16, ntm = (im + 15) / 16;
+  // the forward tiles' activations for the backward, 256-byte aligned after the counter snapshot, when the
+  // workspace holds them (dips_mfma_workspace sizes it so); else the backward recomputes them
+  const int act_f4 = act_f4_of(net);
+  const int64_t act_off = ((int64_t)(gvec + P + 1 - work) + 63) & ~(int64_t)63;
+  const bool stored = act_off + (ntp + ntm) * act_f4 * 256 <= work_floats;
+  f4* act = stored ? reinterpret_cast<f4*>(work + act_off) : nullptr;
   // workgroups split by phase in proportion to the tiles (each at least 1 if it has tiles)
   auto split = [&](int total, int& p, int& m) {
     p = (ntp > 0) ? (int)std::max<int64_t>(1, (int64_t)((double)total * ntp / std::max<int64_t>(1, ntp + ntm))) : 0;
@@ -1102,14 +1378,11 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
 #define X(A, B, C, D)                                                                                              \
   if (!launched && a == A && b == B && c == C && d == D) {                                                      \
     launched = true;                                                                                               \
-    constexpr int PK = ((Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size()                       \
-                                                                     : Packed<A, B>::size()) + 3) & ~3;           \
-    constexpr int PKF = Packed<C, D>::size() > Packed<A, B>::size() ? Packed<C, D>::size() : Packed<A, B>::size(); \
-    constexpr int NBX = D > B ? D : B;                                                                             \
+    constexpr int PKF = FwdPack<C, D>::size() > FwdPack<A, B>::size() ? FwdPack<C, D>::size() : FwdPack<A, B>::size(); \
     const size_t lds_f = sizeof(float) * PKF;                                                                      \
-    const size_t lds_b = sizeof(float) * (PK + kBW * 2 * 16 * NBX * kTP);                                         \
     const size_t red_b = sizeof(float) * std::max(PPp, PPm);                                                       \
-    const size_t bwd_lds = std::max(lds_b, red_b);                                                                 \
+    const size_t bwd_lds = std::max(sizeof(float) * (stored ? BwdLds<A, B, C, D, true>::floats()                   \
+                                                            : BwdLds<A, B, C, D, false>::floats()), red_b);        \
     if (bwd_lds > 160 * 1024 || lds_f > 160 * 1024) {                                                              \
       set_error("dips_mfma_train_step: network too large for the LDS of one workgroup");                            \
       return DIPS_EUNSUPPORTED;                                                                                    \
@@ -1119,23 +1392,30 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
       const size_t red_max = sizeof(float) * std::max(net_params(C, 16 * D), net_params(A, 16 * B));             \
       hipFuncSetAttribute((const void*)mfma_fwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                           (int)(sizeof(float) * PKF));                                                             \
-      hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
-                          (int)std::max(sizeof(float) * (PK + kBW * 2 * 16 * NBX * kTP), red_max));               \
+      hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D, true>,                                          \
+                          hipFuncAttributeMaxDynamicSharedMemorySize,                                              \
+                          (int)std::max(sizeof(float) * BwdLds<A, B, C, D, true>::floats(), red_max));             \
+      hipFuncSetAttribute((const void*)mfma_bwd_kernel<A, B, C, D, false>,                                         \
+                          hipFuncAttributeMaxDynamicSharedMemorySize,                                              \
+                          (int)std::max(sizeof(float) * BwdLds<A, B, C, D, false>::floats(), red_max));            \
       return true;                                                                                                 \
     }();                                                                                                           \
     (void)attrs_set;                                                                                               \
     split(resident_blocks((const void*)mfma_fwd_kernel<A, B, C, D>, kFT, lds_f, kMfmaFwdBlocks), nfp, nfm);       \
     if (nfp + nfm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kFT), lds_f, st, T, params, Hm, Hp,   \
-                         i0, ip, im, u, nfp);                                                                      \
+                         i0, ip, im, u, nfp, act, act_f4);                                                         \
     hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(xcd_grid(nrb)), dim3(kMT), 0, st, rec->row_ptr, rec->row_c,       \
                        bt->row_begin, nrows, rec->ent_w, rec->nnz, u - i0, bt->e_item, r, loss_part);              \
     if (ip + im > 0)                                                                                               \
       hipLaunchKernelGGL(mfma_seed_kernel, dim3(xcd_grid((ip + im + 255) / 256)), dim3(256), 0, st, T, i0,          \
                          ip + im, bt->row_begin, r, u);                                                            \
-    if (nbp + nbm > 0)                                                                                             \
-      hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D>), dim3(nbp + nbm), dim3(kBT), bwd_lds, st, T,                 \
-                         params, Hm, Hp, i0, ip, im, u, gpart, nbp, stride);                                       \
+    if (nbp + nbm > 0 && stored)                                                                                   \
+      hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D, true>), dim3(nbp + nbm), dim3(kBT), bwd_lds, st, T,           \
+                         params, Hm, Hp, i0, ip, im, u, gpart, nbp, stride, act, act_f4);                          \
+    if (nbp + nbm > 0 && !stored)                                                                                  \
+      hipLaunchKernelGGL((mfma_bwd_kernel<A, B, C, D, false>), dim3(nbp + nbm), dim3(kBT), bwd_lds, st, T,          \
+                         params, Hm, Hp, i0, ip, im, u, gpart, nbp, stride, nullptr, act_f4);                      \
   }
   DIPS_MFMA_SHAPES(X)
 #undef X
@@ -1218,7 +1498,7 @@ int mfma_step_noalloc(const dips_grid_t* g, const dips_records_t* rec, const dip
                       const dips_net_t* net, const dips_opt_t* opt, float inv_b, float* params, float* m, float* v,
                       int32_t* count, float* loss_hist, int loss_cap, hipStream_t st) {
   const int64_t nrows = bt->nrows, nent = bt->ent_end - bt->ent_begin;
-  const int64_t floats = mfma_workspace_floats(net, nrows, nent);
+  const int64_t floats = mfma_base_floats(net, nrows, nent);
   const int64_t P = net_params(net->Lm, net->Hm) + net_params(net->Lp, net->Hp);
   const int64_t doubles = (nrows + kMT - 1) / kMT + (P + kRedParams - 1) / kRedParams + 1;
   if (floats > kNoAllocFloats || doubles > kNoAllocDoubles) {

@@ -1,7 +1,9 @@
 """GPU parity of the MFMA training-step kernels (wide networks) with the CPU
 oracle: loss and mean gradient of a batch, and optimizer trajectories.
 
-Tolerance: float32 MFMA accumulation over hidden widths up to 64 against the
+Tolerance: float32 accumulation over hidden widths up to 64 (the forward
+pass's hidden layers from width 32 on as exact three-way bf16 splits on the
+bf16 matrix cores, every other contraction on the float32 ones) against the
 float64 oracle: loss within 1e-5 relative, gradient within 1e-4 relative in the
 2-norm (as for the lane-per-row kernel).
 """
@@ -77,3 +79,46 @@ def test_mfma_training_steps_match_oracle():
     assert int(eng.count.item()) == 12
     assert np.all(np.abs(gl - ol) <= 1e-4 * np.abs(ol))
     assert np.linalg.norm(eng.params.cpu().numpy() - th) <= 1e-3 * np.linalg.norm(th)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(3, 64, 3, 64), (2, 16, 2, 16), (1, 1, 3, 64), (3, 32, 3, 32)])
+def test_stored_and_recomputed_activations_agree(dims):
+    """The backward kernel reads the forward tiles' stored activations when the
+    workspace holds them and recomputes them otherwise; both give the same bits
+    (loss and gradient), and the workspace-size query covers the stored form."""
+    import ctypes
+
+    import torch
+
+    from jax_dips import _native as N
+    from jax_dips.records import Engine
+
+    prob = problems.star(r0=0.55, amp=0.15, centre=(0.0123, -0.0171, 0.0093))
+    axes, P, h, bounds, cb, rec = oracle_level(prob, 16, 0)
+    B = 2048
+    theta = M.init_params(dims, seed=7)
+    grid, drec = device(axes, rec, B)
+    eng = Engine(dims, theta, {}, max_rows=B, max_steps=4)
+    mb = eng.mfma_workspace(drec, grid)
+    L = N.lib()
+    full = int(eng._mwork.numel())
+    # the same workspace without its activation region: 64 alignment floats and max_nnz / 16 + 2 tiles of
+    # max((L - 1) x ceil(H / 16)) f4s per lane (hidden layers 1..L-1)
+    act_f4 = max((dims[0] - 1) * ((dims[1] + 15) // 16), (dims[2] - 1) * ((dims[3] + 15) // 16))
+    base = full - 64 - (mb["max_nnz"] // 16 + 2) * act_f4 * 256
+    assert base > 0
+    for b in range(min(2, P.shape[0] // B)):
+        out = []
+        for wf in (full, base):
+            g = torch.empty(eng.P, dtype=torch.float32, device=eng.device)
+            loss = torch.empty(1, dtype=torch.float64, device=eng.device)
+            N.check(L.dips_mfma_loss_grad(ctypes.byref(grid.c), ctypes.byref(drec.c), ctypes.byref(mb["batches"][b]),
+                                          ctypes.byref(eng.net), ctypes.c_float(1.0 / B), N.ptr(eng.params), N.ptr(g),
+                                          N.ptr(loss), N.ptr(eng._mwork), wf, N.ptr(eng._mdwork),
+                                          int(eng._mdwork.numel()), N.current_stream()), "dips_mfma_loss_grad")
+            torch.cuda.synchronize()
+            out.append((loss.clone(), g.clone()))
+        print(dims, b, "loss", float(out[0][0]), float(out[1][0]))
+        assert torch.equal(out[0][0], out[1][0])
+        assert torch.equal(out[0][1], out[1][1])
