@@ -129,6 +129,18 @@ __device__ __forceinline__ void tile_layer0(const float* __restrict__ W, float x
   }
 }
 
+// layer 0 from this lane's operands held in registers (w0[ob] = the packed value at ob * 64 + lane)
+template <int L, int NB>
+__device__ __forceinline__ void tile_layer0_regs(const float (&w0)[NB], float xin, float (&act)[L][4 * NB]) {
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mfma4(w0[ob], xin, c);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) act[0][ob * 4 + r] = dips_tanh_exp2(c[r]);
+  }
+}
+
 // forward of one tile: act[l][fb*4+r] feature-major; returns nothing (u by caller)
 template <int L, int NB>
 __device__ __forceinline__ void tile_forward(const float* __restrict__ W, float xin, float (&act)[L][4 * NB]) {
@@ -619,24 +631,95 @@ __global__ __launch_bounds__(256) void mfma_seed_kernel(MfmaTileArgs T, int64_t 
 constexpr int kBW = 4;              // wavefronts per backward workgroup
 constexpr int kBT = 64 * kBW;       // its threads
 
+// Transposed split-bf16 copy of the hidden matrices for the backward pass: per hidden
+// layer l, input block ib, chunk c of output blocks (2c, 2c+1) and part p one 64-lane
+// set of 16-byte units, lane 16 g + i holding W_l[16 ib + i][16 (2c + t/4) + 4 g + t%4]
+// (t = 0..7) times kTanhExp2Scale, part p: the A operand of the split-bf16 product
+// W_l d_l, whose B operand is the split delta in its feature-major registers.
+template <int L, int NB>
+struct TransBf {
+  static constexpr int NC = (NB + 1) / 2;
+  static constexpr int units() { return (L > 1 ? L - 1 : 0) * NB * NC * 3 * 64; }
+  static constexpr int size() { return 4 * units(); }  // floats
+  static constexpr int set(int l, int ib, int c, int p) { return (((l - 1) * NB + ib) * NC + c) * 3 + p; }
+};
+
+template <int L, int NB>
+__device__ void pack_trans_bf(const float* __restrict__ params, FlatNet fn, float* __restrict__ lds) {
+  using TB = TransBf<L, NB>;
+  const int Hh = fn.Hh;
+  uint32_t* wb = reinterpret_cast<uint32_t*>(lds);
+  for (int d = threadIdx.x; d < 4 * TB::units(); d += blockDim.x) {
+    const int u = d % 4, lane = (d / 4) % 64;
+    int rest = d / 256;
+    const int p = rest % 3;
+    rest /= 3;
+    const int c = rest % TB::NC;
+    rest /= TB::NC;
+    const int ib = rest % NB, l = rest / NB + 1;
+    const int g = lane / 16, in = 16 * ib + lane % 16;
+    uint32_t word = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = 2 * u + h, ob = 2 * c + t / 4;
+      const int out = 16 * ob + 4 * g + t % 4;
+      float w = 0.0f;
+      if (ob < NB && in < Hh && out < Hh) w = kTanhExp2Scale * params[fn.w(l) + in * Hh + out];
+      word |= bf_part(w, p) << (16 * h);
+    }
+    wb[d] = word;
+  }
+}
+
+// this lane's layer-0 operands (the packed value at ob * 64 + lane) and output weights (feature
+// 16 fb + 4 (lane / 16) + r at fb * 4 + r), for the backward pass without the float32 copy in LDS
+template <int NB>
+__device__ __forceinline__ void lane_weights(const float* __restrict__ params, FlatNet fn, float (&w0)[NB],
+                                             float (&wo)[4 * NB]) {
+  const int lane = threadIdx.x & 63, Hh = fn.Hh;
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) {
+    const int out = 16 * ob + lane % 16, k = lane / 16;
+    w0[ob] = out < Hh ? kTanhExp2Scale * ((k < 3) ? params[fn.w(0) + k * Hh + out] : params[fn.b(0) + out]) : 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < 4 * NB; ++q) {
+    const int h = 16 * (q / 4) + 4 * (lane / 16) + q % 4;
+    wo[q] = h < Hh ? params[fn.wout() + h] : 0.0f;
+  }
+}
+
 template <int L, int NB>
 struct CoopAcc {
   static constexpr int kW = (NB * NB + kBW - 1) / kBW;  // hidden-layer blocks owned per wavefront
   f4 dW[L > 1 ? L - 1 : 1][kW];                         // block q = wave * kW + j: (q / NB, q % NB)
   f4 dW0;                                               // layer 0, out block = wave (wave < NB)
-  float db[L][4 * NB];                                  // per-lane partial bias sums (feature-major)
+  f4 dB[L > 1 ? L - 1 : 1];                             // hidden-layer biases, out block = wave (wave < NB):
+                                                        // delta rows contracted with a row of ones (row 0)
   float dwo[4 * NB];                                    // per-lane partial output-weight sums
   float dbo;
 };
 
+// Operand stage geometry: rows of TP floats; with kPerm the rows of each group of 4
+// are rotated (row 4G + r sits at 4G + (r + G) % 4), which with TP = 16 spreads the
+// transposing writes (lanes of one write instruction hit rows 4g + r, g = 0..3)
+// over all 64 banks; TP = 20 spreads them by padding instead.
+template <int TP, bool kPerm>
+struct Stage {
+  static constexpr int kTPs = TP;
+  static __device__ __forceinline__ int row(int r) { return kPerm ? (r & ~3) | ((r + (r >> 2)) & 3) : r; }
+};
+using StagePad = Stage<kTP, false>;
+using StageRot = Stage<16, true>;
+
 // transposes a feature-major tile matrix (NB*16 features x 16 positions) through LDS
-template <int NB>
+template <int NB, class SG = StagePad>
 __device__ __forceinline__ void to_lds(float* __restrict__ buf, const float (&m)[4 * NB]) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int fb = 0; fb < NB; ++fb) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) buf[(16 * fb + 4 * (lane / 16) + r) * kTP + (lane & 15)] = m[fb * 4 + r];
+    for (int r = 0; r < 4; ++r) buf[SG::row(16 * fb + 4 * (lane / 16) + r) * SG::kTPs + (lane & 15)] = m[fb * 4 + r];
   }
 }
 
@@ -649,23 +732,28 @@ __device__ __forceinline__ float row16_sum(float v) {
   return v;
 }
 
-// D += A^T B over the kBW staged tiles of the workgroup: A rows arow.., B rows brow..
-// The 4 contraction steps of lane group kq take positions 4 kq .. 4 kq + 3 of a
-// tile, one 16-byte LDS read per operand (rows are kTP = 20 floats, 16-byte aligned).
+// D += A^T B over the kBW staged tiles of the workgroup: A rows arow.. (arow < 0: a row
+// of ones and 15 rows of zeros), B rows brow..  The 4 contraction steps of lane group kq
+// take positions 4 kq .. 4 kq + 3 of a tile, one 16-byte LDS read per operand (rows of
+// SG::kTPs floats, 16-byte aligned).
+template <class SG = StagePad>
 __device__ __forceinline__ f4 coop_block(const float* __restrict__ stage, int stride_w, int arow, int brow, bool a_inputs,
                                          f4 acc) {
   const int lane = threadIdx.x & 63;
   const int m = lane & 15, kq = lane / 16;
+  constexpr int TP = SG::kTPs;
 #pragma unroll
   for (int s = 0; s < kBW; ++s) {
     const float* A = stage + s * stride_w;
     const float* D = A + stride_w / 2;
-    const f4 b4 = *reinterpret_cast<const f4*>(&D[(brow + m) * kTP + 4 * kq]);
+    const f4 b4 = *reinterpret_cast<const f4*>(&D[SG::row(brow + m) * TP + 4 * kq]);
     f4 a4;
-    if (a_inputs)
-      a4 = m < 4 ? *reinterpret_cast<const f4*>(&A[m * kTP + 4 * kq]) : f4{0.f, 0.f, 0.f, 0.f};
+    if (arow < 0)  // a row of ones (row 0) and zeros: the delta rows summed over positions
+      a4 = m == 0 ? f4{1.f, 1.f, 1.f, 1.f} : f4{0.f, 0.f, 0.f, 0.f};
+    else if (a_inputs)
+      a4 = m < 4 ? *reinterpret_cast<const f4*>(&A[SG::row(m) * TP + 4 * kq]) : f4{0.f, 0.f, 0.f, 0.f};
     else
-      a4 = *reinterpret_cast<const f4*>(&A[(arow + m) * kTP + 4 * kq]);
+      a4 = *reinterpret_cast<const f4*>(&A[SG::row(arow + m) * TP + 4 * kq]);
 #pragma unroll
     for (int st = 0; st < 4; ++st) acc = mfma4(a4[st], b4[st], acc);
   }
@@ -686,8 +774,6 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
 #pragma unroll
   for (int q = 0; q < 4 * NB; ++q) {
     G.dwo[q] = row16_sum(G.dwo[q]);
-#pragma unroll
-    for (int l = 1; l < L; ++l) G.db[l][q] = row16_sum(G.db[l][q]);
   }
   G.dbo = row16_sum(G.dbo);
   // the delta of hidden layer l carries kTanhExp2Scale^(L-1-l) (weights packed times the factor): inv[l]
@@ -721,6 +807,10 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
           if (row < 3 && o < Hh) red[fn.w(0) - base + row * Hh + o] = G.dW0[r] * inv[0];
           if (row == 3 && o < Hh) red[fn.b(0) - base + o] = G.dW0[r] * inv[0];
         }
+        if (lane < 16 && 16 * wave + lane < Hh) {
+#pragma unroll
+          for (int l = 1; l < L; ++l) red[fn.b(l) - base + 16 * wave + lane] = G.dB[l - 1][0] * inv[l];
+        }
       }
       if ((lane & 15) == 15) {
 #pragma unroll
@@ -730,8 +820,6 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
             const int h = 16 * fb + 4 * (lane / 16) + r;
             if (h < Hh) {
               red[fn.wout() - base + h] += G.dwo[fb * 4 + r];
-#pragma unroll
-              for (int l = 1; l < L; ++l) red[fn.b(l) - base + h] += G.db[l][fb * 4 + r] * inv[l];
             }
           }
         }
@@ -743,12 +831,19 @@ __device__ void flush_coop(CoopAcc<L, NB>& G, FlatNet fn, float* __restrict__ re
   for (int i = threadIdx.x; i < P; i += kBT) out[i] = red[i];
 }
 
-template <int L, int NB>
-__device__ __forceinline__ void coop_backward(const float* __restrict__ W, float seed, float xin,
-                                              const float (&act)[L][4 * NB], float* __restrict__ stage, int stride_w,
-                                              CoopAcc<L, NB>& G) {
+// kBf: W is the TransBf copy (W_l d_l as split bf16 products), the output weights come in wor and the
+// stage rows are 16 floats with rotated row groups; else W is the float32 Packed copy (float32 MFMA) and
+// the stage rows are padded to kTP floats
+// prefetch() is called once per tile, as soon as act is no longer read (after the backprop through
+// layer 1, or at the start for L = 1), so that loads for the next tile can take its registers and their
+// latency overlaps the remaining contractions
+template <int L, int NB, bool kBf, class Prefetch>
+__device__ __forceinline__ void coop_backward(const float* __restrict__ W, const float (&wor)[4 * NB], float seed,
+                                              float xin, const float (&act)[L][4 * NB], float* __restrict__ stage,
+                                              int stride_w, CoopAcc<L, NB>& G, Prefetch&& prefetch) {
   using PK = Packed<L, NB>;
   using CA = CoopAcc<L, NB>;
+  using SG = std::conditional_t<kBf, StageRot, StagePad>;
   const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
   float* bufA = stage + wave * stride_w;
   float* bufD = bufA + stride_w / 2;
@@ -759,42 +854,75 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float a = act[L - 1][fb * 4 + r];
-      const float wo = W[PK::wout() + 16 * fb + 4 * (lane / 16) + r];
+      float wo;
+      if constexpr (kBf)
+        wo = wor[fb * 4 + r];
+      else
+        wo = W[PK::wout() + 16 * fb + 4 * (lane / 16) + r];
       G.dwo[fb * 4 + r] = fmaf(a, seed, G.dwo[fb * 4 + r]);
       d[fb * 4 + r] = seed * wo * (1.0f - a * a);
     }
   }
   if (lane < 16) G.dbo += seed;
+  if constexpr (L == 1) prefetch();
 #pragma unroll
   for (int l = L - 1; l >= 0; --l) {
-    if (l > 0) {  // b_0 is row 3 of the layer-0 weight gradient
-#pragma unroll
-      for (int q = 0; q < 4 * NB; ++q) G.db[l][q] += d[q];
-    }
     // stage this tile's operands: delta rows, and activation rows (or the inputs x, y, z, 1)
-    to_lds<NB>(bufD, d);
+    to_lds<NB, SG>(bufD, d);
     if (l > 0)
-      to_lds<NB>(bufA, act[l - 1]);
+      to_lds<NB, SG>(bufA, act[l - 1]);
     else
-      bufA[(lane / 16) * kTP + (lane & 15)] = xin;
+      bufA[SG::row(lane / 16) * SG::kTPs + (lane & 15)] = xin;
     if (l > 0) {
       // d_{l-1} = (W_l d_l) * (1 - a_{l-1}^2), from this tile's registers (the contraction below reads the
       // staged copy of d_l), before the barrier: a wavefront that arrives early does this matrix work while
-      // it waits.  The A operand of step r over output block fb is W_l[16 ib + m][16 fb + 4 kq + r]
-      // (m = lane%16, kq = lane/16), the float m%4 of unit 16 (m/4) + 4 kq + r of block (fb, ib) -- 4
-      // floats 4 apart from this lane's offset
-      const int m = lane & 15, kq = lane >> 4;
-      const int tofs = 4 * (17 * (m >> 2) + 4 * kq) + (m & 3);
+      // it waits.
       f4 c[NB];
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib) c[ib] = f4{0.f, 0.f, 0.f, 0.f};
+      if constexpr (kBf) {
+        // split-bf16: chunk cc of d_l (blocks 2cc, 2cc+1) is the B operand, split like the forward activations
+        using TB = TransBf<L, NB>;
+        const u4* __restrict__ tb = reinterpret_cast<const u4*>(W);
 #pragma unroll
-      for (int fb = 0; fb < NB; ++fb) {
+        for (int cc = 0; cc < TB::NC; ++cc) {
+          u4 dh, dm, dl;
 #pragma unroll
-        for (int ib = 0; ib < NB; ++ib) {
-          const float* wt = W + PK::fwd(l) + (fb * NB + ib) * PK::kBlk + tofs;
+          for (int u = 0; u < 4; ++u) {
+            const int fb = 2 * cc + u / 2, r = (2 * u) % 4;
+            uint32_t h, mm, lo;
+            bf_split_pair(fb < NB ? d[fb * 4 + r] : 0.0f, fb < NB ? d[fb * 4 + r + 1] : 0.0f, h, mm, lo);
+            dh[u] = h;
+            dm[u] = mm;
+            dl[u] = lo;
+          }
 #pragma unroll
-          for (int r = 0; r < 4; ++r) c[ib] = mfma4(wt[4 * r], d[fb * 4 + r], c[ib]);
+          for (int ib = 0; ib < NB; ++ib) {
+            const u4 w0 = tb[TB::set(l, ib, cc, 0) * 64 + lane];
+            const u4 w1 = tb[TB::set(l, ib, cc, 1) * 64 + lane];
+            const u4 w2 = tb[TB::set(l, ib, cc, 2) * 64 + lane];
+            c[ib] = mfma_bf(w2, dh, c[ib]);
+            c[ib] = mfma_bf(w1, dm, c[ib]);
+            c[ib] = mfma_bf(w0, dl, c[ib]);
+            c[ib] = mfma_bf(w1, dh, c[ib]);
+            c[ib] = mfma_bf(w0, dm, c[ib]);
+            c[ib] = mfma_bf(w0, dh, c[ib]);
+          }
+        }
+      } else {
+        // float32: the A operand of step r over output block fb is W_l[16 ib + m][16 fb + 4 kq + r]
+        // (m = lane%16, kq = lane/16), the float m%4 of unit 16 (m/4) + 4 kq + r of block (fb, ib) -- 4
+        // floats 4 apart from this lane's offset
+        const int m = lane & 15, kq = lane >> 4;
+        const int tofs = 4 * (17 * (m >> 2) + 4 * kq) + (m & 3);
+#pragma unroll
+        for (int fb = 0; fb < NB; ++fb) {
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+            const float* wt = W + PK::fwd(l) + (fb * NB + ib) * PK::kBlk + tofs;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) c[ib] = mfma4(wt[4 * r], d[fb * 4 + r], c[ib]);
+          }
         }
       }
 #pragma unroll
@@ -805,33 +933,42 @@ __device__ __forceinline__ void coop_backward(const float* __restrict__ W, float
           d[ib * 4 + r] = c[ib][r] * (1.0f - a * a);
         }
       }
+      if (l == 1) prefetch();
     }
     __syncthreads();
     if (l > 0) {
 #pragma unroll
       for (int j = 0; j < CA::kW; ++j) {
         const int q = wave * CA::kW + j;
-        if (q < NB * NB) G.dW[l - 1][j] = coop_block(stage, stride_w, 16 * (q / NB), 16 * (q % NB), false, G.dW[l - 1][j]);
+        if (q < NB * NB)
+          G.dW[l - 1][j] = coop_block<SG>(stage, stride_w, 16 * (q / NB), 16 * (q % NB), false, G.dW[l - 1][j]);
       }
-    } else if (wave < NB) {
-      G.dW0 = coop_block(stage, stride_w, 0, 16 * wave, true, G.dW0);
+      // the bias gradient of out block `wave`: the staged deltas against a row of ones
+      if (wave < NB) G.dB[l - 1] = coop_block<SG>(stage, stride_w, -1, 16 * wave, false, G.dB[l - 1]);
+    } else if (wave < NB) {  // b_0 is row 3 of the layer-0 weight gradient
+      G.dW0 = coop_block<SG>(stage, stride_w, 0, 16 * wave, true, G.dW0);
     }
     __syncthreads();  // operands consumed: the buffers may be restaged
   }
 }
 
-// LDS of the backward kernel (floats): the float32 packed weights (PKsz), with kStored false the forward
-// pass's split-bf16 copy where it differs from them (FPsz), then the operand stage
+// LDS of the backward kernel for one phase net (floats): its backprop weights (the TransBf copy from two
+// 16-wide blocks on, the float32 Packed copy below), with kStored false the forward pass's split-bf16 copy
+// (PackedBf, where the forward pass uses one), then the operand stage of the kBW wavefronts (rows of 16
+// floats, rotated, with TransBf; of kTP floats with Packed).  3x64 with kStored: 48 KB + 32 KB = 80 KB,
+// two workgroups per CU.
+template <int L, int NB, bool kStored>
+struct BwdPhase {
+  static constexpr bool kBf = FwdPack<L, NB>::kBf;
+  static constexpr int wr() { return ((kBf ? TransBf<L, NB>::size() : Packed<L, NB>::size()) + 3) & ~3; }
+  static constexpr int fp() { return (!kStored && kBf) ? ((PackedBf<L, NB>::size() + 3) & ~3) : 0; }
+  static constexpr int stride() { return 2 * 16 * NB * (kBf ? StageRot::kTPs : StagePad::kTPs); }
+  static constexpr int floats() { return wr() + fp() + kBW * stride(); }
+};
 template <int Lm, int NBm, int Lp, int NBp, bool kStored>
 struct BwdLds {
-  static constexpr int PKsz = ((Packed<Lp, NBp>::size() > Packed<Lm, NBm>::size() ? Packed<Lp, NBp>::size()
-                                                                                  : Packed<Lm, NBm>::size()) + 3) & ~3;
-  static constexpr int FPp = FwdPack<Lp, NBp>::kBf ? PackedBf<Lp, NBp>::size() : 0;
-  static constexpr int FPm = FwdPack<Lm, NBm>::kBf ? PackedBf<Lm, NBm>::size() : 0;
-  static constexpr int FPsz = kStored ? 0 : (((FPp > FPm ? FPp : FPm) + 3) & ~3);
-  static constexpr int NBx = NBp > NBm ? NBp : NBm;
-  static constexpr int kStrideW = 2 * 16 * NBx * kTP;  // floats of one wavefront's operand pair
-  static constexpr int floats() { return PKsz + FPsz + kBW * kStrideW; }
+  static constexpr int kP = BwdPhase<Lp, NBp, kStored>::floats(), kM = BwdPhase<Lm, NBm, kStored>::floats();
+  static constexpr int floats() { return kP > kM ? kP : kM; }
 };
 
 // kStored: the activations of every tile come from the forward kernel (act_in, act_f4 f4s per lane and
@@ -844,14 +981,9 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
                                                        int nblk_p, int stride, const f4* __restrict__ act_in,
                                                        int act_f4) {
   extern __shared__ __align__(16) float lds[];
-  using BL = BwdLds<Lm, NBm, Lp, NBp, kStored>;
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool plus = (int)blockIdx.x < nblk_p;
   const int PPp = 4 * Hp + (Lp - 1) * (Hp * Hp + Hp) + Hp + 1;
-  // the forward copy and the operand stage start on 16-byte boundaries after the packed weights
-  constexpr int kStrideW = BL::kStrideW;
-  float* fwdW = lds + BL::PKsz;
-  float* stage = lds + BL::PKsz + BL::FPsz;
   const int64_t ntp = (np_ + 15) / 16;
   const int64_t ntiles = plus ? (np_ + 15) / 16 : (nm_ + 15) / 16;
   const int nblk = plus ? nblk_p : (int)gridDim.x - nblk_p;
@@ -863,8 +995,16 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
     constexpr int L = decltype(tagL)::value;
     constexpr int NB = decltype(tagNB)::value;
     using CA = CoopAcc<L, NB>;
-    pack_weights<L, NB>(params, fn, lds);
-    if constexpr (!kStored && FwdPack<L, NB>::kBf) pack_weights_bf<L, NB>(params, fn, fwdW);
+    using BP = BwdPhase<L, NB, kStored>;
+    // the forward copy and the operand stage start on 16-byte boundaries after the backprop weights
+    float* fwdW = lds + BP::wr();
+    float* stage = lds + BP::wr() + BP::fp();
+    if constexpr (BP::kBf) {
+      pack_trans_bf<L, NB>(params, fn, lds);
+      if constexpr (!kStored) pack_weights_bf<L, NB>(params, fn, fwdW);
+    } else {
+      pack_weights<L, NB>(params, fn, lds);
+    }
     __syncthreads();
     CA G;
 #pragma unroll
@@ -873,9 +1013,7 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
       for (int q = 0; q < CA::kW; ++q) G.dW[l][q] = f4{0.f, 0.f, 0.f, 0.f};
     G.dW0 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int l = 0; l < L; ++l)
-#pragma unroll
-      for (int q = 0; q < 4 * NB; ++q) G.db[l][q] = 0.0f;
+    for (int l = 0; l < (L > 1 ? L - 1 : 1); ++l) G.dB[l] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < 4 * NB; ++q) G.dwo[q] = 0.0f;
     G.dbo = 0.0f;
@@ -890,6 +1028,23 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
     auto seed_of = [&](int64_t item) { return item < ee ? seed_item[item - e0] : 0.0f; };
     float xin_next = item_input(T, item_of(0), ee, kq);
     float seed_next = seed_of(item_of(0));
+    // the stored activations of a round's tile (layers 1..L-1), loaded for round rd + 1 inside round rd
+    // once its activations are dead
+    float act_n[L][4 * NB];
+    auto load_round = [&](int64_t rd) {
+      if constexpr (kStored) {
+        const int64_t tile = (rd * nblk + bid) * kBW + wave;
+        if (tile < ntiles) {
+          load_act<L, NB>(act_in + ((plus ? tile : ntp + tile) * act_f4) * 64 + lane, act_n);
+        } else {
+#pragma unroll
+          for (int l = 1; l < L; ++l)
+#pragma unroll
+            for (int q = 0; q < 4 * NB; ++q) act_n[l][q] = 0.0f;
+        }
+      }
+    };
+    load_round(0);
     for (int64_t rd = 0; rd < rounds; ++rd) {
       const float xin = xin_next, seed = seed_next;
       if (rd + 1 < rounds) {
@@ -898,23 +1053,27 @@ __global__ __launch_bounds__(kBT, 2) void mfma_bwd_kernel(MfmaTileArgs T, const 
         seed_next = seed_of(nx);
       }
       float act[L][4 * NB];
+      // with TransBf: layer 0's operands and the output weights of this lane, loaded per round (cached
+      // reads of the parameter vector; held across rounds they would spill)
+      float w0r[NB], wor[4 * NB];
+      if constexpr (BP::kBf) lane_weights<NB>(params, fn, w0r, wor);
       if constexpr (kStored) {
-        const int64_t tile = (rd * nblk + bid) * kBW + wave;
-        if (tile < ntiles) {
-          load_act<L, NB>(act_in + ((plus ? tile : ntp + tile) * act_f4) * 64 + lane, act);
-        } else {
 #pragma unroll
-          for (int l = 1; l < L; ++l)
+        for (int l = 1; l < L; ++l)
 #pragma unroll
-            for (int q = 0; q < 4 * NB; ++q) act[l][q] = 0.0f;
-        }
-        tile_layer0<L, NB>(lds, xin, act);
-      } else if constexpr (FwdPack<L, NB>::kBf) {
+          for (int q = 0; q < 4 * NB; ++q) act[l][q] = act_n[l][q];
+        if constexpr (BP::kBf)
+          tile_layer0_regs<L, NB>(w0r, xin, act);
+        else
+          tile_layer0<L, NB>(lds, xin, act);
+      } else if constexpr (BP::kBf) {
         tile_forward_bf<L, NB>(fwdW, xin, act);
       } else {
         tile_forward<L, NB>(lds, xin, act);
       }
-      coop_backward<L, NB>(lds, seed, xin, act, stage, kStrideW, G);
+      coop_backward<L, NB, BP::kBf>(lds, wor, seed, xin, act, stage, BP::stride(), G, [&] {
+        if (rd + 1 < rounds) load_round(rd + 1);
+      });
     }
     __syncthreads();  // all waves done with the packed weights: reuse LDS for the reduction
     flush_coop<L, NB>(G, fn, lds, out);
