@@ -343,7 +343,11 @@ typedef struct {
 /* 1 if (Lm, ceil(Hm/16), Lp, ceil(Hp/16)) has an MFMA kernel. */
 int dips_mfma_supported(const dips_net_t* net);
 /* float workspace of dips_mfma_train_step for batches of up to max_rows rows
- * and max_nnz entries (the float64 workspace needs rows/256 + P/64 + 2 doubles). */
+ * and max_nnz entries (the float64 workspace needs rows/256 + P/64 + 2 doubles).
+ * It includes the forward tiles' hidden activations of layers 1..L-1, which the
+ * backward then reads (512 B per item at 3x64, max_nnz/16 + 2 tiles); a smaller
+ * workspace that still holds the rest (u, r, the gradient partials, g) makes the
+ * backward recompute them instead -- slower, the same bits. */
 int64_t dips_mfma_workspace(const dips_net_t* net, int64_t max_rows, int64_t max_nnz);
 /* One optimizer step of the batch through the MFMA kernels (forward tiles,
  * row sums, backward tiles with MFMA weight gradients, fixed-order reduction,
@@ -356,9 +360,11 @@ int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
  * no counterpart in the reference; for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops).  The workspace is
  * a fixed device array of the library (8 Mi floats + 8 Ki doubles, 32.1 MiB,
- * one copy per device, held until the process ends): a batch needing more
- * (dips_mfma_workspace floats, rows/256 + P/64 + 1 doubles; e.g. 2^20-row
- * batches) returns DIPS_EINVAL.  The device of `stream` must be current.
+ * one copy per device, held until the process ends): a batch whose workspace
+ * without the stored activations (u per entry, r per row, gradient partials, g)
+ * or float64 partials (rows/256 + P/64 + 1) need more returns DIPS_EINVAL (e.g.
+ * 2^20-row batches); the backward recomputes the activations where the array
+ * has no room for them.  The device of `stream` must be current.
  * Calls on one device share the array: calls that overlap on one device race.
  * Same results as dips_mfma_train_step. */
 int dips_mfma_train_step_noalloc(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
