@@ -18,7 +18,10 @@ error of the trained network against the manufactured solution
 (tests/test_poisson.py:277 of the reference computes the same norms).
 
 --workload sphere: the 128^3 sphere (phi = |x| - 0.5, mu 1/10, k = 0), MLP
-1x1/2x10, batch 65536.  Other workloads: sweep, pb, hashgrid (see --help).
+1x1/2x10, batch 65536.  --workload lpbe: the reference's linearized
+Poisson-Boltzmann benchmark with a known answer (one charged sphere of radius 1
+in [-2.5,2.5]^3, eps 2/80, examples/benchmark_LPBE), 128^3, MLP 3x64, batch
+262144.  Other workloads: sweep, pb, hashgrid (see --help).
 
 Prints one JSON line.  --gpus N > 1 exits: this build targets one GPU.
 """
@@ -73,7 +76,7 @@ def main():
     ap.add_argument("--accuracy-epochs", type=int, default=-1,
                     help="epochs of the end-to-end drop-in trainer run whose error against the manufactured "
                          "solution the line reports (-1: the workload's schedule, 2000; 0: skip)")
-    ap.add_argument("--workload", default="star", choices=["sphere", "star", "sweep", "pb", "hashgrid"],
+    ap.add_argument("--workload", default="star", choices=["sphere", "star", "sweep", "pb", "lpbe", "hashgrid"],
                     help="star: 3x64/3x64 MLP, batch 262144 (default); sphere: 1x1/2x10 MLP, batch 65536; "
                          "sweep: 64 problems of 32^3..96^3 with 2x16/2x16 MLPs, batched vs one at a time; "
                          "pb: Poisson-Boltzmann box of 2000 atoms at 256^3, 3x64/3x64 MLP, batch 2^20; "
@@ -130,6 +133,15 @@ def main():
         prob = problems.pb_box(n_atoms=2000, half_width=20.0, seed=0)
         lo, hi = -20.0, 20.0
         n, B = args.n or 256, B or (1 << 20)
+        mlp = {"hidden_layers_m": 3, "hidden_dim_m": 64, "activation_m": "jnp.tanh",
+               "hidden_layers_p": 3, "hidden_dim_p": 64, "activation_p": "jnp.tanh"}
+    elif args.workload == "lpbe":
+        # the reference's LPBE benchmark (one charged sphere, known answer) on [-2.5, 2.5]^3; beta's normal
+        # from central differences of phi with the training spacing, as the reference computes it
+        lo, hi = -2.5, 2.5
+        xs = problems.grid_axes(n, lo, hi)[0]
+        prob = problems.lpbe_sphere(h=float(xs[1] - xs[0]))
+        B = B or 262144
         mlp = {"hidden_layers_m": 3, "hidden_dim_m": 64, "activation_m": "jnp.tanh",
                "hidden_layers_p": 3, "hidden_dim_p": 64, "activation_p": "jnp.tanh"}
     elif args.workload == "star":

@@ -488,6 +488,71 @@ def pb_box(n_atoms=2000, half_width=20.0, seed=0, mu_m=2.0, mu_p=80.0, kappa2=0.
     )
 
 
+def lpbe_sphere(h, sigma=1.0, charge=1.0, centre=(0.0, 0.0, 0.0), half_width=2.5):
+    """The reference's linearized Poisson-Boltzmann benchmark with a known answer
+    (examples/benchmark_LPBE: one charged sphere, inputs/sphere/single.pqr,
+    radius 1 and charge +1 at the centre of the box [-2.5, 2.5]^3, main.py:139-140).
+
+    Dimensionless constants (units.py:1-16): eps- = alpha_1 = 2, eps+ = alpha_2 = 80,
+    kappa = sqrt(8.4869 Is / alpha_2) with Is = 1e-5, k- = 0, k+ = alpha_2 kappa^2,
+    omega = 24.  With the point charge treated apart (u- = potential - g in the
+    sphere, g = omega / (4 pi alpha_1) q / |x - c|, coefficients.py:125-163):
+    f- = f+ = 0 (coefficients.py:294-301), alpha = [u] = g (coefficients.py:241-246)
+    and beta = alpha_1 (-grad g) . n with n the level-set normal from central
+    differences of phi with the training spacing h (coefficients.py:174-194,
+    :221-252; -(mu+ grad u+ - mu- grad u-) . n in the reference's sign), the
+    Dirichlet value omega / (4 pi alpha_2) q exp(kappa (sigma - d)) / (d (1 + kappa
+    sigma)) at distance d (coefficients.py:35-52).  Exact solution
+    (coefficients.py:307-324): u- = omega / (4 pi sigma) (1 / (alpha_2 (1 + kappa
+    sigma)) - 1 / alpha_1), constant, and u+ = the Dirichlet expression."""
+    alpha_1, alpha_2, omega = 2.0, 80.0, 24.0
+    kappa = (8.4869 * 1e-5 / alpha_2) ** 0.5
+    cx, cy, cz = centre
+    g_coeff = omega / (4.0 * np.pi * alpha_1)
+    bc_coeff = omega / (4.0 * np.pi * alpha_2)
+
+    def dist(r):
+        return xnp.sqrt((r[0] - cx) ** 2 + (r[1] - cy) ** 2 + (r[2] - cz) ** 2)
+
+    def phi_raw(r):
+        return dist(r) - sigma
+
+    phi = perturb(phi_raw)
+
+    def g(r):
+        return xnp.nan_to_num(g_coeff * charge / dist(r))
+
+    def beta(r):
+        # minus grad g = g_coeff q (x - c) / d^3, against the normal of phi from central differences of step h
+        d = dist(r)
+        mg = [xnp.nan_to_num(g_coeff * charge * (r[i] - c) / d ** 3) for i, c in enumerate(centre)]
+        nrm = []
+        for i in range(3):
+            rp = [r[j] + (h if j == i else 0.0) for j in range(3)]
+            rm = [r[j] - (h if j == i else 0.0) for j in range(3)]
+            nrm.append((phi(rp) - phi(rm)) / (2.0 * h))
+        nn = xnp.sqrt(nrm[0] ** 2 + nrm[1] ** 2 + nrm[2] ** 2)
+        return alpha_1 * sum(mg[i] * nrm[i] / nn for i in range(3))
+
+    def up(r):
+        d = dist(r)
+        return xnp.nan_to_num(bc_coeff * charge * xnp.exp(kappa * (sigma - d)) / (d * (1.0 + kappa * sigma)))
+
+    u_in = omega / (4.0 * np.pi * sigma) * (1.0 / (alpha_2 * (1.0 + kappa * sigma)) - 1.0 / alpha_1)
+
+    def um(r):
+        return xnp.zeros_like(r[0]) + u_in
+
+    def exact(r):
+        return xnp.where(phi(r) >= 0, up(r), um(r))
+
+    zero = _const(0.0)
+    return dict(name="lpbe", initial_value_fn=zero, dirichlet_bc_fn=up, lvl_set_fn=phi, mu_m_fn=_const(alpha_1),
+                mu_p_fn=_const(alpha_2), k_m_fn=_const(0.0), k_p_fn=_const(alpha_2 * kappa ** 2), f_m_fn=zero,
+                f_p_fn=zero, alpha_fn=g, beta_fn=beta, exact_fn=exact, exact_m_fn=um, exact_p_fn=up,
+                phi_raw_fn=phi_raw, kappa=kappa, half_width=half_width)
+
+
 SWEEP_SIZES = (32, 48, 64, 96)
 
 

@@ -31,6 +31,8 @@ def _problem(workload):
         return problems.sphere(radius=0.5, mu_m=1.0, mu_p=10.0), None
     if workload == "pb":
         return problems.pb_box(n_atoms=2000, half_width=20.0, seed=0), (3, 64, 3, 64)
+    if workload == "lpbe":
+        return problems.lpbe_sphere(h=5.0 / 127), (3, 64, 3, 64)
     if workload == "star":
         dims = (3, 64, 3, 64)
         return problems.star(r0=0.5, amp=0.1, lobes_t=5, lobes_p=3, mu_m=1.0, mu_p=100.0, k_m=1.0, k_p=1.0), dims
@@ -60,7 +62,8 @@ def _worker(args):
         model = DoubleMLP(mlp=mlp)
     theta = model.flatten(model.init(42))
     cb = D.Callbacks(problems.oracle_callbacks(prob), np.float32)
-    x, y, z = problems.grid_axes(n, -20.0, 20.0) if workload == "pb" else problems.grid_axes(n)
+    half = {"pb": 20.0, "lpbe": 2.5}.get(workload, 1.0)
+    x, y, z = problems.grid_axes(n, -half, half)
     h0 = np.array([x[1] - x[0], y[1] - y[0], z[1] - z[0]], np.float32)
     bounds = (x.min(), x.max(), y.min(), y.max(), z.min(), z.max())
     X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
@@ -77,7 +80,7 @@ def _worker(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "pb", "hashgrid"])
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "star", "pb", "lpbe", "hashgrid"])
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--seconds", type=float, default=15.0)
     ap.add_argument("--batch", type=int, default=2048)

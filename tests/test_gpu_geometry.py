@@ -210,6 +210,45 @@ def test_pb_box_records_match_oracle(level):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("level", [0, 2])
+def test_lpbe_sphere_records_match_oracle(level):
+    """The reference's LPBE benchmark (one charged sphere, Coulomb jump alpha,
+    flux jump beta from the finite-difference normal, mu 2 / 80, k+ = 80 kappa^2,
+    Debye-Hueckel boundary values) at 20^3 on [-2.5, 2.5]^3: device records
+    (callbacks sampled through torch on the device) equal the oracle's, and the
+    MFMA kernels give the oracle's loss and gradient on them."""
+    from jax_dips.level_records import build_level
+    from jax_dips.records import DeviceGrid, Engine
+    from oracle import discretization as D
+    from oracle import mlp as M
+    from oracle import trainer as T
+
+    axes, P, h, bounds = grid_points(20, -2.5, 2.5)
+    prob = problems.lpbe_sphere(float(h[0]))
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    hl = T.level_spacing(h, level)
+    orec = T.LevelRecords(P, hl, bounds, cb, np.float64)
+    grid = DeviceGrid(*axes)
+    n = P.shape[0]
+    drec = build_level(grid, hl, bounds, problems.oracle_callbacks(prob), n)
+    W, C = drec.to_dense()
+    scale = np.maximum(np.abs(orec.W).max(axis=1), 1.0)
+    err = np.abs(W - orec.W).max(axis=1) / scale
+    cerr = np.abs(C - orec.c) / np.maximum(np.abs(orec.c), scale)
+    print("lpbe", level, drec.stats, "max w err", err.max(), "max c err", cerr.max())
+    assert drec.stats["nip"] > 0
+    assert err.max() <= 1e-6 and cerr.max() <= 1e-6
+    dims = (3, 64, 3, 64)
+    theta = M.init_params(dims, seed=13)
+    eng = Engine(dims, theta, {}, max_rows=n, max_steps=2)
+    loss, grad = eng.mfma_loss_grad(grid, drec, 0, 1.0 / n)
+    ol, og, _ = T.loss_and_grad(theta.astype(np.float64), dims, orec, np.arange(n))
+    print(f"loss gpu {loss:.9e} oracle {ol:.9e} grad rel {np.linalg.norm(grad - og) / np.linalg.norm(og):.2e}")
+    assert abs(loss - ol) <= 1e-5 * abs(ol)
+    assert np.linalg.norm(grad - og) <= 1e-4 * np.linalg.norm(og)
+
+
+@pytest.mark.gpu
 def test_atom_field_kernel_matches_host():
     """dips_atoms_eval (cell list on the device, float64) gives the host cell
     list's phi bit for bit (same subtraction order, no contraction) and its
