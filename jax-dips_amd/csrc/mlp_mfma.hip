@@ -536,46 +536,69 @@ __device__ __forceinline__ int64_t xcd_chunk(int64_t n) {
 
 // ---- kernel 2: r per row, loss partials ---------------------------------------
 // u of CSR entry e: u[e] (per-entry values, e_item null), else u[e_item[e]] (per-item values).
-// Launched on xcd_grid(rowsum_blocks(nrows)) workgroups; loss_part[chunk] per chunk of kMT rows.
-__global__ __launch_bounds__(kMT) void mfma_rowsum_kernel(const int32_t* __restrict__ row_ptr,
+// Launched on xcd_grid(rowsum_blocks(nrows)) workgroups of kRST threads; loss_part[chunk] per chunk of
+// kMT rows.
+constexpr int kRST = 1024;
+__global__ __launch_bounds__(kRST) void mfma_rowsum_kernel(const int32_t* __restrict__ row_ptr,
                                                           const float* __restrict__ row_c, int64_t row_begin,
                                                           int64_t nrows, const float* __restrict__ ent_w, int64_t nnz,
                                                           const float* __restrict__ u, const int32_t* __restrict__ e_item,
                                                           float* __restrict__ r_out, double* __restrict__ loss_part) {
-  const int64_t chunk = xcd_chunk((nrows + kMT - 1) / kMT);
-  if (chunk >= (nrows + kMT - 1) / kMT) return;  // the whole workgroup
-  const int64_t i = chunk * kMT + threadIdx.x;
+  const int64_t nch = (nrows + kMT - 1) / kMT;
+  const int64_t chunk = xcd_chunk(nch);
+  if (chunk >= nch) return;  // the whole workgroup
+  // kRowLanes lanes per row (its entries read contiguously across them, so that the long rows of
+  // interface points do not stall a whole wavefront and every load instruction is coalesced),
+  // kRST / kRowLanes rows per pass, kMT rows per workgroup
+  constexpr int kRowLanes = 8, kPer = 2, kRowsPass = kRST / kRowLanes;
+  static_assert(kMT % kRowsPass == 0, "whole passes");
+  const int sub = threadIdx.x % kRowLanes, slot = threadIdx.x / kRowLanes;
   double l = 0.0;
-  if (i < nrows) {
-    const int64_t row = row_begin + i;
-    const int beg = max(row_ptr[row], 0);
-    const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
-    float r = row_c[row];
-    // entries in chunks of 8: every load of a chunk is issued before the first use (the gathers through
-    // e_item depend on one load, not on the previous entry), the sum keeps the entry order
-    for (int e0 = beg; e0 < end; e0 += 8) {
-      float w[8], uv[8];
-      int it[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const bool in = e0 + j < end;
-        w[j] = in ? ent_w[e0 + j] : 0.0f;
-        it[j] = in ? (e_item ? e_item[e0 + j] : e0 + j) : -1;
+  for (int pass = 0; pass < kMT / kRowsPass; ++pass) {
+    const int64_t i = chunk * kMT + pass * kRowsPass + slot;
+    if (i < nrows) {  // uniform over the kRowLanes lanes of a row
+      const int64_t row = row_begin + i;
+      const int beg = max(row_ptr[row], 0);
+      const int end = (int)min((int64_t)row_ptr[row + 1], nnz);
+      // lane `sub` sums entries beg + sub + kRowLanes j in order; every load of an iteration is issued
+      // before the first use (the gathers through e_item depend on one load, not on the previous entry)
+      float acc = 0.0f;
+      for (int e0 = beg + sub; e0 < end; e0 += kRowLanes * kPer) {
+        float w[kPer], uv[kPer];
+        int it[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+          const int e = e0 + kRowLanes * j;
+          w[j] = e < end ? ent_w[e] : 0.0f;
+          it[j] = e < end ? (e_item ? e_item[e] : e) : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) uv[j] = it[j] >= 0 ? u[it[j]] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+          if (it[j] >= 0) acc = fmaf(w[j], uv[j], acc);
       }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) uv[j] = it[j] >= 0 ? u[it[j]] : 0.0f;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (e0 + j < end) r = fmaf(w[j], uv[j], r);
+      // the lanes' partial sums in a fixed tree (addition is commutative: all lanes of the row agree)
+      acc += __shfl_xor(acc, 1, 64);
+      acc += __shfl_xor(acc, 2, 64);
+      acc += __shfl_xor(acc, 4, 64);
+      const float r = row_c[row] + acc;
+      if (sub == 0) {
+        r_out[i] = r;
+        l += 0.5 * (double)r * (double)r;
+      }
     }
-    r_out[i] = r;
-    l = 0.5 * (double)r * (double)r;
   }
   l = dips_wave_sum_d(l);
-  __shared__ double red[kMT / 64];
+  __shared__ double red[kRST / 64];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = l;
   __syncthreads();
-  if (threadIdx.x == 0) loss_part[chunk] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kRST / 64; ++w) t += red[w];
+    loss_part[chunk] = t;
+  }
 }
 
 // ---- kernel 2b: seed per item ----------------------------------------------------
@@ -1353,7 +1376,7 @@ __global__ __launch_bounds__(256) void mfma_loss_out_kernel(const double* __rest
 void rowsum_launch(const int32_t* row_ptr, const float* row_c, int64_t row_begin, int64_t nrows, const float* ent_w,
                    int64_t nnz, const float* u, float* r, double* loss_part, hipStream_t st) {
   const int nrb = (int)((nrows + kMT - 1) / kMT);
-  hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(xcd_grid(nrb)), dim3(kMT), 0, st, row_ptr, row_c, row_begin, nrows, ent_w,
+  hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(xcd_grid(nrb)), dim3(kRST), 0, st, row_ptr, row_c, row_begin, nrows, ent_w,
                      nnz, u, (const int32_t*)nullptr, r, loss_part);
 }
 
@@ -1564,7 +1587,7 @@ int mfma_step(const dips_grid_t* g, const dips_records_t* rec, const dips_mfma_b
     if (nfp + nfm > 0)                                                                                             \
       hipLaunchKernelGGL((mfma_fwd_kernel<A, B, C, D>), dim3(nfp + nfm), dim3(kFT), lds_f, st, T, params, Hm, Hp,   \
                          i0, ip, im, u, nfp, act, act_f4);                                                         \
-    hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(xcd_grid(nrb)), dim3(kMT), 0, st, rec->row_ptr, rec->row_c,       \
+    hipLaunchKernelGGL(mfma_rowsum_kernel, dim3(xcd_grid(nrb)), dim3(kRST), 0, st, rec->row_ptr, rec->row_c,      \
                        bt->row_begin, nrows, rec->ent_w, rec->nnz, u - i0, bt->e_item, r, loss_part);              \
     if (ip + im > 0)                                                                                               \
       hipLaunchKernelGGL(mfma_seed_kernel, dim3(xcd_grid((ip + im + 255) / 256)), dim3(256), 0, st, T, i0,          \
