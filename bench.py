@@ -439,9 +439,24 @@ def accuracy_run(prob, n, lo, hi, mlp, B, epochs, opt):
     exact2 = np.asarray(prob["exact_fn"](R2.T), np.float64)
     err2 = np.asarray(u2, np.float64) - exact2
     del R2, u2
+    # the judge: the matrix-free BiCGSTAB solve of the same cut-cell equations on the grid values (the method
+    # without the network), against the same exact solution at the training nodes
+    from jax_dips.judge import GridJudge
+
+    torch.cuda.synchronize()
+    t2 = time.time()
+    ug, iters, relres = GridJudge(g, g, problems_callbacks(prob)).solve(rtol=1e-6, max_iter=5000)
+    torch.cuda.synchronize()
+    judge_s = time.time() - t2
+    errg = np.asarray(ug, np.float64) - exact
+    judge = {"solver": "BiCGSTAB on grid values (the same discretisation, spacing h)", "iterations": int(iters),
+             "relres": float(relres), "converged": bool(relres <= 1e-5), "solve_s": judge_s,
+             "linf": float(np.abs(errg).max()), "rms": float(np.sqrt(np.mean(errg ** 2))),
+             "rel_l2": float(np.sqrt(np.sum(errg ** 2) / np.sum(exact ** 2)))}
     print(f"# accuracy run: {epochs} epochs in {wall:.1f} s (records {tr.timing['records_s']:.1f} s, training "
           f"{tr.timing['train_s']:.1f} s); L-inf {np.abs(err).max():.4e} at the training nodes, "
-          f"{np.abs(err2).max():.4e} on the {2 * n}^3 eval grid ({eval_s:.2f} s)", file=sys.stderr, flush=True)
+          f"{np.abs(err2).max():.4e} on the {2 * n}^3 eval grid ({eval_s:.2f} s); grid solve L-inf "
+          f"{judge['linf']:.4e} ({iters} iterations, relres {relres:.1e})", file=sys.stderr, flush=True)
     return {"epochs": epochs, "steps": int(tr.steps), "end_to_end_s": wall,
             "records_s": tr.timing["records_s"], "train_s": tr.timing["train_s"],
             "train_points_per_s": npts * epochs / tr.timing["train_s"],
@@ -452,13 +467,20 @@ def accuracy_run(prob, n, lo, hi, mlp, B, epochs, opt):
             "eval_2n": {"grid": f"{2 * n}^3 on the same box", "linf": float(np.abs(err2).max()),
                         "rms": float(np.sqrt(np.mean(err2 ** 2))),
                         "rel_l2": float(np.sqrt(np.sum(err2 ** 2) / np.sum(exact2 ** 2))),
-                        "eval_s": eval_s, "points": int(err2.size)}}
+                        "eval_s": eval_s, "points": int(err2.size)},
+            "judge": judge}
 
 
 def problems_axes(n, lo, hi):
     from jax_dips import problems
 
     return problems.grid_axes(n, lo, hi)
+
+
+def problems_callbacks(prob):
+    from jax_dips import problems
+
+    return problems.oracle_callbacks(prob)
 
 
 def run_sweep(args):
