@@ -2,10 +2,23 @@
 #include <cmath>
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "dips_common.h"
 #include "dips_hip.h"
 
 namespace dips {
+
+// A roctx range over an entry point, named for the part of the step it enqueues (K1 record build, K2-K4
+// training steps, K5 the grid-value judge, K6 the batched sweep, F1 evaluation).  Ranges are host-side
+// markers: no-ops unless a tracer such as `rocprofv3 --marker-trace` is attached, and free of stream work
+// (so the entry points stay capturable in a HIP graph).
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 static thread_local std::string g_last_error;
 
@@ -81,6 +94,7 @@ int dips_net_supported(const dips_net_t* net) {
 int dips_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, int64_t row_begin, int64_t nrows,
                    int64_t nbulk, const dips_net_t* net, const float* params, float* partial, int64_t partial_cap,
                    int want_grad, int32_t* nblk, void* stream) {
+  const Range range("K2 dips_loss_grad");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -98,6 +112,7 @@ int dips_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, int64_t r
 int dips_opt_step(const dips_net_t* net, const dips_opt_t* opt, const float* partial, int32_t nblk,
                   float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
                   int32_t loss_cap, void* stream) {
+  const Range range("K4 dips_opt_step");
   int st = check_net(net);
   if (st) return st;
   if (!opt || !partial || !params || !m || !v || !count || !loss_hist) {
@@ -112,6 +127,7 @@ int dips_train_step(const dips_grid_t* grid, const dips_records_t* rec, int64_t 
                     int64_t nbulk, const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params,
                     float* m, float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* partial,
                     int64_t partial_cap, uint32_t* sync, void* stream) {
+  const Range range("K2-K4 dips_train_step");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -133,6 +149,7 @@ int dips_train_level(const dips_grid_t* grid, const dips_records_t* rec, const i
                      const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
                      int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
                      uint32_t* sync, void* stream) {
+  const Range range("K2-K4 dips_train_level");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -170,6 +187,7 @@ int64_t dips_pack_tiles(const int32_t* row_ptr, int64_t nrows, const int64_t* ba
 int dips_tile_slots(const dips_grid_t* grid, const dips_records_t* rec, const int32_t* tile_row, int64_t nbounds,
                     float* slot_x, float* slot_y, float* slot_z, float* slot_w, float* slot_c, uint32_t* slot_info,
                     int32_t* slot_row, void* stream) {
+  const Range range("K1 dips_tile_slots");
   int st = check_records(grid, rec);
   if (st) return st;
   if (!tile_row || nbounds < 1 || !slot_x || !slot_y || !slot_z || !slot_w || !slot_c || !slot_info) {
@@ -193,6 +211,7 @@ int64_t dips_tile_partial_size(const dips_net_t* net) {
 int dips_tile_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_tiles_t* tiles,
                         int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const float* params,
                         float* partial, int64_t partial_cap, int want_grad, int32_t* nblk, void* stream) {
+  const Range range("K2-K3 dips_tile_loss_grad");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -211,6 +230,7 @@ int dips_tile_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
                          int64_t tile_begin, int64_t ntiles, const dips_net_t* net, const dips_opt_t* opt,
                          float inv_batch, float* params, float* m, float* v, int32_t* count, float* loss_hist,
                          int32_t loss_cap, float* partial, int64_t partial_cap, uint32_t* sync, void* stream) {
+  const Range range("K2-K4 dips_tile_train_step");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -227,6 +247,7 @@ int dips_tile_train_level(const dips_grid_t* grid, const dips_records_t* rec, co
                           const dips_net_t* net, const dips_opt_t* opt, float* params, float* m, float* v,
                           int32_t* count, float* loss_hist, int32_t loss_cap, float* partial, int64_t partial_cap,
                           uint32_t* sync, void* stream) {
+  const Range range("K2-K4 dips_tile_train_level");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -267,6 +288,7 @@ int64_t dips_batch_workspace_cfg(const dips_batch_problem_t* probs, int32_t npro
 int dips_batch_setup(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
                      const dips_opt_t* opt, const dips_batch_cfg_t* cfg, void* work, int64_t work_bytes,
                      dips_batch_plan_t* plan, void* stream) {
+  const Range range("K6 dips_batch_setup");
   int st = check_net(net);
   if (st) return st;
   if (!probs || nprob < 1) {
@@ -279,6 +301,7 @@ int dips_batch_setup(const dips_batch_problem_t* probs, int32_t nprob, int32_t n
 }
 
 int dips_batch_steps(const dips_net_t* net, const dips_batch_plan_t* plan, int32_t nsteps, void* work, void* stream) {
+  const Range range("K6 dips_batch_steps");
   int st = check_net(net);
   if (st) return st;
   return batch_steps(net, plan, nsteps, work, (hipStream_t)stream);
@@ -286,6 +309,7 @@ int dips_batch_steps(const dips_net_t* net, const dips_batch_plan_t* plan, int32
 
 int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t nepochs, const dips_net_t* net,
                      const dips_opt_t* opt, void* work, int64_t work_bytes, void* stream) {
+  const Range range("K6 dips_batch_train");
   int st = check_net(net);
   if (st) return st;
   if (!probs || nprob < 1) {
@@ -300,11 +324,13 @@ int dips_batch_train(const dips_batch_problem_t* probs, int32_t nprob, int32_t n
 int dips_atoms_eval(const double* atoms, int32_t natoms, const int32_t* cell_start, const int32_t* cell_atoms,
                     int32_t nc, double lo, double cell, double cap, double sigma, const double* pos, int64_t m,
                     double* phi, double* charge, void* stream) {
+  const Range range("K1 dips_atoms_eval");
   return atoms_launch(atoms, natoms, cell_start, cell_atoms, nc, lo, cell, cap, sigma, pos, m, phi, charge,
                       (hipStream_t)stream);
 }
 
 int dips_geom_regions(const double* phic, int64_t n7, int8_t* region, void* stream) {
+  const Range range("K1 dips_geom_regions");
   if (n7 < 0 || (n7 > 0 && (!phic || !region))) {
     set_error("dips_geom_regions: invalid arguments");
     return DIPS_EINVAL;
@@ -314,6 +340,7 @@ int dips_geom_regions(const double* phic, int64_t n7, int8_t* region, void* stre
 
 int dips_geom_cells(const dips_grid_t* grid, float hx, float hy, float hz, const int32_t* cell_pt, int64_t ncell,
                     const double* phic, double* vol_m, double* area_m, double* tri, double* tri_area, void* stream) {
+  const Range range("K1 dips_geom_cells");
   if (!grid || ncell < 0 || (ncell > 0 && (!cell_pt || !phic || !vol_m || !area_m || !tri || !tri_area))) {
     set_error("dips_geom_cells: invalid arguments");
     return DIPS_EINVAL;
@@ -324,6 +351,7 @@ int dips_geom_cells(const dips_grid_t* grid, float hx, float hy, float hz, const
 int dips_geom_regression(const dips_grid_t* grid, float hx, float hy, float hz, const int32_t* cs_pt,
                          const int8_t* cs_e, int64_t ncs, const double* phi_cube, const double* mu_s, double rcond,
                          double* a, double* k, int8_t* sel, double* r_gamma, double* delta, void* stream) {
+  const Range range("K1 dips_geom_regression");
   if (!grid || ncs < 0 || (ncs > 0 && (!cs_pt || !cs_e || !phi_cube || !mu_s || !a || !k || !sel || !r_gamma || !delta))) {
     set_error("dips_geom_regression: invalid arguments");
     return DIPS_EINVAL;
@@ -333,6 +361,7 @@ int dips_geom_regression(const dips_grid_t* grid, float hx, float hy, float hz, 
 }
 
 int dips_geom_assemble(const dips_assemble_t* args, void* stream) {
+  const Range range("K1 dips_geom_assemble");
   if (!args || args->npts < 0 || (args->mode != 0 && args->mode != 1) || !args->xs || !args->region ||
       !args->phis || !args->mu_face || !args->pt_vals || (args->mode == 0 && !args->nnz) ||
       (args->mode == 1 && (!args->ent_off || !args->row_of || !args->ent_code || !args->ent_w || !args->row_c))) {
@@ -344,6 +373,7 @@ int dips_geom_assemble(const dips_assemble_t* args, void* stream) {
 
 int dips_eval(const dips_net_t* net, const float* params, const float* pts, const uint8_t* phase, const float* normals,
               int64_t m, float* u, float* grad, float* grad_n, void* stream) {
+  const Range range("F1 dips_eval");
   int st = check_net(net);
   if (st) return st;
   if (m < 0 || (m > 0 && (!params || !pts || !phase || !u))) {
@@ -355,6 +385,7 @@ int dips_eval(const dips_net_t* net, const float* params, const float* pts, cons
 
 int dips_grid_apply(const dips_grid_t* grid, const dips_records_t* rec, const float* u, float* y, int with_c,
                     void* stream) {
+  const Range range("K5 dips_grid_apply");
   int st = check_records(grid, rec);
   if (st) return st;
   if (grid->extra && grid->nextra > 0) {
@@ -370,6 +401,7 @@ int dips_grid_apply(const dips_grid_t* grid, const dips_records_t* rec, const fl
 
 int dips_bicgstab(const dips_grid_t* grid, const dips_records_t* rec, float* x, float* work, double* dwork,
                   int32_t max_iter, double rtol, int32_t* iters, double* relres, void* stream) {
+  const Range range("K5 dips_bicgstab");
   int st = check_records(grid, rec);
   if (st) return st;
   if (grid->extra && grid->nextra > 0) {
@@ -402,6 +434,7 @@ int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
                          const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
                          float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* work,
                          int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  const Range range("K2-K4 dips_mfma_train_step");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -424,6 +457,7 @@ int dips_mfma_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
 int dips_mfma_train_step_noalloc(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
                                  const dips_net_t* net, const dips_opt_t* opt, float inv_batch, float* params, float* m,
                                  float* v, int32_t* count, float* loss_hist, int32_t loss_cap, void* stream) {
+  const Range range("K2-K4 dips_mfma_train_step_noalloc");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -445,6 +479,7 @@ int dips_mfma_train_step_noalloc(const dips_grid_t* grid, const dips_records_t* 
 int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
                         const dips_net_t* net, float inv_batch, const float* params, float* grad, double* loss,
                         float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  const Range range("K2-K3 dips_mfma_loss_grad");
   int st = check_net(net);
   if (st) return st;
   if ((st = check_records(grid, rec))) return st;
@@ -464,6 +499,7 @@ int dips_mfma_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
 
 int dips_mfma_items(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch, float* i_pos,
                     int32_t* e_item, void* stream) {
+  const Range range("K1 dips_mfma_items");
   int st = check_records(grid, rec);
   if (st) return st;
   if (!batch || batch->ent_begin < 0 || batch->ent_end > rec->nnz || batch->ent_begin > batch->ent_end ||
@@ -495,6 +531,7 @@ int dips_hash_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
                          const dips_hashgrid_t* hg, const dips_opt_t* opt, float inv_batch, float* params, float* m,
                          float* v, int32_t* count, float* loss_hist, int32_t loss_cap, float* work,
                          int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  const Range range("K2-K4 dips_hash_train_step");
   int st = check_records(grid, rec);
   if (st) return st;
   if (!check_hash_batch(rec, batch) || !opt || !params || !m || !v || !count || !loss_hist || !work || !dwork) {
@@ -508,6 +545,7 @@ int dips_hash_train_step(const dips_grid_t* grid, const dips_records_t* rec, con
 int dips_hash_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, const dips_mfma_batch_t* batch,
                         const dips_hashgrid_t* hg, float inv_batch, const float* params, float* grad, double* loss,
                         float* work, int64_t work_floats, double* dwork, int64_t dwork_doubles, void* stream) {
+  const Range range("K2-K3 dips_hash_loss_grad");
   int st = check_records(grid, rec);
   if (st) return st;
   if (!check_hash_batch(rec, batch) || !params || !grad || !loss || !work || !dwork) {
@@ -520,6 +558,7 @@ int dips_hash_loss_grad(const dips_grid_t* grid, const dips_records_t* rec, cons
 
 int dips_hash_eval(const dips_hashgrid_t* hg, const float* params, const float* pts, const uint8_t* phase, int64_t m,
                    float* u, float* grad, void* stream) {
+  const Range range("F1 dips_hash_eval");
   if (m < 0 || (m > 0 && (!params || !pts || !phase || !u))) {
     set_error("dips_hash_eval: invalid arguments");
     return DIPS_EINVAL;

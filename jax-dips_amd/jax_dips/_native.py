@@ -200,6 +200,35 @@ def lib():
     return _lib
 
 
+_roctx = None
+
+
+class trace_range:
+    """A roctx range around a block of host code (the training loop's levels and epochs); the library's
+    entry points carry their own.  A no-op unless a tracer (rocprofv3 --marker-trace) is attached, or
+    where the roctx runtime is absent."""
+
+    def __init__(self, name):
+        self.name = name.encode()
+
+    def __enter__(self):
+        global _roctx
+        if _roctx is None:
+            try:
+                _roctx = ctypes.CDLL("librocprofiler-sdk-roctx.so.1")
+                _roctx.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            except OSError:
+                _roctx = False
+        if _roctx:
+            _roctx.roctxRangePushA(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        if _roctx:
+            _roctx.roctxRangePop()
+        return False
+
+
 def check(status, what):
     if status != DIPS_OK:
         msg = lib().dips_last_error().decode()

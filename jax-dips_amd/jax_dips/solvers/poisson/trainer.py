@@ -245,7 +245,8 @@ class Trainer:
                     last += 1
                 h = np.array(self.TD.alternate_res_sequentially(E, epoch, tr.dx, tr.dy, tr.dz), np.float32)
                 tr0 = time.time()
-                rec = build_level(grid, h, bounds, self.cbs, B, coeffs=self.precond is not None)
+                with _native.trace_range(f"K1 records of zoom level {lvl}"):
+                    rec = build_level(grid, h, bounds, self.cbs, B, coeffs=self.precond is not None)
                 torch.cuda.synchronize()
                 self.timing["records_s"] += time.time() - tr0
                 tt0 = time.time()
@@ -256,7 +257,8 @@ class Trainer:
                 for e in range(epoch, last + 1):
                     if stop_training:
                         break
-                    self._epoch(engine, grid, rec, graph)
+                    with _native.trace_range(f"epoch {e} (zoom level {lvl})"):
+                        self._epoch(engine, grid, rec, graph)
                     done = e + 1
                 del graph
                 torch.cuda.synchronize()
@@ -270,7 +272,10 @@ class Trainer:
                 del rec
                 epoch = last + 1
         finally:
-            signal.signal(signal.SIGINT, prev)
+            # a handler installed outside Python (a profiler's preloaded library, an embedding host) reads back
+            # as None and cannot be reinstalled from here: leave ours in place then (it only sets a flag)
+            if prev is not None:
+                signal.signal(signal.SIGINT, prev)
         logger.info(f"solve took {time.time() - t0} (sec)")
         self.steps = int(engine.count.item()) - count0
         self.params = self.unflatten(engine.params.cpu().numpy())
