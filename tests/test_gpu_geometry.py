@@ -249,6 +249,46 @@ def test_lpbe_sphere_records_match_oracle(level):
 
 
 @pytest.mark.gpu
+def test_nan_callback_records_match_oracle():
+    """A beta callback that returns NaN at the origin (where the padding triangles of
+    the interface quadrature sit) makes the interface integral NaN; the reference
+    masks it with nan_to_num(rhs / diag) (discretization.py:418-419), which zeroes the
+    jump contribution of every crossed cell.  Device records built with that callback
+    (sampled through torch on the device) equal the oracle's, and differ from the
+    records of the finite callback exactly on the crossed cells."""
+    from jax_dips import xnp
+    from jax_dips.level_records import build_level
+    from jax_dips.records import DeviceGrid
+    from oracle import discretization as D
+    from oracle import trainer as T
+
+    prob = problems.sphere()
+    fns = problems.oracle_callbacks(prob)
+    bad_fns = dict(fns)
+    beta = fns["beta"]
+    bad_fns["beta"] = lambda r: xnp.where((r[0] == 0) & (r[1] == 0) & (r[2] == 0), float("nan"), beta(r))
+    axes, P, h, bounds = grid_points(12)
+    grid = DeviceGrid(*axes)
+    n = P.shape[0]
+    out = {}
+    for name, f in (("good", fns), ("bad", bad_fns)):
+        orec = T.LevelRecords(P, h, bounds, D.Callbacks(f, np.float64), np.float64)
+        drec = build_level(grid, h, bounds, f, n)
+        W, C = drec.to_dense()
+        scale = np.maximum(np.abs(orec.W).max(axis=1), 1.0)
+        err = np.abs(W - orec.W).max(axis=1) / scale
+        cerr = np.abs(C - orec.c) / np.maximum(np.abs(orec.c), scale)
+        print(name, "max w err", err.max(), "max c err", cerr.max(), "finite", np.isfinite(C).all())
+        assert np.isfinite(C).all() and np.isfinite(W).all()
+        assert err.max() <= 1e-6 and cerr.max() <= 1e-6
+        out[name] = (C, orec)
+    cut = (out["good"][1].slots["geo"]["region"] == 0) & ~out["good"][1].boundary
+    assert cut.sum() > 10
+    assert np.any(out["good"][0][cut] != out["bad"][0][cut])
+    assert np.array_equal(out["good"][0][~cut], out["bad"][0][~cut])
+
+
+@pytest.mark.gpu
 def test_atom_field_kernel_matches_host():
     """dips_atoms_eval (cell list on the device, float64) gives the host cell
     list's phi bit for bit (same subtraction order, no contraction) and its
