@@ -48,8 +48,11 @@ constexpr int kStageFloats = 2 * 16 * kLdsStride;     // A and B operand stacks 
 // hidden widths (PC1, PC2; 0 = no preconditioner, PC2 = 0 = one hidden layer)
 // and hidden activation (0 = tanh, 1 = CELU: the hash network's coordinate
 // MLPs with the identity encoder, 3 -> [16] -> 1 in the reference's LPBE
-// configuration, examples/benchmark_LPBE/conf/lpbe.yaml:48-60).  [8, 4] is the preconditioner of the reference's LPBE
-// configuration (examples/benchmark_LPBE/conf/lpbe.yaml:63-67).
+// configuration, examples/benchmark_LPBE/conf/lpbe.yaml:48-60).  [8, 4] is the
+// preconditioner of the reference's LPBE configuration
+// (examples/benchmark_LPBE/conf/lpbe.yaml:63-67); its hidden activation is
+// tanh whatever the networks' activation, so (1, 16, 1, 16, 8, 4, 1) is that
+// configuration in full.
 #define DIPS_TILE_SHAPES(X)  \
   X(1, 1, 2, 10, 0, 0, 0)    \
   X(1, 3, 2, 10, 0, 0, 0)    \
@@ -60,7 +63,8 @@ constexpr int kStageFloats = 2 * 16 * kLdsStride;     // A and B operand stacks 
   X(1, 1, 2, 10, 8, 0, 0)    \
   X(2, 10, 2, 10, 8, 4, 0)   \
   X(1, 16, 1, 16, 0, 0, 1)   \
-  X(2, 16, 2, 16, 0, 0, 1)
+  X(2, 16, 2, 16, 0, 0, 1)   \
+  X(1, 16, 1, 16, 8, 4, 1)
 
 struct TileStepArgs {
   PointSet ps;

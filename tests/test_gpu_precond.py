@@ -229,3 +229,72 @@ def test_trainer_with_preconditioner_matches_oracle(tmp_path):
     assert dev.size == full.size and set(tr.params["preconditioner"]["params"]) == {"Dense_0", "Dense_1", "Dense_2"}
     assert np.linalg.norm(dev - th) <= 1e-4 * np.linalg.norm(th)
     assert np.linalg.norm(dev[nn:] - th[nn:]) <= 1e-4 * np.linalg.norm(th[nn:])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [0, 1])
+def test_precond_with_celu_networks_loss_grad(level):
+    """The reference's LPBE model in full (lpbe.yaml:48-67): CELU coordinate
+    networks 3 -> [16] -> 1 (hash network, identity encoder) and the tanh
+    preconditioner [8, 4], on the tile kernel of shape (1, 16, 1, 16, 8, 4, CELU)."""
+    from jax_dips.nn.preconditioner import Preconditioner
+    from jax_dips.records import Engine
+    from test_gpu_hash_network import _glorot
+
+    dims, widths = (1, 16, 1, 16), (8, 4)
+    prob = problems.sphere(mu_m=1.0, mu_p=10.0)
+    axes, P, h, bounds, cb, rec = oracle_level(prob, 16, level)
+    n = P.shape[0]
+    _, theta = _glorot(dims, 7)
+    pcm = Preconditioner(Ds=list(widths))
+    full = np.concatenate([theta, pcm.flatten(pcm.init(17))]).astype(np.float32)
+    nn = theta.size
+    grid, drec = _device(axes, rec, n)
+    pc = dict(widths=list(widths), scale=1.0)
+    eng = Engine(dims, full, {}, max_rows=n, max_steps=4, precond=pc, act=1)
+    assert eng.tile_path and eng.P == full.size
+    loss, grad = eng.tile_loss_grad(grid, drec, 0, 1.0 / n)
+    ol, og, _ = T.loss_and_grad(full.astype(np.float64), dims, rec, np.arange(n), precond=pc, act="celu")
+    _check(loss, grad, ol, og, nn, f"celu {dims} {widths} level {level}")
+
+
+@pytest.mark.gpu
+def test_trainer_lpbe_model_with_preconditioner_matches_oracle(tmp_path):
+    """The drop-in trainer with the reference's LPBE model_dict (hash network with
+    the identity encoder, widths [16], preconditioner [8, 4], scaling 1.0) on the
+    LPBE problem at 16^3 on [-2.5, 2.5]^3, against the oracle epoch loop with the
+    same initial parameters: per-epoch losses and final parameters (networks and
+    preconditioner) to 1e-4 relative."""
+    from jax_dips.domain import mesh
+    from jax_dips.solvers.poisson import trainer
+    from test_gpu_hash_network import LPBE, OPT, _glorot
+
+    dims, widths = (1, 16, 1, 16), (8, 4)
+    axes = problems.grid_axes(16, -2.5, 2.5)
+    h = float(axes[0][1] - axes[0][0])
+    prob = problems.lpbe_sphere(h)
+    _, theta = _glorot(dims, 3)
+    full = np.concatenate([theta, PC.init_params(widths, seed=4)]).astype(np.float32)
+    nn = theta.size
+    model = {"model_type": "multiresolution_hash_network", "multiresolution_hash_network": LPBE,
+             "preconditioner": {"enable": True, "layer_widths": list(widths), "scaling_coeff": 1.0}}
+    g = mesh.construct(3)[0](*axes)
+    init_fn = trainer.setup(prob["initial_value_fn"], prob["dirichlet_bc_fn"], prob["lvl_set_fn"], prob["mu_m_fn"],
+                            prob["mu_p_fn"], prob["k_m_fn"], prob["k_p_fn"], prob["f_m_fn"], prob["f_p_fn"],
+                            prob["alpha_fn"], prob["beta_fn"])
+    state, solve_fn = init_fn(lvl_gstate=g, tr_gstate=g, eval_gstate=g, num_epochs=8, batch_size=2048,
+                              checkpoint_dir=str(tmp_path / "ck"), results_dir=str(tmp_path), optimizer_dict=OPT,
+                              model_dict=model, init_params=full)
+    _, es, le = solve_fn(sim_state=state)
+    tr = solve_fn.trainer
+    _, P, hh, bounds = grid_points(16, -2.5, 2.5)
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    th, ole, _, _ = T.train(P, hh, bounds, cb, dims, full, 8, 2048, OPT, precond=dict(widths=list(widths), scale=1.0),
+                            act="celu")
+    print("device", le, "oracle", ole)
+    assert np.all(np.abs(le - ole) <= 1e-4 * np.abs(ole))
+    dev = tr.flatten(tr.params)
+    assert dev.size == full.size and set(tr.params["params"]) == {"sol_p", "sol_m"}
+    assert set(tr.params["preconditioner"]["params"]) == {"Dense_0", "Dense_1", "Dense_2"}
+    assert np.linalg.norm(dev - th) <= 1e-4 * np.linalg.norm(th)
+    assert np.linalg.norm(dev[nn:] - th[nn:]) <= 1e-4 * np.linalg.norm(th[nn:])
