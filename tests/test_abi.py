@@ -117,6 +117,11 @@ def test_precond_shapes_and_counts():
     assert lib.dips_tile_supported(ctypes.byref(one)) == 1
     bad = _native.Net(1, 1, 2, 10, 8, 4, float("nan"))
     assert lib.dips_param_count(ctypes.byref(bad)) < 0
+    # the LPBE model in full: CELU coordinate networks 3 -> [16] -> 1 and the [8, 4] preconditioner
+    lpbe = _native.Net(1, 16, 1, 16, 8, 4, 1.0, 1)
+    assert lib.dips_param_count(ctypes.byref(lpbe)) == 2 * (4 * 16 + 17) + 216 + 36 + 5
+    assert lib.dips_tile_supported(ctypes.byref(lpbe)) == 1
+    assert lib.dips_tile_supported(ctypes.byref(_native.Net(1, 16, 1, 16, 8, 0, 1.0, 1))) == 0
 
 
 def test_hashgrid_table_and_param_counts():
