@@ -430,35 +430,31 @@ __device__ __forceinline__ void hash_scatter(const Sink& sink, int32_t row, floa
 }
 
 // The 8 corner records of every lane in live on a dense (coarse) level, where a
-// wavefront's entries fall into a few cells: lanes whose 8 corner rows all
-// equal are summed over the wavefront and their leader sends the 8 records.
-// Wave-uniform call.
+// wavefront's entries fall into a few cells.  The entries are in table order
+// (x fastest), so the lanes of one cell form runs of adjacent lanes: a run is a
+// maximal range of live lanes whose corner-0 row (which fixes the cell on a
+// dense level) equals the previous lane's.  One segmented scan per value sums
+// every run at once (fixed DPP tree), and the last lane of each run sends the
+// run's 8 records, all runs in parallel.  A cell met in two runs (the wavefront
+// wrapping to the next row of cells) sends two sets of records, summed later
+// like any others.  Wave-uniform call.
 template <typename Sink>
 __device__ __forceinline__ void hash_scatter_cell(const Sink& sink, const int32_t (&rows)[8], const float (&v0)[8],
                                                   const float (&v1)[8], uint64_t live, int lane) {
-  uint64_t rem = live;
-#pragma unroll 1
-  while (rem != 0) {
-    const int leader = __builtin_ctzll(rem);
-    bool match = true;
+  const bool on = (live >> lane) & 1;
+  const int32_t prev_key = __shfl(rows[0], lane > 0 ? lane - 1 : 0, DIPS_WAVE);
+  const bool prev_on = lane > 0 && ((live >> (lane - 1)) & 1);
+  const uint64_t starts = __ballot(on && !(prev_on && prev_key == rows[0]));
+  // the run of this lane begins at the last start at or below it (lanes outside live carry zeros and
+  // close the run before them; they send nothing)
+  const uint64_t below = starts & (~0ull >> (63 - lane));
+  const int head = below ? 63 - __builtin_clzll(below) : lane;
+  const bool last = on && (lane == 63 || !((live >> (lane + 1)) & 1) || ((starts >> (lane + 1)) & 1));
 #pragma unroll
-    for (int c = 0; c < 8; ++c) match = match && rows[c] == __builtin_amdgcn_readlane(rows[c], leader);
-    const uint64_t same = rem & __ballot(match);
-    if (__builtin_popcountll(same) == 1) {
-      if (lane == leader) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) sink.put(rows[c], v0[c], v1[c]);
-      }
-    } else {
-      const bool in = (same >> lane) & 1;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const float s0 = dips_wave_sum(in ? v0[c] : 0.0f);
-        const float s1 = dips_wave_sum(in ? v1[c] : 0.0f);
-        if (lane == leader) sink.put(rows[c], s0, s1);
-      }
-    }
-    rem &= ~same;
+  for (int c = 0; c < 8; ++c) {
+    const float s0 = dips_segmented_scan(on ? v0[c] : 0.0f, lane, head);
+    const float s1 = dips_segmented_scan(on ? v1[c] : 0.0f, lane, head);
+    if (last) sink.put(rows[c], s0, s1);
   }
 }
 
