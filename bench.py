@@ -21,7 +21,10 @@ error of the trained network against the manufactured solution
 1x1/2x10, batch 65536.  --workload lpbe: the reference's linearized
 Poisson-Boltzmann benchmark with a known answer (one charged sphere of radius 1
 in [-2.5,2.5]^3, eps 2/80, examples/benchmark_LPBE), 128^3, MLP 3x64, batch
-262144.  Other workloads: sweep, pb, hashgrid (see --help).
+262144; its line also carries an end-to-end run of that benchmark's own model
+(hash network with the identity encoder, CELU [16], learned preconditioner
+[8, 4], lr 1e-2, 20000 epochs on a 32^3 grid) with its error against the known
+answer.  Other workloads: sweep, pb, hashgrid (see --help).
 
 Prints one JSON line.  --gpus N > 1 exits: this build targets one GPU.
 """
@@ -332,6 +335,26 @@ def main():
         del levels, eng
         torch.cuda.empty_cache()
         accuracy = accuracy_run(prob, n, lo, hi, mlp, B, acc_epochs, opt)
+    reference_model = None
+    if args.workload == "lpbe" and acc_epochs > 0:
+        # the reference's own LPBE benchmark model in full (examples/benchmark_LPBE/conf/lpbe.yaml:17-67): the
+        # hash network with the identity encoder (CELU 3 -> [16] -> 1 per phase) and the learned
+        # preconditioner [8, 4], learning rate 1e-2 with exponential decay, 20000 epochs, on its uniform 32^3
+        # training grid (lpbe.yaml:78-81) as one batch of 32^3 points
+        xs32 = problems.grid_axes(32, lo, hi)[0]
+        ref_model = {"model_type": "multiresolution_hash_network",
+                     "multiresolution_hash_network": {
+                         "bound": 1.0, "pos_enc": "identity", "N_min": 16, "N_max": 2048, "F": 2, "T": 524288,
+                         "tv_scale": 0.0, "pos_levels": 16, "layer_widths": [16], "sol_skip_in_layers": [],
+                         "sol_act": "linear", "highest_sh_order": 2},
+                     "preconditioner": {"enable": True, "layer_widths": [8, 4], "scaling_coeff": 1.0}}
+        ref_opt = {"optimizer_name": "custom", "learning_rate": 1e-2,
+                   "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
+        reference_model = accuracy_run(problems.lpbe_sphere(h=float(xs32[1] - xs32[0])), 32, lo, hi, None,
+                                       32 ** 3, 20000 if args.accuracy_epochs < 0 else args.accuracy_epochs,
+                                       ref_opt, model_dict=ref_model)
+        reference_model["model"] = ("the reference's LPBE model: hash network, identity encoder, CELU [16] per "
+                                    "phase, preconditioner [8, 4]; 32^3 grid, one batch of 32768, lr 1e-2")
     result = {
         "metric": "NBM residual+grad points/s vs JAX ref on same GPU at equal Linf error",
         "value": value,
@@ -369,16 +392,18 @@ def main():
     }
     if accuracy is not None:
         result["accuracy"] = accuracy
+    if reference_model is not None:
+        result["accuracy_reference_model"] = reference_model
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.workload, n, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 
-def accuracy_run(prob, n, lo, hi, mlp, B, epochs, opt):
+def accuracy_run(prob, n, lo, hi, mlp, B, epochs, opt, model_dict=None):
     """The workload trained end to end through the drop-in trainer
     (setup -> init_fn -> solve_fn, initial parameters from the model's own
     initialiser, seed 42) for ``epochs`` epochs with the zoom schedule, the
-    training grid as eval grid.  Returns the wall time of solve_fn (records of
+    training grid as eval grid; model_dict defaults to the two-phase MLP mlp.  Returns the wall time of solve_fn (records of
     the four zoom levels, training, evaluation), its parts, the training
     throughput and the error of the trained network against the manufactured
     solution at the eval nodes: L-inf = max |u - u*|, RMS, L2 = sqrt(sum (u - u*)^2)
@@ -402,7 +427,7 @@ def accuracy_run(prob, n, lo, hi, mlp, B, epochs, opt):
                                 prob["f_p_fn"], prob["alpha_fn"], prob["beta_fn"])
         state, solve_fn = init_fn(lvl_gstate=g, tr_gstate=g, eval_gstate=g, num_epochs=epochs, batch_size=B,
                                   checkpoint_dir=os.path.join(work, "ck"), results_dir=work, optimizer_dict=opt,
-                                  model_dict={"model_type": "mlp", "mlp": mlp}, print_rate=epochs)
+                                  model_dict=model_dict or {"model_type": "mlp", "mlp": mlp}, print_rate=epochs)
         torch.cuda.synchronize()
         t0 = time.time()
         new_state, _, loss_epochs = solve_fn(sim_state=state)
