@@ -43,7 +43,10 @@ extern "C" {
  * C order) for n < nx*ny*nz, followed by nextra extra points (device float32
  * (nextra,3), or NULL with nextra = 0): the random points that pad a short
  * last batch (data_management.py:70-76, :103-173); extra point q has index
- * nx*ny*nz + q. */
+ * nx*ny*nz + q.  nx = ny = nz = 0 (axes NULL) with nextra > 0: a set of
+ * scattered points only (a training set built by
+ * domain/mesh.py:init_gstate_3d_manually, as examples/benchmark_LPBE/main.py:149-187
+ * builds its sampled training points). */
 typedef struct {
   const float* xs;
   const float* ys;

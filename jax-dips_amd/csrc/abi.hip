@@ -50,8 +50,14 @@ static int check_net(const dips_net_t* net) {
 }
 
 static int check_records(const dips_grid_t* grid, const dips_records_t* rec) {
-  if (!grid || !rec || !grid->xs || !grid->ys || !grid->zs || grid->nx < 1 || grid->ny < 1 || grid->nz < 1 ||
-      grid->nextra < 0 || (grid->nextra > 0 && !grid->extra)) {
+  if (!grid || !rec) {
+    set_error("invalid grid");
+    return DIPS_EINVAL;
+  }
+  // grid nodes (all three axes non-empty), or none (nx = ny = nz = 0) with the points all extra points
+  const bool nodes = grid->nx != 0 || grid->ny != 0 || grid->nz != 0;
+  if ((nodes && (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || !grid->xs || !grid->ys || !grid->zs)) ||
+      grid->nextra < 0 || (grid->nextra > 0 && !grid->extra) || (!nodes && grid->nextra < 1)) {
     set_error("invalid grid");
     return DIPS_EINVAL;
   }
@@ -362,7 +368,8 @@ int dips_geom_regression(const dips_grid_t* grid, float hx, float hy, float hz, 
 
 int dips_geom_assemble(const dips_assemble_t* args, void* stream) {
   const Range range("K1 dips_geom_assemble");
-  if (!args || args->npts < 0 || (args->mode != 0 && args->mode != 1) || !args->xs || !args->region ||
+  if (!args || args->npts < 0 || (args->mode != 0 && args->mode != 1) ||
+      ((args->nx != 0 || args->ny != 0 || args->nz != 0) && (!args->xs || !args->ys || !args->zs)) || !args->region ||
       !args->phis || !args->mu_face || !args->pt_vals || (args->mode == 0 && !args->nnz) ||
       (args->mode == 1 && (!args->ent_off || !args->row_of || !args->ent_code || !args->ent_w || !args->row_c))) {
     set_error("dips_geom_assemble: invalid arguments");

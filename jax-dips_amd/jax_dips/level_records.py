@@ -84,6 +84,10 @@ def stencil_corner_phi(grid, S, ht, half, phi):
     device = S.device
     nx, ny, nz = grid.shape
     ng = grid.ngrid
+    if ng == 0:  # scattered points only: every sample taken directly
+        phis = sample(phi, S).contiguous()
+        phic = sample(phi, S[:, :, None, :] + _offsets(CORNER_SIGNS, device)[None, None] * half).contiguous()
+        return phis, phic
     sgn = torch.tensor([-1.0, 0.0, 1.0], dtype=torch.float32, device=device)
     tabs = []
     for a, axv in enumerate((grid.xs, grid.ys, grid.zs)):

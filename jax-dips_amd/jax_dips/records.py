@@ -94,7 +94,16 @@ def pack_tiles(row_ptr, nrows, batch_begin, batch_rows):
 class DeviceGrid:
     """Training points on the device: the nodes of the float32 axes ('ij' C
     order) followed by optional extra points (nextra, 3), the random points that
-    pad a short last batch."""
+    pad a short last batch.  Empty axes and only extra points: a scattered
+    training set (from_points)."""
+
+    @classmethod
+    def from_points(cls, R, device="cuda"):
+        """A set of scattered points (N, 3), no grid nodes: the training points
+        of a gstate built by mesh.init_gstate_3d_manually (reference
+        domain/mesh.py:179-213)."""
+        empty = np.zeros(0, np.float32)
+        return cls(empty, empty, empty, extra=np.asarray(R, np.float32).reshape(-1, 3), device=device)
 
     def __init__(self, xs, ys, zs, extra=None, device="cuda"):
         import torch
@@ -108,7 +117,11 @@ class DeviceGrid:
         self.extra = torch.as_tensor(np.ascontiguousarray(ex), device=device) if len(ex) else None
         self.nextra = int(len(ex))
         self.npts = self.ngrid + self.nextra
-        self.c = N.Grid(self.xs.data_ptr(), self.ys.data_ptr(), self.zs.data_ptr(), *self.shape,
+        if self.ngrid == 0 and self.nextra == 0:
+            raise ValueError("DeviceGrid: no points")
+        nodes = self.ngrid > 0
+        self.c = N.Grid(self.xs.data_ptr() if nodes else None, self.ys.data_ptr() if nodes else None,
+                        self.zs.data_ptr() if nodes else None, *(self.shape if nodes else (0, 0, 0)),
                         self.extra.data_ptr() if self.extra is not None else None, self.nextra)
 
     def points(self):

@@ -211,7 +211,13 @@ class Trainer:
         # (DatasetDict.gpu_padded_batches, data_management.py:103-173); they are
         # training points like the others and sit after the grid nodes
         extra = generate_random_points_like(np.asarray(tr.R), nb * B - n) if nb * B > n else None
-        grid = DeviceGrid(tr.x, tr.y, tr.z, extra=extra)
+        if len(tr.x) * len(tr.y) * len(tr.z) == n:
+            grid = DeviceGrid(tr.x, tr.y, tr.z, extra=extra)
+        else:
+            # scattered training points (mesh.init_gstate_3d_manually: x, y, z are the points' coordinates,
+            # R their rows), e.g. the sampled sets of examples/benchmark_LPBE/main.py:149-187
+            R = np.asarray(tr.R, np.float32).reshape(-1, 3)
+            grid = DeviceGrid.from_points(R if extra is None else np.concatenate([R, np.asarray(extra, np.float32)]))
         lv = self.lvl_gstate
         bounds = (lv.xmin(), lv.xmax(), lv.ymin(), lv.ymax(), lv.zmin(), lv.zmax())
         E = self.num_epochs

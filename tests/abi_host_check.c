@@ -138,6 +138,18 @@ static void argument_checks(void) {
   CHECK(dips_hash_train_step(NULL, NULL, NULL, NULL, &opt, 1.0f, &f, &f, &f, &i32, &f, 1, &f, 1, &d, 1, NULL) !=
         0);
   CHECK(dips_batch_workspace(NULL, 0, &net) < 0);
+  /* a points-only set (no axes) needs extra points; axes must be all set or all empty */
+  dips_grid_t pts;
+  memset(&pts, 0, sizeof pts);
+  dips_records_t r1;
+  memset(&r1, 0, sizeof r1);
+  r1.row_point = &i32, r1.row_ptr = &i32, r1.row_c = &f, r1.nrows = 1;
+  CHECK(dips_loss_grad(&pts, &r1, 0, 1, 0, &net, &f, &f, 1, 1, &i32, NULL) != 0);
+  CHECK(strstr(dips_last_error(), "invalid grid") != NULL);
+  pts.ny = 1;
+  pts.extra = &f, pts.nextra = 1;
+  CHECK(dips_loss_grad(&pts, &r1, 0, 1, 0, &net, &f, &f, 1, 1, &i32, NULL) != 0);
+  CHECK(strstr(dips_last_error(), "invalid grid") != NULL);
 }
 
 int main(void) {

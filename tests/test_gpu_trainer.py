@@ -228,3 +228,42 @@ def test_wide_network_eval_at_256_cubed():
         Q = P[sub].astype(np.float64)
         fd = (M.forward(th, model.dims, Q + e, phase[sub]) - M.forward(th, model.dims, Q - e, phase[sub])) / (2 * h)
         assert np.max(np.abs(g[sub, a] - fd)) <= 1e-3 * max(1.0, np.max(np.abs(fd))), a
+
+
+@pytest.mark.gpu
+def test_trainer_scattered_points_matches_oracle_loop(tmp_path):
+    """Scattered training points, as the reference's LPBE example builds its sampled sets
+    (examples/benchmark_LPBE/main.py:149-187 with domain/mesh.py:init_gstate_3d_manually): x, y, z
+    are the points' coordinates, not axes.  The drop-in trainer places them as a points-only set and
+    matches the oracle epoch loop on the same points (losses and parameters to 1e-4 relative), and its
+    evaluation at the points matches the oracle network."""
+    from jax_dips.domain import mesh
+    from jax_dips.solvers.poisson import trainer
+
+    prob = problems.sphere()
+    dims = (1, 1, 2, 10)
+    theta0 = M.init_params(dims, seed=43)
+    rng = np.random.default_rng(5)
+    P = rng.uniform(-0.95, 0.95, size=(4096, 3)).astype(np.float32)
+    d = np.float32(2.0 / 15)
+    g_tr = mesh.init_gstate_3d_manually(P[:, 0], P[:, 1], P[:, 2], d, d, d, None, None, None, None, None, None)
+    init_mesh_fn, _ = mesh.construct(3)
+    g_lvl = init_mesh_fn(*problems.grid_axes(16))
+    init_fn = trainer.setup(prob["initial_value_fn"], prob["dirichlet_bc_fn"], prob["lvl_set_fn"], prob["mu_m_fn"],
+                            prob["mu_p_fn"], prob["k_m_fn"], prob["k_p_fn"], prob["f_m_fn"], prob["f_p_fn"],
+                            prob["alpha_fn"], prob["beta_fn"])
+    state, solve_fn = init_fn(lvl_gstate=g_lvl, tr_gstate=g_tr, eval_gstate=g_tr, num_epochs=8, batch_size=2048,
+                              checkpoint_dir=str(tmp_path / "ck"), results_dir=str(tmp_path), optimizer_dict=OPT,
+                              model_dict=MODEL, init_params=theta0)
+    st, es, le = solve_fn(sim_state=state)
+    tr = solve_fn.trainer
+    h = np.array([d, d, d], np.float32)
+    bounds = (-1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    th, ole, _, _ = T.train(P, h, bounds, cb, dims, theta0, 8, 2048, OPT)
+    print("device", le, "oracle", ole)
+    assert np.all(np.abs(le - ole) <= 1e-4 * np.abs(ole))
+    dev = tr.model.flatten(tr.params)
+    assert np.linalg.norm(dev - th) <= 1e-4 * np.linalg.norm(th)
+    phase = cb("phi", P) >= 0
+    assert np.max(np.abs(np.asarray(st.solution) - M.forward(dev, dims, P, phase))) <= 1e-5
