@@ -31,7 +31,7 @@ encoder (CELU coordinate MLPs in the tile kernel), the hashgrid encoder
 (dips_hash_train_step per batch: the MLPs and the encoder's table train
 together) or the frequency and spherical-harmonics encoders (the same entry
 points, no table), nn/hash_network.py.  Unsupported reference options raise
-NotImplementedError: algorithm 1, lbfgs, nonlinear operators, activations
+NotImplementedError: algorithm 1, lbfgs, nonlinear operators other than zero, activations
 other than tanh in "mlp", and the preconditioner with the hashgrid, frequency
 or spherical-harmonics encoders.
 """
@@ -99,8 +99,8 @@ class Trainer:
         if pc.get("enable", False):
             self.precond = Preconditioner(Ds=pc["layer_widths"], out_dim=1,
                                           scaling_coeff=pc.get("scaling_coeff", 1.0))
-        if sim_state_fn.nonlinear_op_m is not None or sim_state_fn.nonlinear_op_p is not None:
-            raise NotImplementedError("nonlinear operators have no device implementation")
+        if not (_zero_operator(sim_state_fn.nonlinear_op_m) and _zero_operator(sim_state_fn.nonlinear_op_p)):
+            raise NotImplementedError("nonlinear operators other than zero have no device implementation")
         if multi_gpu:
             logger.warning("multi_gpu=True: one GPU is used; the batch walk matches the single-GPU path")
         self.lvl_gstate, self.tr_gstate, self.eval_gstate = lvl_gstate, tr_gstate, eval_gstate
@@ -337,9 +337,25 @@ class Trainer:
         return u, g, gn
 
 
+def _zero_operator(fn):
+    """True for a missing operator or one that returns 0 wherever it is probed (the linear problems of the
+    reference's examples pass such operators, e.g. examples/benchmark_LPBE/coefficients.py:110-117); its term
+    N(u) V in the residual (discretization.py:369) then vanishes.  Any other operator, or one that fails on a
+    float, is taken as non-zero."""
+    if fn is None:
+        return True
+    try:
+        return all(float(np.asarray(fn(u), np.float64).reshape(-1)[0]) == 0.0 for u in (0.0, 1.0, -1.0, 0.37, -25.0))
+    except Exception:  # noqa: BLE001 -- any failure: not provably zero
+        return False
+
+
 def setup(initial_value_fn, dirichlet_bc_fn, lvl_set_fn, mu_m_fn_, mu_p_fn_, k_m_fn_, k_p_fn_, f_m_fn_, f_p_fn_,
           alpha_fn_, beta_fn_, nonlinear_op_m=None, nonlinear_op_p=None):
-    """Returns init_fn (trainer.py:980-1137).  Missing nonlinear operators are zero."""
+    """Returns init_fn (trainer.py:980-1137).  Missing nonlinear operators are zero; operators that are not
+    identically zero raise NotImplementedError here, before any device work."""
+    if not (_zero_operator(nonlinear_op_m) and _zero_operator(nonlinear_op_p)):
+        raise NotImplementedError("nonlinear operators other than zero have no device implementation")
     if nonlinear_op_m is None:
         logger.warning("nonlinear_op_m(u) is not defined. Setting it to zero.")
     if nonlinear_op_p is None:

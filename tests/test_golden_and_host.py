@@ -100,3 +100,26 @@ def test_data_management_and_optimizer_config():
     assert get_optimizer("custom", learning_rate=1e-3, decay_rate=0.9)["optimizer_name"] == "custom"
     with pytest.raises(ValueError):
         get_optimizer("sgd")
+
+
+def test_trainer_accepts_zero_nonlinear_operators():
+    """The linear problems of the reference's examples pass nonlinear operators that
+    return 0 (examples/benchmark_LPBE/coefficients.py:110-117): the drop-in trainer
+    takes them as linear; any other operator raises NotImplementedError, before any
+    device work (the device path has no N(u) term)."""
+    import pytest
+
+    from jax_dips import problems
+    from jax_dips.solvers.poisson import trainer
+
+    prob = problems.sphere()
+    cbs = [prob[k] for k in ("initial_value_fn", "dirichlet_bc_fn", "lvl_set_fn", "mu_m_fn", "mu_p_fn", "k_m_fn",
+                             "k_p_fn", "f_m_fn", "f_p_fn", "alpha_fn", "beta_fn")]
+    # the check itself (the trainer past it allocates on the GPU)
+    assert trainer._zero_operator(None) and trainer._zero_operator(lambda u: 0.0)
+    assert trainer._zero_operator(lambda u: 0.0 * u)
+    assert not trainer._zero_operator(lambda u: np.sinh(u)) and not trainer._zero_operator(lambda u: u * u)
+    assert not trainer._zero_operator(lambda u: 1.0 / 0.0)
+    with pytest.raises(NotImplementedError):
+        trainer.setup(*cbs, lambda u: np.sinh(u), None)
+    assert callable(trainer.setup(*cbs, lambda u: 0.0, lambda u: 0.0 * u))
