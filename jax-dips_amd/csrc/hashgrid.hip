@@ -47,10 +47,12 @@ static_assert(kHE * DIPS_WAVE <= kHStage, "staging area too small");
 
 constexpr int kSHMax = 8;  // highest order of the spherical-harmonics encoder (SphericalHarmonicsEncoder, L <= 8)
 
+constexpr uint32_t kNoMask = 0xffffffffu;
 struct HashLevels {
   int L;    // levels (hashgrid), frequencies (frequency encoding) or highest order (spherical harmonics)
   int enc;  // 0 = hashgrid, 1 = frequency encoding, 2 = spherical harmonics
   uint32_t T;
+  uint32_t tmask;  // T - 1 when T is a power of two (idx % T = idx & tmask), else kNoMask: the modulo
   int64_t rows;
   float bound, two_bound;
   float scale[kHL];
@@ -112,6 +114,7 @@ static int hash_levels(const dips_hashgrid_t* hg, HashLevels& H) {
   H.L = hg->levels;
   H.enc = 0;
   H.T = (uint32_t)hg->table_size;
+  H.tmask = (H.T & (H.T - 1)) == 0 ? H.T - 1 : kNoMask;
   H.bound = hg->bound;
   H.two_bound = 2.0f * hg->bound;
   const double b = exp((log((double)hg->n_max) - log((double)hg->n_min)) / (double)(hg->levels - 1));
@@ -176,7 +179,12 @@ __device__ __forceinline__ int64_t hash_row(const HashLevels& H, int l, const ui
   } else {
     idx = vx ^ (vy * 2654435761u) ^ (vz * 805459861u);
   }
-  const int64_t row = H.offset[l] + (int64_t)(idx % H.T);
+  // the reference's tables are powers of two (lpbe.yaml:54, 2^14 to 2^24): a mask, not a division
+  if (H.tmask != kNoMask)
+    idx &= H.tmask;
+  else
+    idx %= H.T;
+  const int64_t row = H.offset[l] + (int64_t)idx;
   return row < H.rows ? row : H.rows - 1;
 }
 
