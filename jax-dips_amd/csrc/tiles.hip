@@ -136,8 +136,9 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
       sched[2] = 1.0f - powf(0.999f, t1);
     }
   }
-  // one round of loads when the wavefront's parameters fit (<= 16 per wavefront)
-  constexpr int kChunk = kPer <= 2 * kReduceChunk ? kPer : kReduceChunk;
+  // one round of loads when the wavefront's parameters fit (<= 32 per wavefront: the preconditioned
+  // networks' 420 parameters load in one round instead of four; the summation order is the same)
+  constexpr int kChunk = kPer <= 4 * kReduceChunk ? kPer : kReduceChunk;
   for (int q0 = 0; q0 < kPer; q0 += kChunk) {
     float vals[kChunk][kBlocksPerLane];
 #pragma unroll
