@@ -137,8 +137,9 @@ __device__ void tile_reduce_apply(const float* partial, int nblk, const FusedOpt
     }
   }
   // one round of loads when the wavefront's parameters fit (<= 32 per wavefront: the preconditioned
-  // networks' 420 parameters load in one round instead of four; the summation order is the same)
-  constexpr int kChunk = kPer <= 4 * kReduceChunk ? kPer : kReduceChunk;
+  // networks' 420 parameters load in one round instead of four), else rounds of 16 (the 707 of the 2x16
+  // networks: three); the summation order is the same
+  constexpr int kChunk = kPer <= 4 * kReduceChunk ? kPer : 2 * kReduceChunk;
   for (int q0 = 0; q0 < kPer; q0 += kChunk) {
     float vals[kChunk][kBlocksPerLane];
 #pragma unroll
