@@ -298,3 +298,54 @@ def test_trainer_lpbe_model_with_preconditioner_matches_oracle(tmp_path):
     assert set(tr.params["preconditioner"]["params"]) == {"Dense_0", "Dense_1", "Dense_2"}
     assert np.linalg.norm(dev - th) <= 1e-4 * np.linalg.norm(th)
     assert np.linalg.norm(dev[nn:] - th[nn:]) <= 1e-4 * np.linalg.norm(th[nn:])
+
+
+@pytest.mark.gpu
+def test_lpbe_example_default_path_matches_oracle(tmp_path):
+    """The reference LPBE example's default path (examples/benchmark_LPBE/main.py:149-187, lpbe.yaml:68-96):
+    scattered training points sampled inside and outside the sphere, on the box boundary and on the
+    interface (as its inn sampler draws them; numpy draws here), a gstate from mesh.init_gstate_3d_manually
+    with the 32^3 training spacing, the zero nonlinear operators it passes, and its model_dict (identity
+    hash network, CELU [16], preconditioner [8, 4]) -- against the oracle epoch loop on the same points:
+    per-epoch losses and final parameters to 1e-4 relative."""
+    from jax_dips.domain import mesh
+    from jax_dips.solvers.poisson import trainer
+    from test_gpu_hash_network import LPBE, _glorot
+
+    rng = np.random.default_rng(11)
+
+    def ball(m, r0, r1):
+        v = rng.normal(size=(m, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        return v * rng.uniform(r0, r1, size=(m, 1))
+
+    face = rng.uniform(-2.5, 2.5, size=(996, 3))
+    face[np.arange(996), rng.integers(0, 3, 996)] = rng.choice([-2.5, 2.5], 996)
+    P = np.concatenate([ball(1000, 0.05, 0.95), ball(1000, 1.05, 2.4), face, ball(1000, 1.0, 1.0)]).astype(np.float32)
+    d = np.float32(5.0 / 32)
+    g_tr = mesh.init_gstate_3d_manually(P[:, 0], P[:, 1], P[:, 2], d, d, d, None, None, None, None, None, None)
+    g_lvl = mesh.construct(3)[0](*problems.grid_axes(32, -2.5, 2.5))
+    prob = problems.lpbe_sphere(h=float(d))
+    dims, widths = (1, 16, 1, 16), (8, 4)
+    _, theta = _glorot(dims, 8)
+    full = np.concatenate([theta, PC.init_params(widths, seed=9)]).astype(np.float32)
+    model = {"model_type": "multiresolution_hash_network", "multiresolution_hash_network": LPBE,
+             "preconditioner": {"enable": True, "layer_widths": list(widths), "scaling_coeff": 1.0}}
+    opt = {"optimizer_name": "custom", "learning_rate": 1e-2, "sched": {"scheduler_name": "exponential", "decay_rate": 0.9}}
+    zero = lambda u: 0.0  # noqa: E731 -- the example's nonlinear_operator_m / _p (coefficients.py:110-117)
+    init_fn = trainer.setup(prob["initial_value_fn"], prob["dirichlet_bc_fn"], prob["lvl_set_fn"], prob["mu_m_fn"],
+                            prob["mu_p_fn"], prob["k_m_fn"], prob["k_p_fn"], prob["f_m_fn"], prob["f_p_fn"],
+                            prob["alpha_fn"], prob["beta_fn"], zero, zero)
+    state, solve_fn = init_fn(lvl_gstate=g_lvl, tr_gstate=g_tr, eval_gstate=g_tr, num_epochs=8, batch_size=3996,
+                              checkpoint_dir=str(tmp_path / "ck"), results_dir=str(tmp_path), optimizer_dict=opt,
+                              model_dict=model, init_params=full)
+    _, es, le = solve_fn(sim_state=state)
+    tr = solve_fn.trainer
+    cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
+    bounds = (-2.5, 2.5, -2.5, 2.5, -2.5, 2.5)
+    th, ole, _, _ = T.train(P, np.array([d, d, d], np.float32), bounds, cb, dims, full, 8, 3996, opt,
+                            precond=dict(widths=list(widths), scale=1.0), act="celu")
+    print("device", le, "oracle", ole)
+    assert np.all(np.abs(le - ole) <= 1e-4 * np.abs(ole))
+    dev = tr.flatten(tr.params)
+    assert np.linalg.norm(dev - th) <= 1e-4 * np.linalg.norm(th)
