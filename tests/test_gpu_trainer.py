@@ -4,8 +4,9 @@ batches, optimizer chain and explicit initial parameters.
 
 Tolerance: the device path builds records in FP64 from float64 callback samples
 and trains in float32; the oracle trains in float64 arithmetic with float32
-parameters and optimizer state.  Per-epoch losses agree to 1e-4 relative over
-a short run; the final evaluation matches the oracle network to 1e-5 absolute.
+parameters and optimizer state.  Per-epoch losses agree to 1e-5 relative over
+a short run (1e-4 with a padded last batch); the final evaluation matches the
+oracle network to 1e-5 absolute.
 """
 import numpy as np
 import pytest
@@ -55,7 +56,7 @@ def test_trainer_matches_oracle_loop(tmp_path):
     th, ole, steps, _ = T.train(P, h, bounds, cb, dims, theta0, 8, 2048, OPT)
     print("device", le, "oracle", ole)
     assert np.array_equal(es, np.arange(8))
-    assert np.all(np.abs(le - ole) <= 1e-4 * np.abs(ole))
+    assert np.all(np.abs(le - ole) <= 1e-5 * np.abs(ole))
     dev_theta = tr.model.flatten(tr.params)
     assert np.linalg.norm(dev_theta - th) <= 1e-4 * np.linalg.norm(th)
     # evaluation on the eval grid = network of the final parameters
@@ -235,7 +236,7 @@ def test_trainer_scattered_points_matches_oracle_loop(tmp_path):
     """Scattered training points, as the reference's LPBE example builds its sampled sets
     (examples/benchmark_LPBE/main.py:149-187 with domain/mesh.py:init_gstate_3d_manually): x, y, z
     are the points' coordinates, not axes.  The drop-in trainer places them as a points-only set and
-    matches the oracle epoch loop on the same points (losses and parameters to 1e-4 relative), and its
+    matches the oracle epoch loop on the same points (losses to 1e-5, parameters to 1e-4 relative), and its
     evaluation at the points matches the oracle network."""
     from jax_dips.domain import mesh
     from jax_dips.solvers.poisson import trainer
@@ -262,7 +263,7 @@ def test_trainer_scattered_points_matches_oracle_loop(tmp_path):
     cb = D.Callbacks(problems.oracle_callbacks(prob), np.float64)
     th, ole, _, _ = T.train(P, h, bounds, cb, dims, theta0, 8, 2048, OPT)
     print("device", le, "oracle", ole)
-    assert np.all(np.abs(le - ole) <= 1e-4 * np.abs(ole))
+    assert np.all(np.abs(le - ole) <= 1e-5 * np.abs(ole))
     dev = tr.model.flatten(tr.params)
     assert np.linalg.norm(dev - th) <= 1e-4 * np.linalg.norm(th)
     phase = cb("phi", P) >= 0
