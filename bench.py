@@ -718,8 +718,10 @@ def run_hashgrid(args):
                                f"{eng.P} parameters",
                    "global_batch": B, "grid": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "whole dips_hash_train_step (9 launches), compulsory bytes",
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(f"hashgrid sphere{n}") if args.batch == 0 else None,
+                     "kernel": "whole dips_hash_train_step (9 launches), compulsory bytes; traffic: the measured "
+                               "HBM bytes of one step (its 9 kernels)",
                      "bytes_per_step": bytes_step, "entries_per_step": ents / K},
         "records_build_s": t_records,
         "loss_tail": eng.loss_hist[: int(eng.count.item())][-4:].cpu().numpy().tolist(),
